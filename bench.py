@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Closest-hit throughput on one MI355X.
+
+Workload: the random-triangle microbenchmark -- 1,000,000 triangles (centre
+~ U[0,1)^3, vertices centre + 0.01 * U[-1,1)^3, seed 2) in a SAH 4-wide QBVH,
+4,194,304 rays (origin ~ U[0,1)^3, direction uniform on S^2, tnear 0,
+tfar +inf, seed 3).  One step = one closest-hit launch over all rays, inputs
+resident in HBM.  Also reports any-hit (tfar ~ U[0.01, 2], seed 4).
+
+Prints one JSON line.  The roofline uses algorithmic bytes per ray
+  B_ray = 32 (ray) + 16 (hit) + 128 * nodes/ray + 48 * prims/ray
+with nodes/ray and prims/ray counted by the CPU oracle's traversal of the
+same QBVH on a sample of the same rays.  The CPU baseline is that oracle
+traversal (a multi-threaded C restatement of LiAR's QBVH query) timed on the
+host cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s closest-hit at 1 GPU; time-to-equal-MSE guided vs plain GPU tracer"
+HBM_PEAK_GBS = 8000.0
+N_TRIS = 1_000_000
+N_RAYS = 1 << 22
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="target duration of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--count-sample", type=int, default=1 << 18,
+                    help="rays used for the oracle's nodes/prims-per-ray counts and parity")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        log("bench.py measures one GPU only: this build has no multi-GPU path "
+            "(the ray query shards by rays with a replicated QBVH; see DESIGN.md).")
+        return 2
+
+    import numpy as np
+    import torch
+
+    import liar_amd as la
+    import oracle
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    ctx = la.Context(0)
+
+    log("[bench] generating scene and rays")
+    verts, tris = la.random_triangles(N_TRIS, seed=2)
+    bvh = la.Qbvh(ctx, verts, tris)
+    info = bvh.info()
+    rays_h = la.random_rays(N_RAYS, seed=3)
+    rays_any_h = la.random_rays(N_RAYS, seed=4, tfar_lo=0.01, tfar_hi=2.0)
+    rays = torch.from_numpy(rays_h).to(dev)
+    rays_any = torch.from_numpy(rays_any_h).to(dev)
+    prim = torch.empty(N_RAYS, dtype=torch.int32, device=dev)
+    tuv = torch.empty((3, N_RAYS), dtype=torch.float32, device=dev)
+    occ = torch.empty(N_RAYS, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a, b in evs:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kern_ms = [a.elapsed_time(b) for a, b in evs]
+        return wall, float(np.mean(kern_ms))
+
+    log(f"[bench] closest-hit: {args.warmup} warmup + {args.steps} timed launches of {N_RAYS} rays")
+    wall_c, kern_c = timed(lambda: bvh.trace_closest(rays, prim, tuv), args.steps, args.warmup)
+    log("[bench] any-hit")
+    wall_a, kern_a = timed(lambda: bvh.trace_any(rays_any, occ), args.steps, args.warmup)
+
+    mrays = N_RAYS * args.steps / wall_c / 1e6
+    mrays_any = N_RAYS * args.steps / wall_a / 1e6
+
+    # oracle counts + parity on a sample of the same rays
+    nodes, prims = bvh.export()
+    S = min(args.count_sample, N_RAYS)
+    log(f"[bench] oracle counts / parity on {S} rays")
+    t0 = time.perf_counter()
+    ref = oracle.qbvh_closest(nodes, prims, rays_h[:, :S])
+    t_count = time.perf_counter() - t0
+    ref_any = oracle.qbvh_any(nodes, prims, rays_any_h[:, :S])
+    g_prim = prim[:S].cpu().numpy().view(np.uint32)
+    g_tuv = tuv[:, :S].cpu().numpy()
+    mism = int((g_prim != ref[0]).sum()) + sum(
+        int((g_tuv[k].view(np.uint32) != ref[1 + k].view(np.uint32)).sum()) for k in range(3))
+    mism_any = int((occ[:S].cpu().numpy() != ref_any[0]).sum())
+    nodes_per_ray = ref[4] / S
+    prims_per_ray = ref[5] / S
+    b_ray = 32 + 16 + 128 * nodes_per_ray + 48 * prims_per_ray
+    any_nodes = ref_any[1] / S
+    any_prims = ref_any[2] / S
+    b_ray_any = 32 + 1 + 128 * any_nodes + 48 * any_prims
+    achieved = N_RAYS * b_ray / (kern_c * 1e-3) / 1e9
+    achieved_any = N_RAYS * b_ray_any / (kern_a * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(oracle.default_threads(), 16)
+        rate0 = S / max(t_count, 1e-6)  # rays/s with default threads
+        n_cpu = int(min(N_RAYS, max(1 << 16, rate0 * args.cpu_seconds * threads / oracle.default_threads())))
+        log(f"[bench] CPU baseline: oracle QBVH closest-hit on {n_cpu} rays, {threads} threads")
+        t0 = time.perf_counter()
+        oracle.qbvh_closest(nodes, prims, rays_h[:, :n_cpu], threads=threads)
+        t_cpu = time.perf_counter() - t0
+        cpu = {
+            "value": round(n_cpu / t_cpu / 1e6, 4),
+            "unit": "Mrays/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {n_cpu} of the {N_RAYS} closest-hit rays, same QBVH, "
+                      f"{t_cpu:.1f} s on {threads} host threads",
+        }
+
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_c / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "random triangle soup, closest-hit",
+            "triangles": N_TRIS,
+            "rays": N_RAYS,
+            "scene": "centre U[0,1)^3, vertices centre + 0.01*U[-1,1)^3, seed 2",
+            "ray_gen": "origin U[0,1)^3, direction uniform on S^2, tnear 0, tfar inf, seed 3",
+            "qbvh": f"SAH 4-wide, leaf<=4, {info['n_nodes']} nodes, depth {info['max_depth']}",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "detail": {
+            "kernel_ms_closest": round(kern_c, 4),
+            "bytes_per_ray_closest": round(b_ray, 1),
+            "nodes_per_ray_closest": round(nodes_per_ray, 3),
+            "prims_per_ray_closest": round(prims_per_ray, 3),
+            "any_hit_mrays_s": round(mrays_any, 3),
+            "kernel_ms_any": round(kern_a, 4),
+            "bytes_per_ray_any": round(b_ray_any, 1),
+            "achieved_gbs_any": round(achieved_any, 1),
+            "frac_any": round(achieved_any / HBM_PEAK_GBS, 4),
+            "parity_sample": S,
+            "parity_mismatches_closest": mism,
+            "parity_mismatches_any": mism_any,
+            "hit_rate_closest": round(float((ref[0] != oracle.MISS).mean()), 4),
+            "qbvh_build_s": round(info["build_seconds"], 3),
+            "trace_config": ctx.trace_config(),
+        },
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

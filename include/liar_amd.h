@@ -1,0 +1,191 @@
+/*
+ * liar_amd.h -- C-ABI of the MI355X (gfx950) ray-tracing hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.
+ * Every entry point returns LR_OK (0) on success or a negative LR_E* code;
+ * lr_last_error() then returns a thread-local, human-readable message.
+ *
+ * Pointer conventions
+ *   - "host" pointers are borrowed for the duration of the call only.
+ *   - "device" pointers address HBM of the context's device (hipMalloc'd or
+ *     torch CUDA tensors); they are borrowed for the duration of the call and
+ *     the work is ordered on `stream` (a hipStream_t, NULL = default stream).
+ *   - Calls on distinct streams are thread-safe; one lr_qbvh may serve
+ *     concurrent read-only traces.
+ *
+ * Geometry semantics follow LiAR (reference paths are relative to the
+ * reference repository root):
+ *   - ray directions are unit vectors, so t is a world distance
+ *     (src/liar/kernel/kernel_common.h:116);
+ *   - a hit t is "in range" iff t > tnear*(1+eps) && t < tfar*(1-eps),
+ *     eps = LR_TOLERANCE (src/liar/kernel/bounded_ray.h:69-77,
+ *     src/liar/kernel/kernel_common.cpp:29-33);
+ *   - closest-hit on the triangle set takes the nearest t > tnear over ALL
+ *     triangles and then reports a miss if that t is not in range
+ *     (src/liar/scenery/triangle_mesh.cpp:291-317); each sphere takes its
+ *     nearest root t > tnear and keeps it iff in range
+ *     (src/liar/scenery/sphere.cpp:104-118); the overall result is the
+ *     minimum over the triangle set and the spheres
+ *     (src/liar/scenery/list.cpp:106-133);
+ *   - any-hit: a triangle with tnear < t < tfar, or a sphere whose nearest
+ *     root t > tnear is in range (src/liar/scenery/triangle_mesh.cpp:321-330,
+ *     src/liar/scenery/sphere.cpp:122-127, src/liar/scenery/list.cpp:137-150);
+ *   - equal t is broken by the smaller primitive id (LiAR keeps the
+ *     first-found, which depends on traversal order; this library fixes the
+ *     rule so that every traversal order gives the same answer).
+ *   - primitive ids: triangle k -> k; sphere j -> n_triangles + j
+ *     (src/liar/scenery/triangle_mesh.cpp:310-311).
+ */
+#ifndef LIAR_AMD_H
+#define LIAR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LR_OK 0
+#define LR_E_INVALID (-1)   /* bad argument (null pointer, size, handle)     */
+#define LR_E_DEVICE (-2)    /* HIP runtime error                            */
+#define LR_E_NOMEM (-3)     /* allocation failed                            */
+#define LR_E_UNSUPPORTED (-4)
+
+#define LR_MISS 0xFFFFFFFFu /* prim id of a miss                           */
+#define LR_TOLERANCE 1e-6f  /* LiAR's global tolerance (kernel_common.cpp:31) */
+
+/* Flattened 4-wide QBVH node, 128 B (one HBM/L2 line).  SoA over the four
+ * children so one lane fetches a whole node with 8 dwordx4 loads.
+ * child[i] encoding:
+ *   LR_CHILD_EMPTY           no child
+ *   bit31 == 0               inner node index
+ *   bit31 == 1               leaf: first prim record = (c >> 2) & 0x1FFFFFFF,
+ *                            prim count = (c & 3) + 1 (1..4)
+ * Bounds are padded outward at build time so FP32 slab tests are
+ * conservative. */
+#define LR_CHILD_EMPTY 0xFFFFFFFFu
+typedef struct lr_node {
+    float lo_x[4], hi_x[4];
+    float lo_y[4], hi_y[4];
+    float lo_z[4], hi_z[4];
+    uint32_t child[4];
+    uint32_t reserved[4];
+} lr_node;
+
+/* Primitive record, 48 B, stored in leaf order.
+ *   triangle: a = (v0, id), b = (v1 - v0, kind=0), c = (v2 - v0, 0)
+ *   sphere:   a = (centre, id), b = (radius, radius^2, 0, kind=1), c = 0
+ * id / kind are stored as uint32 bit patterns in the .w lanes. */
+typedef struct lr_prim {
+    float a[4];
+    float b[4];
+    float c[4];
+} lr_prim;
+
+#define LR_PRIM_TRIANGLE 0u
+#define LR_PRIM_SPHERE 1u
+
+/* Device SoA ray batch, 32 B per ray (all device pointers, length n). */
+typedef struct lr_rays_soa {
+    const float *ox, *oy, *oz;
+    const float *dx, *dy, *dz;
+    const float *tnear, *tfar;
+} lr_rays_soa;
+
+/* Device SoA hit batch, 16 B per ray. Miss: prim = LR_MISS, t = +inf,
+ * u = v = 0.  Triangles: (u, v) barycentrics, point = v0 + u e1 + v e2
+ * (src/liar/scenery/triangle.cpp:85-98).  Spheres: u = v = 0. */
+typedef struct lr_hits_soa {
+    uint32_t *prim;
+    float *t, *u, *v;
+} lr_hits_soa;
+
+typedef struct lr_qbvh_info {
+    uint32_t n_nodes;
+    uint32_t n_prims;
+    uint32_t n_triangles;
+    uint32_t n_spheres;
+    uint32_t max_depth;   /* levels below the root                        */
+    uint32_t max_stack;   /* traversal stack bound = 3*(max_depth+1) + 1  */
+    double sah_cost;      /* SAH cost, node cost 1, prim cost 1           */
+    double build_seconds;
+    float scene_lo[3], scene_hi[3];
+} lr_qbvh_info;
+
+typedef struct lr_context lr_context;
+typedef struct lr_qbvh lr_qbvh;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* lr_last_error(void);
+
+/* Library version string. */
+const char* lr_version(void);
+
+/* Replaces the process-global RenderEngine set-up of LiAR
+ * (src/liar/kernel/render_engine.cpp:170-247): binds one HIP device. */
+int lr_context_create(int device, lr_context** out);
+int lr_context_destroy(lr_context* ctx);
+
+/* Replaces the TriangleMesh ctor + Lass TriangleMesh3D QBVH build and the
+ * object-level QbvhTree::doPreProcess over Sphere children
+ * (src/liar/scenery/triangle_mesh.cpp:69-85, src/liar/scenery/object_trees.h:100-108).
+ *   verts   host, 3*nv floats (x, y, z per vertex)
+ *   tris    host, 3*nt uint32 vertex indices
+ *   spheres host, 4*ns floats (cx, cy, cz, radius); may be NULL if ns == 0
+ * SAH 4-wide build on the host, then the node and prim arrays are uploaded
+ * to the context's device.  ctx == NULL builds a host-only handle (no device
+ * memory; lr_qbvh_get_info / lr_qbvh_export work, tracing it is an error). */
+int lr_qbvh_create(lr_context* ctx, const float* verts, uint32_t nv,
+                   const uint32_t* tris, uint32_t nt, const float* spheres,
+                   uint32_t ns, lr_qbvh** out);
+int lr_qbvh_destroy(lr_qbvh* bvh);
+int lr_qbvh_get_info(const lr_qbvh* bvh, lr_qbvh_info* info);
+/* Copies the flattened arrays back to host memory (nodes: n_nodes records,
+ * prims: n_prims records); either pointer may be NULL. */
+int lr_qbvh_export(const lr_qbvh* bvh, lr_node* nodes, lr_prim* prims);
+
+/* Closest hit for n rays.  Replaces SceneObject::intersect over the scene
+ * (src/liar/kernel/scene_object.h:110-113) and its TriangleMesh / Sphere /
+ * QbvhTree / List implementations (triangle_mesh.cpp:291-317,
+ * sphere.cpp:104-118, object_trees.h:110-135, list.cpp:106-133). */
+int lr_trace_closest(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n,
+                     lr_hits_soa* hits, void* stream);
+
+/* Any hit (occlusion) for n rays; occluded[i] = 1 or 0.  Replaces
+ * SceneObject::isIntersecting (scene_object.h:110-113, triangle_mesh.cpp:321-330,
+ * sphere.cpp:122-127, object_trees.h:137-143, list.cpp:137-150). */
+int lr_trace_any(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n,
+                 uint8_t* occluded, void* stream);
+
+/* Traversal launch parameters (block size in threads, blocks per CU, LDS
+ * short-stack depth).  0 keeps the current value.  The defaults are the
+ * tuned ones. */
+typedef struct lr_trace_config {
+    uint32_t block;
+    uint32_t blocks_per_cu;
+    uint32_t lds_stack;
+} lr_trace_config;
+int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg);
+int lr_get_trace_config(const lr_context* ctx, lr_trace_config* cfg);
+
+/* ---- deterministic synthetic inputs (host, counter-based Philox4x32-10) --- */
+
+/* Random triangle soup: centre c ~ U[0,1)^3, v_k = c + scale * xi_k,
+ * xi_k ~ U[-1,1)^3.  verts: 9*nt floats, tris: 3*nt uint32 (identity). */
+int lr_gen_random_triangles(uint64_t seed, uint32_t nt, float scale,
+                            float* verts, uint32_t* tris);
+/* Rays: origin ~ U[0,1)^3, direction uniform on S^2 (z = 1 - 2u,
+ * phi = 2 pi v), tnear = 0, tfar = +inf or ~ U[tfar_lo, tfar_hi) when
+ * tfar_hi > 0.  All outputs host arrays of n floats (SoA). */
+int lr_gen_random_rays(uint64_t seed, uint32_t n, float tfar_lo, float tfar_hi,
+                       float* ox, float* oy, float* oz, float* dx, float* dy,
+                       float* dz, float* tnear, float* tfar);
+/* Philox4x32-10 block (Salmon et al., SC'11) for tests: out[4]. */
+void lr_philox4x32(uint64_t key, uint32_t c0, uint32_t c1, uint32_t c2,
+                   uint32_t c3, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,11 @@
+"""liar_amd -- MI355X (gfx950) native implementation of LiAR's ray-tracing hot
+path: QBVH closest/any-hit traversal and the spectral path-tracing integrand.
+
+The compute path is hand-written HIP in ``liar_amd/csrc`` behind the C-ABI of
+``include/liar_amd.h``; this package is the host-side mirror of LiAR's
+interfaces on top of that C-ABI.
+"""
+from ._native import LiarError, LIB_PATH  # noqa: F401
+from .qbvh import Context, Qbvh, random_rays, random_triangles, philox4x32  # noqa: F401
+
+__version__ = "0.1.0"

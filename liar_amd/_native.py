@@ -1,0 +1,107 @@
+"""ctypes binding of the C-ABI in ``include/liar_amd.h``.
+
+The shared library is built in-tree by ``python -m liar_amd.build``.  Loading
+fails loudly when it is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libliar_amd.so"
+
+LR_OK = 0
+LR_MISS = 0xFFFFFFFF
+LR_CHILD_EMPTY = 0xFFFFFFFF
+NODE_BYTES = 128
+PRIM_BYTES = 48
+
+
+class LiarError(RuntimeError):
+    """A non-zero status from the native library (message from lr_last_error)."""
+
+
+class RaysSoa(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tnear", "tfar")]
+
+
+class HitsSoa(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("prim", "t", "u", "v")]
+
+
+class QbvhInfo(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32),
+        ("n_prims", C.c_uint32),
+        ("n_triangles", C.c_uint32),
+        ("n_spheres", C.c_uint32),
+        ("max_depth", C.c_uint32),
+        ("max_stack", C.c_uint32),
+        ("sah_cost", C.c_double),
+        ("build_seconds", C.c_double),
+        ("scene_lo", C.c_float * 3),
+        ("scene_hi", C.c_float * 3),
+    ]
+
+
+class TraceConfig(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_stack", C.c_uint32)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_F = C.c_float
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "lr_last_error": (C.c_char_p, []),
+    "lr_version": (C.c_char_p, []),
+    "lr_context_create": (_I, [_I, C.POINTER(_P)]),
+    "lr_context_destroy": (_I, [_P]),
+    "lr_qbvh_create": (_I, [_P, _P, _U32, _P, _U32, _P, _U32, C.POINTER(_P)]),
+    "lr_qbvh_destroy": (_I, [_P]),
+    "lr_qbvh_get_info": (_I, [_P, C.POINTER(QbvhInfo)]),
+    "lr_qbvh_export": (_I, [_P, _P, _P]),
+    "lr_trace_closest": (_I, [_P, C.POINTER(RaysSoa), _U32, C.POINTER(HitsSoa), _P]),
+    "lr_trace_any": (_I, [_P, C.POINTER(RaysSoa), _U32, _P, _P]),
+    "lr_set_trace_config": (_I, [_P, C.POINTER(TraceConfig)]),
+    "lr_get_trace_config": (_I, [_P, C.POINTER(TraceConfig)]),
+    "lr_gen_random_triangles": (_I, [_U64, _U32, _F, _P, _P]),
+    "lr_gen_random_rays": (_I, [_U64, _U32, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "lr_philox4x32": (None, [_U64, _U32, _U32, _U32, _U32, _P]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libliar_amd.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"native library {LIB_PATH} is missing; build it with `python -m liar_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != LR_OK:
+        msg = lib().lr_last_error().decode(errors="replace")
+        raise LiarError(f"{what}: status {rc}: {msg}" if what else f"status {rc}: {msg}")
+
+
+def ptr(a) -> int:
+    """Address of a numpy array or torch tensor (contiguous)."""
+    if a is None:
+        return 0
+    if hasattr(a, "data_ptr"):
+        return int(a.data_ptr())
+    return int(a.ctypes.data)

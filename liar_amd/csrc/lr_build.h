@@ -1,0 +1,25 @@
+// lr_build.h -- host-side QBVH build interface (internal to the library).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/liar_amd.h"
+
+namespace lr {
+
+struct BuiltQbvh {
+    std::vector<lr_node> nodes;
+    std::vector<lr_prim> prims;
+    uint32_t n_triangles = 0, n_spheres = 0, max_depth = 0;
+    double sah_cost = 0.0, build_seconds = 0.0;
+    float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {0, 0, 0};
+};
+
+// Returns 0 on success, -1 with `err` set on invalid input.
+int build_qbvh(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, const float* spheres,
+               uint32_t ns, BuiltQbvh& out, std::string& err);
+
+}  // namespace lr

@@ -1,0 +1,597 @@
+// trace.hip -- gfx950 QBVH closest-hit / any-hit traversal and the C-ABI for
+// contexts, QBVH handles and ray batches.
+//
+// Execution model: persistent wave64 kernel, one ray per lane, grid sized to
+// blocks_per_cu x #CUs.  Each lane walks the 4-wide QBVH front to back with a
+// short stack of (entry, entry distance) pairs in LDS; pushes past the LDS
+// depth spill to a per-thread global region, so any tree depth is handled.
+//
+// Closest-hit keeps two candidates, following LiAR's per-object filtering
+// (src/liar/scenery/list.cpp:106-133): the nearest triangle with t > tnear
+// (src/liar/scenery/triangle_mesh.cpp:291-317, whose interval check is
+// applied once at the end) and the nearest in-range sphere
+// (src/liar/scenery/sphere.cpp:104-118).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "lr_geom.h"
+#include "lr_internal.h"
+
+namespace {
+
+// Box tests use (plane - origin) * inv_dir; scaling the far distance by
+// 1 + 2*gamma(3) makes the test conservative (Ize, "Robust BVH Ray
+// Traversal", JCGT 2013).
+constexpr float kIze = 1.0000003576278687f;
+// Slack on the current best distance so that FP32 rounding of a primitive's
+// hit distance can never cull the box that holds it.
+constexpr float kCullSlack = 1.0f + 1.0f / 65536.0f;
+
+struct RaysArg {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tnear, *tfar;
+};
+struct HitsArg {
+    uint32_t* prim;
+    float *t, *u, *v;
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+    return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
+}
+
+// Slab test of the four children of a node; returns per-child entry distance
+// (+inf when culled).
+__device__ __forceinline__ void node_test(const float4* __restrict__ np, float ox, float oy, float oz, float ix,
+                                          float iy, float iz, float tnear, float cull, float t[4], uint32_t e[4]) {
+    const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+    const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float ax = (lx[i] - ox) * ix, bx = (hx[i] - ox) * ix;
+        const float ay = (ly[i] - oy) * iy, by = (hy[i] - oy) * iy;
+        const float az = (lz[i] - oz) * iz, bz = (hz[i] - oz) * iz;
+        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tnear));
+        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)) , fmaxf(az, bz)) * kIze;
+        const bool hit = tn <= fminf(tf, cull) && c[i] != LR_CHILD_EMPTY;
+        t[i] = hit ? tn : INFINITY;
+        e[i] = c[i];
+    }
+}
+
+#define LR_CSWAP(a, b)                     \
+    {                                      \
+        const bool s_ = t[b] < t[a];       \
+        const float ta_ = t[a], tb_ = t[b]; \
+        const uint32_t ea_ = e[a], eb_ = e[b]; \
+        t[a] = s_ ? tb_ : ta_;             \
+        t[b] = s_ ? ta_ : tb_;             \
+        e[a] = s_ ? eb_ : ea_;             \
+        e[b] = s_ ? ea_ : eb_;             \
+    }
+
+template <int BLOCK, int SHORT>
+struct Stack {
+    uint32_t* s_e;
+    float* s_t;
+    uint2* spill;
+    uint32_t stride;  // threads in the grid
+    uint32_t gtid, tid;
+    int sp;
+    int cap;
+
+    __device__ __forceinline__ void push(uint32_t e, float t) {
+        if (sp < SHORT) {
+            s_e[sp * BLOCK + tid] = e;
+            s_t[sp * BLOCK + tid] = t;
+        } else if (sp < cap) {
+            spill[(size_t)(sp - SHORT) * stride + gtid] = make_uint2(e, __float_as_uint(t));
+        } else {
+            return;  // unreachable: cap >= 3 * (max_depth + 1) + 1
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(uint32_t& e, float& t) {
+        --sp;
+        if (sp < SHORT) {
+            e = s_e[sp * BLOCK + tid];
+            t = s_t[sp * BLOCK + tid];
+        } else {
+            const uint2 v = spill[(size_t)(sp - SHORT) * stride + gtid];
+            e = v.x;
+            t = __uint_as_float(v.y);
+        }
+    }
+};
+
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK) void k_closest(const lr_node* __restrict__ nodes,
+                                                   const lr_prim* __restrict__ prims, RaysArg rays,
+                                                   uint32_t n, HitsArg hits, uint2* __restrict__ spill,
+                                                   int cap) {
+    __shared__ uint32_t s_e[SHORT * BLOCK];
+    __shared__ float s_t[SHORT * BLOCK];
+    Stack<BLOCK, SHORT> st;
+    st.s_e = s_e;
+    st.s_t = s_t;
+    st.spill = spill;
+    st.stride = gridDim.x * BLOCK;
+    st.tid = threadIdx.x;
+    st.gtid = blockIdx.x * BLOCK + threadIdx.x;
+    st.cap = cap;
+    const float4* __restrict__ nf = reinterpret_cast<const float4*>(nodes);
+    const float4* __restrict__ pf = reinterpret_cast<const float4*>(prims);
+
+    for (uint32_t r = st.gtid; r < n; r += st.stride) {
+        const float ox = rays.ox[r], oy = rays.oy[r], oz = rays.oz[r];
+        const float dx = rays.dx[r], dy = rays.dy[r], dz = rays.dz[r];
+        const float tnear = rays.tnear[r], tfar = rays.tfar[r];
+        const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+        const float far_lim = tfar * (1.0f - 1e-6f);
+        const float near_lim = tnear * (1.0f + 1e-6f);
+
+        float tri_t = far_lim, tri_u = 0.0f, tri_v = 0.0f;
+        uint32_t tri_id = LR_MISS;
+        bool doomed = false;
+        float sph_t = far_lim;
+        uint32_t sph_id = LR_MISS;
+
+        st.sp = 0;
+        uint32_t cur = 0;  // root
+        for (;;) {
+            const float cull = fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
+            if (!(cur & 0x80000000u)) {
+                float t[4];
+                uint32_t e[4];
+                node_test(nf + (size_t)cur * 8, ox, oy, oz, ix, iy, iz, tnear, cull, t, e);
+                LR_CSWAP(0, 1);
+                LR_CSWAP(2, 3);
+                LR_CSWAP(0, 2);
+                LR_CSWAP(1, 3);
+                LR_CSWAP(1, 2);
+                if (t[0] != INFINITY) {
+                    if (t[3] != INFINITY) st.push(e[3], t[3]);
+                    if (t[2] != INFINITY) st.push(e[2], t[2]);
+                    if (t[1] != INFINITY) st.push(e[1], t[1]);
+                    cur = e[0];
+                    continue;
+                }
+            } else {
+                const uint32_t first = (cur >> 2) & 0x1FFFFFFFu;
+                const uint32_t cnt = (cur & 3u) + 1u;
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const float4* q = pf + (size_t)(first + k) * 3;
+                    const float4 a = q[0], b = q[1];
+                    const uint32_t id = __float_as_uint(a.w);
+                    if (__float_as_uint(b.w) == LR_PRIM_TRIANGLE) {
+                        const float4 c = q[2];
+                        float th, uh, vh;
+                        if (lr::tri_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z,
+                                              th, uh, vh) &&
+                            th > tnear && th < far_lim && (th < tri_t || (th == tri_t && id < tri_id))) {
+                            tri_t = th;
+                            tri_u = uh;
+                            tri_v = vh;
+                            tri_id = id;
+                            doomed = !(th > near_lim);
+                        }
+                    } else {
+                        float th;
+                        if (lr::sphere_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.y, tnear, th) &&
+                            lr::in_range(th, tnear, tfar) && (th < sph_t || (th == sph_t && id < sph_id))) {
+                            sph_t = th;
+                            sph_id = id;
+                        }
+                    }
+                }
+            }
+            // pop the next entry that can still hold a better hit
+            const float cull2 = fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
+            bool found = false;
+            while (st.sp > 0) {
+                uint32_t e;
+                float te;
+                st.pop(e, te);
+                if (te <= cull2) {
+                    cur = e;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) break;
+        }
+        uint32_t prim = LR_MISS;
+        float t = INFINITY, u = 0.0f, v = 0.0f;
+        const bool tri_ok = tri_id != LR_MISS && !doomed;
+        if (tri_ok && (sph_id == LR_MISS || tri_t < sph_t || (tri_t == sph_t && tri_id < sph_id))) {
+            prim = tri_id;
+            t = tri_t;
+            u = tri_u;
+            v = tri_v;
+        } else if (sph_id != LR_MISS) {
+            prim = sph_id;
+            t = sph_t;
+        }
+        hits.prim[r] = prim;
+        hits.t[r] = t;
+        hits.u[r] = u;
+        hits.v[r] = v;
+    }
+}
+
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK) void k_any(const lr_node* __restrict__ nodes, const lr_prim* __restrict__ prims,
+                                               RaysArg rays, uint32_t n, uint8_t* __restrict__ occluded,
+                                               uint2* __restrict__ spill, int cap) {
+    __shared__ uint32_t s_e[SHORT * BLOCK];
+    __shared__ float s_t[SHORT * BLOCK];
+    Stack<BLOCK, SHORT> st;
+    st.s_e = s_e;
+    st.s_t = s_t;
+    st.spill = spill;
+    st.stride = gridDim.x * BLOCK;
+    st.tid = threadIdx.x;
+    st.gtid = blockIdx.x * BLOCK + threadIdx.x;
+    st.cap = cap;
+    const float4* __restrict__ nf = reinterpret_cast<const float4*>(nodes);
+    const float4* __restrict__ pf = reinterpret_cast<const float4*>(prims);
+
+    for (uint32_t r = st.gtid; r < n; r += st.stride) {
+        const float ox = rays.ox[r], oy = rays.oy[r], oz = rays.oz[r];
+        const float dx = rays.dx[r], dy = rays.dy[r], dz = rays.dz[r];
+        const float tnear = rays.tnear[r], tfar = rays.tfar[r];
+        const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+        const float cull = tfar * kCullSlack;
+        bool hit = false;
+        st.sp = 0;
+        uint32_t cur = 0;
+        for (;;) {
+            if (!(cur & 0x80000000u)) {
+                float t[4];
+                uint32_t e[4];
+                node_test(nf + (size_t)cur * 8, ox, oy, oz, ix, iy, iz, tnear, cull, t, e);
+                // any order is valid; visit the nearest child first
+                int best = -1;
+                float bt = INFINITY;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (t[i] < bt) {
+                        bt = t[i];
+                        best = i;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (i != best && t[i] != INFINITY) st.push(e[i], t[i]);
+                if (best >= 0) {
+                    cur = e[best];
+                    continue;
+                }
+            } else {
+                const uint32_t first = (cur >> 2) & 0x1FFFFFFFu;
+                const uint32_t cnt = (cur & 3u) + 1u;
+                for (uint32_t k = 0; k < cnt && !hit; ++k) {
+                    const float4* q = pf + (size_t)(first + k) * 3;
+                    const float4 a = q[0], b = q[1];
+                    if (__float_as_uint(b.w) == LR_PRIM_TRIANGLE) {
+                        const float4 c = q[2];
+                        float th, uh, vh;
+                        hit = lr::tri_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y,
+                                                c.z, th, uh, vh) &&
+                              th > tnear && th < tfar;
+                    } else {
+                        float th;
+                        hit = lr::sphere_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.y, tnear, th) &&
+                              lr::in_range(th, tnear, tfar);
+                    }
+                }
+                if (hit) break;
+            }
+            if (st.sp == 0) break;
+            float te;
+            st.pop(cur, te);
+        }
+        occluded[r] = hit ? 1 : 0;
+    }
+}
+
+__global__ void k_fill_miss(uint32_t n, HitsArg hits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        hits.prim[i] = LR_MISS;
+        hits.t[i] = INFINITY;
+        hits.u[i] = 0.0f;
+        hits.v[i] = 0.0f;
+    }
+}
+
+__global__ void k_fill_u8(uint32_t n, uint8_t* p, uint8_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+using ClosestFn = void (*)(const lr_node*, const lr_prim*, RaysArg, uint32_t, HitsArg, uint2*, int);
+using AnyFn = void (*)(const lr_node*, const lr_prim*, RaysArg, uint32_t, uint8_t*, uint2*, int);
+
+
+struct KernelEntry {
+    uint32_t block, lds_stack;
+    ClosestFn closest;
+    AnyFn any;
+};
+
+const KernelEntry kKernels[] = {
+    {64, 8, k_closest<64, 8>, k_any<64, 8>},       {64, 16, k_closest<64, 16>, k_any<64, 16>},
+    {64, 24, k_closest<64, 24>, k_any<64, 24>},    {128, 8, k_closest<128, 8>, k_any<128, 8>},
+    {128, 16, k_closest<128, 16>, k_any<128, 16>}, {128, 24, k_closest<128, 24>, k_any<128, 24>},
+    {256, 8, k_closest<256, 8>, k_any<256, 8>},    {256, 16, k_closest<256, 16>, k_any<256, 16>},
+    {256, 24, k_closest<256, 24>, k_any<256, 24>},
+};
+
+const KernelEntry* find_kernel(uint32_t block, uint32_t lds_stack) {
+    for (const auto& k : kKernels)
+        if (k.block == block && k.lds_stack == lds_stack) return &k;
+    return nullptr;
+}
+
+int check_rays(const lr_rays_soa* rays, uint32_t n) {
+    if (!rays) return lr::fail(LR_E_INVALID, "rays is NULL");
+    if (n && (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz || !rays->tnear ||
+              !rays->tfar))
+        return lr::fail(LR_E_INVALID, "a ray array pointer is NULL");
+    return LR_OK;
+}
+
+struct Launch {
+    const KernelEntry* k;
+    uint32_t grid;
+    int cap;
+    uint2* spill;
+};
+
+int prepare(lr_qbvh* bvh, uint32_t n, hipStream_t stream, Launch& L) {
+    lr_context* ctx = bvh->ctx;
+    lr_trace_config cfg;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        cfg = ctx->cfg;
+    }
+    L.k = find_kernel(cfg.block, cfg.lds_stack);
+    if (!L.k) return lr::fail(LR_E_INVALID, "no traversal kernel for this block / lds_stack");
+    const uint64_t want = ((uint64_t)n + cfg.block - 1) / cfg.block;
+    const uint64_t maxg = (uint64_t)ctx->n_cu * cfg.blocks_per_cu;
+    L.grid = (uint32_t)std::max<uint64_t>(1, std::min(want, maxg));
+    L.cap = (int)bvh->info.max_stack;
+    L.spill = nullptr;
+    if ((uint32_t)L.cap > cfg.lds_stack) {
+        const size_t bytes = (size_t)(L.cap - cfg.lds_stack) * L.grid * cfg.block * sizeof(uint2);
+        void* p = nullptr;
+        const int rc = ctx->scratch(stream, bytes, &p);
+        if (rc) return rc;
+        L.spill = static_cast<uint2*>(p);
+    }
+    return LR_OK;
+}
+
+thread_local std::string g_last_error;
+
+}  // namespace
+
+namespace lr {
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int bind(const lr_context* ctx) {
+    LR_HIP_CHECK(hipSetDevice(ctx->device));
+    return LR_OK;
+}
+
+}  // namespace lr
+
+int lr_context::scratch(hipStream_t stream, size_t bytes, void** out) {
+    std::lock_guard<std::mutex> g(mu);
+    lr_scratch& s = spill[stream];
+    if (s.bytes < bytes) {
+        if (s.ptr) {
+            LR_HIP_CHECK(hipStreamSynchronize(stream));
+            LR_HIP_CHECK(hipFree(s.ptr));
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        LR_HIP_CHECK(hipMalloc(&s.ptr, bytes));
+        s.bytes = bytes;
+    }
+    *out = s.ptr;
+    return LR_OK;
+}
+
+extern "C" {
+
+const char* lr_last_error(void) { return g_last_error.c_str(); }
+
+const char* lr_version(void) { return "liar_amd 0.1.0 (gfx950)"; }
+
+int lr_context_create(int device, lr_context** out) {
+    if (!out) return lr::fail(LR_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    LR_HIP_CHECK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return lr::fail(LR_E_INVALID, "device " + std::to_string(device) + " out of range (" +
+                                          std::to_string(count) + " devices)");
+    LR_HIP_CHECK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    LR_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+        return lr::fail(LR_E_UNSUPPORTED, std::string("device is ") + prop.gcnArchName + ", built for gfx950 only");
+    lr_context* ctx = new (std::nothrow) lr_context();
+    if (!ctx) return lr::fail(LR_E_NOMEM, "out of host memory");
+    ctx->device = device;
+    ctx->n_cu = prop.multiProcessorCount;
+    *out = ctx;
+    return LR_OK;
+}
+
+int lr_context_destroy(lr_context* ctx) {
+    if (!ctx) return LR_OK;
+    (void)hipSetDevice(ctx->device);
+    for (auto& kv : ctx->spill) {
+        if (kv.second.ptr) {
+            (void)hipStreamSynchronize(kv.first);
+            (void)hipFree(kv.second.ptr);
+        }
+    }
+    delete ctx;
+    return LR_OK;
+}
+
+int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg) {
+    if (!ctx || !cfg) return lr::fail(LR_E_INVALID, "NULL argument");
+    lr_trace_config c;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        c = ctx->cfg;
+    }
+    if (cfg->block) c.block = cfg->block;
+    if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
+    if (cfg->lds_stack) c.lds_stack = cfg->lds_stack;
+    if (!find_kernel(c.block, c.lds_stack))
+        return lr::fail(LR_E_INVALID, "unsupported (block, lds_stack) = (" + std::to_string(c.block) + ", " +
+                                          std::to_string(c.lds_stack) +
+                                          "); block in {64,128,256}, lds_stack in {8,16,24}");
+    if (c.blocks_per_cu > 64) return lr::fail(LR_E_INVALID, "blocks_per_cu must be <= 64");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->cfg = c;
+    return LR_OK;
+}
+
+int lr_get_trace_config(const lr_context* ctx, lr_trace_config* cfg) {
+    if (!ctx || !cfg) return lr::fail(LR_E_INVALID, "NULL argument");
+    *cfg = ctx->cfg;
+    return LR_OK;
+}
+
+int lr_qbvh_create(lr_context* ctx, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                   const float* spheres, uint32_t ns, lr_qbvh** out) {
+    if (!out) return lr::fail(LR_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if ((nt && (!tris || !verts)) || (ns && !spheres)) return lr::fail(LR_E_INVALID, "NULL geometry array");
+    lr_qbvh* b = new (std::nothrow) lr_qbvh();
+    if (!b) return lr::fail(LR_E_NOMEM, "out of host memory");
+    b->ctx = ctx;
+    std::string err;
+    if (lr::build_qbvh(verts, nv, tris, nt, spheres, ns, b->host, err)) {
+        delete b;
+        return lr::fail(LR_E_INVALID, "lr_qbvh_create: " + err);
+    }
+    lr_qbvh_info& I = b->info;
+    I.n_nodes = (uint32_t)b->host.nodes.size();
+    I.n_prims = (uint32_t)b->host.prims.size();
+    I.n_triangles = nt;
+    I.n_spheres = ns;
+    I.max_depth = b->host.max_depth;
+    I.max_stack = 3 * (b->host.max_depth + 1) + 1;
+    I.sah_cost = b->host.sah_cost;
+    I.build_seconds = b->host.build_seconds;
+    for (int k = 0; k < 3; ++k) {
+        I.scene_lo[k] = b->host.scene_lo[k];
+        I.scene_hi[k] = b->host.scene_hi[k];
+    }
+    int rc = ctx ? lr::bind(ctx) : LR_OK;
+    if (!rc && ctx && I.n_nodes) {
+        hipError_t e = hipMalloc(&b->d_nodes, sizeof(lr_node) * I.n_nodes);
+        if (e == hipSuccess) e = hipMalloc(&b->d_prims, sizeof(lr_prim) * I.n_prims);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_nodes, b->host.nodes.data(), sizeof(lr_node) * I.n_nodes, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_prims, b->host.prims.data(), sizeof(lr_prim) * I.n_prims, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = lr::fail(LR_E_DEVICE, std::string("lr_qbvh_create upload: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        lr_qbvh_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return LR_OK;
+}
+
+int lr_qbvh_destroy(lr_qbvh* b) {
+    if (!b) return LR_OK;
+    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    if (b->d_nodes) (void)hipFree(b->d_nodes);
+    if (b->d_prims) (void)hipFree(b->d_prims);
+    delete b;
+    return LR_OK;
+}
+
+int lr_qbvh_get_info(const lr_qbvh* b, lr_qbvh_info* info) {
+    if (!b || !info) return lr::fail(LR_E_INVALID, "NULL argument");
+    *info = b->info;
+    return LR_OK;
+}
+
+int lr_qbvh_export(const lr_qbvh* b, lr_node* nodes, lr_prim* prims) {
+    if (!b) return lr::fail(LR_E_INVALID, "NULL qbvh");
+    if (nodes && !b->host.nodes.empty())
+        std::memcpy(nodes, b->host.nodes.data(), sizeof(lr_node) * b->host.nodes.size());
+    if (prims && !b->host.prims.empty())
+        std::memcpy(prims, b->host.prims.data(), sizeof(lr_prim) * b->host.prims.size());
+    return LR_OK;
+}
+
+int lr_trace_closest(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits_soa* hits, void* stream) {
+    if (!bvh) return lr::fail(LR_E_INVALID, "NULL qbvh");
+    if (!bvh->ctx) return lr::fail(LR_E_INVALID, "QBVH was built host-only (NULL context); it cannot be traced");
+    if (int rc = check_rays(rays, n)) return rc;
+    if (!hits || (n && (!hits->prim || !hits->t || !hits->u || !hits->v)))
+        return lr::fail(LR_E_INVALID, "a hit array pointer is NULL");
+    if (n == 0) return LR_OK;
+    if (int rc = lr::bind(bvh->ctx)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar};
+    const HitsArg ha{hits->prim, hits->t, hits->u, hits->v};
+    if (bvh->info.n_nodes == 0) {
+        hipLaunchKernelGGL(k_fill_miss, dim3((n + 255) / 256), dim3(256), 0, s, n, ha);
+        LR_HIP_CHECK(hipGetLastError());
+        return LR_OK;
+    }
+    Launch L;
+    if (int rc = prepare(bvh, n, s, L)) return rc;
+    hipLaunchKernelGGL(L.k->closest, dim3(L.grid), dim3(L.k->block), 0, s, bvh->d_nodes, bvh->d_prims, ra, n, ha,
+                       L.spill, L.cap);
+    LR_HIP_CHECK(hipGetLastError());
+    return LR_OK;
+}
+
+int lr_trace_any(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, uint8_t* occluded, void* stream) {
+    if (!bvh) return lr::fail(LR_E_INVALID, "NULL qbvh");
+    if (!bvh->ctx) return lr::fail(LR_E_INVALID, "QBVH was built host-only (NULL context); it cannot be traced");
+    if (int rc = check_rays(rays, n)) return rc;
+    if (n && !occluded) return lr::fail(LR_E_INVALID, "occluded is NULL");
+    if (n == 0) return LR_OK;
+    if (int rc = lr::bind(bvh->ctx)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar};
+    if (bvh->info.n_nodes == 0) {
+        hipLaunchKernelGGL(k_fill_u8, dim3((n + 255) / 256), dim3(256), 0, s, n, occluded, (uint8_t)0);
+        LR_HIP_CHECK(hipGetLastError());
+        return LR_OK;
+    }
+    Launch L;
+    if (int rc = prepare(bvh, n, s, L)) return rc;
+    hipLaunchKernelGGL(L.k->any, dim3(L.grid), dim3(L.k->block), 0, s, bvh->d_nodes, bvh->d_prims, ra, n, occluded,
+                       L.spill, L.cap);
+    LR_HIP_CHECK(hipGetLastError());
+    return LR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+"""CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, by ``__graft_entry__.smoke()`` and by the CPU-baseline leg
+of ``bench.py``, always as the checker (or the timed CPU baseline), never as a
+code path of the library.  Wraps ``oracle/liboracle.so`` (built by
+``oracle/Makefile``) and the pure-Python/numpy restatements in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+MISS = 0xFFFFFFFF
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing; run `make -C oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        P, U32, I = C.c_void_p, C.c_uint32, C.c_int
+        RAYS = P  # const float* const[8]
+        L.or_brute_closest.argtypes = [P, P, U32, P, U32, RAYS, U32, P, P, P, P, I]
+        L.or_brute_any.argtypes = [P, P, U32, P, U32, RAYS, U32, P, I]
+        L.or_brute_closest_f64.argtypes = [P, P, U32, P, U32, RAYS, U32, P, P, P, P, I]
+        L.or_qbvh_closest.argtypes = [P, U32, P, RAYS, U32, P, P, P, P, I, P, P]
+        L.or_qbvh_any.argtypes = [P, U32, P, RAYS, U32, P, I, P, P]
+        L.or_qbvh_check.argtypes = [P, U32, P, U32, P, P, U32, P, U32, P]
+        L.or_last_message.restype = C.c_char_p
+        F4 = C.POINTER(C.c_float)
+        L.or_tri_f32.argtypes = [P, P, P, P, P, P, P, P]
+        L.or_sphere_f32.argtypes = [P, P, P, C.c_float, C.c_float, P]
+        for name in ("or_brute_closest", "or_brute_any", "or_brute_closest_f64", "or_qbvh_closest",
+                     "or_qbvh_any", "or_qbvh_check", "or_tri_f32", "or_sphere_f32"):
+            getattr(L, name).restype = C.c_int
+        del F4
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _rays_arg(rays: np.ndarray):
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    assert rays.ndim == 2 and rays.shape[0] == 8
+    arr = (C.c_void_p * 8)(*[rays[k].ctypes.data for k in range(8)])
+    return rays, arr
+
+
+def _scene(verts, tris, spheres):
+    verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+    tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+    if spheres is None:
+        spheres = np.zeros((0, 4), np.float32)
+    spheres = np.ascontiguousarray(spheres, dtype=np.float32).reshape(-1, 4)
+    return verts, tris, spheres
+
+
+def default_threads() -> int:
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def brute_closest(verts, tris, spheres, rays, threads: int | None = None):
+    """LiAR List{TriangleMesh, Spheres...}::intersect by brute force (FP32).
+    Returns (prim uint32 (n,), t, u, v float32 (n,))."""
+    verts, tris, spheres = _scene(verts, tris, spheres)
+    rays, ra = _rays_arg(rays)
+    n = rays.shape[1]
+    prim = np.empty(n, np.uint32)
+    t, u, v = (np.empty(n, np.float32) for _ in range(3))
+    rc = lib().or_brute_closest(_p(verts), _p(tris), len(tris), _p(spheres), len(spheres), ra, n,
+                                _p(prim), _p(t), _p(u), _p(v), threads or default_threads())
+    assert rc == 0
+    return prim, t, u, v
+
+
+def brute_any(verts, tris, spheres, rays, threads: int | None = None):
+    verts, tris, spheres = _scene(verts, tris, spheres)
+    rays, ra = _rays_arg(rays)
+    n = rays.shape[1]
+    occ = np.empty(n, np.uint8)
+    rc = lib().or_brute_any(_p(verts), _p(tris), len(tris), _p(spheres), len(spheres), ra, n,
+                            _p(occ), threads or default_threads())
+    assert rc == 0
+    return occ
+
+
+def brute_closest_f64(verts, tris, spheres, rays, threads: int | None = None):
+    """Same candidates, all arithmetic in FP64 (error reference)."""
+    verts, tris, spheres = _scene(verts, tris, spheres)
+    rays, ra = _rays_arg(rays)
+    n = rays.shape[1]
+    prim = np.empty(n, np.uint32)
+    t, u, v = (np.empty(n, np.float64) for _ in range(3))
+    rc = lib().or_brute_closest_f64(_p(verts), _p(tris), len(tris), _p(spheres), len(spheres), ra, n,
+                                    _p(prim), _p(t), _p(u), _p(v), threads or default_threads())
+    assert rc == 0
+    return prim, t, u, v
+
+
+def qbvh_closest(nodes, prims, rays, threads: int | None = None):
+    """Closest hit over flattened QBVH arrays (uint8 (n,128), (m,48)).
+    Returns (prim, t, u, v, nodes_visited, prims_tested)."""
+    nodes = np.ascontiguousarray(nodes)
+    prims = np.ascontiguousarray(prims)
+    rays, ra = _rays_arg(rays)
+    n = rays.shape[1]
+    prim = np.empty(n, np.uint32)
+    t, u, v = (np.empty(n, np.float32) for _ in range(3))
+    nv, npr = C.c_uint64(), C.c_uint64()
+    rc = lib().or_qbvh_closest(_p(nodes), len(nodes), _p(prims), ra, n, _p(prim), _p(t), _p(u), _p(v),
+                               threads or default_threads(), C.byref(nv), C.byref(npr))
+    assert rc == 0, "oracle traversal stack overflow"
+    return prim, t, u, v, nv.value, npr.value
+
+
+def qbvh_any(nodes, prims, rays, threads: int | None = None):
+    nodes = np.ascontiguousarray(nodes)
+    prims = np.ascontiguousarray(prims)
+    rays, ra = _rays_arg(rays)
+    n = rays.shape[1]
+    occ = np.empty(n, np.uint8)
+    nv, npr = C.c_uint64(), C.c_uint64()
+    rc = lib().or_qbvh_any(_p(nodes), len(nodes), _p(prims), ra, n, _p(occ), threads or default_threads(),
+                           C.byref(nv), C.byref(npr))
+    assert rc == 0, "oracle traversal stack overflow"
+    return occ, nv.value, npr.value
+
+
+def qbvh_check(nodes, prims, verts, tris, spheres=None):
+    """Structural invariants of a flattened QBVH; returns (ok, message, max_depth)."""
+    verts, tris, spheres = _scene(verts, tris, spheres)
+    nodes = np.ascontiguousarray(nodes)
+    prims = np.ascontiguousarray(prims)
+    md = C.c_uint32()
+    rc = lib().or_qbvh_check(_p(nodes), len(nodes), _p(prims), len(prims), _p(verts), _p(tris), len(tris),
+                             _p(spheres), len(spheres), C.byref(md))
+    return rc == 0, lib().or_last_message().decode(), md.value
+
+
+def tri_f32(o, d, v0, e1, e2):
+    """Single FP32 ray-triangle test -> (hit, t, u, v)."""
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, v0, e1, e2)]
+    t, u, v = (np.zeros(1, np.float32) for _ in range(3))
+    hit = lib().or_tri_f32(*[_p(x) for x in a], _p(t), _p(u), _p(v))
+    return bool(hit), float(t[0]), float(u[0]), float(v[0])
+
+
+def sphere_f32(o, d, c, r, tmin=0.0):
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, c)]
+    t = np.zeros(1, np.float32)
+    r2 = np.float32(r) * np.float32(r)
+    hit = lib().or_sphere_f32(*[_p(x) for x in a], C.c_float(r2), C.c_float(tmin), _p(t))
+    return bool(hit), float(t[0])

@@ -1,0 +1,218 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+No reference test pins LiAR's ray-query arithmetic (its QBVH and primitive
+tests live in the unvendored Lass library), so the oracle is pinned by:
+published known-answer vectors (Philox4x32-10), hand-solved closed forms,
+FP64 geometric invariants (the hit point lies on the ray AND on the
+triangle's plane at the reported barycentrics), LiAR's interval and
+per-object rules on constructed cases, and brute force vs QBVH traversal.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import liar_amd as la
+import oracle
+
+MISS = oracle.MISS
+
+
+def rays_of(o, d, tnear=0.0, tfar=math.inf):
+    o = np.asarray(o, np.float32).reshape(-1, 3)
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    n = len(o)
+    r = np.empty((8, n), np.float32)
+    r[0:3] = o.T
+    r[3:6] = d.T
+    r[6] = tnear
+    r[7] = tfar
+    return r
+
+
+# -- Philox4x32-10: Random123 known-answer vectors (kat_vectors) ------------
+@pytest.mark.parametrize("key,ctr,expect", [
+    (0, (0, 0, 0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    (0xFFFFFFFFFFFFFFFF, (0xFFFFFFFF,) * 4, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    (0xa4093822 | (0x299f31d0 << 32), (0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+])
+def test_philox_kat(key, ctr, expect):
+    assert tuple(int(x) for x in la.philox4x32(key, *ctr)) == expect
+
+
+# -- ray-triangle closed forms ----------------------------------------------
+V0, E1, E2 = (0, 0, 0), (1, 0, 0), (0, 1, 0)
+
+
+def test_triangle_known_answer_barycentrics_not_swapped():
+    hit, t, u, v = oracle.tri_f32((0.5, 0.25, 2.0), (0, 0, -1), V0, E1, E2)
+    assert hit and t == 2.0 and u == 0.5 and v == 0.25
+
+
+def test_triangle_two_sided_and_line_semantics():
+    hit, t, u, v = oracle.tri_f32((0.25, 0.25, -1.0), (0, 0, 1), V0, E1, E2)  # from below
+    assert hit and t == 1.0 and (u, v) == (0.25, 0.25)
+    hit, t, _, _ = oracle.tri_f32((0.25, 0.25, 1.0), (0, 0, 1), V0, E1, E2)  # plane behind origin
+    assert hit and t == -1.0
+
+
+@pytest.mark.parametrize("o,d", [
+    ((0.6, 0.6, 1.0), (0, 0, -1)),   # u + v > 1
+    ((-0.1, 0.5, 1.0), (0, 0, -1)),  # u < 0
+    ((0.5, -0.1, 1.0), (0, 0, -1)),  # v < 0
+    ((0.2, 0.2, 1.0), (1, 0, 0)),    # parallel: det == 0
+])
+def test_triangle_misses(o, d):
+    assert not oracle.tri_f32(o, d, V0, E1, E2)[0]
+
+
+def test_triangle_random_fp32_vs_fp64_geometry():
+    """FP64 hits lie on the ray and at (u, v) on the triangle; FP32 agrees."""
+    rng = np.random.default_rng(11)
+    n = 4000
+    v0 = rng.random((n, 3)).astype(np.float32)
+    e1 = (rng.random((n, 3)) - 0.5).astype(np.float32)
+    e2 = (rng.random((n, 3)) - 0.5).astype(np.float32)
+    verts = np.stack([v0, v0 + e1, v0 + e2], 1).reshape(-1, 3).astype(np.float32)
+    tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    # one ray per triangle aimed at a random interior point
+    a, b = rng.random(n), rng.random(n)
+    flip = a + b > 1
+    a[flip], b[flip] = 1 - a[flip], 1 - b[flip]
+    p = v0 + a[:, None] * e1 + b[:, None] * e2
+    o = rng.random((n, 3)) * 3 - 1
+    d = p - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    hits32, hits64 = 0, 0
+    for i in range(0, n, 97):
+        r = rays_of(o[i], d[i])
+        v3 = verts[3 * i:3 * i + 3]
+        p32, t32, u32, w32 = oracle.brute_closest(v3, [[0, 1, 2]], None, r)
+        p64, t64, u64, w64 = oracle.brute_closest_f64(v3, [[0, 1, 2]], None, r)
+        if p64[0] != MISS:
+            hits64 += 1
+            o64, d64 = r[0:3, 0].astype(np.float64), r[3:6, 0].astype(np.float64)
+            q0 = v3[0].astype(np.float64)
+            q = q0 + u64[0] * (v3[1] - v3[0]).astype(np.float64) + w64[0] * (v3[2] - v3[0]).astype(np.float64)
+            np.testing.assert_allclose(o64 + t64[0] * d64, q, rtol=0, atol=1e-12)
+            if p32[0] != MISS:
+                hits32 += 1
+                assert abs(t32[0] - t64[0]) <= 1e-4 * abs(t64[0]) + 1e-6
+                assert abs(u32[0] - u64[0]) <= 1e-4 and abs(w32[0] - w64[0]) <= 1e-4
+    assert hits64 >= 40 and hits32 >= hits64 - 2
+
+
+# -- ray-sphere closed forms -------------------------------------------------
+def test_sphere_known_answers():
+    assert oracle.sphere_f32((0, 0, -5), (0, 0, 1), (0, 0, 0), 1.0) == (True, 4.0)
+    assert oracle.sphere_f32((0, 0, 0), (0, 0, 1), (0, 0, 0), 1.0) == (True, 1.0)  # from inside
+    assert oracle.sphere_f32((0, 0, -5), (0, 0, 1), (0, 0, 0), 1.0, tmin=4.5) == (True, 6.0)
+    assert oracle.sphere_f32((0, 1.5, -5), (0, 0, 1), (0, 0, 0), 1.0)[0] is False
+    assert oracle.sphere_f32((0, 0, 5), (0, 0, 1), (0, 0, 0), 1.0)[0] is False  # behind
+    hit, t = oracle.sphere_f32((0, 0.6, -5), (0, 0, 1), (0, 0, 0), 1.0)
+    assert hit and abs(t - (5 - 0.8)) < 1e-6  # chord: sqrt(1 - 0.36) = 0.8
+
+
+# -- LiAR interval / per-object rules ---------------------------------------
+def _quad_at(z, x0=0.0):
+    """Two triangles covering [x0, x0+1]^2 at height z (ids 2k, 2k+1)."""
+    v = np.array([[x0, 0, z], [x0 + 1, 0, z], [x0 + 1, 1, z], [x0, 1, z]], np.float32)
+    return v, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+def _stack(*quads):
+    vs, ts, off = [], [], 0
+    for v, t in quads:
+        vs.append(v)
+        ts.append(t + off)
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(ts)
+
+
+def test_interval_quirk_nearest_triangle_out_of_range_means_mesh_miss():
+    # rays go up from z = 0 with tnear = 1; triangle A at t = 1 + 2.5e-7
+    # (> near, but <= near*(1+1e-6)), triangle B at t = 2.  LiAR's mesh takes
+    # the nearest t > near (A) and rejects it: miss (triangle_mesh.cpp:301-302).
+    zA = np.float32(1.0) + np.float32(2.5e-7)
+    verts, tris = _stack(_quad_at(zA), _quad_at(2.0))
+    r = rays_of([0.3, 0.2, 0.0], [0, 0, 1], tnear=1.0)
+    p, t, _, _ = oracle.brute_closest(verts, tris, None, r)
+    assert p[0] == MISS
+    # a sphere further along is still found: List takes min over children
+    sph = np.array([[0.3, 0.2, 3.5, 0.5]], np.float32)
+    p, t, _, _ = oracle.brute_closest(verts, tris, sph, r)
+    assert p[0] == len(tris) and t[0] == 3.0
+    # the QBVH traversal reproduces both answers
+    for s in (None, sph):
+        b = la.Qbvh(None, verts, tris, s)
+        nodes, prims = b.export()
+        q = oracle.qbvh_closest(nodes, prims, r)
+        ref = oracle.brute_closest(verts, tris, s, r)
+        assert q[0][0] == ref[0][0] and q[1][0] == ref[1][0]
+
+
+def test_far_limit_is_open_and_shrunk():
+    verts, tris = _quad_at(2.0)
+    assert oracle.brute_closest(verts, tris, None, rays_of([0.3, 0.2, 0], [0, 0, 1], tfar=2.0))[0][0] == MISS
+    assert oracle.brute_closest(verts, tris, None, rays_of([0.3, 0.2, 0], [0, 0, 1], tfar=2.000001))[0][0] == MISS
+    assert oracle.brute_closest(verts, tris, None, rays_of([0.3, 0.2, 0], [0, 0, 1], tfar=2.01))[0][0] == 0
+    # any-hit uses the plain open interval (near, far)
+    assert oracle.brute_any(verts, tris, None, rays_of([0.3, 0.2, 0], [0, 0, 1], tfar=2.000001))[0] == 1
+    assert oracle.brute_any(verts, tris, None, rays_of([0.3, 0.2, 0], [0, 0, 1], tfar=2.0))[0] == 0
+
+
+def test_ties_break_to_smaller_id():
+    verts, tris = _stack(_quad_at(1.0), _quad_at(1.0))  # ids 2,3 duplicate 0,1
+    tris = tris[::-1].copy()  # now the duplicates come first
+    p, t, _, _ = oracle.brute_closest(verts, tris, None, rays_of([0.7, 0.2, 0], [0, 0, 1]))
+    assert t[0] == 1.0
+    # both triangle 1 and 3 (reversed order) cover (0.7, 0.2): smallest id wins
+    ids = [k for k in range(4) if oracle.tri_f32([0.7, 0.2, 0], [0, 0, 1], verts[tris[k][0]],
+                                                 verts[tris[k][1]] - verts[tris[k][0]],
+                                                 verts[tris[k][2]] - verts[tris[k][0]])[0]]
+    assert p[0] == min(ids) and len(ids) == 2
+
+
+# -- QBVH traversal == brute force ------------------------------------------
+@pytest.mark.parametrize("nt,ns", [(0, 0), (1, 0), (3, 0), (4, 0), (5, 0), (0, 7), (2000, 0), (5000, 300)])
+def test_qbvh_traversal_equals_brute_force(nt, ns):
+    rng = np.random.default_rng(nt * 7 + ns)
+    v, t = la.random_triangles(nt, seed=5 + nt)
+    sph = np.concatenate([rng.random((ns, 3)), rng.uniform(0.002, 0.05, (ns, 1))], 1).astype(np.float32)
+    b = la.Qbvh(None, v, t, sph if ns else None)
+    nodes, prims = b.export()
+    ok, msg, _ = oracle.qbvh_check(nodes, prims, v, t, sph if ns else None)
+    assert ok, msg
+    r = la.random_rays(8192, seed=9)
+    ref = oracle.brute_closest(v, t, sph if ns else None, r)
+    got = oracle.qbvh_closest(nodes, prims, r)
+    for a, c in zip(ref, got[:4]):
+        np.testing.assert_array_equal(a, c)
+    r2 = la.random_rays(8192, seed=10, tfar_lo=0.01, tfar_hi=2.0)
+    np.testing.assert_array_equal(oracle.brute_any(v, t, sph if ns else None, r2),
+                                  oracle.qbvh_any(nodes, prims, r2)[0])
+
+
+def test_qbvh_degenerate_all_centroids_equal():
+    # 600 identical triangles force the median-split fallback
+    v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (600, 1))
+    t = np.arange(1800, dtype=np.uint32).reshape(600, 3)
+    b = la.Qbvh(None, v, t)
+    nodes, prims = b.export()
+    assert oracle.qbvh_check(nodes, prims, v, t)[0]
+    r = rays_of(np.tile([0.2, 0.2, 1.0], (4, 1)), np.tile([0, 0, -1.0], (4, 1)))
+    p = oracle.qbvh_closest(nodes, prims, r)[0]
+    assert (p == 0).all()  # all tie at t = 1: smallest id
+
+
+def test_qbvh_check_detects_a_planted_error():
+    v, t = la.random_triangles(500, seed=1)
+    b = la.Qbvh(None, v, t)
+    nodes, prims = b.export()
+    bad = prims.copy()
+    bad[7, 0:4] = np.frombuffer(np.float32(5.0).tobytes(), np.uint8)  # move a vertex
+    assert not oracle.qbvh_check(nodes, bad, v, t)[0]
+    bad_nodes = nodes.copy()
+    bad_nodes[0, 0:4] = np.frombuffer(np.float32(0.5).tobytes(), np.uint8)  # shrink root child box
+    assert not oracle.qbvh_check(bad_nodes, prims, v, t)[0]
