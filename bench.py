@@ -36,6 +36,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _traffic(kind: str):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary of this
+    workload (FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy)."""
+    f = ROOT / "profiles" / "traffic_random1M.json"
+    try:
+        return json.loads(f.read_text())[kind]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +56,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--count-sample", type=int, default=1 << 18,
                     help="rays used for the oracle's nodes/prims-per-ray counts and parity")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -91,6 +103,14 @@ def main() -> int:
         wall = time.perf_counter() - t0
         kern_ms = [a.elapsed_time(b) for a, b in evs]
         return wall, float(np.mean(kern_ms))
+
+    if args.calibrate_copy:
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     log(f"[bench] closest-hit: {args.warmup} warmup + {args.steps} timed launches of {N_RAYS} rays")
     wall_c, kern_c = timed(lambda: bvh.trace_closest(rays, prim, tuv), args.steps, args.warmup)
@@ -168,7 +188,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": _traffic("closest"),
         },
         "cpu_baseline": cpu,
         "detail": {
@@ -180,6 +200,7 @@ def main() -> int:
             "kernel_ms_any": round(kern_a, 4),
             "bytes_per_ray_any": round(b_ray_any, 1),
             "achieved_gbs_any": round(achieved_any, 1),
+            "traffic_any": _traffic("any"),
             "frac_any": round(achieved_any / HBM_PEAK_GBS, 4),
             "parity_sample": S,
             "parity_mismatches_closest": mism,

@@ -158,13 +158,19 @@ int lr_trace_closest(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n,
 int lr_trace_any(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n,
                  uint8_t* occluded, void* stream);
 
-/* Traversal launch parameters (block size in threads, blocks per CU, LDS
- * short-stack depth).  0 keeps the current value.  The defaults are the
- * tuned ones. */
+/* Traversal launch parameters.  0 keeps the current value.
+ *   block          threads per workgroup (64, 128 or 256)
+ *   blocks_per_cu  cap on resident workgroups per CU (the grid is the
+ *                  occupancy limit x #CUs, persistent)
+ *   lds_stack      traversal stack entries kept in LDS (8, 16 or 24); deeper
+ *                  entries spill to global memory
+ *   refill_min     a wave fetches new rays once at least this many of its 64
+ *                  lanes are idle (1..64) */
 typedef struct lr_trace_config {
     uint32_t block;
     uint32_t blocks_per_cu;
     uint32_t lds_stack;
+    uint32_t refill_min;
 } lr_trace_config;
 int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg);
 int lr_get_trace_config(const lr_context* ctx, lr_trace_config* cfg);

@@ -45,7 +45,8 @@ class QbvhInfo(C.Structure):
 
 
 class TraceConfig(C.Structure):
-    _fields_ = [("block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_stack", C.c_uint32)]
+    _fields_ = [("block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_stack", C.c_uint32),
+                ("refill_min", C.c_uint32)]
 
 
 _P = C.c_void_p

@@ -110,11 +110,16 @@ struct Stack {
     }
 };
 
-template <int BLOCK, int SHORT>
-__global__ __launch_bounds__(BLOCK) void k_closest(const lr_node* __restrict__ nodes,
-                                                   const lr_prim* __restrict__ prims, RaysArg rays,
-                                                   uint32_t n, HitsArg hits, uint2* __restrict__ spill,
-                                                   int cap) {
+// Lane-refill persistent traversal.  Every lane runs one traversal step per
+// iteration (a node visit or a leaf's primitive tests); a lane whose ray is
+// finished takes the next ray index from a global counter, one atomic per
+// wave per refill (ballot + mbcnt), so no lane idles while rays remain.
+template <int BLOCK, int SHORT, bool ANY>
+__global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nodes,
+                                                 const lr_prim* __restrict__ prims, RaysArg rays, uint32_t n,
+                                                 HitsArg hits, uint8_t* __restrict__ occluded,
+                                                 uint2* __restrict__ spill, int cap,
+                                                 uint32_t* __restrict__ counter, uint32_t refill_min) {
     __shared__ uint32_t s_e[SHORT * BLOCK];
     __shared__ float s_t[SHORT * BLOCK];
     Stack<BLOCK, SHORT> st;
@@ -125,182 +130,171 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const lr_node* __restrict__ n
     st.tid = threadIdx.x;
     st.gtid = blockIdx.x * BLOCK + threadIdx.x;
     st.cap = cap;
+    st.sp = 0;
     const float4* __restrict__ nf = reinterpret_cast<const float4*>(nodes);
     const float4* __restrict__ pf = reinterpret_cast<const float4*>(prims);
+    const uint32_t lane = threadIdx.x & 63u;
 
-    for (uint32_t r = st.gtid; r < n; r += st.stride) {
-        const float ox = rays.ox[r], oy = rays.oy[r], oz = rays.oz[r];
-        const float dx = rays.dx[r], dy = rays.dy[r], dz = rays.dz[r];
-        const float tnear = rays.tnear[r], tfar = rays.tfar[r];
-        const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
-        const float far_lim = tfar * (1.0f - 1e-6f);
-        const float near_lim = tnear * (1.0f + 1e-6f);
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t ray = kNone;
+    bool no_more = false;  // wave-uniform
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
+    float tnear = 0, tfar = 0, far_lim = 0, near_lim = 0;
+    float tri_t = 0, tri_u = 0, tri_v = 0, sph_t = 0;
+    uint32_t tri_id = LR_MISS, sph_id = LR_MISS, cur = 0;
+    bool doomed = false;
 
-        float tri_t = far_lim, tri_u = 0.0f, tri_v = 0.0f;
-        uint32_t tri_id = LR_MISS;
-        bool doomed = false;
-        float sph_t = far_lim;
-        uint32_t sph_id = LR_MISS;
-
-        st.sp = 0;
-        uint32_t cur = 0;  // root
-        for (;;) {
-            const float cull = fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
-            if (!(cur & 0x80000000u)) {
-                float t[4];
-                uint32_t e[4];
-                node_test(nf + (size_t)cur * 8, ox, oy, oz, ix, iy, iz, tnear, cull, t, e);
-                LR_CSWAP(0, 1);
-                LR_CSWAP(2, 3);
-                LR_CSWAP(0, 2);
-                LR_CSWAP(1, 3);
-                LR_CSWAP(1, 2);
-                if (t[0] != INFINITY) {
-                    if (t[3] != INFINITY) st.push(e[3], t[3]);
-                    if (t[2] != INFINITY) st.push(e[2], t[2]);
-                    if (t[1] != INFINITY) st.push(e[1], t[1]);
-                    cur = e[0];
-                    continue;
+    for (;;) {
+        if (!no_more) {
+            const uint64_t need = __ballot(ray == kNone);
+            const uint32_t idle = (uint32_t)__popcll(need);
+            // Refill only when enough lanes are idle (or all are): each refill
+            // costs the wave one atomic and one ray-load round trip.
+            if (idle && (idle >= refill_min || idle == 64u)) {
+                const uint32_t cnt = (uint32_t)__popcll(need);
+                const int leader = __ffsll((unsigned long long)need) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(counter, cnt);
+                base = __shfl(base, leader);
+                if (ray == kNone) {
+                    const uint32_t my = base + __builtin_amdgcn_mbcnt_hi(
+                                                   (uint32_t)(need >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+                    if (my < n) {
+                        ray = my;
+                        ox = rays.ox[my];
+                        oy = rays.oy[my];
+                        oz = rays.oz[my];
+                        dx = rays.dx[my];
+                        dy = rays.dy[my];
+                        dz = rays.dz[my];
+                        tnear = rays.tnear[my];
+                        tfar = rays.tfar[my];
+                        ix = safe_inv(dx);
+                        iy = safe_inv(dy);
+                        iz = safe_inv(dz);
+                        far_lim = tfar * (1.0f - 1e-6f);
+                        near_lim = tnear * (1.0f + 1e-6f);
+                        tri_t = far_lim;
+                        sph_t = far_lim;
+                        tri_u = tri_v = 0.0f;
+                        tri_id = sph_id = LR_MISS;
+                        doomed = false;
+                        cur = 0;
+                        st.sp = 0;
+                    }
                 }
-            } else {
-                const uint32_t first = (cur >> 2) & 0x1FFFFFFFu;
-                const uint32_t cnt = (cur & 3u) + 1u;
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    const float4* q = pf + (size_t)(first + k) * 3;
-                    const float4 a = q[0], b = q[1];
-                    const uint32_t id = __float_as_uint(a.w);
-                    if (__float_as_uint(b.w) == LR_PRIM_TRIANGLE) {
-                        const float4 c = q[2];
-                        float th, uh, vh;
-                        if (lr::tri_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z,
-                                              th, uh, vh) &&
-                            th > tnear && th < far_lim && (th < tri_t || (th == tri_t && id < tri_id))) {
+                if ((uint64_t)base + cnt >= n) no_more = true;
+            }
+        }
+        if (__ballot(ray != kNone) == 0) break;
+        if (ray == kNone) continue;
+
+        bool finished = false;
+        bool need_pop = true;
+        bool any_hit = false;
+        const float cull = ANY ? tfar * kCullSlack : fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
+        if (!(cur & 0x80000000u)) {
+            float t[4];
+            uint32_t e[4];
+            node_test(nf + (size_t)cur * 8, ox, oy, oz, ix, iy, iz, tnear, cull, t, e);
+            LR_CSWAP(0, 1);
+            LR_CSWAP(2, 3);
+            LR_CSWAP(0, 2);
+            LR_CSWAP(1, 3);
+            LR_CSWAP(1, 2);
+            if (t[0] != INFINITY) {
+                if (t[3] != INFINITY) st.push(e[3], t[3]);
+                if (t[2] != INFINITY) st.push(e[2], t[2]);
+                if (t[1] != INFINITY) st.push(e[1], t[1]);
+                cur = e[0];
+                need_pop = false;
+            }
+        } else {
+            const uint32_t first = (cur >> 2) & 0x1FFFFFFFu;
+            const uint32_t cnt = (cur & 3u) + 1u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4* q = pf + (size_t)(first + k) * 3;
+                const float4 a = q[0], b = q[1];
+                const uint32_t id = __float_as_uint(a.w);
+                if (__float_as_uint(b.w) == LR_PRIM_TRIANGLE) {
+                    const float4 c = q[2];
+                    float th, uh, vh;
+                    if (lr::tri_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, th,
+                                          uh, vh)) {
+                        if (ANY) {
+                            if (th > tnear && th < tfar) {
+                                any_hit = true;
+                                break;
+                            }
+                        } else if (th > tnear && th < far_lim && (th < tri_t || (th == tri_t && id < tri_id))) {
                             tri_t = th;
                             tri_u = uh;
                             tri_v = vh;
                             tri_id = id;
                             doomed = !(th > near_lim);
                         }
-                    } else {
-                        float th;
-                        if (lr::sphere_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.y, tnear, th) &&
-                            lr::in_range(th, tnear, tfar) && (th < sph_t || (th == sph_t && id < sph_id))) {
+                    }
+                } else {
+                    float th;
+                    if (lr::sphere_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.y, tnear, th) &&
+                        lr::in_range(th, tnear, tfar)) {
+                        if (ANY) {
+                            any_hit = true;
+                            break;
+                        } else if (th < sph_t || (th == sph_t && id < sph_id)) {
                             sph_t = th;
                             sph_id = id;
                         }
                     }
                 }
             }
+            if (ANY && any_hit) {
+                finished = true;
+                need_pop = false;
+            }
+        }
+        if (need_pop) {
             // pop the next entry that can still hold a better hit
-            const float cull2 = fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
-            bool found = false;
+            const float cull2 =
+                ANY ? cull : fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
+            finished = true;
             while (st.sp > 0) {
                 uint32_t e;
                 float te;
                 st.pop(e, te);
                 if (te <= cull2) {
                     cur = e;
-                    found = true;
+                    finished = false;
                     break;
                 }
             }
-            if (!found) break;
         }
-        uint32_t prim = LR_MISS;
-        float t = INFINITY, u = 0.0f, v = 0.0f;
-        const bool tri_ok = tri_id != LR_MISS && !doomed;
-        if (tri_ok && (sph_id == LR_MISS || tri_t < sph_t || (tri_t == sph_t && tri_id < sph_id))) {
-            prim = tri_id;
-            t = tri_t;
-            u = tri_u;
-            v = tri_v;
-        } else if (sph_id != LR_MISS) {
-            prim = sph_id;
-            t = sph_t;
-        }
-        hits.prim[r] = prim;
-        hits.t[r] = t;
-        hits.u[r] = u;
-        hits.v[r] = v;
-    }
-}
-
-template <int BLOCK, int SHORT>
-__global__ __launch_bounds__(BLOCK) void k_any(const lr_node* __restrict__ nodes, const lr_prim* __restrict__ prims,
-                                               RaysArg rays, uint32_t n, uint8_t* __restrict__ occluded,
-                                               uint2* __restrict__ spill, int cap) {
-    __shared__ uint32_t s_e[SHORT * BLOCK];
-    __shared__ float s_t[SHORT * BLOCK];
-    Stack<BLOCK, SHORT> st;
-    st.s_e = s_e;
-    st.s_t = s_t;
-    st.spill = spill;
-    st.stride = gridDim.x * BLOCK;
-    st.tid = threadIdx.x;
-    st.gtid = blockIdx.x * BLOCK + threadIdx.x;
-    st.cap = cap;
-    const float4* __restrict__ nf = reinterpret_cast<const float4*>(nodes);
-    const float4* __restrict__ pf = reinterpret_cast<const float4*>(prims);
-
-    for (uint32_t r = st.gtid; r < n; r += st.stride) {
-        const float ox = rays.ox[r], oy = rays.oy[r], oz = rays.oz[r];
-        const float dx = rays.dx[r], dy = rays.dy[r], dz = rays.dz[r];
-        const float tnear = rays.tnear[r], tfar = rays.tfar[r];
-        const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
-        const float cull = tfar * kCullSlack;
-        bool hit = false;
-        st.sp = 0;
-        uint32_t cur = 0;
-        for (;;) {
-            if (!(cur & 0x80000000u)) {
-                float t[4];
-                uint32_t e[4];
-                node_test(nf + (size_t)cur * 8, ox, oy, oz, ix, iy, iz, tnear, cull, t, e);
-                // any order is valid; visit the nearest child first
-                int best = -1;
-                float bt = INFINITY;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    if (t[i] < bt) {
-                        bt = t[i];
-                        best = i;
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (i != best && t[i] != INFINITY) st.push(e[i], t[i]);
-                if (best >= 0) {
-                    cur = e[best];
-                    continue;
-                }
+        if (finished) {
+            if (ANY) {
+                occluded[ray] = any_hit ? 1 : 0;
             } else {
-                const uint32_t first = (cur >> 2) & 0x1FFFFFFFu;
-                const uint32_t cnt = (cur & 3u) + 1u;
-                for (uint32_t k = 0; k < cnt && !hit; ++k) {
-                    const float4* q = pf + (size_t)(first + k) * 3;
-                    const float4 a = q[0], b = q[1];
-                    if (__float_as_uint(b.w) == LR_PRIM_TRIANGLE) {
-                        const float4 c = q[2];
-                        float th, uh, vh;
-                        hit = lr::tri_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y,
-                                                c.z, th, uh, vh) &&
-                              th > tnear && th < tfar;
-                    } else {
-                        float th;
-                        hit = lr::sphere_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.y, tnear, th) &&
-                              lr::in_range(th, tnear, tfar);
-                    }
+                uint32_t prim = LR_MISS;
+                float t = INFINITY, u = 0.0f, v = 0.0f;
+                const bool tri_ok = tri_id != LR_MISS && !doomed;
+                if (tri_ok && (sph_id == LR_MISS || tri_t < sph_t || (tri_t == sph_t && tri_id < sph_id))) {
+                    prim = tri_id;
+                    t = tri_t;
+                    u = tri_u;
+                    v = tri_v;
+                } else if (sph_id != LR_MISS) {
+                    prim = sph_id;
+                    t = sph_t;
                 }
-                if (hit) break;
+                hits.prim[ray] = prim;
+                hits.t[ray] = t;
+                hits.u[ray] = u;
+                hits.v[ray] = v;
             }
-            if (st.sp == 0) break;
-            float te;
-            st.pop(cur, te);
+            ray = kNone;
         }
-        occluded[r] = hit ? 1 : 0;
     }
 }
-
 __global__ void k_fill_miss(uint32_t n, HitsArg hits) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -316,23 +310,23 @@ __global__ void k_fill_u8(uint32_t n, uint8_t* p, uint8_t v) {
     if (i < n) p[i] = v;
 }
 
-using ClosestFn = void (*)(const lr_node*, const lr_prim*, RaysArg, uint32_t, HitsArg, uint2*, int);
-using AnyFn = void (*)(const lr_node*, const lr_prim*, RaysArg, uint32_t, uint8_t*, uint2*, int);
+__global__ void k_zero_u32(uint32_t* p) { p[threadIdx.x] = 0u; }
 
+using TraceFn = void (*)(const lr_node*, const lr_prim*, RaysArg, uint32_t, HitsArg, uint8_t*, uint2*, int,
+                         uint32_t*, uint32_t);
 
 struct KernelEntry {
     uint32_t block, lds_stack;
-    ClosestFn closest;
-    AnyFn any;
+    TraceFn closest;
+    TraceFn any;
 };
 
+#define LR_K(B, S) {B, S, k_trace<B, S, false>, k_trace<B, S, true>}
 const KernelEntry kKernels[] = {
-    {64, 8, k_closest<64, 8>, k_any<64, 8>},       {64, 16, k_closest<64, 16>, k_any<64, 16>},
-    {64, 24, k_closest<64, 24>, k_any<64, 24>},    {128, 8, k_closest<128, 8>, k_any<128, 8>},
-    {128, 16, k_closest<128, 16>, k_any<128, 16>}, {128, 24, k_closest<128, 24>, k_any<128, 24>},
-    {256, 8, k_closest<256, 8>, k_any<256, 8>},    {256, 16, k_closest<256, 16>, k_any<256, 16>},
-    {256, 24, k_closest<256, 24>, k_any<256, 24>},
+    LR_K(64, 8),  LR_K(64, 16),  LR_K(64, 24),  LR_K(128, 8), LR_K(128, 16),
+    LR_K(128, 24), LR_K(256, 8), LR_K(256, 16), LR_K(256, 24),
 };
+#undef LR_K
 
 const KernelEntry* find_kernel(uint32_t block, uint32_t lds_stack) {
     for (const auto& k : kKernels)
@@ -349,33 +343,55 @@ int check_rays(const lr_rays_soa* rays, uint32_t n) {
 }
 
 struct Launch {
-    const KernelEntry* k;
+    TraceFn fn;
+    uint32_t block;
     uint32_t grid;
     int cap;
     uint2* spill;
+    uint32_t* counter;
+    uint32_t refill_min;
 };
 
-int prepare(lr_qbvh* bvh, uint32_t n, hipStream_t stream, Launch& L) {
+// Scratch layout per stream: [work counter, 256 B][spill stack: (cap - lds) x threads x 8 B].
+constexpr size_t kCounterBytes = 256;
+
+int prepare(lr_qbvh* bvh, uint32_t n, bool any, hipStream_t stream, Launch& L) {
     lr_context* ctx = bvh->ctx;
     lr_trace_config cfg;
     {
         std::lock_guard<std::mutex> g(ctx->mu);
         cfg = ctx->cfg;
     }
-    L.k = find_kernel(cfg.block, cfg.lds_stack);
-    if (!L.k) return lr::fail(LR_E_INVALID, "no traversal kernel for this block / lds_stack");
+    const KernelEntry* k = find_kernel(cfg.block, cfg.lds_stack);
+    if (!k) return lr::fail(LR_E_INVALID, "no traversal kernel for this block / lds_stack");
+    L.fn = any ? k->any : k->closest;
+    L.block = cfg.block;
+    L.refill_min = cfg.refill_min;
+    int occ = 0;
+    LR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(L.fn),
+                                                              (int)cfg.block, 0));
+    occ = std::max(1, std::min(occ, (int)cfg.blocks_per_cu));
     const uint64_t want = ((uint64_t)n + cfg.block - 1) / cfg.block;
-    const uint64_t maxg = (uint64_t)ctx->n_cu * cfg.blocks_per_cu;
+    const uint64_t maxg = (uint64_t)ctx->n_cu * (uint64_t)occ;
     L.grid = (uint32_t)std::max<uint64_t>(1, std::min(want, maxg));
     L.cap = (int)bvh->info.max_stack;
-    L.spill = nullptr;
-    if ((uint32_t)L.cap > cfg.lds_stack) {
-        const size_t bytes = (size_t)(L.cap - cfg.lds_stack) * L.grid * cfg.block * sizeof(uint2);
-        void* p = nullptr;
-        const int rc = ctx->scratch(stream, bytes, &p);
-        if (rc) return rc;
-        L.spill = static_cast<uint2*>(p);
-    }
+    size_t bytes = kCounterBytes;
+    if ((uint32_t)L.cap > cfg.lds_stack) bytes += (size_t)(L.cap - cfg.lds_stack) * L.grid * cfg.block * sizeof(uint2);
+    void* p = nullptr;
+    if (int rc = ctx->scratch(stream, bytes, &p)) return rc;
+    L.counter = static_cast<uint32_t*>(p);
+    L.spill = (uint32_t)L.cap > cfg.lds_stack
+                  ? reinterpret_cast<uint2*>(static_cast<char*>(p) + kCounterBytes)
+                  : nullptr;
+    return LR_OK;
+}
+
+int launch(lr_qbvh* bvh, const Launch& L, const RaysArg& ra, uint32_t n, const HitsArg& ha, uint8_t* occ,
+           hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, L.counter);
+    hipLaunchKernelGGL(L.fn, dim3(L.grid), dim3(L.block), 0, s, bvh->d_nodes, bvh->d_prims, ra, n, ha, occ,
+                       L.spill, L.cap, L.counter, L.refill_min);
+    LR_HIP_CHECK(hipGetLastError());
     return LR_OK;
 }
 
@@ -464,6 +480,8 @@ int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg) {
     if (cfg->block) c.block = cfg->block;
     if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
     if (cfg->lds_stack) c.lds_stack = cfg->lds_stack;
+    if (cfg->refill_min) c.refill_min = cfg->refill_min;
+    if (c.refill_min > 64) return lr::fail(LR_E_INVALID, "refill_min must be in 1..64");
     if (!find_kernel(c.block, c.lds_stack))
         return lr::fail(LR_E_INVALID, "unsupported (block, lds_stack) = (" + std::to_string(c.block) + ", " +
                                           std::to_string(c.lds_stack) +
@@ -565,11 +583,8 @@ int lr_trace_closest(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits_
         return LR_OK;
     }
     Launch L;
-    if (int rc = prepare(bvh, n, s, L)) return rc;
-    hipLaunchKernelGGL(L.k->closest, dim3(L.grid), dim3(L.k->block), 0, s, bvh->d_nodes, bvh->d_prims, ra, n, ha,
-                       L.spill, L.cap);
-    LR_HIP_CHECK(hipGetLastError());
-    return LR_OK;
+    if (int rc = prepare(bvh, n, false, s, L)) return rc;
+    return launch(bvh, L, ra, n, ha, nullptr, s);
 }
 
 int lr_trace_any(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, uint8_t* occluded, void* stream) {
@@ -587,11 +602,9 @@ int lr_trace_any(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, uint8_t* occ
         return LR_OK;
     }
     Launch L;
-    if (int rc = prepare(bvh, n, s, L)) return rc;
-    hipLaunchKernelGGL(L.k->any, dim3(L.grid), dim3(L.k->block), 0, s, bvh->d_nodes, bvh->d_prims, ra, n, occluded,
-                       L.spill, L.cap);
-    LR_HIP_CHECK(hipGetLastError());
-    return LR_OK;
+    if (int rc = prepare(bvh, n, true, s, L)) return rc;
+    const HitsArg none{nullptr, nullptr, nullptr, nullptr};
+    return launch(bvh, L, ra, n, none, occluded, s);
 }
 
 }  // extern "C"

@@ -56,10 +56,12 @@ class Context:
     def trace_config(self) -> dict:
         cfg = N.TraceConfig()
         N.check(N.lib().lr_get_trace_config(self.handle, C.byref(cfg)), "lr_get_trace_config")
-        return {"block": cfg.block, "blocks_per_cu": cfg.blocks_per_cu, "lds_stack": cfg.lds_stack}
+        return {"block": cfg.block, "blocks_per_cu": cfg.blocks_per_cu, "lds_stack": cfg.lds_stack,
+                "refill_min": cfg.refill_min}
 
-    def set_trace_config(self, block: int = 0, blocks_per_cu: int = 0, lds_stack: int = 0) -> None:
-        cfg = N.TraceConfig(block, blocks_per_cu, lds_stack)
+    def set_trace_config(self, block: int = 0, blocks_per_cu: int = 0, lds_stack: int = 0,
+                         refill_min: int = 0) -> None:
+        cfg = N.TraceConfig(block, blocks_per_cu, lds_stack, refill_min)
         N.check(N.lib().lr_set_trace_config(self.handle, C.byref(cfg)), "lr_set_trace_config")
 
 
