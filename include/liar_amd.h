@@ -175,6 +175,127 @@ typedef struct lr_trace_config {
 int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg);
 int lr_get_trace_config(const lr_context* ctx, lr_trace_config* cfg);
 
+/* ---- spectral path tracing (wavefront) ------------------------------------ */
+
+#define LR_NONE 0xFFFFFFFFu
+
+/* Spectra are evaluated at one sampled wavelength per path, as LiAR's
+ * SPECTRAL_MODE=Single (src/liar/kernel/spectral.h:64-65).  Wavelengths are
+ * in nanometres. */
+#define LR_SPEC_CONSTANT 0u  /* value                                         */
+#define LR_SPEC_SAMPLED 1u   /* piecewise linear through (w[i], v[i]), 0 outside
+                                (src/liar/kernel/spectral.cpp:185-210)         */
+#define LR_SPEC_BINNED 2u    /* `count` equal bins over [lo, hi), constant per
+                                bin, 0 outside                                 */
+#define LR_SPEC_BLACKBODY 3u /* Planck at `temperature` K, peak = `scale`
+                                (src/liar/spectra/black_body.cpp:47-59,96-118) */
+typedef struct lr_spectrum {
+    uint32_t kind;
+    uint32_t count;
+    const float* w; /* host, SAMPLED: count strictly increasing wavelengths   */
+    const float* v; /* host, SAMPLED / BINNED: count values                   */
+    float value;
+    float lo, hi;
+    float temperature;
+    float scale;
+} lr_spectrum;
+
+/* Surface shaders (src/liar/shaders/). Spectrum slots index the
+ * lr_spectrum array given to lr_scene_create; LR_NONE = unused.  Reflectance
+ * slots use LiAR's Reflectant convention (a SAMPLED/BINNED spectrum whose
+ * maximum exceeds 1 is divided by that maximum). */
+#define LR_MAT_NONE 0u          /* no BSDF (absorbs)                         */
+#define LR_MAT_LAMBERT 1u       /* lambert.cpp:76-124                        */
+#define LR_MAT_MIRROR 2u        /* mirror.cpp:99-161 (no fuzz)              */
+#define LR_MAT_DIELECTRIC 3u    /* dielectric.cpp:148-247                    */
+#define LR_MAT_CONDUCTOR 4u     /* conductor.cpp:125-198                     */
+#define LR_MAT_COOK_TORRANCE 5u /* cook_torrance.cpp:189-294 with the
+                                   Trowbridge-Reitz (GGX) distribution,
+                                   microfacet_trowbridge_reitz.cpp:46-84     */
+#define LR_MAT_COOK_TORRANCE_BECKMANN 6u /* same with the Beckmann
+                                   distribution (LiAR's default mdf),
+                                   microfacet_beckmann.cpp:50-106            */
+typedef struct lr_material {
+    uint32_t type;
+    uint32_t reflectance;   /* Lambert diffuse; Mirror/Dielectric/Conductor/
+                               CookTorrance reflectance                      */
+    uint32_t transmittance; /* Dielectric                                    */
+    uint32_t eta;           /* Conductor/CookTorrance n; Dielectric inner IOR */
+    uint32_t kappa;         /* Conductor/CookTorrance k; Dielectric outer IOR */
+    uint32_t emission;      /* Unshaded emission seen at generation 0        */
+    uint32_t light;         /* index of the area light this surface is       */
+    float roughness;        /* CookTorrance: alpha = max(roughness, 1e-3)^2  */
+} lr_material;
+
+/* Lights (src/liar/scenery/light_area.cpp, light_sky.cpp). */
+#define LR_LIGHT_PARALLELOGRAM 0u /* LightArea over a Parallelogram         */
+#define LR_LIGHT_SKY 1u           /* LightSky with a constant radiance      */
+#define LR_LIGHT_SUN 2u           /* distant disc of `angle` diameter      */
+typedef struct lr_light {
+    uint32_t type;
+    uint32_t radiance;      /* spectrum index                               */
+    uint32_t double_sided;  /* parallelogram: emit on both faces            */
+    float origin[3], dx[3], dy[3]; /* parallelogram: origin + u dx + v dy,
+                                      front face normal = cross(dx, dy)     */
+    float direction[3];     /* sun: unit vector towards the sun             */
+    float angle;            /* sun: apparent diameter [rad]                 */
+} lr_light;
+
+/* PerspectiveCamera (src/liar/cameras/perspective_camera.cpp:406-434),
+ * pinhole, default film width 0.036. */
+typedef struct lr_camera {
+    float position[3];
+    float direction[3];
+    float sky[3];
+    float fov;        /* horizontal field of view [rad]                     */
+    float aspect;     /* width / height                                     */
+    float near_limit; /* rays start at max(1e-6, near_limit)                 */
+} lr_camera;
+
+typedef struct lr_render_params {
+    uint32_t width, height;
+    uint32_t spp;          /* camera samples per pixel                       */
+    uint32_t wavelengths;  /* stratified wavelengths (paths) per camera sample */
+    uint32_t max_depth;    /* ray generations per path (0: 64)              */
+    uint32_t rr_depth;     /* Russian roulette from this generation on       */
+    float rr_survival;     /* survival probability (LiAR: 0.5)               */
+    float lambda_min, lambda_max; /* wavelength sampling domain [nm]       */
+    uint64_t seed;
+    uint32_t chunk_paths;  /* paths per wavefront chunk (0: 4M)              */
+} lr_render_params;
+
+typedef struct lr_render_stats {
+    uint64_t paths;
+    uint64_t rays;         /* extension rays traced (all generations)        */
+    uint32_t chunks;
+    double seconds;        /* host wall time of lr_render (synchronised)     */
+} lr_render_stats;
+
+typedef struct lr_scene lr_scene;
+
+/* Standard observer (LiAR: liar.Observer.setStandard, src/liar/__init__.py:51-69,
+ * src/liar/kernel/observer.cpp:57-129): n wavelengths [nm] and x, y, z
+ * colour matching values (xyz: 3*n, interleaved).  Process-global. */
+int lr_set_standard_observer(const float* w, const float* xyz, uint32_t n);
+
+/* A renderable scene: geometry (an lr_qbvh of the same context), one
+ * material index per primitive (triangles then spheres), materials,
+ * spectra and lights.  All arrays are host, copied. */
+int lr_scene_create(lr_context* ctx, lr_qbvh* geometry, const uint32_t* prim_material,
+                    const lr_material* materials, uint32_t n_materials,
+                    const lr_spectrum* spectra, uint32_t n_spectra,
+                    const lr_light* lights, uint32_t n_lights, lr_scene** out);
+int lr_scene_destroy(lr_scene* scene);
+
+/* Renders with LiAR's AdjointPhotonTracer integrand
+ * (src/liar/tracers/adjoint_photon_tracer.cpp:77-292) as a wavefront path
+ * tracer and accumulates into a Raster-style film
+ * (src/liar/output/raster.cpp:547-606): film_xyz (device, 3*W*H floats,
+ * planes X, Y, Z) += sum of sample XYZ, film_weight (device, W*H) += sample
+ * count.  Deterministic for a given seed.  stats may be NULL. */
+int lr_render(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
+              float* film_xyz, float* film_weight, lr_render_stats* stats, void* stream);
+
 /* ---- deterministic synthetic inputs (host, counter-based Philox4x32-10) --- */
 
 /* Random triangle soup: centre c ~ U[0,1)^3, v_k = c + scale * xi_k,

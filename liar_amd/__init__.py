@@ -3,9 +3,13 @@ path: QBVH closest/any-hit traversal and the spectral path-tracing integrand.
 
 The compute path is hand-written HIP in ``liar_amd/csrc`` behind the C-ABI of
 ``include/liar_amd.h``; this package is the host-side mirror of LiAR's
-interfaces on top of that C-ABI.
+interfaces on top of that C-ABI (``scenery``, ``shaders``, ``textures``,
+``spectra``, ``cameras``, ``samplers``, ``tracers``, ``output``,
+``RenderEngine``).
 """
-from ._native import LiarError, LIB_PATH  # noqa: F401
-from .qbvh import Context, Qbvh, random_rays, random_triangles, philox4x32  # noqa: F401
+from . import cameras, output, samplers, scenery, shaders, spectra, textures, tracers  # noqa: F401
+from ._native import LIB_PATH, LiarError  # noqa: F401
+from .engine import RenderEngine, compile_scene  # noqa: F401
+from .qbvh import Context, Qbvh, philox4x32, random_rays, random_triangles  # noqa: F401
 
 __version__ = "0.1.0"

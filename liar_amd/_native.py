@@ -49,6 +49,46 @@ class TraceConfig(C.Structure):
                 ("refill_min", C.c_uint32)]
 
 
+class Spectrum(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("count", C.c_uint32), ("w", C.c_void_p), ("v", C.c_void_p),
+                ("value", C.c_float), ("lo", C.c_float), ("hi", C.c_float), ("temperature", C.c_float),
+                ("scale", C.c_float)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("reflectance", C.c_uint32), ("transmittance", C.c_uint32),
+                ("eta", C.c_uint32), ("kappa", C.c_uint32), ("emission", C.c_uint32), ("light", C.c_uint32),
+                ("roughness", C.c_float)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("radiance", C.c_uint32), ("double_sided", C.c_uint32),
+                ("origin", C.c_float * 3), ("dx", C.c_float * 3), ("dy", C.c_float * 3),
+                ("direction", C.c_float * 3), ("angle", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("sky", C.c_float * 3),
+                ("fov", C.c_float), ("aspect", C.c_float), ("near_limit", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("wavelengths", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
+                ("rr_survival", C.c_float), ("lambda_min", C.c_float), ("lambda_max", C.c_float),
+                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("chunks", C.c_uint32), ("seconds", C.c_double)]
+
+
+LR_NONE = 0xFFFFFFFF
+SPEC_CONSTANT, SPEC_SAMPLED, SPEC_BINNED, SPEC_BLACKBODY = 0, 1, 2, 3
+MAT_NONE, MAT_LAMBERT, MAT_MIRROR, MAT_DIELECTRIC, MAT_CONDUCTOR, MAT_COOK_TORRANCE = 0, 1, 2, 3, 4, 5
+MAT_COOK_TORRANCE_BECKMANN = 6
+LIGHT_PARALLELOGRAM, LIGHT_SKY, LIGHT_SUN = 0, 1, 2
+
 _P = C.c_void_p
 _I = C.c_int
 _U32 = C.c_uint32
@@ -72,6 +112,10 @@ SIGNATURES = {
     "lr_gen_random_triangles": (_I, [_U64, _U32, _F, _P, _P]),
     "lr_gen_random_rays": (_I, [_U64, _U32, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P]),
     "lr_philox4x32": (None, [_U64, _U32, _U32, _U32, _U32, _P]),
+    "lr_set_standard_observer": (_I, [_P, _P, _U32]),
+    "lr_scene_create": (_I, [_P, _P, _P, _P, _U32, _P, _U32, _P, _U32, C.POINTER(_P)]),
+    "lr_scene_destroy": (_I, [_P]),
+    "lr_render": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, _P, C.POINTER(RenderStats), _P]),
 }
 
 _lib = None

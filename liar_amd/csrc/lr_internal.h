@@ -41,4 +41,9 @@ struct lr_qbvh {
 namespace lr {
 // Sets the device of `ctx` current on the calling thread.
 int bind(const lr_context* ctx);
+
+// Closest-hit over a device ray queue whose length (<= n_max) is read from
+// device memory `n_dev` by the kernel; no host synchronisation.
+int trace_closest_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, const uint32_t* n_dev,
+                        lr_hits_soa* hits, hipStream_t stream);
 }

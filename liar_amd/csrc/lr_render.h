@@ -1,0 +1,447 @@
+// lr_render.h -- device-side data layout and inline functions of the spectral
+// wavefront path tracer: spectra at one sampled wavelength, the standard
+// observer, sampling warps, the five BSDFs and the lights.
+//
+// Semantics follow LiAR (paths relative to the reference repository root);
+// every function cites the file it restates.  Sampling warps whose exact
+// form lives in the unvendored Lass library (cosine hemisphere, uniform
+// sphere) use the published maps named at each function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/liar_amd.h"
+#include "lr_geom.h"
+
+namespace lr {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTolerance = 1e-6f;  // src/liar/kernel/kernel_common.cpp:29-33
+
+struct DevSpectrum {
+    uint32_t kind, count, offset, pad;
+    float value, lo, hi, refl_scale;  // refl_scale: Reflectant normalisation
+    float bb_t, bb_k, pad1, pad2;     // BlackBody temperature and 1/planck(peak)*scale
+};
+
+struct DevLight {
+    uint32_t type, radiance, double_sided, pad;
+    float o[3], area;
+    float dx[3], cos_max;   // cos_max: sun cone
+    float dy[3], inv_solid;  // inv_solid: 1 / sun cone solid angle
+    float n[3], sel_pdf;     // n: unit front normal (parallelogram) / sun direction
+    float du[3], cdf;        // du, dv: dual basis, (p - o).du = u
+    float dv[3], pad2;
+};
+
+struct DevMaterial {
+    uint32_t type, refl, trans, eta, kappa, emission, light;
+    float roughness;
+};
+
+// Observer tables restricted to the sampling domain (n entries).
+struct DevObserver {
+    const float* w;    // wavelengths [nm]
+    const float* cdf;  // cumulative, cdf[0] = 0, cdf[n-1] = 1
+    const float* xyz;  // 3*n: x, y, z sensitivities (Y of a unit spectrum = 1)
+    const float* dxyz; // 3*n: slopes d(xyz)/dw on [w[k], w[k+1])
+    uint32_t n;
+};
+
+struct DevScene {
+    const lr_node* nodes;
+    const lr_prim* prims;
+    const uint32_t* prim_mat;
+    const DevMaterial* mats;
+    const DevSpectrum* spectra;
+    const float* table;
+    const DevLight* lights;
+    uint32_t n_lights;
+    uint32_t n_triangles;
+    DevObserver obs;
+};
+
+// ---------------------------------------------------------------------------
+// spectra
+// ---------------------------------------------------------------------------
+
+// Planck's law with the constants of src/liar/spectra/black_body.cpp:47-59.
+__device__ __host__ inline float planck(float w_m, float temperature) {
+    const float h = 6.626070040e-34f, c = 299792458.0f, kb = 1.3806488e-23f;
+    const float c1 = 2.0f * h * c * c;
+    const float c2 = h * c / kb;
+    const float w5 = (w_m * w_m) * (w_m * w_m) * w_m;
+    return c1 / (w5 * expm1f(c2 / (w_m * temperature)));
+}
+
+// Spectrum value at wavelength w [nm].  `reflectant` applies LiAR's
+// Reflectant normalisation (src/liar/kernel/spectral.cpp:200-207).
+__device__ inline float spectrum_eval(const DevScene& S, uint32_t idx, float w, bool reflectant) {
+    if (idx == LR_NONE) return 0.0f;
+    const DevSpectrum sp = S.spectra[idx];
+    float v = 0.0f;
+    switch (sp.kind) {
+    case LR_SPEC_CONSTANT:
+        return sp.value;
+    case LR_SPEC_SAMPLED: {
+        const float* ws = S.table + sp.offset;
+        const float* vs = ws + sp.count;
+        // upper_bound
+        uint32_t lo = 0, hi = sp.count;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ws[mid] <= w) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo == 0 || lo == sp.count) return 0.0f;
+        const float t = (w - ws[lo - 1]) / (ws[lo] - ws[lo - 1]);
+        v = vs[lo - 1] + t * (vs[lo] - vs[lo - 1]);
+        break;
+    }
+    case LR_SPEC_BINNED: {
+        if (!(w >= sp.lo && w < sp.hi)) return 0.0f;
+        uint32_t k = (uint32_t)((w - sp.lo) / (sp.hi - sp.lo) * (float)sp.count);
+        k = k < sp.count ? k : sp.count - 1;
+        v = S.table[sp.offset + k];
+        break;
+    }
+    case LR_SPEC_BLACKBODY:
+        return sp.bb_k * planck(w * 1e-9f, sp.bb_t);
+    default:
+        return 0.0f;
+    }
+    return reflectant ? v * sp.refl_scale : v;
+}
+
+// Observer::sample (src/liar/kernel/observer.cpp:283-317): piecewise-constant
+// pdf per table interval, proportional to the trapezoid of x+y+z.
+__device__ inline float observer_sample(const DevObserver& O, float u, float& pdf) {
+    uint32_t lo = 0, hi = O.n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (O.cdf[mid] <= u) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo == 0 || lo >= O.n) {
+        pdf = 0.0f;
+        return O.w[0];
+    }
+    const float c1 = O.cdf[lo - 1], c2 = O.cdf[lo];
+    const float dc = c2 - c1;
+    const float x = (u - c1) / dc;
+    const float w1 = O.w[lo - 1], w2 = O.w[lo];
+    pdf = dc / (w2 - w1);
+    return w1 + x * (w2 - w1);
+}
+
+// Observer::sensitivity (src/liar/kernel/observer.cpp:156-171).
+__device__ inline void observer_xyz(const DevObserver& O, float w, float xyz[3]) {
+    uint32_t lo = 0, hi = O.n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (O.w[mid] <= w) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo == 0 || lo == O.n) {
+        // w == last wavelength is the only in-domain value that lands here
+        const bool last = lo == O.n && w == O.w[O.n - 1];
+        for (int a = 0; a < 3; ++a) xyz[a] = last ? O.xyz[3 * (O.n - 1) + a] : 0.0f;
+        return;
+    }
+    const uint32_t k = lo - 1;
+    const float dw = w - O.w[k];
+    for (int a = 0; a < 3; ++a) xyz[a] = O.xyz[3 * k + a] + O.dxyz[3 * k + a] * dw;
+}
+
+// ---------------------------------------------------------------------------
+// vectors and warps
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+__device__ inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline V3 operator*(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__device__ inline V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ inline V3 normalize(V3 a) { return (1.0f / sqrtf(dot(a, a))) * a; }
+__device__ inline V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
+
+// Orthonormal basis around unit n: Duff et al., "Building an Orthonormal
+// Basis, Revisited", JCGT 2017.
+__device__ inline void onb(V3 n, V3& t, V3& b) {
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float bb = n.x * n.y * a;
+    t = v3(1.0f + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+    b = v3(bb, sign + n.y * n.y * a, -n.y);
+}
+
+// Cosine-weighted hemisphere via Shirley-Chiu concentric disk + Malley's
+// method; pdf = cos/pi.
+__device__ inline V3 cosine_hemisphere(float u1, float u2, float& pdf) {
+    const float a = 2.0f * u1 - 1.0f, b = 2.0f * u2 - 1.0f;
+    float r, phi;
+    if (a == 0.0f && b == 0.0f) {
+        r = 0.0f;
+        phi = 0.0f;
+    } else if (a * a > b * b) {
+        r = a;
+        phi = (kPi / 4.0f) * (b / a);
+    } else {
+        r = b;
+        phi = (kPi / 2.0f) - (kPi / 4.0f) * (a / b);
+    }
+    const float x = r * cosf(phi), y = r * sinf(phi);
+    const float z = sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y));
+    pdf = z / kPi;
+    return v3(x, y, z);
+}
+
+// Uniform sphere: z = 1 - 2 u1, phi = 2 pi u2; pdf = 1 / (4 pi).
+__device__ inline V3 uniform_sphere(float u1, float u2) {
+    const float z = 1.0f - 2.0f * u1;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float phi = 2.0f * kPi * u2;
+    return v3(r * cosf(phi), r * sinf(phi), z);
+}
+
+// ---------------------------------------------------------------------------
+// BSDFs in the local frame (z = normal flipped towards the incoming ray,
+// omega_in.z > 0).  Shading normal == geometric normal, so LiAR's
+// geometric-hemisphere caps masking (src/liar/kernel/bsdf.cpp:52-76) reduces
+// to the sign of omega_out.z.
+// ---------------------------------------------------------------------------
+struct BsdfParams {
+    uint32_t type;
+    float refl, trans;  // spectral values at the path's wavelength
+    float eta, kappa;   // conductor n, k
+    float ior;          // dielectric n_incident / n_transmitted
+    float alpha;        // Cook-Torrance roughness^2
+};
+
+__device__ inline bool bsdf_is_diffuse_or_glossy(uint32_t type) {
+    return type == LR_MAT_LAMBERT || type == LR_MAT_COOK_TORRANCE || type == LR_MAT_COOK_TORRANCE_BECKMANN;
+}
+
+// Conductor Fresnel of src/liar/shaders/conductor.cpp:184-194 and
+// cook_torrance.cpp:242-256.
+__device__ inline float fresnel_conductor(float eta, float kappa, float cos_i) {
+    const float cos2 = cos_i * cos_i;
+    const float a = eta * eta + kappa * kappa;
+    const float b = 2.0f * cos_i * eta;
+    const float a_orth = a + cos2;
+    const float a_par = a * cos2 + 1.0f;
+    const float r_orth2 = (a_orth - b) / (a_orth + b);
+    const float r_par2 = (a_par - b) / (a_par + b);
+    return 0.5f * (r_orth2 + r_par2);
+}
+
+// Trowbridge-Reitz D and pdf of h (microfacet_trowbridge_reitz.cpp:46-58),
+// isotropic alpha.
+__device__ inline float ggx_d(V3 h, float alpha, float& pdf_h) {
+    const float hx = h.x / alpha, hy = h.y / alpha;
+    const float q = hx * hx + hy * hy + h.z * h.z;
+    const float d = 1.0f / (kPi * alpha * alpha * q * q);
+    pdf_h = d * h.z;
+    return d;
+}
+
+// Beckmann D and pdf of h (microfacet_beckmann.cpp:50-72), isotropic alpha.
+__device__ inline float beckmann_d(V3 h, float alpha, float& pdf_h) {
+    const float c2 = h.z * h.z;
+    if (c2 == 0.0f) {
+        pdf_h = 0.0f;
+        return 0.0f;
+    }
+    const float a2 = alpha * alpha;
+    const float d = expf(-(h.x * h.x + h.y * h.y) / (a2 * c2)) / (kPi * a2 * c2 * c2);
+    pdf_h = d * h.z;
+    return d;
+}
+
+// Cook-Torrance V-cavity G (cook_torrance.cpp:234-238).
+__device__ inline float ct_g(V3 wi, V3 wo, V3 h) {
+    const float doh = fabsf(dot(wi, h));
+    return fminf(1.0f, fminf(2.0f * h.z * wi.z / doh, 2.0f * h.z * wo.z / doh));
+}
+
+// Bsdf::evaluate for the non-delta BSDFs; returns false for "no value".
+__device__ inline bool bsdf_eval(const BsdfParams& P, V3 wi, V3 wo, float& value, float& pdf) {
+    if (wo.z == 0.0f) return false;
+    if (!(wo.z > 0.0f)) return false;  // transmission masked for reflective BSDFs
+    if (P.type == LR_MAT_LAMBERT) {  // lambert.cpp:105-114
+        value = P.refl / kPi;
+        pdf = fabsf(wo.z) / kPi;
+        return pdf > 0.0f;
+    }
+    if (P.type == LR_MAT_COOK_TORRANCE || P.type == LR_MAT_COOK_TORRANCE_BECKMANN) {  // cook_torrance.cpp:262-280
+        const V3 h = normalize(wi + wo);
+        float pdf_h;
+        const float d = P.type == LR_MAT_COOK_TORRANCE ? ggx_d(h, P.alpha, pdf_h) : beckmann_d(h, P.alpha, pdf_h);
+        const float g = ct_g(wi, wo, h);
+        const float woh = dot(wi, h);
+        pdf = pdf_h / (4.0f * woh);
+        value = P.refl * (d * g / (4.0f * wi.z * wo.z)) *
+                fresnel_conductor(P.eta, P.kappa, fabsf(woh));
+        return pdf > 0.0f;
+    }
+    return false;  // delta BSDFs evaluate to zero (mirror.cpp:135-138 etc.)
+}
+
+// Bsdf::sample.  u1, u2: 2D sample; uc: component sample.
+__device__ inline bool bsdf_sample(const BsdfParams& P, V3 wi, float u1, float u2, float uc, V3& wo, float& value,
+                                   float& pdf) {
+    switch (P.type) {
+    case LR_MAT_LAMBERT: {  // lambert.cpp:116-124
+        wo = cosine_hemisphere(u1, u2, pdf);
+        value = P.refl / kPi;
+        return pdf > 0.0f;
+    }
+    case LR_MAT_MIRROR: {  // mirror.cpp:142-161: value = reflectance, pdf = cos
+        wo = v3(-wi.x, -wi.y, wi.z);
+        value = P.refl;
+        pdf = wo.z;
+        return pdf > 0.0f;
+    }
+    case LR_MAT_CONDUCTOR: {  // conductor.cpp:179-198
+        const float cos_i = wi.z;
+        wo = v3(-wi.x, -wi.y, wi.z);
+        value = P.refl * fresnel_conductor(P.eta, P.kappa, cos_i) / cos_i;
+        pdf = 1.0f;
+        return true;
+    }
+    case LR_MAT_DIELECTRIC: {  // dielectric.cpp:209-247
+        const float cos_i = wi.z;
+        if (cos_i <= 0.0f) return false;
+        const float ior = P.ior;
+        const float sin_t2 = ior * ior * (1.0f - cos_i * cos_i);
+        const float cos_t = sqrtf(fmaxf(1.0f - sin_t2, 0.0f));
+        const float r_orth = (ior * cos_i - cos_t) / (ior * cos_i + cos_t);
+        const float r_par = (cos_i - ior * cos_t) / (cos_i + ior * cos_t);
+        const float fr = 0.5f * (r_orth * r_orth + r_par * r_par);
+        const float pow_r = fabsf(P.refl) * fr;
+        const float pow_t = fabsf(P.trans) * (1.0f - fr);
+        if (pow_r + pow_t == 0.0f) return false;
+        const float prob_r = pow_r / (pow_r + pow_t);
+        if (uc < prob_r) {
+            wo = v3(-wi.x, -wi.y, wi.z);
+            value = P.refl * fr / cos_i;
+            pdf = prob_r;
+        } else {
+            wo = v3(-ior * wi.x, -ior * wi.y, -cos_t);
+            value = P.trans * (1.0f - fr) / cos_t;
+            pdf = 1.0f - prob_r;
+        }
+        return pdf > 0.0f;
+    }
+    case LR_MAT_COOK_TORRANCE: {  // cook_torrance.cpp:284-294, GGX sampleH
+        const float phi = 2.0f * kPi * u1;
+        const float s = sqrtf(u2 / (1.0f - u2));
+        const V3 h = normalize(v3(s * P.alpha * cosf(phi), s * P.alpha * sinf(phi), 1.0f));
+        wo = (2.0f * dot(wi, h)) * h - wi;
+        if (!(wo.z > 0.0f)) return false;
+        return bsdf_eval(P, wi, wo, value, pdf);
+    }
+    case LR_MAT_COOK_TORRANCE_BECKMANN: {  // microfacet_beckmann.cpp:76-106 (isotropic)
+        const float s = logf(1.0f - u1);
+        const float phi = 2.0f * kPi * u2;
+        const float tan2 = -s * P.alpha * P.alpha;
+        const float cos2 = 1.0f / (1.0f + tan2);
+        const float ct = sqrtf(cos2), st = sqrtf(fmaxf(1.0f - cos2, 0.0f));
+        const V3 h = v3(cosf(phi) * st, sinf(phi) * st, ct);
+        wo = (2.0f * dot(wi, h)) * h - wi;
+        if (!(wo.z > 0.0f)) return false;
+        return bsdf_eval(P, wi, wo, value, pdf);
+    }
+    default:
+        return false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// lights (src/liar/scenery/light_area.cpp:189-252, light_sky.cpp:239-297,
+// src/liar/kernel/light_context.cpp:309-331)
+// ---------------------------------------------------------------------------
+
+// Ray (origin p, unit dir d, tnear) against a parallelogram; returns t or -1.
+__device__ inline float parallelogram_hit(const DevLight& L, V3 p, V3 d, float tnear) {
+    const V3 n = ld3(L.n);
+    const float dn = dot(d, n);
+    if (dn == 0.0f) return -1.0f;
+    const float t = dot(ld3(L.o) - p, n) / dn;
+    if (!(t > tnear)) return -1.0f;
+    const V3 q = (p + t * d) - ld3(L.o);
+    const float u = dot(q, ld3(L.du)), v = dot(q, ld3(L.dv));
+    if (!(u >= 0.0f && u <= 1.0f && v >= 0.0f && v <= 1.0f)) return -1.0f;
+    return t;
+}
+
+// LightContext::emission pdf (solid angle) of direction d from point p.
+__device__ inline float light_pdf(const DevLight& L, V3 p, V3 d) {
+    if (L.type == LR_LIGHT_PARALLELOGRAM) {
+        const float t = parallelogram_hit(L, p, d, kTolerance);
+        if (t < 0.0f) return 0.0f;
+        const float c = dot(d, ld3(L.n));
+        if (!L.double_sided && c > 0.0f) return 0.0f;
+        return t * t / (L.area * fabsf(c));
+    }
+    if (L.type == LR_LIGHT_SKY) return 1.0f / (4.0f * kPi);
+    // sun
+    return dot(d, ld3(L.n)) >= L.cos_max ? L.inv_solid : 0.0f;
+}
+
+// Radiance of light L seen along d (for the sky / sun: any ray that escapes;
+// for a parallelogram: the caller already hit it).
+__device__ inline bool light_sees(const DevLight& L, V3 d) {
+    if (L.type == LR_LIGHT_PARALLELOGRAM) return L.double_sided || dot(d, ld3(L.n)) < 0.0f;
+    if (L.type == LR_LIGHT_SKY) return true;
+    return dot(d, ld3(L.n)) >= L.cos_max;
+}
+
+// LightContext::sampleEmission: direction from p towards the light; pdf in
+// solid angle (0 = failed).
+__device__ inline V3 light_sample(const DevLight& L, V3 p, float u1, float u2, float& pdf) {
+    if (L.type == LR_LIGHT_PARALLELOGRAM) {
+        const V3 q = ld3(L.o) + u1 * ld3(L.dx) + u2 * ld3(L.dy);
+        V3 to = q - p;
+        const float d2 = dot(to, to);
+        to = (1.0f / sqrtf(d2)) * to;
+        const float c = dot(ld3(L.n), to);
+        pdf = (1.0f / L.area) * d2 / fabsf(c);
+        if (!L.double_sided && c > 0.0f) pdf = 0.0f;
+        if (!(pdf < INFINITY)) pdf = 0.0f;
+        return to;
+    }
+    if (L.type == LR_LIGHT_SKY) {
+        pdf = 1.0f / (4.0f * kPi);
+        return uniform_sphere(u1, u2);
+    }
+    // sun: uniform cone around L.n
+    const float cos_t = 1.0f - u1 * (1.0f - L.cos_max);
+    const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = 2.0f * kPi * u2;
+    V3 t, b;
+    const V3 n = ld3(L.n);
+    onb(n, t, b);
+    pdf = L.inv_solid;
+    return (sin_t * cosf(phi)) * t + (sin_t * sinf(phi)) * b + cos_t * n;
+}
+
+// LightContexts::sample (light_context.cpp:309-319): lower_bound on the cdf.
+__device__ inline uint32_t light_pick(const DevScene& S, float u) {
+    uint32_t lo = 0, hi = S.n_lights;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (S.lights[mid].cdf < u) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < S.n_lights ? lo : S.n_lights - 1;
+}
+
+}  // namespace lr

@@ -1,0 +1,835 @@
+// render.hip -- spectral wavefront path tracer for gfx950 and its C-ABI.
+//
+// Integrand: LiAR's AdjointPhotonTracer (src/liar/tracers/adjoint_photon_tracer.cpp:77-292)
+// in SPECTRAL_MODE=Single: every path carries one sampled wavelength; at each
+// surface a one-sample MIS choice between BSDF sampling and light sampling
+// picks the next direction (power heuristic over both pdfs), emission is
+// added when a later generation hits a light, Russian roulette applies from
+// a configurable generation on.
+//
+// Wavefront: per chunk of paths (whole pixels), generation g runs
+//   trace  : closest hit of queue[g&1] (persistent QBVH kernel, length read
+//            from device memory)
+//   shade  : emission + BSDF/light scatter; survivors appended to
+//            queue[(g+1)&1] with one atomic per wave (ballot + mbcnt)
+// and a final per-pixel reduction adds the paths' XYZ to the film in a fixed
+// order, so renders are deterministic for a seed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lr_internal.h"
+#include "lr_render.h"
+
+using namespace lr;
+
+namespace {
+
+std::mutex g_obs_mu;
+std::vector<float> g_obs_w, g_obs_xyz;  // standard observer (nm, xyz interleaved)
+
+struct DevCamera {
+    float pos[3], dir[3], right[3], down[3], dir_n[3];
+    float near_limit;
+};
+
+struct Params {
+    uint32_t width, height, spp, nl, max_gen, rr_depth;
+    float rr_p;
+    uint32_t seed_lo, seed_hi;
+    uint32_t strata;  // sqrt(spp) if spp is a square, else 0
+};
+
+struct Queue {
+    float* r[8];  // ox oy oz dx dy dz tnear tfar
+    uint32_t* pid;
+};
+
+struct PathState {
+    float* thr;
+    float* L;
+    float* lam;
+    float* lpdf;
+};
+
+struct Hits {
+    uint32_t* prim;
+    float *t, *u, *v;
+};
+
+__device__ inline lr::u4 rng(const Params& P, uint64_t path, uint32_t gen, uint32_t purpose) {
+    return lr::philox4x32(P.seed_lo, P.seed_hi, lr::u4{(uint32_t)path, (uint32_t)(path >> 32), gen, purpose});
+}
+
+// -- generation 0: camera rays (perspective_camera.cpp:406-434) -------------
+__global__ void k_camera(DevScene S, DevCamera C, Params P, uint64_t path0, uint32_t n, PathState ps, Queue q,
+                         uint32_t* count, uint64_t* rays_total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t p = path0 + i;
+    const uint32_t l = (uint32_t)(p % P.nl);
+    const uint64_t sidx = p / P.nl;
+    const uint32_t s = (uint32_t)(sidx % P.spp);
+    const uint64_t pixel = sidx / P.spp;
+    const uint32_t px = (uint32_t)(pixel % P.width), py = (uint32_t)(pixel / P.width);
+    const lr::u4 rs = rng(P, sidx, 0xFFFFu, 0u);  // per camera sample (shared by its wavelengths)
+    const lr::u4 rp = rng(P, p, 0xFFFEu, 0u);     // per path
+    float sx = lr::u01(rs.x), sy = lr::u01(rs.y);
+    if (P.strata) {
+        sx = ((float)(s % P.strata) + sx) / (float)P.strata;
+        sy = ((float)(s / P.strata) + sy) / (float)P.strata;
+    }
+    const float X = ((float)px + sx) / (float)P.width;
+    const float Y = ((float)py + sy) / (float)P.height;
+    V3 d = ld3(C.dir) + (X - 0.5f) * ld3(C.right) + (Y - 0.5f) * ld3(C.down);
+    d = normalize(d);
+    const float ray_factor = 1.0f / dot(d, ld3(C.dir_n));
+    // stratified wavelength (stratifier.cpp:191-196 with nl strata per sample)
+    const float uw = ((float)l + lr::u01(rp.x)) / (float)P.nl;
+    float lpdf;
+    const float lam = observer_sample(S.obs, uw, lpdf);
+    float thr = lpdf > 0.0f ? 1.0f : 0.0f;
+    // generation guard / roulette of tracePhoton(0) (adjoint_photon_tracer.cpp:84-101)
+    if (P.max_gen == 0) thr = 0.0f;
+    if (thr > 0.0f && P.rr_depth == 0) {
+        if (lr::u01(rp.y) < P.rr_p) thr /= P.rr_p;
+        else thr = 0.0f;
+    }
+    ps.thr[i] = thr;
+    ps.L[i] = 0.0f;
+    ps.lam[i] = lam;
+    ps.lpdf[i] = lpdf;
+    const bool alive = thr > 0.0f;
+    const uint64_t mask = __ballot(alive);
+    if (mask) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) {
+            base = atomicAdd(count, (uint32_t)__popcll(mask));
+            atomicAdd((unsigned long long*)rays_total, (unsigned long long)__popcll(mask));
+        }
+        base = __shfl(base, leader);
+        if (alive) {
+            const uint32_t pos =
+                base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            q.r[0][pos] = C.pos[0];
+            q.r[1][pos] = C.pos[1];
+            q.r[2][pos] = C.pos[2];
+            q.r[3][pos] = d.x;
+            q.r[4][pos] = d.y;
+            q.r[5][pos] = d.z;
+            q.r[6][pos] = fmaxf(kTolerance, ray_factor * C.near_limit);
+            q.r[7][pos] = INFINITY;
+            q.pid[pos] = i;
+        }
+    }
+}
+
+__global__ void k_zero_words(uint32_t* c, uint32_t n) {
+    if (threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+__global__ void k_fill_miss_queue(const uint32_t* count, Hits h) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *count) {
+        h.prim[i] = LR_MISS;
+        h.t[i] = INFINITY;
+    }
+}
+
+// -- one generation: emission + scatter (adjoint_photon_tracer.cpp:140-292) --
+__global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queue q, const uint32_t* count,
+                        Hits h, const float4* __restrict__ prim_geom, PathState ps, Queue qn, uint32_t* count_next,
+                        uint64_t* rays_total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < *count;
+    bool alive = false;
+    V3 start = v3(0, 0, 0), dir = v3(0, 0, 1);
+    uint32_t pid = 0;
+    if (valid) {
+        pid = q.pid[i];
+        const V3 o = v3(q.r[0][i], q.r[1][i], q.r[2][i]);
+        const V3 d = v3(q.r[3][i], q.r[4][i], q.r[5][i]);
+        const float tnear = q.r[6][i];
+        float thr = ps.thr[pid];
+        float L = ps.L[pid];
+        const float lam = ps.lam[pid];
+        const uint32_t prim = h.prim[i];
+        if (prim == LR_MISS) {
+            // escaped: the sky / sun are lights, counted from generation 1 on
+            if (gen > 0) {
+                for (uint32_t k = 0; k < S.n_lights; ++k) {
+                    const DevLight& Lk = S.lights[k];
+                    if (Lk.type != LR_LIGHT_PARALLELOGRAM && light_sees(Lk, d))
+                        L += thr * spectrum_eval(S, Lk.radiance, lam, false);
+                }
+            }
+        } else {
+            const float t = h.t[i];
+            const V3 x = o + t * d;
+            const float4 g = prim_geom[prim];
+            V3 ng;
+            bool leaving;
+            if (prim < S.n_triangles) {
+                ng = v3(g.x, g.y, g.z);
+                leaving = dot(d, ng) > 0.0f;  // triangle_mesh.cpp:304-309
+            } else {
+                const V3 c = v3(g.x, g.y, g.z);
+                ng = (1.0f / g.w) * (x - c);
+                const V3 s0 = (o + tnear * d) - c;
+                leaving = dot(s0, s0) < g.w * g.w;  // sphere.cpp:110-111
+            }
+            const V3 nf = dot(ng, d) < 0.0f ? ng : -ng;  // flipped towards the viewer
+            const DevMaterial m = S.mats[S.prim_mat[prim]];
+            if (gen > 0 && m.light != LR_NONE) {
+                const DevLight& Lk = S.lights[m.light];
+                if (light_sees(Lk, d)) L += thr * spectrum_eval(S, Lk.radiance, lam, false);
+            }
+            if ((gen == 0 || m.light == LR_NONE) && m.emission != LR_NONE)
+                L += thr * spectrum_eval(S, m.emission, lam, false);
+
+            if (m.type != LR_MAT_NONE) {
+                BsdfParams B;
+                B.type = m.type;
+                B.refl = spectrum_eval(S, m.refl, lam, true);
+                B.trans = spectrum_eval(S, m.trans, lam, true);
+                B.eta = fmaxf(spectrum_eval(S, m.eta, lam, false), 1e-9f);
+                B.kappa = fmaxf(spectrum_eval(S, m.kappa, lam, false), 1e-9f);
+                B.ior = leaving ? B.eta / B.kappa : B.kappa / B.eta;  // dielectric.cpp:150-155
+                const float r = fmaxf(m.roughness, 1e-3f);
+                B.alpha = r * r;
+
+                V3 tt, bb;
+                onb(nf, tt, bb);
+                const V3 wiw = -d;
+                const V3 wi = v3(dot(wiw, tt), dot(wiw, bb), fmaxf(dot(wiw, nf), 1.1920929e-7f));
+                const uint64_t gp = path0 + pid;
+                const lr::u4 r0 = rng(P, gp, gen, 1u);
+                const lr::u4 r1 = rng(P, gp, gen, 2u);
+                const float u_strat = lr::u01(r0.x), u1 = lr::u01(r0.y), u2 = lr::u01(r0.z);
+                const float u_comp = lr::u01(r0.w), u_light = lr::u01(r1.x), ul1 = lr::u01(r1.y),
+                            ul2 = lr::u01(r1.z), u_rr = lr::u01(r1.w);
+                const float strategy = bsdf_is_diffuse_or_glossy(m.type) ? 0.5f : 1.0f;
+                V3 wo;
+                float value = 0.0f, pdf = 0.0f;
+                bool ok = false;
+                if (u_strat < strategy) {
+                    // BSDF sampling, MIS-weighted against all lights
+                    ok = bsdf_sample(B, wi, u1, u2, u_comp, wo, value, pdf);
+                    if (ok) {
+                        pdf *= strategy;
+                        float pdf_lights = 0.0f;
+                        if (strategy < 1.0f) {
+                            const V3 dw = wo.x * tt + wo.y * bb + wo.z * nf;
+                            for (uint32_t k = 0; k < S.n_lights; ++k)
+                                pdf_lights += S.lights[k].sel_pdf * light_pdf(S.lights[k], x, dw);
+                            pdf_lights *= (1.0f - strategy);
+                        }
+                        value *= (pdf * pdf) / (pdf * pdf + pdf_lights * pdf_lights);
+                    }
+                } else if (S.n_lights > 0) {
+                    // light sampling
+                    const uint32_t k = light_pick(S, u_light);
+                    const DevLight& Lk = S.lights[k];
+                    float pdf_l = 0.0f;
+                    const V3 dw = light_sample(Lk, x, ul1, ul2, pdf_l);
+                    if (Lk.sel_pdf > 0.0f && pdf_l > 0.0f) {
+                        float pl = Lk.sel_pdf * pdf_l;
+                        for (uint32_t j = 0; j < S.n_lights; ++j)
+                            if (j != k) pl += S.lights[j].sel_pdf * light_pdf(S.lights[j], x, dw);
+                        pl *= (1.0f - strategy);
+                        wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
+                        float vb, pb;
+                        if (bsdf_eval(B, wi, wo, vb, pb) && vb != 0.0f) {
+                            const float pbs = pb * strategy;
+                            value = vb * (pl * pl) / (pl * pl + pbs * pbs);
+                            pdf = pl;
+                            ok = true;
+                        }
+                    }
+                }
+                if (ok && pdf > 0.0f) {
+                    const float cos_t = wo.z;
+                    thr *= value * fabsf(cos_t) / pdf;
+                    start = x + ((cos_t > 0.0f ? 10.0f : -10.0f) * kTolerance) * nf;
+                    dir = normalize(wo.x * tt + wo.y * bb + wo.z * nf);
+                    const uint32_t next = gen + 1;
+                    if (next >= P.max_gen) thr = 0.0f;
+                    else if (next >= P.rr_depth) {
+                        if (u_rr < P.rr_p) thr /= P.rr_p;
+                        else thr = 0.0f;
+                    }
+                    alive = thr > 0.0f && isfinite(thr);
+                }
+            }
+        }
+        ps.L[pid] = L;
+        ps.thr[pid] = thr;
+    }
+    const uint64_t mask = __ballot(alive);
+    if (mask) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) {
+            base = atomicAdd(count_next, (uint32_t)__popcll(mask));
+            atomicAdd((unsigned long long*)rays_total, (unsigned long long)__popcll(mask));
+        }
+        base = __shfl(base, leader);
+        if (alive) {
+            const uint32_t pos =
+                base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            qn.r[0][pos] = start.x;
+            qn.r[1][pos] = start.y;
+            qn.r[2][pos] = start.z;
+            qn.r[3][pos] = dir.x;
+            qn.r[4][pos] = dir.y;
+            qn.r[5][pos] = dir.z;
+            qn.r[6][pos] = kTolerance;
+            qn.r[7][pos] = INFINITY;
+            qn.pid[pos] = pid;
+        }
+    }
+}
+
+// -- film: Raster box filter, fixed-order per-pixel sums (raster.cpp:547-606)
+__global__ void k_splat(DevScene S, Params P, uint64_t pix0, uint32_t npix, PathState ps, float* film_xyz,
+                        float* film_w) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npix) return;
+    const uint32_t ppp = P.spp * P.nl;
+    float X = 0.0f, Y = 0.0f, Z = 0.0f;
+    for (uint32_t k = 0; k < ppp; ++k) {
+        const uint32_t i = j * ppp + k;
+        const float L = ps.L[i];
+        if (L == 0.0f) continue;
+        float xyz[3];
+        observer_xyz(S.obs, ps.lam[i], xyz);
+        const float s = L / ps.lpdf[i];  // Spectral::xyz, spectral.cpp:212-221
+        X += s * xyz[0];
+        Y += s * xyz[1];
+        Z += s * xyz[2];
+    }
+    const uint64_t px = pix0 + j;
+    const uint64_t np = (uint64_t)P.width * P.height;
+    film_xyz[px] += X;
+    film_xyz[np + px] += Y;
+    film_xyz[2 * np + px] += Z;
+    film_w[px] += (float)ppp;
+}
+
+// -- host helpers --------------------------------------------------------------
+
+template <class T>
+int upload(std::vector<T>& v, T** d) {
+    *d = nullptr;
+    if (v.empty()) return LR_OK;
+    LR_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(d), sizeof(T) * v.size()));
+    LR_HIP_CHECK(hipMemcpy(*d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return LR_OK;
+}
+
+float host_planck(float w_m, float temp) { return lr::planck(w_m, temp); }
+
+// Host evaluation of a user spectrum (for luminance / normalisation).
+double host_spectrum(const lr_spectrum& s, double w) {
+    switch (s.kind) {
+    case LR_SPEC_CONSTANT:
+        return s.value;
+    case LR_SPEC_SAMPLED: {
+        const float* ws = s.w;
+        const uint32_t n = s.count;
+        const float* it = std::upper_bound(ws, ws + n, (float)w);
+        if (it == ws || it == ws + n) return 0.0;
+        const size_t k = (size_t)(it - ws);
+        const double t = (w - ws[k - 1]) / ((double)ws[k] - ws[k - 1]);
+        return s.v[k - 1] + t * ((double)s.v[k] - s.v[k - 1]);
+    }
+    case LR_SPEC_BINNED: {
+        if (!(w >= s.lo && w < s.hi)) return 0.0;
+        uint32_t k = (uint32_t)((w - s.lo) / ((double)s.hi - s.lo) * s.count);
+        return s.v[std::min(k, s.count - 1)];
+    }
+    case LR_SPEC_BLACKBODY: {
+        const float peak = 2.897771955e-3f / s.temperature;
+        return (double)s.scale * host_planck((float)(w * 1e-9), s.temperature) / host_planck(peak, s.temperature);
+    }
+    }
+    return 0.0;
+}
+
+// Luminance Y of a spectrum under the standard observer (trapezoid over the
+// observer table, normalised so that Y(1) = 1; observer.cpp:105-123).
+double host_luminance(const lr_spectrum& s) {
+    std::lock_guard<std::mutex> g(g_obs_mu);
+    const size_t n = g_obs_w.size();
+    double num = 0.0, den = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+        const double wp = g_obs_w[k > 0 ? k - 1 : 0], wn = g_obs_w[k + 1 < n ? k + 1 : n - 1];
+        const double dw = (wn - wp) / 2.0;
+        num += host_spectrum(s, g_obs_w[k]) * g_obs_xyz[3 * k + 1] * dw;
+        den += g_obs_xyz[3 * k + 1] * dw;
+    }
+    return den > 0.0 ? num / den : 0.0;
+}
+
+}  // namespace
+
+struct lr_scene {
+    lr_context* ctx = nullptr;
+    lr_qbvh* bvh = nullptr;
+    uint32_t n_triangles = 0, n_lights = 0;
+    uint32_t* d_prim_mat = nullptr;
+    float4* d_prim_geom = nullptr;
+    DevMaterial* d_mats = nullptr;
+    DevSpectrum* d_spectra = nullptr;
+    float* d_table = nullptr;
+    DevLight* d_lights = nullptr;
+    // observer restricted to the last render's sampling domain
+    float obs_lo = -1.0f, obs_hi = -1.0f;
+    uint32_t obs_n = 0;
+    float* d_obs = nullptr;
+    // wavefront workspace (grow-only)
+    size_t cap = 0;
+    float* d_work = nullptr;
+    uint32_t* d_counts = nullptr;  // [0..1] queue lengths, [4..5] u64 ray counter
+};
+
+namespace {
+
+void free_scene(lr_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    void* ptrs[] = {s->d_prim_mat, s->d_prim_geom, s->d_mats, s->d_spectra, s->d_table, s->d_lights,
+                    s->d_obs,      s->d_work,      s->d_counts};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+int build_observer(lr_scene* sc, float lo, float hi) {
+    if (sc->d_obs && sc->obs_lo == lo && sc->obs_hi == hi) return LR_OK;
+    std::vector<double> w, x, y, z;
+    {
+        std::lock_guard<std::mutex> g(g_obs_mu);
+        const size_t n = g_obs_w.size();
+        if (n < 2) return lr::fail(LR_E_INVALID, "no standard observer: call lr_set_standard_observer first");
+        auto sens = [&](double wl, int a) -> double {
+            const float* it = std::upper_bound(g_obs_w.data(), g_obs_w.data() + n, (float)wl);
+            if (it == g_obs_w.data()) return 0.0;
+            if (it == g_obs_w.data() + n) return wl == g_obs_w[n - 1] ? g_obs_xyz[3 * (n - 1) + a] : 0.0;
+            const size_t k = (size_t)(it - g_obs_w.data()) - 1;
+            const double t = (wl - g_obs_w[k]) / ((double)g_obs_w[k + 1] - g_obs_w[k]);
+            return g_obs_xyz[3 * k + a] + t * ((double)g_obs_xyz[3 * (k + 1) + a] - g_obs_xyz[3 * k + a]);
+        };
+        const double a = std::max<double>(lo, g_obs_w[0]), b = std::min<double>(hi, g_obs_w[n - 1]);
+        if (!(b > a)) return lr::fail(LR_E_INVALID, "wavelength domain does not overlap the observer");
+        w.push_back(a);
+        for (size_t k = 0; k < n; ++k)
+            if (g_obs_w[k] > a && g_obs_w[k] < b) w.push_back(g_obs_w[k]);
+        w.push_back(b);
+        for (double wl : w) {
+            x.push_back(sens(wl, 0));
+            y.push_back(sens(wl, 1));
+            z.push_back(sens(wl, 2));
+        }
+    }
+    const size_t n = w.size();
+    std::vector<double> cdf(n, 0.0);
+    for (size_t k = 0; k + 1 < n; ++k)
+        cdf[k + 1] = cdf[k] + (x[k] + y[k] + z[k] + x[k + 1] + y[k + 1] + z[k + 1]) * (w[k + 1] - w[k]);
+    double Y = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+        const double wp = w[k > 0 ? k - 1 : 0], wn = w[k + 1 < n ? k + 1 : n - 1];
+        Y += y[k] * (wn - wp) / 2.0;
+    }
+    if (!(Y > 0.0) || !(cdf[n - 1] > 0.0)) return lr::fail(LR_E_INVALID, "observer has no sensitivity in domain");
+    std::vector<float> buf(8 * n);
+    float* W = buf.data();
+    float* CDF = W + n;
+    float* XYZ = CDF + n;
+    float* DXYZ = XYZ + 3 * n;
+    for (size_t k = 0; k < n; ++k) {
+        W[k] = (float)w[k];
+        CDF[k] = (float)(cdf[k] / cdf[n - 1]);
+        XYZ[3 * k + 0] = (float)(x[k] / Y);
+        XYZ[3 * k + 1] = (float)(y[k] / Y);
+        XYZ[3 * k + 2] = (float)(z[k] / Y);
+    }
+    CDF[n - 1] = 1.0f;
+    for (size_t k = 0; k < n; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            DXYZ[3 * k + a] = k + 1 < n ? (XYZ[3 * (k + 1) + a] - XYZ[3 * k + a]) / (W[k + 1] - W[k]) : 0.0f;
+        }
+    }
+    if (sc->d_obs) LR_HIP_CHECK(hipFree(sc->d_obs));
+    sc->d_obs = nullptr;
+    LR_HIP_CHECK(hipMalloc(&sc->d_obs, sizeof(float) * buf.size()));
+    LR_HIP_CHECK(hipMemcpy(sc->d_obs, buf.data(), sizeof(float) * buf.size(), hipMemcpyHostToDevice));
+    sc->obs_lo = lo;
+    sc->obs_hi = hi;
+    sc->obs_n = (uint32_t)n;
+    return LR_OK;
+}
+
+int ensure_work(lr_scene* sc, size_t paths, hipStream_t s) {
+    if (sc->cap >= paths) return LR_OK;
+    LR_HIP_CHECK(hipStreamSynchronize(s));
+    if (sc->d_work) LR_HIP_CHECK(hipFree(sc->d_work));
+    sc->d_work = nullptr;
+    // path state 4, two queues 2 * 9, hits 4 (32-bit words per path)
+    LR_HIP_CHECK(hipMalloc(&sc->d_work, sizeof(float) * 26 * paths));
+    if (!sc->d_counts) LR_HIP_CHECK(hipMalloc(&sc->d_counts, 64));
+    sc->cap = paths;
+    return LR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lr_set_standard_observer(const float* w, const float* xyz, uint32_t n) {
+    if (!w || !xyz || n < 2) return lr::fail(LR_E_INVALID, "observer needs at least two wavelengths");
+    for (uint32_t k = 0; k + 1 < n; ++k)
+        if (!(w[k + 1] > w[k])) return lr::fail(LR_E_INVALID, "observer wavelengths must be strictly increasing");
+    std::lock_guard<std::mutex> g(g_obs_mu);
+    g_obs_w.assign(w, w + n);
+    g_obs_xyz.assign(xyz, xyz + 3 * (size_t)n);
+    return LR_OK;
+}
+
+int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material, const lr_material* materials,
+                    uint32_t n_materials, const lr_spectrum* spectra, uint32_t n_spectra, const lr_light* lights,
+                    uint32_t n_lights, lr_scene** out) {
+    if (!ctx || !bvh || !out) return lr::fail(LR_E_INVALID, "NULL context, geometry or out");
+    *out = nullptr;
+    if (bvh->ctx != ctx) return lr::fail(LR_E_INVALID, "geometry belongs to another context");
+    const uint32_t n_prims = bvh->info.n_prims;
+    if (n_prims && !prim_material) return lr::fail(LR_E_INVALID, "prim_material is NULL");
+    if ((n_materials && !materials) || (n_spectra && !spectra) || (n_lights && !lights))
+        return lr::fail(LR_E_INVALID, "NULL materials / spectra / lights array");
+    {
+        std::lock_guard<std::mutex> g(g_obs_mu);
+        if (g_obs_w.size() < 2) return lr::fail(LR_E_INVALID, "no standard observer: call lr_set_standard_observer first");
+    }
+    auto spec_ok = [&](uint32_t i) { return i == LR_NONE || i < n_spectra; };
+    // spectra -> device table
+    std::vector<DevSpectrum> dspec(n_spectra);
+    std::vector<float> table;
+    for (uint32_t i = 0; i < n_spectra; ++i) {
+        const lr_spectrum& s = spectra[i];
+        DevSpectrum& d = dspec[i];
+        std::memset(&d, 0, sizeof(d));
+        d.kind = s.kind;
+        d.refl_scale = 1.0f;
+        switch (s.kind) {
+        case LR_SPEC_CONSTANT:
+            d.value = s.value;
+            break;
+        case LR_SPEC_SAMPLED: {
+            if (s.count < 2 || !s.w || !s.v) return lr::fail(LR_E_INVALID, "sampled spectrum needs >= 2 points");
+            for (uint32_t k = 0; k + 1 < s.count; ++k)
+                if (!(s.w[k + 1] > s.w[k])) return lr::fail(LR_E_INVALID, "sampled spectrum wavelengths must increase");
+            d.count = s.count;
+            d.offset = (uint32_t)table.size();
+            table.insert(table.end(), s.w, s.w + s.count);
+            table.insert(table.end(), s.v, s.v + s.count);
+            const float mx = *std::max_element(s.v, s.v + s.count);
+            d.refl_scale = mx > 1.0f ? 1.0f / mx : 1.0f;
+            break;
+        }
+        case LR_SPEC_BINNED: {
+            if (s.count < 1 || !s.v || !(s.hi > s.lo)) return lr::fail(LR_E_INVALID, "binned spectrum needs bins and lo < hi");
+            d.count = s.count;
+            d.lo = s.lo;
+            d.hi = s.hi;
+            d.offset = (uint32_t)table.size();
+            table.insert(table.end(), s.v, s.v + s.count);
+            const float mx = *std::max_element(s.v, s.v + s.count);
+            d.refl_scale = mx > 1.0f ? 1.0f / mx : 1.0f;
+            break;
+        }
+        case LR_SPEC_BLACKBODY: {
+            if (!(s.temperature > 0.0f)) return lr::fail(LR_E_INVALID, "blackbody temperature must be > 0");
+            d.bb_t = s.temperature;
+            d.bb_k = s.scale / host_planck(2.897771955e-3f / s.temperature, s.temperature);
+            break;
+        }
+        default:
+            return lr::fail(LR_E_INVALID, "unknown spectrum kind " + std::to_string(s.kind));
+        }
+    }
+    std::vector<DevMaterial> dmat(n_materials);
+    for (uint32_t i = 0; i < n_materials; ++i) {
+        const lr_material& m = materials[i];
+        if (m.type > LR_MAT_COOK_TORRANCE_BECKMANN) return lr::fail(LR_E_INVALID, "unknown material type");
+        if (!spec_ok(m.reflectance) || !spec_ok(m.transmittance) || !spec_ok(m.eta) || !spec_ok(m.kappa) ||
+            !spec_ok(m.emission))
+            return lr::fail(LR_E_INVALID, "material " + std::to_string(i) + " references a missing spectrum");
+        if (m.light != LR_NONE && m.light >= n_lights)
+            return lr::fail(LR_E_INVALID, "material " + std::to_string(i) + " references a missing light");
+        dmat[i] = DevMaterial{m.type, m.reflectance, m.transmittance, m.eta, m.kappa, m.emission, m.light, m.roughness};
+    }
+    std::vector<uint32_t> pm(prim_material, prim_material + n_prims);
+    for (uint32_t i = 0; i < n_prims; ++i)
+        if (pm[i] >= n_materials) return lr::fail(LR_E_INVALID, "prim " + std::to_string(i) + " has no material");
+    // per-prim geometry by primitive id: triangle unit normal / sphere (c, r)
+    std::vector<float4> geom(n_prims);
+    for (const lr_prim& q : bvh->host.prims) {
+        const uint32_t id = lr::as_u32(q.a[3]);
+        if (lr::as_u32(q.b[3]) == LR_PRIM_TRIANGLE) {
+            double nx = (double)q.b[1] * q.c[2] - (double)q.b[2] * q.c[1];
+            double ny = (double)q.b[2] * q.c[0] - (double)q.b[0] * q.c[2];
+            double nz = (double)q.b[0] * q.c[1] - (double)q.b[1] * q.c[0];
+            const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+            if (len > 0) {
+                nx /= len;
+                ny /= len;
+                nz /= len;
+            } else {
+                nz = 1.0;
+            }
+            geom[id] = make_float4((float)nx, (float)ny, (float)nz, 0.0f);
+        } else {
+            geom[id] = make_float4(q.a[0], q.a[1], q.a[2], q.b[0]);
+        }
+    }
+    // lights: geometry, power (light_area.cpp:290-294, light_sky.cpp:93-134)
+    const lr_qbvh_info& I = bvh->info;
+    double R = 0.0;
+    for (int k = 0; k < 3; ++k) R += 0.25 * ((double)I.scene_hi[k] - I.scene_lo[k]) * ((double)I.scene_hi[k] - I.scene_lo[k]);
+    R = std::sqrt(R);
+    const double far_away = R > 0 ? 1e3 * R : 1e3;
+    std::vector<DevLight> dl(n_lights);
+    std::vector<double> power(n_lights, 0.0);
+    for (uint32_t i = 0; i < n_lights; ++i) {
+        const lr_light& L = lights[i];
+        DevLight& d = dl[i];
+        std::memset(&d, 0, sizeof(d));
+        d.type = L.type;
+        d.radiance = L.radiance;
+        d.double_sided = L.double_sided ? 1u : 0u;
+        if (L.radiance >= n_spectra) return lr::fail(LR_E_INVALID, "light " + std::to_string(i) + " has no radiance");
+        const double lum = host_luminance(spectra[L.radiance]);
+        if (L.type == LR_LIGHT_PARALLELOGRAM) {
+            const double ax = L.dx[0], ay = L.dx[1], az = L.dx[2], bx = L.dy[0], by = L.dy[1], bz = L.dy[2];
+            const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+            const double area = std::sqrt(cx * cx + cy * cy + cz * cz);
+            if (!(area > 0)) return lr::fail(LR_E_INVALID, "degenerate parallelogram light");
+            for (int k = 0; k < 3; ++k) {
+                d.o[k] = L.origin[k];
+                d.dx[k] = L.dx[k];
+                d.dy[k] = L.dy[k];
+            }
+            d.n[0] = (float)(cx / area);
+            d.n[1] = (float)(cy / area);
+            d.n[2] = (float)(cz / area);
+            d.area = (float)area;
+            // dual basis: du = (dy x n) / (dx . (dy x n)), dv = (n x dx) / (dy . (n x dx))
+            const double nx = cx / area, ny = cy / area, nz = cz / area;
+            const double ux = by * nz - bz * ny, uy = bz * nx - bx * nz, uz = bx * ny - by * nx;
+            const double ud = ax * ux + ay * uy + az * uz;
+            const double vx = ny * az - nz * ay, vy = nz * ax - nx * az, vz = nx * ay - ny * ax;
+            const double vd = bx * vx + by * vy + bz * vz;
+            d.du[0] = (float)(ux / ud);
+            d.du[1] = (float)(uy / ud);
+            d.du[2] = (float)(uz / ud);
+            d.dv[0] = (float)(vx / vd);
+            d.dv[1] = (float)(vy / vd);
+            d.dv[2] = (float)(vz / vd);
+            power[i] = 3.14159265358979 * area * lum * (L.double_sided ? 2.0 : 1.0);
+        } else if (L.type == LR_LIGHT_SKY) {
+            power[i] = lum * 3.14159265358979 * far_away * far_away * 4.0 * 3.14159265358979;
+        } else if (L.type == LR_LIGHT_SUN) {
+            double dx = L.direction[0], dy = L.direction[1], dz = L.direction[2];
+            const double len = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (!(len > 0) || !(L.angle > 0 && L.angle < 3.14159265)) return lr::fail(LR_E_INVALID, "bad sun light");
+            d.n[0] = (float)(dx / len);
+            d.n[1] = (float)(dy / len);
+            d.n[2] = (float)(dz / len);
+            d.cos_max = (float)std::cos(0.5 * L.angle);
+            const double solid = 2.0 * 3.14159265358979 * (1.0 - std::cos(0.5 * L.angle));
+            d.inv_solid = (float)(1.0 / solid);
+            power[i] = lum * solid * 3.14159265358979 * far_away * far_away;
+        } else {
+            return lr::fail(LR_E_INVALID, "unknown light type");
+        }
+    }
+    double total = 0.0;
+    for (double p : power) total += p;
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n_lights; ++i) {
+        acc += power[i];
+        dl[i].cdf = total > 0 ? (float)(acc / total) : (float)(i + 1) / n_lights;
+        dl[i].sel_pdf = total > 0 ? (float)(power[i] / total) : 1.0f / n_lights;
+    }
+    if (n_lights) dl[n_lights - 1].cdf = 1.0f;
+
+    lr_scene* sc = new (std::nothrow) lr_scene();
+    if (!sc) return lr::fail(LR_E_NOMEM, "out of host memory");
+    sc->ctx = ctx;
+    sc->bvh = bvh;
+    sc->n_triangles = bvh->info.n_triangles;
+    sc->n_lights = n_lights;
+    int rc = lr::bind(ctx);
+    if (!rc) rc = upload(pm, &sc->d_prim_mat);
+    if (!rc) rc = upload(geom, &sc->d_prim_geom);
+    if (!rc) rc = upload(dmat, &sc->d_mats);
+    if (!rc) rc = upload(dspec, &sc->d_spectra);
+    if (!rc) rc = upload(table, &sc->d_table);
+    if (!rc) rc = upload(dl, &sc->d_lights);
+    if (rc) {
+        free_scene(sc);
+        return rc;
+    }
+    *out = sc;
+    return LR_OK;
+}
+
+int lr_scene_destroy(lr_scene* sc) {
+    free_scene(sc);
+    return LR_OK;
+}
+
+int lr_render(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, float* film_xyz, float* film_w,
+              lr_render_stats* stats, void* stream) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!sc || !cam || !prm) return lr::fail(LR_E_INVALID, "NULL scene, camera or params");
+    if (!film_xyz || !film_w) return lr::fail(LR_E_INVALID, "NULL film buffer");
+    const lr_render_params& R = *prm;
+    if (!R.width || !R.height || !R.spp || !R.wavelengths)
+        return lr::fail(LR_E_INVALID, "width, height, spp and wavelengths must be > 0");
+    if (!(R.rr_survival > 0.0f && R.rr_survival <= 1.0f)) return lr::fail(LR_E_INVALID, "rr_survival must be in (0, 1]");
+    if (!(R.lambda_max > R.lambda_min)) return lr::fail(LR_E_INVALID, "lambda_max must exceed lambda_min");
+    if (!(cam->fov > 0.0f && cam->fov < 3.14159f) || !(cam->aspect > 0.0f))
+        return lr::fail(LR_E_INVALID, "camera fov must be in (0, pi) and aspect > 0");
+    if (int rc = lr::bind(sc->ctx)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (int rc = build_observer(sc, R.lambda_min, R.lambda_max)) return rc;
+
+    // camera frame (perspective_camera.cpp:126-128, 259, 466-468)
+    DevCamera C;
+    {
+        double d[3] = {cam->direction[0], cam->direction[1], cam->direction[2]};
+        double sk[3] = {cam->sky[0], cam->sky[1], cam->sky[2]};
+        auto nrm = [](double* v) {
+            const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            if (l > 0)
+                for (int k = 0; k < 3; ++k) v[k] /= l;
+            return l;
+        };
+        if (!(nrm(d) > 0)) return lr::fail(LR_E_INVALID, "camera direction is zero");
+        double r[3] = {d[1] * sk[2] - d[2] * sk[1], d[2] * sk[0] - d[0] * sk[2], d[0] * sk[1] - d[1] * sk[0]};
+        if (!(nrm(r) > 0)) return lr::fail(LR_E_INVALID, "camera sky is parallel to its direction");
+        double dn[3] = {d[1] * r[2] - d[2] * r[1], d[2] * r[0] - d[0] * r[2], d[0] * r[1] - d[1] * r[0]};
+        nrm(dn);
+        const double width = 0.036, height = width / cam->aspect;
+        const double focal = (width / 2) / std::tan(cam->fov / 2);
+        for (int k = 0; k < 3; ++k) {
+            C.pos[k] = cam->position[k];
+            C.dir[k] = (float)(d[k] * focal);
+            C.right[k] = (float)(r[k] * width);
+            C.down[k] = (float)(dn[k] * height);
+            C.dir_n[k] = (float)d[k];
+        }
+        C.near_limit = cam->near_limit;
+    }
+    Params P;
+    P.width = R.width;
+    P.height = R.height;
+    P.spp = R.spp;
+    P.nl = R.wavelengths;
+    P.max_gen = R.max_depth ? R.max_depth : 64u;
+    P.rr_depth = R.rr_depth;
+    P.rr_p = R.rr_survival;
+    P.seed_lo = (uint32_t)R.seed;
+    P.seed_hi = (uint32_t)(R.seed >> 32);
+    {
+        const uint32_t q = (uint32_t)std::lround(std::sqrt((double)R.spp));
+        P.strata = q * q == R.spp ? q : 0u;
+    }
+    const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
+    const uint64_t n_pix = (uint64_t)R.width * R.height;
+    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 22);
+    const uint64_t pix_per_chunk = std::max<uint64_t>(1, chunk_req / ppp);
+    const uint64_t cap64 = pix_per_chunk * ppp;
+    if (cap64 >= (1ull << 31)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");
+    const uint32_t cap = (uint32_t)cap64;
+    if (int rc = ensure_work(sc, cap, s)) return rc;
+
+    DevScene S;
+    S.nodes = sc->bvh->d_nodes;
+    S.prims = sc->bvh->d_prims;
+    S.prim_mat = sc->d_prim_mat;
+    S.mats = sc->d_mats;
+    S.spectra = sc->d_spectra;
+    S.table = sc->d_table;
+    S.lights = sc->d_lights;
+    S.n_lights = sc->n_lights;
+    S.n_triangles = sc->n_triangles;
+    S.obs.n = sc->obs_n;
+    S.obs.w = sc->d_obs;
+    S.obs.cdf = sc->d_obs + sc->obs_n;
+    S.obs.xyz = sc->d_obs + 2 * sc->obs_n;
+    S.obs.dxyz = sc->d_obs + 5 * sc->obs_n;
+
+    float* w = sc->d_work;
+    PathState ps{w, w + cap, w + 2 * (size_t)cap, w + 3 * (size_t)cap};
+    Queue q[2];
+    for (int b = 0; b < 2; ++b) {
+        float* base = w + (4 + 9 * b) * (size_t)cap;
+        for (int k = 0; k < 8; ++k) q[b].r[k] = base + k * (size_t)cap;
+        q[b].pid = reinterpret_cast<uint32_t*>(base + 8 * (size_t)cap);
+    }
+    float* hb = w + 22 * (size_t)cap;
+    Hits H{reinterpret_cast<uint32_t*>(hb), hb + cap, hb + 2 * (size_t)cap, hb + 3 * (size_t)cap};
+    uint32_t* counts = sc->d_counts;
+    uint64_t* rays_total = reinterpret_cast<uint64_t*>(sc->d_counts + 4);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 16u);
+
+    uint32_t chunks = 0;
+    for (uint64_t pix0 = 0; pix0 < n_pix; pix0 += pix_per_chunk) {
+        const uint32_t npix = (uint32_t)std::min<uint64_t>(pix_per_chunk, n_pix - pix0);
+        const uint32_t n = (uint32_t)(npix * ppp);
+        const uint64_t path0 = pix0 * ppp;
+        const dim3 blk(256), grd((n + 255) / 256);
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 1u);
+        hipLaunchKernelGGL(k_camera, grd, blk, 0, s, S, C, P, path0, n, ps, q[0], counts, rays_total);
+        for (uint32_t gen = 0; gen < P.max_gen; ++gen) {
+            const int cur = gen & 1, nxt = cur ^ 1;
+            hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + nxt, 1u);
+            lr_rays_soa rays{q[cur].r[0], q[cur].r[1], q[cur].r[2], q[cur].r[3],
+                             q[cur].r[4], q[cur].r[5], q[cur].r[6], q[cur].r[7]};
+            lr_hits_soa hits{H.prim, H.t, H.u, H.v};
+            if (sc->bvh->info.n_nodes == 0) {
+                hipLaunchKernelGGL(k_fill_miss_queue, grd, blk, 0, s, counts + cur, H);
+            } else if (int rc = lr::trace_closest_queue(sc->bvh, &rays, n, counts + cur, &hits, s)) {
+                return rc;
+            }
+            hipLaunchKernelGGL(k_shade, grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
+                               sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total);
+        }
+        hipLaunchKernelGGL(k_splat, dim3((npix + 255) / 256), blk, 0, s, S, P, pix0, npix, ps, film_xyz, film_w);
+        LR_HIP_CHECK(hipGetLastError());
+        ++chunks;
+    }
+    if (stats) {
+        uint64_t rays = 0;
+        LR_HIP_CHECK(hipMemcpyAsync(&rays, rays_total, sizeof(rays), hipMemcpyDeviceToHost, s));
+        LR_HIP_CHECK(hipStreamSynchronize(s));
+        stats->paths = n_pix * ppp;
+        stats->rays = rays;
+        stats->chunks = chunks;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return LR_OK;
+}
+
+}  // extern "C"

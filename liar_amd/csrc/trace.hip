@@ -119,7 +119,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
                                                  const lr_prim* __restrict__ prims, RaysArg rays, uint32_t n,
                                                  HitsArg hits, uint8_t* __restrict__ occluded,
                                                  uint2* __restrict__ spill, int cap,
-                                                 uint32_t* __restrict__ counter, uint32_t refill_min) {
+                                                 uint32_t* __restrict__ counter, uint32_t refill_min,
+                                                 const uint32_t* __restrict__ n_dev) {
+    if (n_dev) n = *n_dev;  // queue length produced on the device
     __shared__ uint32_t s_e[SHORT * BLOCK];
     __shared__ float s_t[SHORT * BLOCK];
     Stack<BLOCK, SHORT> st;
@@ -313,7 +315,7 @@ __global__ void k_fill_u8(uint32_t n, uint8_t* p, uint8_t v) {
 __global__ void k_zero_u32(uint32_t* p) { p[threadIdx.x] = 0u; }
 
 using TraceFn = void (*)(const lr_node*, const lr_prim*, RaysArg, uint32_t, HitsArg, uint8_t*, uint2*, int,
-                         uint32_t*, uint32_t);
+                         uint32_t*, uint32_t, const uint32_t*);
 
 struct KernelEntry {
     uint32_t block, lds_stack;
@@ -387,10 +389,10 @@ int prepare(lr_qbvh* bvh, uint32_t n, bool any, hipStream_t stream, Launch& L) {
 }
 
 int launch(lr_qbvh* bvh, const Launch& L, const RaysArg& ra, uint32_t n, const HitsArg& ha, uint8_t* occ,
-           hipStream_t s) {
+           hipStream_t s, const uint32_t* n_dev = nullptr) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, L.counter);
     hipLaunchKernelGGL(L.fn, dim3(L.grid), dim3(L.block), 0, s, bvh->d_nodes, bvh->d_prims, ra, n, ha, occ,
-                       L.spill, L.cap, L.counter, L.refill_min);
+                       L.spill, L.cap, L.counter, L.refill_min, n_dev);
     LR_HIP_CHECK(hipGetLastError());
     return LR_OK;
 }
@@ -409,6 +411,20 @@ int fail(int code, const std::string& msg) {
 int bind(const lr_context* ctx) {
     LR_HIP_CHECK(hipSetDevice(ctx->device));
     return LR_OK;
+}
+
+}  // namespace lr
+
+namespace lr {
+
+int trace_closest_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, const uint32_t* n_dev,
+                        lr_hits_soa* hits, hipStream_t s) {
+    if (n_max == 0 || bvh->info.n_nodes == 0) return LR_OK;  // callers treat an empty scene separately
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar};
+    const HitsArg ha{hits->prim, hits->t, hits->u, hits->v};
+    Launch L;
+    if (int rc = prepare(bvh, n_max, false, s, L)) return rc;
+    return launch(bvh, L, ra, n_max, ha, nullptr, s, n_dev);
 }
 
 }  // namespace lr

@@ -1,0 +1,271 @@
+"""RenderEngine (mirrors ``liar.RenderEngine``, src/liar/kernel/render_engine.cpp:35-247)
+and the flattening of a scene graph into the C-ABI scene description.
+
+``compile_scene`` turns the object graph into plain arrays (triangles,
+spheres, one material per primitive, spectra, lights); ``RenderEngine.render``
+builds the QBVH and scene on the GPU and runs ``lr_render`` into its target.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from . import cameras, scenery, shaders, spectra, textures
+from .observer import ensure_standard_observer
+
+
+@dataclass
+class CompiledScene:
+    verts: np.ndarray          # (nv, 3) float32
+    tris: np.ndarray           # (nt, 3) uint32
+    spheres: np.ndarray        # (ns, 4) float32
+    prim_material: np.ndarray  # (nt + ns,) uint32
+    materials: list = field(default_factory=list)   # dicts (type, slots, roughness)
+    spectra: list = field(default_factory=list)     # spectra.Spectrum objects
+    lights: list = field(default_factory=list)      # dicts
+
+
+def _spectrum_of(tex, table, index):
+    if tex is None:
+        return N.LR_NONE
+    tex = textures.as_texture(tex)
+    sp = tex.spectrum
+    key = id(sp)
+    if key not in index:
+        index[key] = len(table)
+        table.append(sp)
+    return index[key]
+
+
+def _scalar_of(tex, default):
+    if tex is None:
+        return default
+    return textures.as_texture(tex).scalar()
+
+
+def _material_of(shader, table, index):
+    m = {"type": N.MAT_NONE, "reflectance": N.LR_NONE, "transmittance": N.LR_NONE, "eta": N.LR_NONE,
+         "kappa": N.LR_NONE, "emission": N.LR_NONE, "light": N.LR_NONE, "roughness": 0.0}
+    parts = shader.children if isinstance(shader, shaders.Sum) else ([shader] if shader is not None else [])
+    bsdfs = [p for p in parts if not isinstance(p, shaders.Unshaded)]
+    emit = [p for p in parts if isinstance(p, shaders.Unshaded)]
+    if len(bsdfs) > 1:
+        raise NotImplementedError("Sum of more than one BSDF shader is not supported")
+    if len(emit) > 1:
+        raise NotImplementedError("Sum of more than one Unshaded shader is not supported")
+    if emit:
+        m["emission"] = _spectrum_of(emit[0].colour, table, index)
+    if bsdfs:
+        s = bsdfs[0]
+        if isinstance(s, shaders.Lambert):
+            m.update(type=N.MAT_LAMBERT, reflectance=_spectrum_of(s.diffuse, table, index))
+        elif isinstance(s, shaders.Mirror):
+            if s.fuzz is not None and _scalar_of(s.fuzz, 0.0) > 0:
+                raise NotImplementedError("Mirror.fuzz is not supported")
+            m.update(type=N.MAT_MIRROR, reflectance=_spectrum_of(s.reflectance, table, index))
+        elif isinstance(s, shaders.Dielectric):
+            m.update(type=N.MAT_DIELECTRIC, reflectance=_spectrum_of(s.reflectance, table, index),
+                     transmittance=_spectrum_of(s.transmittance, table, index),
+                     eta=_spectrum_of(s.innerRefractionIndex, table, index),
+                     kappa=_spectrum_of(s.outerRefractionIndex, table, index))
+        elif isinstance(s, shaders.Conductor):
+            m.update(type=N.MAT_CONDUCTOR, reflectance=_spectrum_of(s.reflectance, table, index),
+                     eta=_spectrum_of(s.refractionIndex, table, index),
+                     kappa=_spectrum_of(s.absorptionCoefficient, table, index))
+        elif isinstance(s, shaders.CookTorrance):
+            ru, rv = _scalar_of(s.roughnessU, 0.1), _scalar_of(s.roughnessV, 0.1)
+            if ru != rv:
+                raise NotImplementedError("anisotropic CookTorrance roughness is not supported")
+            t = N.MAT_COOK_TORRANCE if isinstance(s.mdf, shaders.MicrofacetTrowbridgeReitz) \
+                else N.MAT_COOK_TORRANCE_BECKMANN
+            m.update(type=t, reflectance=_spectrum_of(s.reflectance, table, index),
+                     eta=_spectrum_of(s.refractionIndex, table, index),
+                     kappa=_spectrum_of(s.absorptionCoefficient, table, index), roughness=float(ru))
+        else:
+            raise NotImplementedError(f"shader {type(s).__name__} is not supported")
+    return m
+
+
+def compile_scene(scene) -> CompiledScene:
+    """Flattens a scene graph into arrays (LiAR's object graph -> one QBVH)."""
+    verts, tris, tri_mat = [], [], []
+    sph, sph_mat = [], []
+    materials, mat_index = [], {}
+    table, spec_index = [], {}
+    lights = []
+    nverts = [0]
+
+    def material(shader, light=N.LR_NONE):
+        key = (id(shader), light)
+        if key not in mat_index:
+            m = _material_of(shader, table, spec_index)
+            m["light"] = light
+            mat_index[key] = len(materials)
+            materials.append(m)
+        return mat_index[key]
+
+    def add_mesh(v, t, mat):
+        verts.append(np.asarray(v, np.float32))
+        tris.append(np.asarray(t, np.uint32) + nverts[0])
+        tri_mat.append(np.full(len(t), mat, np.uint32))
+        nverts[0] += len(v)
+
+    def walk(obj, inherited):
+        shader = obj.shader if obj.shader is not None else inherited
+        if isinstance(obj, scenery.List):
+            for c in obj.children:
+                walk(c, shader)
+        elif isinstance(obj, scenery.TriangleMesh):
+            add_mesh(obj.vertices(), obj.triangles(), material(shader))
+        elif isinstance(obj, scenery.Parallelogram):
+            add_mesh(*obj._mesh(), material(shader))
+        elif isinstance(obj, scenery.Sphere):
+            sph.append([*obj.center, obj.radius])
+            sph_mat.append(material(shader))
+        elif isinstance(obj, scenery.LightArea):
+            surf = obj.surface
+            li = len(lights)
+            lights.append({"type": N.LIGHT_PARALLELOGRAM,
+                           "radiance": _spectrum_of(obj.radiance, table, spec_index),
+                           "double_sided": bool(obj.isDoubleSided),
+                           "origin": surf.support, "dx": surf.sizeU, "dy": surf.sizeV})
+            s = obj.shader if obj.shader is not None else surf.shader
+            add_mesh(*surf._mesh(), material(s, li))
+        elif isinstance(obj, scenery.LightSky):
+            lights.append({"type": N.LIGHT_SKY, "radiance": _spectrum_of(obj.radiance, table, spec_index)})
+        elif isinstance(obj, scenery.LightSun):
+            lights.append({"type": N.LIGHT_SUN, "radiance": _spectrum_of(obj.radiance, table, spec_index),
+                           "direction": obj.direction, "angle": obj.angle})
+        else:
+            raise NotImplementedError(f"scene object {type(obj).__name__} is not supported")
+
+    walk(scene, None)
+    V = np.concatenate(verts) if verts else np.zeros((0, 3), np.float32)
+    T = np.concatenate(tris) if tris else np.zeros((0, 3), np.uint32)
+    TM = np.concatenate(tri_mat) if tri_mat else np.zeros(0, np.uint32)
+    S = np.asarray(sph, np.float32).reshape(-1, 4)
+    PM = np.concatenate([TM, np.asarray(sph_mat, np.uint32)]).astype(np.uint32)
+    return CompiledScene(V, T, S, PM, materials, table, lights)
+
+
+def camera_struct(cam: cameras.PerspectiveCamera) -> N.Camera:
+    c = N.Camera()
+    c.position[:] = [float(x) for x in cam.position]
+    c.direction[:] = [float(x) for x in cam.direction]
+    c.sky[:] = [float(x) for x in cam.sky]
+    c.fov = float(cam.fovAngle)
+    c.aspect = float(cam.aspectRatio)
+    c.near_limit = float(cam.nearLimit)
+    return c
+
+
+class GpuScene:
+    """A compiled scene resident on one GPU (QBVH + lr_scene)."""
+
+    def __init__(self, ctx, cs: CompiledScene):
+        from .qbvh import Qbvh
+
+        ensure_standard_observer()
+        self.ctx = ctx
+        self.compiled = cs
+        self.qbvh = Qbvh(ctx, cs.verts, cs.tris, cs.spheres)
+        keep = []
+        specs = (N.Spectrum * max(1, len(cs.spectra)))(*[s._describe(keep) for s in cs.spectra])
+        mats = (N.Material * max(1, len(cs.materials)))(*[
+            N.Material(m["type"], m["reflectance"], m["transmittance"], m["eta"], m["kappa"], m["emission"],
+                       m["light"], m["roughness"]) for m in cs.materials])
+        ls = (N.Light * max(1, len(cs.lights)))()
+        for i, L in enumerate(cs.lights):
+            ls[i].type = L["type"]
+            ls[i].radiance = L["radiance"]
+            ls[i].double_sided = int(L.get("double_sided", False))
+            for name in ("origin", "dx", "dy", "direction"):
+                if name in L:
+                    getattr(ls[i], name)[:] = [float(x) for x in L[name]]
+            ls[i].angle = float(L.get("angle", 0.0))
+        pm = np.ascontiguousarray(cs.prim_material, np.uint32)
+        h = C.c_void_p()
+        N.check(N.lib().lr_scene_create(ctx.handle, self.qbvh.handle, N.ptr(pm), mats, len(cs.materials), specs,
+                                        len(cs.spectra), ls, len(cs.lights), C.byref(h)), "lr_scene_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            N.lib().lr_scene_destroy(self._h)
+            self._h = None
+        if getattr(self, "qbvh", None) is not None:
+            self.qbvh.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, camera: N.Camera, params: N.RenderParams, film_xyz, film_w, stream=None) -> dict:
+        import torch
+
+        s = torch.cuda.current_stream(film_xyz.device).cuda_stream if stream is None else stream
+        st = N.RenderStats()
+        N.check(N.lib().lr_render(self._h, C.byref(camera), C.byref(params), C.c_void_p(film_xyz.data_ptr()),
+                                  C.c_void_p(film_w.data_ptr()), C.byref(st), C.c_void_p(s)), "lr_render")
+        return {"paths": st.paths, "rays": st.rays, "chunks": st.chunks, "seconds": st.seconds}
+
+
+def render_params(width, height, spp, wavelengths=1, max_depth=0, rr_depth=8, rr_survival=0.5,
+                  lambda_range=(380.0, 780.0), seed=1, chunk_paths=0) -> N.RenderParams:
+    return N.RenderParams(int(width), int(height), int(spp), int(wavelengths), int(max_depth), int(rr_depth),
+                          float(rr_survival), float(lambda_range[0]), float(lambda_range[1]), int(seed),
+                          int(chunk_paths))
+
+
+class RenderEngine:
+    """``engine.render()`` renders ``scene`` through ``camera`` with ``tracer``
+    and ``sampler`` into ``target`` on GPU ``device``."""
+
+    def __init__(self):
+        self.camera = None
+        self.tracer = None
+        self.sampler = None
+        self.target = None
+        self.scene = None
+        self.numberOfThreads = 0  # accepted for API compatibility
+        self.seed = 1
+        self.device = 0
+        self.wavelengthsPerSample = 1
+        self.wavelengthRange = None  # (lo, hi) nm; None = the observer's full range
+        self.lastStats = None
+
+    def render(self, time=0.0, bucket=None):
+        import torch
+
+        from .observer import standard_observer
+        from .qbvh import Context
+
+        for name in ("camera", "tracer", "sampler", "target", "scene"):
+            if getattr(self, name) is None:
+                raise RuntimeError(f"RenderEngine.{name} is not set")
+        w, h = self.sampler.resolution
+        if tuple(self.target.resolution) != (w, h):
+            raise RuntimeError("sampler and target resolutions differ")
+        tr = self.tracer
+        rr_start = tr.maxRayGeneration if tr.russianRouletteStart is None else tr.russianRouletteStart
+        lr = self.wavelengthRange or (float(standard_observer()[0][0]), float(standard_observer()[0][-1]))
+        params = render_params(w, h, self.sampler.samplesPerPixel, self.wavelengthsPerSample,
+                               tr.maxDepth or 0, rr_start, tr.survivalProbability, lr, self.seed)
+        ctx = Context(self.device)
+        gs = GpuScene(ctx, compile_scene(self.scene))
+        dev = torch.device("cuda", self.device)
+        film = torch.zeros((3, h, w), dtype=torch.float32, device=dev)
+        wgt = torch.zeros((h, w), dtype=torch.float32, device=dev)
+        self.lastStats = gs.render(camera_struct(self.camera), params, film, wgt)
+        torch.cuda.synchronize(dev)
+        self.target.accumulate(film.cpu().numpy().astype(np.float64), wgt.cpu().numpy().astype(np.float64))
+        if hasattr(self.target, "save"):
+            self.target.save()
+        gs.close()
+        ctx.close()

@@ -1,0 +1,123 @@
+"""Scene objects (mirrors ``liar.scenery``: src/liar/scenery/).
+
+Geometry is flattened by the RenderEngine into one QBVH over triangles and
+spheres; aggregates (List, QbvhTree) only group children.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .textures import as_texture
+
+
+class SceneObject:
+    def __init__(self):
+        self.shader = None
+
+
+class TriangleMesh(SceneObject):
+    """``TriangleMesh(vertices, normals, uvs, triangles)``; each triangle is a
+    3-tuple of vertex indices or of ``(vertex[, normal[, uv]])`` index tuples
+    (src/liar/scenery/triangle_mesh.cpp:39-85).  Shading uses the geometric
+    normal."""
+
+    def __init__(self, vertices, normals=(), uvs=(), triangles=()):
+        super().__init__()
+        self._vertices = np.asarray(vertices, dtype=np.float32).reshape(-1, 3)
+        self._normals = list(normals)
+        self._uvs = list(uvs)
+        idx = []
+        for tri in triangles:
+            if len(tri) != 3:
+                raise ValueError("a triangle needs three corners")
+            idx.append([c[0] if isinstance(c, (tuple, list)) else c for c in tri])
+        self._triangles = np.asarray(idx, dtype=np.uint32).reshape(-1, 3)
+        if len(self._triangles) and self._triangles.max() >= len(self._vertices):
+            raise ValueError("triangle references a missing vertex")
+
+    def vertices(self):
+        return self._vertices
+
+    def triangles(self):
+        return self._triangles
+
+    def normals(self):
+        return self._normals
+
+    def uvs(self):
+        return self._uvs
+
+
+class Sphere(SceneObject):
+    def __init__(self, center=(0.0, 0.0, 0.0), radius=1.0):
+        super().__init__()
+        self.center = tuple(float(c) for c in center)
+        self.radius = float(radius)
+
+
+class Parallelogram(SceneObject):
+    """support + u * sizeU + v * sizeV, u, v in [0, 1]; normal = sizeU x sizeV
+    (src/liar/scenery/parallelogram.cpp)."""
+
+    def __init__(self, support, sizeU, sizeV):
+        super().__init__()
+        self.support = np.asarray(support, dtype=np.float32)
+        self.sizeU = np.asarray(sizeU, dtype=np.float32)
+        self.sizeV = np.asarray(sizeV, dtype=np.float32)
+
+    def _mesh(self):
+        o, a, b = self.support, self.sizeU, self.sizeV
+        v = np.stack([o, o + a, o + a + b, o + b]).astype(np.float32)
+        return v, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+class List(SceneObject):
+    def __init__(self, children=()):
+        super().__init__()
+        self.children = list(children)
+
+    def add(self, child):
+        self.children.append(child)
+
+
+class QbvhTree(List):
+    """Object-level QBVH (src/liar/scenery/object_trees.h); here the engine
+    builds one QBVH over all primitives, so this is a grouping node."""
+
+
+class AabbTree(List):
+    pass
+
+
+class LightArea(SceneObject):
+    """Area light over a surface (src/liar/scenery/light_area.cpp); the
+    surface must be a Parallelogram."""
+
+    def __init__(self, surface):
+        super().__init__()
+        if not isinstance(surface, Parallelogram):
+            raise TypeError("LightArea surfaces are limited to Parallelogram")
+        self.surface = surface
+        self.radiance = as_texture(1.0)
+        self.isDoubleSided = False
+        self.numberOfEmissionSamples = 1
+
+
+class LightSky(SceneObject):
+    """Constant environment light (src/liar/scenery/light_sky.cpp)."""
+
+    def __init__(self, radiance=None):
+        super().__init__()
+        self.radiance = as_texture(1.0 if radiance is None else radiance)
+
+
+class LightSun(SceneObject):
+    """Distant disc light: radiance within ``angle`` (apparent diameter, rad)
+    of ``direction`` (towards the sun).  Not a LiAR class; models the sun disc
+    of the terrain workload."""
+
+    def __init__(self, direction=(0.0, 0.0, 1.0), angle=np.radians(0.5), radiance=None):
+        super().__init__()
+        self.direction = tuple(float(x) for x in direction)
+        self.angle = float(angle)
+        self.radiance = as_texture(1.0 if radiance is None else radiance)

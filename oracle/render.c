@@ -1,0 +1,705 @@
+/*
+ * render.c -- CPU oracle of the spectral path-tracing integrand.  TEST
+ * INFRASTRUCTURE ONLY (see oracle.c).
+ *
+ * A scalar, recursive restatement in FP64 of LiAR's AdjointPhotonTracer
+ * (src/liar/tracers/adjoint_photon_tracer.cpp:84-292) over the flattened
+ * scene description of include/liar_amd.h, with:
+ *   - Observer wavelength sampling and the Single-mode XYZ estimator
+ *     (src/liar/kernel/observer.cpp:57-129,283-317, spectral.cpp:212-221),
+ *   - Lambert / Mirror / Dielectric / Conductor / CookTorrance(GGX, Beckmann)
+ *     BSDFs (src/liar/shaders/, as cited per function),
+ *   - LightArea (parallelogram), LightSky (constant) and a distant sun disc,
+ *     light choice proportional to power (src/liar/kernel/light_context.cpp:272-331),
+ *   - the Raster box filter, accumulated in FP64 (src/liar/output/raster.cpp:547-606).
+ * Intersections are brute force over all primitives in FP64 with LiAR's
+ * per-object interval rules.
+ *
+ * The random numbers are the same Philox4x32-10 streams, keyed the same way,
+ * as the GPU renderer; sampling warps are the published maps named in
+ * liar_amd/csrc/lr_render.h (Lass's own maps are parity unpinned).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define RMISS 0xFFFFFFFFu
+#define RNONE 0xFFFFFFFFu
+static const double PI = 3.14159265358979323846;
+static const double TOL = 1e-6;
+
+/* Test hook: 1 = drop both one-sample MIS weights (a planted mistake that
+ * the furnace tests must detect).  Never set outside tests. */
+static int g_plant = 0;
+void or_set_planted_mistake(int k) { g_plant = k; }
+
+/* ---- Philox4x32-10 (Salmon et al., SC'11) ---------------------------- */
+static void philox(uint32_t k0, uint32_t k1, const uint32_t ctr[4], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+void or_philox(uint32_t k0, uint32_t k1, const uint32_t* ctr, uint32_t* out) { philox(k0, k1, ctr, out); }
+static double u01(uint32_t x) { return (double)(x >> 8) * (1.0 / 16777216.0); }
+
+/* ---- scene description (layouts of include/liar_amd.h) ---------------- */
+typedef struct { uint32_t kind, count; const float *w, *v; float value, lo, hi, temperature, scale; } or_spectrum;
+typedef struct { uint32_t type, refl, trans, eta, kappa, emission, light; float roughness; } or_material;
+typedef struct { uint32_t type, radiance, double_sided; float origin[3], dx[3], dy[3], direction[3], angle; } or_light;
+typedef struct { float position[3], direction[3], sky[3], fov, aspect, near_limit; } or_camera;
+typedef struct {
+    uint32_t width, height, spp, wavelengths, max_depth, rr_depth;
+    float rr_survival, lambda_min, lambda_max;
+    uint64_t seed;
+    uint32_t chunk_paths;
+} or_params;
+
+typedef struct { double x, y, z; } V;
+static V v(double x, double y, double z) { V r = {x, y, z}; return r; }
+static V add(V a, V b) { return v(a.x + b.x, a.y + b.y, a.z + b.z); }
+static V sub(V a, V b) { return v(a.x - b.x, a.y - b.y, a.z - b.z); }
+static V mul(double s, V a) { return v(s * a.x, s * a.y, s * a.z); }
+static double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static V cross(V a, V b) { return v(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static V nrm(V a) { return mul(1.0 / sqrt(dot(a, a)), a); }
+static V f3(const float* p) { return v(p[0], p[1], p[2]); }
+
+typedef struct {
+    const float* verts; const uint32_t* tris; uint32_t nt;
+    const float* spheres; uint32_t ns;
+    const uint32_t* prim_mat;
+    const or_material* mats;
+    const or_spectrum* spectra; uint32_t n_spectra;
+    const or_light* lights; uint32_t n_lights;
+    /* derived */
+    double* light_sel; double* light_cdf; double* light_area; V* light_n; V* light_du; V* light_dv;
+    double* light_cos; double* light_invsolid;
+    double* spec_reflscale;
+    /* observer restricted to the domain */
+    int on; double *ow, *ocdf, *oxyz;
+    double R;
+} Scene;
+
+/* ---- spectra --------------------------------------------------------------- */
+static double planck(double w_m, double t) {
+    const double h = 6.626070040e-34, c = 299792458.0, kb = 1.3806488e-23;
+    return 2 * h * c * c / (pow(w_m, 5) * expm1(h * c / (kb * w_m * t)));
+}
+static double spec_eval(const Scene* S, uint32_t i, double w, int reflectant) {
+    if (i == RNONE) return 0.0;
+    const or_spectrum* s = &S->spectra[i];
+    double val = 0.0;
+    switch (s->kind) {
+    case 0: return s->value;
+    case 1: {
+        uint32_t k = 0;
+        while (k < s->count && s->w[k] <= w) ++k;  /* upper_bound */
+        if (k == 0 || k == s->count) return 0.0;
+        double t = (w - s->w[k - 1]) / ((double)s->w[k] - s->w[k - 1]);
+        val = s->v[k - 1] + t * ((double)s->v[k] - s->v[k - 1]);
+        break;
+    }
+    case 2: {
+        if (!(w >= s->lo && w < s->hi)) return 0.0;
+        uint32_t k = (uint32_t)((w - s->lo) / ((double)s->hi - s->lo) * s->count);
+        if (k >= s->count) k = s->count - 1;
+        val = s->v[k];
+        break;
+    }
+    case 3: {
+        double peak = 2.897771955e-3 / s->temperature;
+        return s->scale * planck(w * 1e-9, s->temperature) / planck(peak, s->temperature);
+    }
+    }
+    return reflectant ? val * S->spec_reflscale[i] : val;
+}
+
+/* Observer restricted to [lo, hi]: CDF from the trapezoid of x+y+z, XYZ
+ * normalised so that a unit spectrum has Y = 1 (observer.cpp:57-129). */
+static int build_observer(Scene* S, const float* w, const float* xyz, int n, double lo, double hi) {
+    double a = lo > w[0] ? lo : w[0], b = hi < w[n - 1] ? hi : w[n - 1];
+    if (!(b > a)) return -1;
+    int m = 0;
+    S->ow = (double*)malloc(sizeof(double) * (n + 2));
+    S->ocdf = (double*)malloc(sizeof(double) * (n + 2));
+    S->oxyz = (double*)malloc(sizeof(double) * 3 * (n + 2));
+    S->ow[m++] = a;
+    for (int k = 0; k < n; ++k)
+        if (w[k] > a && w[k] < b) S->ow[m++] = w[k];
+    S->ow[m++] = b;
+    for (int j = 0; j < m; ++j) {
+        double wl = S->ow[j];
+        int k = 0;
+        while (k < n && w[k] <= wl) ++k;
+        for (int c = 0; c < 3; ++c) {
+            double val;
+            if (k == 0) val = 0;
+            else if (k == n) val = (wl == w[n - 1]) ? xyz[3 * (n - 1) + c] : 0;
+            else {
+                double t = (wl - w[k - 1]) / ((double)w[k] - w[k - 1]);
+                val = xyz[3 * (k - 1) + c] + t * ((double)xyz[3 * k + c] - xyz[3 * (k - 1) + c]);
+            }
+            S->oxyz[3 * j + c] = val;
+        }
+    }
+    S->ocdf[0] = 0;
+    for (int j = 0; j + 1 < m; ++j) {
+        double s0 = S->oxyz[3 * j] + S->oxyz[3 * j + 1] + S->oxyz[3 * j + 2];
+        double s1 = S->oxyz[3 * j + 3] + S->oxyz[3 * j + 4] + S->oxyz[3 * j + 5];
+        S->ocdf[j + 1] = S->ocdf[j] + (s0 + s1) * (S->ow[j + 1] - S->ow[j]);
+    }
+    double Y = 0;
+    for (int j = 0; j < m; ++j) {
+        double wp = S->ow[j > 0 ? j - 1 : 0], wn = S->ow[j + 1 < m ? j + 1 : m - 1];
+        Y += S->oxyz[3 * j + 1] * (wn - wp) / 2;
+    }
+    for (int j = 0; j < m; ++j) {
+        S->ocdf[j] /= S->ocdf[m - 1];
+        for (int c = 0; c < 3; ++c) S->oxyz[3 * j + c] /= Y;
+    }
+    S->ocdf[m - 1] = 1.0;
+    S->on = m;
+    return 0;
+}
+
+static double obs_sample(const Scene* S, double u, double* pdf) {
+    int k = 0;
+    while (k < S->on && S->ocdf[k] <= u) ++k;
+    if (k == 0 || k >= S->on) { *pdf = 0; return S->ow[0]; }
+    double c1 = S->ocdf[k - 1], c2 = S->ocdf[k], dc = c2 - c1;
+    double x = (u - c1) / dc;
+    *pdf = dc / (S->ow[k] - S->ow[k - 1]);
+    return S->ow[k - 1] + x * (S->ow[k] - S->ow[k - 1]);
+}
+
+static void obs_xyz(const Scene* S, double w, double out[3]) {
+    int k = 0;
+    while (k < S->on && S->ow[k] <= w) ++k;
+    if (k == 0 || k == S->on) {
+        int last = (k == S->on && w == S->ow[S->on - 1]);
+        for (int c = 0; c < 3; ++c) out[c] = last ? S->oxyz[3 * (S->on - 1) + c] : 0.0;
+        return;
+    }
+    double t = (w - S->ow[k - 1]) / (S->ow[k] - S->ow[k - 1]);
+    for (int c = 0; c < 3; ++c)
+        out[c] = S->oxyz[3 * (k - 1) + c] + t * (S->oxyz[3 * k + c] - S->oxyz[3 * (k - 1) + c]);
+}
+
+/* ---- intersection (FP64, LiAR per-object rules) ---------------------------- */
+static int tri_hit(const Scene* S, uint32_t k, V o, V d, double* t) {
+    const float* a = S->verts + 3 * (size_t)S->tris[3 * k];
+    const float* b = S->verts + 3 * (size_t)S->tris[3 * k + 1];
+    const float* c = S->verts + 3 * (size_t)S->tris[3 * k + 2];
+    V v0 = f3(a), e1 = sub(f3(b), v0), e2 = sub(f3(c), v0);
+    V p = cross(d, e2);
+    double det = dot(e1, p);
+    if (det == 0) return 0;
+    V s = sub(o, v0);
+    double u = dot(s, p) / det;
+    if (!(u >= 0 && u <= 1)) return 0;
+    V q = cross(s, e1);
+    double vv = dot(d, q) / det;
+    if (!(vv >= 0 && u + vv <= 1)) return 0;
+    *t = dot(e2, q) / det;
+    return 1;
+}
+static int sph_hit(const Scene* S, uint32_t j, V o, V d, double tmin, double* t) {
+    const float* s = S->spheres + 4 * (size_t)j;
+    V oc = sub(o, f3(s));
+    double b = dot(oc, d), c = dot(oc, oc) - (double)s[3] * s[3];
+    double disc = b * b - c;
+    if (disc < 0) return 0;
+    double sq = sqrt(disc), t0 = -b - sq, t1 = -b + sq;
+    if (t0 > tmin) { *t = t0; return 1; }
+    if (t1 > tmin) { *t = t1; return 1; }
+    return 0;
+}
+static int in_range(double t, double tn, double tf) { return t > tn * (1 + TOL) && t < tf * (1 - TOL); }
+
+/* List{TriangleMesh, Spheres}: returns prim id or RMISS. */
+static uint32_t intersect(const Scene* S, V o, V d, double tn, double* th) {
+    double bt = INFINITY;
+    uint32_t bid = RMISS;
+    for (uint32_t k = 0; k < S->nt; ++k) {
+        double t;
+        if (tri_hit(S, k, o, d, &t) && t > tn && (t < bt || (t == bt && k < bid))) { bt = t; bid = k; }
+    }
+    uint32_t rid = RMISS;
+    double rt = INFINITY;
+    if (bid != RMISS && in_range(bt, tn, INFINITY)) { rid = bid; rt = bt; }
+    for (uint32_t j = 0; j < S->ns; ++j) {
+        double t;
+        if (sph_hit(S, j, o, d, tn, &t) && in_range(t, tn, INFINITY)) {
+            uint32_t id = S->nt + j;
+            if (t < rt || (t == rt && id < rid)) { rid = id; rt = t; }
+        }
+    }
+    *th = rt;
+    return rid;
+}
+
+/* ---- lights ------------------------------------------------------------------ */
+static double para_hit(const Scene* S, uint32_t k, V p, V d, double tn) {
+    V n = S->light_n[k];
+    double dn = dot(d, n);
+    if (dn == 0) return -1;
+    double t = dot(sub(f3(S->lights[k].origin), p), n) / dn;
+    if (!(t > tn)) return -1;
+    V q = sub(add(p, mul(t, d)), f3(S->lights[k].origin));
+    double u = dot(q, S->light_du[k]), vv = dot(q, S->light_dv[k]);
+    if (!(u >= 0 && u <= 1 && vv >= 0 && vv <= 1)) return -1;
+    return t;
+}
+static double light_pdf(const Scene* S, uint32_t k, V p, V d) {
+    const or_light* L = &S->lights[k];
+    if (L->type == 0) {
+        double t = para_hit(S, k, p, d, TOL);
+        if (t < 0) return 0;
+        double c = dot(d, S->light_n[k]);
+        if (!L->double_sided && c > 0) return 0;
+        return t * t / (S->light_area[k] * fabs(c));
+    }
+    if (L->type == 1) return 1.0 / (4 * PI);
+    return dot(d, S->light_n[k]) >= S->light_cos[k] ? S->light_invsolid[k] : 0.0;
+}
+static int light_sees(const Scene* S, uint32_t k, V d) {
+    const or_light* L = &S->lights[k];
+    if (L->type == 0) return L->double_sided || dot(d, S->light_n[k]) < 0;
+    if (L->type == 1) return 1;
+    return dot(d, S->light_n[k]) >= S->light_cos[k];
+}
+static void onb(V n, V* t, V* b) {  /* Duff et al., JCGT 2017 */
+    double sign = copysign(1.0, n.z), a = -1.0 / (sign + n.z), bb = n.x * n.y * a;
+    *t = v(1 + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+    *b = v(bb, sign + n.y * n.y * a, -n.y);
+}
+static V uniform_sphere(double u1, double u2) {
+    double z = 1 - 2 * u1, r = sqrt(fmax(0, 1 - z * z)), phi = 2 * PI * u2;
+    return v(r * cos(phi), r * sin(phi), z);
+}
+static V light_sample(const Scene* S, uint32_t k, V p, double u1, double u2, double* pdf) {
+    const or_light* L = &S->lights[k];
+    if (L->type == 0) {
+        V q = add(f3(L->origin), add(mul(u1, f3(L->dx)), mul(u2, f3(L->dy))));
+        V to = sub(q, p);
+        double d2 = dot(to, to);
+        to = mul(1 / sqrt(d2), to);
+        double c = dot(S->light_n[k], to);
+        *pdf = d2 / (S->light_area[k] * fabs(c));
+        if (!L->double_sided && c > 0) *pdf = 0;
+        if (!(*pdf < INFINITY)) *pdf = 0;
+        return to;
+    }
+    if (L->type == 1) { *pdf = 1 / (4 * PI); return uniform_sphere(u1, u2); }
+    double ct = 1 - u1 * (1 - S->light_cos[k]), st = sqrt(fmax(0, 1 - ct * ct)), phi = 2 * PI * u2;
+    V t, b, n = S->light_n[k];
+    onb(n, &t, &b);
+    *pdf = S->light_invsolid[k];
+    return add(add(mul(st * cos(phi), t), mul(st * sin(phi), b)), mul(ct, n));
+}
+static uint32_t light_pick(const Scene* S, double u) {
+    uint32_t k = 0;
+    while (k < S->n_lights && S->light_cdf[k] < u) ++k;  /* lower_bound */
+    return k < S->n_lights ? k : S->n_lights - 1;
+}
+
+/* ---- BSDFs (local frame, z = normal towards the viewer) -------------------- */
+typedef struct { uint32_t type; double refl, trans, eta, kappa, ior, alpha; } Bsdf;
+
+static double fresnel_cond(double eta, double kappa, double ci) {  /* conductor.cpp:184-194 */
+    double c2 = ci * ci, a = eta * eta + kappa * kappa, b = 2 * ci * eta;
+    double ao = a + c2, ap = a * c2 + 1;
+    return ((ao - b) / (ao + b) + (ap - b) / (ap + b)) / 2;
+}
+static int bsdf_eval(const Bsdf* B, V wi, V wo, double* val, double* pdf) {
+    if (wo.z == 0 || !(wo.z > 0)) return 0;
+    if (B->type == 1) { *val = B->refl / PI; *pdf = fabs(wo.z) / PI; return *pdf > 0; }
+    if (B->type == 5 || B->type == 6) {  /* cook_torrance.cpp:262-280 */
+        V h = nrm(add(wi, wo));
+        double a = B->alpha, d;
+        if (B->type == 5) {
+            double q = (h.x / a) * (h.x / a) + (h.y / a) * (h.y / a) + h.z * h.z;
+            d = 1 / (PI * a * a * q * q);
+        } else {
+            double c2 = h.z * h.z;
+            if (c2 == 0) return 0;
+            d = exp(-(h.x * h.x + h.y * h.y) / (a * a * c2)) / (PI * a * a * c2 * c2);
+        }
+        double pdfh = d * h.z, doh = fabs(dot(wi, h));
+        double g = fmin(1, fmin(2 * h.z * wi.z / doh, 2 * h.z * wo.z / doh));
+        double woh = dot(wi, h);
+        *pdf = pdfh / (4 * woh);
+        *val = B->refl * d * g / (4 * wi.z * wo.z) * fresnel_cond(B->eta, B->kappa, fabs(woh));
+        return *pdf > 0;
+    }
+    return 0;
+}
+static int bsdf_sample(const Bsdf* B, V wi, double u1, double u2, double uc, V* wo, double* val, double* pdf) {
+    switch (B->type) {
+    case 1: {  /* concentric disk + Malley */
+        double a = 2 * u1 - 1, b = 2 * u2 - 1, r, phi;
+        if (a == 0 && b == 0) { r = 0; phi = 0; }
+        else if (a * a > b * b) { r = a; phi = (PI / 4) * (b / a); }
+        else { r = b; phi = PI / 2 - (PI / 4) * (a / b); }
+        double x = r * cos(phi), y = r * sin(phi), z = sqrt(fmax(0, 1 - x * x - y * y));
+        *wo = v(x, y, z);
+        *pdf = z / PI;
+        *val = B->refl / PI;
+        return *pdf > 0;
+    }
+    case 2:
+        *wo = v(-wi.x, -wi.y, wi.z);
+        *val = B->refl;
+        *pdf = wo->z;
+        return *pdf > 0;
+    case 4:
+        *wo = v(-wi.x, -wi.y, wi.z);
+        *val = B->refl * fresnel_cond(B->eta, B->kappa, wi.z) / wi.z;
+        *pdf = 1;
+        return 1;
+    case 3: {  /* dielectric.cpp:209-247 */
+        double ci = wi.z, ior = B->ior;
+        if (ci <= 0) return 0;
+        double st2 = ior * ior * (1 - ci * ci), ct = sqrt(fmax(1 - st2, 0));
+        double ro = (ior * ci - ct) / (ior * ci + ct), rp = (ci - ior * ct) / (ci + ior * ct);
+        double fr = (ro * ro + rp * rp) / 2;
+        double pr = fabs(B->refl) * fr, pt = fabs(B->trans) * (1 - fr);
+        if (pr + pt == 0) return 0;
+        double prob = pr / (pr + pt);
+        if (uc < prob) { *wo = v(-wi.x, -wi.y, wi.z); *val = B->refl * fr / ci; *pdf = prob; }
+        else { *wo = v(-ior * wi.x, -ior * wi.y, -ct); *val = B->trans * (1 - fr) / ct; *pdf = 1 - prob; }
+        return *pdf > 0;
+    }
+    case 5: {  /* GGX sampleH, microfacet_trowbridge_reitz.cpp:62-72 */
+        double phi = 2 * PI * u1, s = sqrt(u2 / (1 - u2));
+        V h = nrm(v(s * B->alpha * cos(phi), s * B->alpha * sin(phi), 1));
+        *wo = sub(mul(2 * dot(wi, h), h), wi);
+        if (!(wo->z > 0)) return 0;
+        return bsdf_eval(B, wi, *wo, val, pdf);
+    }
+    case 6: {  /* Beckmann sampleH, microfacet_beckmann.cpp:76-106 */
+        double s = log(1 - u1), phi = 2 * PI * u2, t2 = -s * B->alpha * B->alpha;
+        double c2 = 1 / (1 + t2), ct = sqrt(c2), st = sqrt(fmax(1 - c2, 0));
+        V h = v(cos(phi) * st, sin(phi) * st, ct);
+        *wo = sub(mul(2 * dot(wi, h), h), wi);
+        if (!(wo->z > 0)) return 0;
+        return bsdf_eval(B, wi, *wo, val, pdf);
+    }
+    }
+    return 0;
+}
+
+/* ---- one path -------------------------------------------------------------- */
+typedef struct { const Scene* S; const or_params* P; uint32_t k0, k1; } Ctx;
+
+static void rng(const Ctx* c, uint64_t path, uint32_t gen, uint32_t purpose, uint32_t out[4]) {
+    uint32_t ctr[4] = {(uint32_t)path, (uint32_t)(path >> 32), gen, purpose};
+    philox(c->k0, c->k1, ctr, out);
+}
+
+/* Radiance L(lambda) of path `p` (tracePhoton, iteratively: the estimator
+ * is a product of per-vertex weights plus per-vertex emission). */
+static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double lam, uint32_t max_gen,
+                         double thr) {
+    const Scene* S = c->S;
+    const or_params* P = c->P;
+    double L = 0;
+    for (uint32_t gen = 0;; ++gen) {
+        double t;
+        uint32_t prim = intersect(S, o, d, tn, &t);
+        if (prim == RMISS) {
+            if (gen > 0)
+                for (uint32_t k = 0; k < S->n_lights; ++k)
+                    if (S->lights[k].type != 0 && light_sees(S, k, d))
+                        L += thr * spec_eval(S, S->lights[k].radiance, lam, 0);
+            return L;
+        }
+        V x = add(o, mul(t, d));
+        V ng;
+        int leaving;
+        if (prim < S->nt) {
+            const float* a = S->verts + 3 * (size_t)S->tris[3 * prim];
+            const float* b = S->verts + 3 * (size_t)S->tris[3 * prim + 1];
+            const float* cc = S->verts + 3 * (size_t)S->tris[3 * prim + 2];
+            ng = nrm(cross(sub(f3(b), f3(a)), sub(f3(cc), f3(a))));
+            leaving = dot(d, ng) > 0;
+        } else {
+            const float* s = S->spheres + 4 * (size_t)(prim - S->nt);
+            ng = mul(1.0 / s[3], sub(x, f3(s)));
+            V s0 = sub(add(o, mul(tn, d)), f3(s));
+            leaving = dot(s0, s0) < (double)s[3] * s[3];
+        }
+        V nf = dot(ng, d) < 0 ? ng : mul(-1, ng);
+        const or_material* m = &S->mats[S->prim_mat[prim]];
+        if (gen > 0 && m->light != RNONE && light_sees(S, m->light, d))
+            L += thr * spec_eval(S, S->lights[m->light].radiance, lam, 0);
+        if ((gen == 0 || m->light == RNONE) && m->emission != RNONE) L += thr * spec_eval(S, m->emission, lam, 0);
+        if (m->type == 0) return L;
+        Bsdf B;
+        B.type = m->type;
+        B.refl = spec_eval(S, m->refl, lam, 1);
+        B.trans = spec_eval(S, m->trans, lam, 1);
+        B.eta = fmax(spec_eval(S, m->eta, lam, 0), 1e-9);
+        B.kappa = fmax(spec_eval(S, m->kappa, lam, 0), 1e-9);
+        B.ior = leaving ? B.eta / B.kappa : B.kappa / B.eta;
+        double r = fmax(m->roughness, 1e-3);
+        B.alpha = r * r;
+        V tt, bb;
+        onb(nf, &tt, &bb);
+        V wiw = mul(-1, d);
+        V wi = v(dot(wiw, tt), dot(wiw, bb), fmax(dot(wiw, nf), 1.1920929e-7));
+        uint32_t r0[4], r1[4];
+        rng(c, p, gen, 1u, r0);
+        rng(c, p, gen, 2u, r1);
+        double us = u01(r0[0]), u1 = u01(r0[1]), u2 = u01(r0[2]), uc = u01(r0[3]);
+        double ul = u01(r1[0]), ul1 = u01(r1[1]), ul2 = u01(r1[2]), urr = u01(r1[3]);
+        double strategy = (m->type == 1 || m->type == 5 || m->type == 6) ? 0.5 : 1.0;
+        V wo = v(0, 0, 1);
+        double val = 0, pdf = 0;
+        int ok = 0;
+        if (us < strategy) {
+            ok = bsdf_sample(&B, wi, u1, u2, uc, &wo, &val, &pdf);
+            if (ok) {
+                pdf *= strategy;
+                double pl = 0;
+                if (strategy < 1) {
+                    V dw = add(add(mul(wo.x, tt), mul(wo.y, bb)), mul(wo.z, nf));
+                    for (uint32_t k = 0; k < S->n_lights; ++k) pl += S->light_sel[k] * light_pdf(S, k, x, dw);
+                    pl *= 1 - strategy;
+                }
+                if (g_plant != 1) val *= pdf * pdf / (pdf * pdf + pl * pl);
+            }
+        } else if (S->n_lights > 0) {
+            uint32_t k = light_pick(S, ul);
+            double pdfl;
+            V dw = light_sample(S, k, x, ul1, ul2, &pdfl);
+            if (S->light_sel[k] > 0 && pdfl > 0) {
+                double pl = S->light_sel[k] * pdfl;
+                for (uint32_t j = 0; j < S->n_lights; ++j)
+                    if (j != k) pl += S->light_sel[j] * light_pdf(S, j, x, dw);
+                pl *= 1 - strategy;
+                wo = v(dot(dw, tt), dot(dw, bb), dot(dw, nf));
+                double vb, pb;
+                if (bsdf_eval(&B, wi, wo, &vb, &pb) && vb != 0) {
+                    double pbs = pb * strategy;
+                    val = g_plant == 1 ? vb : vb * pl * pl / (pl * pl + pbs * pbs);
+                    pdf = pl;
+                    ok = 1;
+                }
+            }
+        }
+        if (!(ok && pdf > 0)) return L;
+        double ct = wo.z;
+        thr *= val * fabs(ct) / pdf;
+        V start = add(x, mul((ct > 0 ? 10.0 : -10.0) * TOL, nf));
+        d = nrm(add(add(mul(wo.x, tt), mul(wo.y, bb)), mul(wo.z, nf)));
+        o = start;
+        tn = TOL;
+        uint32_t next = gen + 1;
+        if (next >= max_gen) return L;
+        if (next >= P->rr_depth) {
+            if (urr < P->rr_survival) thr /= P->rr_survival;
+            else return L;
+        }
+        if (!(thr > 0) || !isfinite(thr)) return L;
+    }
+}
+
+typedef struct {
+    const Ctx* c;
+    V pos, dir, right, down, dirn;
+    double near_limit;
+    uint32_t strata;
+    double* xyz;     /* 3 planes, W*H each, sums */
+    double* weight;
+    uint32_t row0, row1;
+} Job;
+
+static void* run_rows(void* arg) {
+    Job* J = (Job*)arg;
+    const Ctx* c = J->c;
+    const or_params* P = c->P;
+    uint32_t max_gen = P->max_depth ? P->max_depth : 64u;
+    uint64_t np = (uint64_t)P->width * P->height;
+    for (uint32_t py = J->row0; py < J->row1; ++py)
+        for (uint32_t px = 0; px < P->width; ++px) {
+            uint64_t pixel = (uint64_t)py * P->width + px;
+            double X = 0, Y = 0, Z = 0;
+            for (uint32_t s = 0; s < P->spp; ++s) {
+                uint64_t sidx = pixel * P->spp + s;
+                uint32_t rs[4];
+                rng(c, sidx, 0xFFFFu, 0u, rs);
+                double sx = u01(rs[0]), sy = u01(rs[1]);
+                if (J->strata) {
+                    sx = ((double)(s % J->strata) + sx) / J->strata;
+                    sy = ((double)(s / J->strata) + sy) / J->strata;
+                }
+                double Xs = (px + sx) / P->width, Ys = (py + sy) / P->height;
+                V d = nrm(add(J->dir, add(mul(Xs - 0.5, J->right), mul(Ys - 0.5, J->down))));
+                double rf = 1.0 / dot(d, J->dirn);
+                double tn = fmax(TOL, rf * J->near_limit);
+                for (uint32_t l = 0; l < P->wavelengths; ++l) {
+                    uint64_t p = sidx * P->wavelengths + l;
+                    uint32_t rp[4];
+                    rng(c, p, 0xFFFEu, 0u, rp);
+                    double uw = (l + u01(rp[0])) / P->wavelengths, lpdf;
+                    double lam = obs_sample(c->S, uw, &lpdf);
+                    if (!(lpdf > 0) || max_gen == 0) continue;
+                    double thr = 1;
+                    if (P->rr_depth == 0) {
+                        if (u01(rp[1]) < P->rr_survival) thr /= P->rr_survival;
+                        else continue;
+                    }
+                    double L = trace_path(c, p, J->pos, d, tn, lam, max_gen, thr);
+                    if (L == 0) continue;
+                    double cmf[3];
+                    obs_xyz(c->S, lam, cmf);
+                    X += L / lpdf * cmf[0];
+                    Y += L / lpdf * cmf[1];
+                    Z += L / lpdf * cmf[2];
+                }
+            }
+            J->xyz[pixel] += X;
+            J->xyz[np + pixel] += Y;
+            J->xyz[2 * np + pixel] += Z;
+            J->weight[pixel] += (double)P->spp * P->wavelengths;
+        }
+    return NULL;
+}
+
+static double lum(const Scene* S, uint32_t i, const float* w, const float* xyz, int n) {
+    double num = 0, den = 0;
+    for (int k = 0; k < n; ++k) {
+        double wp = w[k > 0 ? k - 1 : 0], wn = w[k + 1 < n ? k + 1 : n - 1];
+        num += spec_eval(S, i, w[k], 0) * xyz[3 * k + 1] * (wn - wp) / 2;
+        den += xyz[3 * k + 1] * (wn - wp) / 2;
+    }
+    return den > 0 ? num / den : 0;
+}
+
+/* Renders into xyz (3*W*H doubles, += sums) and weight (W*H, += counts). */
+int or_render(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres, uint32_t ns,
+              const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
+              uint32_t n_spectra, const or_light* lights, uint32_t n_lights, const float* obs_w,
+              const float* obs_xyz, uint32_t obs_n, const or_camera* cam, const or_params* P, double* xyz,
+              double* weight, int nthreads) {
+    Scene S;
+    memset(&S, 0, sizeof S);
+    S.verts = verts; S.tris = tris; S.nt = nt; S.spheres = spheres; S.ns = ns; S.prim_mat = prim_mat;
+    S.mats = mats; S.spectra = spectra; S.n_spectra = n_spectra; S.lights = lights; S.n_lights = n_lights;
+    if (build_observer(&S, obs_w, obs_xyz, (int)obs_n, P->lambda_min, P->lambda_max)) return -1;
+    S.spec_reflscale = (double*)malloc(sizeof(double) * (n_spectra + 1));
+    for (uint32_t i = 0; i < n_spectra; ++i) {
+        double mx = -INFINITY;
+        if (spectra[i].kind == 1 || spectra[i].kind == 2)
+            for (uint32_t k = 0; k < spectra[i].count; ++k) mx = fmax(mx, spectra[i].v[k]);
+        S.spec_reflscale[i] = mx > 1 ? 1 / mx : 1;
+    }
+    /* scene bounding sphere radius (half the AABB diagonal) */
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = 0; k < 3 * nt; ++k) {
+        const float* p = verts + 3 * (size_t)tris[k];
+        for (int a = 0; a < 3; ++a) { lo[a] = fmin(lo[a], p[a]); hi[a] = fmax(hi[a], p[a]); }
+    }
+    for (uint32_t j = 0; j < ns; ++j)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmin(lo[a], spheres[4 * j + a] - spheres[4 * j + 3]);
+            hi[a] = fmax(hi[a], spheres[4 * j + a] + spheres[4 * j + 3]);
+        }
+    double R = 0;
+    if (nt + ns)
+        for (int a = 0; a < 3; ++a) R += 0.25 * (hi[a] - lo[a]) * (hi[a] - lo[a]);
+    R = sqrt(R);
+    double far_away = R > 0 ? 1e3 * R : 1e3;
+    uint32_t nl = n_lights ? n_lights : 1;
+    S.light_sel = calloc(nl, sizeof(double)); S.light_cdf = calloc(nl, sizeof(double));
+    S.light_area = calloc(nl, sizeof(double)); S.light_n = calloc(nl, sizeof(V));
+    S.light_du = calloc(nl, sizeof(V)); S.light_dv = calloc(nl, sizeof(V));
+    S.light_cos = calloc(nl, sizeof(double)); S.light_invsolid = calloc(nl, sizeof(double));
+    double total = 0;
+    double* power = calloc(nl, sizeof(double));
+    for (uint32_t k = 0; k < n_lights; ++k) {
+        const or_light* L = &lights[k];
+        double Y = lum(&S, L->radiance, obs_w, obs_xyz, (int)obs_n);
+        if (L->type == 0) {
+            V a = f3(L->dx), b = f3(L->dy), cr = cross(a, b);
+            double area = sqrt(dot(cr, cr));
+            V n = mul(1 / area, cr);
+            S.light_area[k] = area;
+            S.light_n[k] = n;
+            V u = cross(b, n), w2 = cross(n, a);
+            S.light_du[k] = mul(1 / dot(a, u), u);
+            S.light_dv[k] = mul(1 / dot(b, w2), w2);
+            power[k] = PI * area * Y * (L->double_sided ? 2 : 1);
+        } else if (L->type == 1) {
+            power[k] = Y * PI * far_away * far_away * 4 * PI;
+        } else {
+            S.light_n[k] = nrm(f3(L->direction));
+            S.light_cos[k] = cos(0.5 * L->angle);
+            double solid = 2 * PI * (1 - cos(0.5 * L->angle));
+            S.light_invsolid[k] = 1 / solid;
+            power[k] = Y * solid * PI * far_away * far_away;
+        }
+        total += power[k];
+    }
+    double acc = 0;
+    for (uint32_t k = 0; k < n_lights; ++k) {
+        acc += power[k];
+        S.light_cdf[k] = total > 0 ? acc / total : (double)(k + 1) / n_lights;
+        S.light_sel[k] = total > 0 ? power[k] / total : 1.0 / n_lights;
+    }
+    if (n_lights) S.light_cdf[n_lights - 1] = 1;
+    free(power);
+
+    Ctx c = {&S, P, (uint32_t)P->seed, (uint32_t)(P->seed >> 32)};
+    Job proto;
+    memset(&proto, 0, sizeof proto);
+    proto.c = &c;
+    {
+        V d = nrm(f3(cam->direction)), sk = f3(cam->sky);
+        V r = nrm(cross(d, sk)), dn = nrm(cross(d, r));
+        double width = 0.036, height = width / cam->aspect, focal = (width / 2) / tan(cam->fov / 2);
+        proto.pos = f3(cam->position);
+        proto.dir = mul(focal, d);
+        proto.right = mul(width, r);
+        proto.down = mul(height, dn);
+        proto.dirn = d;
+        proto.near_limit = cam->near_limit;
+    }
+    {
+        uint32_t q = (uint32_t)lround(sqrt((double)P->spp));
+        proto.strata = q * q == P->spp ? q : 0;
+    }
+    proto.xyz = xyz;
+    proto.weight = weight;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    Job jobs[256];
+    /* interleave rows across threads in small blocks for balance */
+    uint32_t H = P->height;
+    uint32_t per = (H + (uint32_t)nthreads - 1) / (uint32_t)nthreads;
+    int used = 0;
+    for (int k = 0; k < nthreads; ++k) {
+        jobs[k] = proto;
+        jobs[k].row0 = (uint32_t)k * per < H ? (uint32_t)k * per : H;
+        jobs[k].row1 = jobs[k].row0 + per < H ? jobs[k].row0 + per : H;
+        if (jobs[k].row0 >= jobs[k].row1) break;
+        pthread_create(&th[k], NULL, run_rows, &jobs[k]);
+        ++used;
+    }
+    for (int k = 0; k < used; ++k) pthread_join(th[k], NULL);
+    free(S.ow); free(S.ocdf); free(S.oxyz); free(S.spec_reflscale);
+    free(S.light_sel); free(S.light_cdf); free(S.light_area); free(S.light_n); free(S.light_du);
+    free(S.light_dv); free(S.light_cos); free(S.light_invsolid);
+    return 0;
+}

@@ -1,0 +1,166 @@
+"""GPU parity of the spectral wavefront path tracer.
+
+1. Same seed, same sample streams: the GPU render (FP32) and the CPU oracle
+   render (FP64) follow the same paths, so their images agree far below the
+   Monte Carlo noise; only paths whose FP32/FP64 decisions differ diverge.
+2. White furnace (test/white_furnace/ashikhmin_shirley.py geometry): a
+   convex unit sphere under a constant sky of radiance 1 gives Y = rho for
+   Lambert and Mirror, and Y = 1 for a lossless dielectric.
+3. Determinism: two renders with the same seed are bit-identical.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import liar_amd as la
+import oracle
+from liar_amd import engine as E
+from liar_amd import scenery, shaders, spectra, textures
+from liar_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_render(ctx, cs, cam, w, h, spp, nl=1, max_depth=0, rr_depth=8, seed=1, lam=(380.0, 780.0)):
+    import torch
+
+    gs = E.GpuScene(ctx, cs)
+    film = torch.zeros((3, h, w), dtype=torch.float32, device="cuda")
+    wgt = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    stats = gs.render(E.camera_struct(cam), E.render_params(w, h, spp, nl, max_depth, rr_depth, 0.5, lam, seed),
+                      film, wgt)
+    torch.cuda.synchronize()
+    out = film.cpu().numpy().astype(np.float64), wgt.cpu().numpy().astype(np.float64), stats
+    gs.close()
+    return out
+
+
+def compare(gx, gw, ox, ow, label):
+    assert np.array_equal(gw, ow), "sample counts differ"
+    g = gx / gw
+    o = ox / ow
+    tot_g, tot_o = g[1].sum(), o[1].sum()
+    rel_total = abs(tot_g - tot_o) / tot_o
+    scale = max(o[1].max(), 1e-12)
+    diff = np.abs(g - o) / scale
+    frac_bad = float((diff.max(0) > 1e-4).mean())
+    say(f"[gpu] {label}: total Y gpu {tot_g:.6f} oracle {tot_o:.6f} rel {rel_total:.2e}; "
+        f"max |diff|/max {diff.max():.2e}; pixels > 1e-4: {frac_bad:.4f}")
+    return rel_total, frac_bad, diff
+
+
+def test_cornell_same_seed_matches_oracle(gpu_ctx):
+    say("[gpu] cornell 64x64 16spp x 4 lambda vs oracle")
+    sc, cam = W.cornell_box()
+    cs = la.compile_scene(sc)
+    gx, gw, st = gpu_render(gpu_ctx, cs, cam, 64, 64, 16, nl=4, max_depth=8, rr_depth=3, seed=1)
+    ox, ow = oracle.render(cs, cam, 64, 64, 16, wavelengths=4, max_depth=8, rr_depth=3, seed=1)
+    rel_total, frac_bad, _ = compare(gx, gw, ox, ow, "cornell")
+    assert st["paths"] == 64 * 64 * 16 * 4 and st["rays"] >= st["paths"]
+    assert rel_total < 2e-3
+    assert frac_bad < 0.02
+
+
+def _mixed_scene():
+    au_w = np.array([281.7, 359.3, 413.2, 476.8, 516.6, 563.5, 619.9, 826.5])
+    au_n = np.array([1.848, 1.967, 1.950, 1.541, 0.878, 0.613, 0.487, 0.418])
+    au_k = np.array([1.87, 1.84, 1.86, 1.71, 2.01, 2.64, 3.31, 5.13])
+    floor = scenery.Parallelogram((-4, -4, 0), (8, 0, 0), (0, 8, 0))
+    floor.shader = shaders.Lambert(textures.Constant(spectra.Binned(380, 780, np.linspace(0.2, 0.8, 10))))
+    objs = [floor]
+    mats = [shaders.Lambert(textures.Constant(0.7)), shaders.Mirror(textures.Constant(0.9)),
+            shaders.Dielectric(textures.Constant(1.5)),
+            shaders.Conductor(textures.Constant(spectra.Sampled(au_w, au_n)),
+                              textures.Constant(spectra.Sampled(au_w, au_k)))]
+    ct = shaders.CookTorrance(textures.Constant(spectra.Sampled(au_w, au_n)),
+                              textures.Constant(spectra.Sampled(au_w, au_k)))
+    ct.mdf = shaders.MicrofacetTrowbridgeReitz()
+    ct.roughnessU = ct.roughnessV = textures.Constant(0.3)
+    mats.append(ct)
+    ctb = shaders.CookTorrance(textures.Constant(1.2), textures.Constant(2.0))
+    ctb.roughnessU = ctb.roughnessV = textures.Constant(0.2)
+    mats.append(ctb)
+    for i, m in enumerate(mats):
+        s = scenery.Sphere((-2.5 + i, 0.0, 0.5), 0.45)
+        s.shader = m
+        objs.append(s)
+    light = scenery.LightArea(scenery.Parallelogram((-1, -1, 4), (2, 0, 0), (0, -2, 0)))
+    light.radiance = textures.Constant(spectra.BlackBody(4500, 8.0))
+    sky = scenery.LightSky(textures.Constant(0.3))
+    sun = scenery.LightSun((0.3, -0.4, 0.87), np.radians(0.53), textures.Constant(spectra.BlackBody(5800, 2e3)))
+    scene = scenery.List(objs + [light, sky, sun])
+    cam = la.cameras.PerspectiveCamera()
+    cam.position = (0.0, -7.0, 2.5)
+    cam.lookAt((0.0, 0.0, 0.4))
+    cam.fovAngle = np.radians(45)
+    cam.aspectRatio = 1.5
+    return scene, cam
+
+
+def test_mixed_materials_same_seed_matches_oracle(gpu_ctx):
+    say("[gpu] mixed BSDFs / lights vs oracle")
+    sc, cam = _mixed_scene()
+    cs = la.compile_scene(sc)
+    gx, gw, _ = gpu_render(gpu_ctx, cs, cam, 96, 64, 16, nl=2, max_depth=12, rr_depth=4, seed=5)
+    ox, ow = oracle.render(cs, cam, 96, 64, 16, wavelengths=2, max_depth=12, rr_depth=4, seed=5)
+    rel_total, frac_bad, _ = compare(gx, gw, ox, ow, "mixed")
+    assert rel_total < 5e-3
+    assert frac_bad < 0.05
+
+
+@pytest.mark.parametrize("name,shader,expect", [
+    ("lambert_1", shaders.Lambert(textures.Constant(1.0)), 1.0),
+    ("lambert_0.5", shaders.Lambert(textures.Constant(0.5)), 0.5),
+    ("mirror_1", shaders.Mirror(textures.Constant(1.0)), 1.0),
+    ("dielectric_lossless", shaders.Dielectric(textures.Constant(1.5)), 1.0),
+])
+def test_white_furnace(gpu_ctx, name, shader, expect):
+    say(f"[gpu] white furnace {name}")
+    sc, cam = W.furnace_sphere(shader)
+    cs = la.compile_scene(sc)
+    gx, gw, _ = gpu_render(gpu_ctx, cs, cam, 64, 64, 1024, nl=1, max_depth=0, rr_depth=8, seed=3)
+    img = gx / gw
+    # the sphere covers a disc of radius ~30 px at the image centre (fov 11
+    # degrees at distance sqrt(125)); test pixels well inside it
+    yy, xx = np.mgrid[0:64, 0:64]
+    centre = (xx - 31.5) ** 2 + (yy - 31.5) ** 2 < 18 ** 2
+    Y = img[1][centre]
+    say(f"[gpu] furnace {name}: mean Y {Y.mean():.6f} (expect {expect}), std {Y.std():.4f}, n {Y.size}")
+    assert abs(Y.mean() - expect) < 1e-3 * max(expect, 1.0) + 3 * Y.std() / np.sqrt(Y.size)
+    assert abs(Y.mean() - expect) < 2e-3
+    corner = img[1][:4, :4]
+    assert (corner == 0).all()  # camera rays that miss see no sky (generation 0)
+
+
+def test_render_deterministic(gpu_ctx):
+    sc, cam = W.cornell_box()
+    cs = la.compile_scene(sc)
+    a = gpu_render(gpu_ctx, cs, cam, 48, 48, 4, nl=4, max_depth=8, rr_depth=3, seed=9)
+    b = gpu_render(gpu_ctx, cs, cam, 48, 48, 4, nl=4, max_depth=8, rr_depth=3, seed=9)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    c = gpu_render(gpu_ctx, cs, cam, 48, 48, 4, nl=4, max_depth=8, rr_depth=3, seed=10)
+    assert not np.array_equal(a[0], c[0])
+
+
+def test_chunking_is_invisible(gpu_ctx):
+    """Splitting the render into many wavefront chunks gives the same image."""
+    import torch
+
+    sc, cam = W.cornell_box()
+    cs = la.compile_scene(sc)
+    gs = E.GpuScene(gpu_ctx, cs)
+    out = []
+    for chunk in (0, 5000):
+        film = torch.zeros((3, 40, 40), dtype=torch.float32, device="cuda")
+        wgt = torch.zeros((40, 40), dtype=torch.float32, device="cuda")
+        gs.render(E.camera_struct(cam), E.render_params(40, 40, 4, 4, 8, 3, 0.5, (380, 780), 2, chunk), film, wgt)
+        torch.cuda.synchronize()
+        out.append(film.cpu().numpy())
+    gs.close()
+    assert np.array_equal(out[0], out[1])
