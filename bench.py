@@ -46,6 +46,55 @@ def _traffic(kind: str):
         return None
 
 
+def bench_render(args, la, oracle, torch, np):
+    """Path-tracer leg: the Cornell-box workload (256x256, 16 spp, 4 stratified
+    wavelengths, depth <= 8, Russian roulette from generation 3) rendered
+    whole per step; CPU baseline = the oracle renderer on a bounded crop."""
+    from liar_amd import engine as E
+    from liar_amd import workloads as W
+
+    W_, H_, SPP, NL = 256, 256, 16, 4
+    sc, cam = W.cornell_box()
+    cs = la.compile_scene(sc)
+    ctx = la.Context(0)
+    gs = E.GpuScene(ctx, cs)
+    camera = E.camera_struct(cam)
+    film = torch.zeros((3, H_, W_), dtype=torch.float32, device="cuda")
+    wgt = torch.zeros((H_, W_), dtype=torch.float32, device="cuda")
+    stats = []
+    for k in range(args.warmup + args.render_steps):
+        film.zero_()
+        wgt.zero_()
+        torch.cuda.synchronize()
+        st = gs.render(camera, E.render_params(W_, H_, SPP, NL, 8, 3, 0.5, (380.0, 780.0), 1 + k), film, wgt)
+        if k >= args.warmup:
+            stats.append(st)
+    secs = float(np.mean([s["seconds"] for s in stats]))
+    paths = stats[0]["paths"]
+    rays = float(np.mean([s["rays"] for s in stats]))
+    out = {
+        "workload": "Cornell box 36 tris, 256x256, 16 spp x 4 wavelengths, depth <= 8, RR from 3",
+        "ms_per_frame": round(secs * 1e3, 3),
+        "paths_per_s": round(paths / secs, 1),
+        "camera_samples_per_s": round(paths / NL / secs, 1),
+        "mrays_per_s": round(rays / secs / 1e6, 2),
+        "rays_per_path": round(rays / paths, 3),
+    }
+    if not args.no_cpu:
+        cw, ch = 64, 64
+        threads = min(oracle.default_threads(), 16)
+        t0 = time.perf_counter()
+        oracle.render(cs, cam, cw, ch, SPP, wavelengths=NL, max_depth=8, rr_depth=3, seed=1, threads=threads)
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cw * ch * SPP * NL / t_cpu, 1), "unit": "paths/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{cw}x{ch} pixels of the same frame geometry (FP64 oracle), "
+                                         f"{t_cpu:.2f} s"}
+    gs.close()
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +105,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--count-sample", type=int, default=1 << 18,
                     help="rays used for the oracle's nodes/prims-per-ray counts and parity")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-tracer leg")
+    ap.add_argument("--render-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -160,6 +211,8 @@ def main() -> int:
                       f"{t_cpu:.1f} s on {threads} host threads",
         }
 
+    render = None if args.no_render else bench_render(args, la, oracle, torch, np)
+
     out = {
         "metric": METRIC,
         "value": round(mrays, 3),
@@ -209,6 +262,7 @@ def main() -> int:
             "qbvh_build_s": round(info["build_seconds"], 3),
             "trace_config": ctx.trace_config(),
         },
+        "render": render,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -308,6 +308,12 @@ int lr_gen_random_triangles(uint64_t seed, uint32_t nt, float scale,
 int lr_gen_random_rays(uint64_t seed, uint32_t n, float tfar_lo, float tfar_hi,
                        float* ox, float* oy, float* oz, float* dx, float* dy,
                        float* dz, float* tnear, float* tfar);
+/* Heightfield terrain over [0,1]^2: n_side^2 vertices, 2 (n_side-1)^2
+ * triangles (two per cell, counter-clockwise seen from +z), z = height *
+ * fBm(x, y): `octaves` octaves of gradient noise, frequency base_frequency *
+ * 2^o, amplitude gain^o.  verts: 3 n_side^2 floats, tris: 6 (n_side-1)^2. */
+int lr_gen_terrain(uint64_t seed, uint32_t n_side, uint32_t octaves, float gain,
+                   float base_frequency, float height, float* verts, uint32_t* tris);
 /* Philox4x32-10 block (Salmon et al., SC'11) for tests: out[4]. */
 void lr_philox4x32(uint64_t key, uint32_t c0, uint32_t c1, uint32_t c2,
                    uint32_t c3, uint32_t* out);

@@ -93,4 +93,58 @@ int lr_gen_random_rays(uint64_t seed, uint32_t n, float tfar_lo, float tfar_hi, 
     return LR_OK;
 }
 
+// Gradient noise on the integer lattice (Perlin's 2002 quintic fade), unit
+// gradients at angles drawn from Philox keyed by (seed, octave, lattice).
+static float grad_noise(uint64_t seed, uint32_t octave, float x, float y) {
+    const float fx = floorf(x), fy = floorf(y);
+    const int32_t ix = (int32_t)fx, iy = (int32_t)fy;
+    const float tx = x - fx, ty = y - fy;
+    float dots[4];
+    for (int c = 0; c < 4; ++c) {
+        const int32_t cx = ix + (c & 1), cy = iy + (c >> 1);
+        const lr::u4 r = draw(seed, 100u + octave, (uint32_t)cx, (uint32_t)cy);
+        const float a = 6.283185307179586f * lr::u01(r.x);
+        dots[c] = cosf(a) * (tx - (float)(c & 1)) + sinf(a) * (ty - (float)(c >> 1));
+    }
+    auto fade = [](float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); };
+    const float u = fade(tx), v = fade(ty);
+    const float a = dots[0] + u * (dots[1] - dots[0]);
+    const float b = dots[2] + u * (dots[3] - dots[2]);
+    return a + v * (b - a);
+}
+
+int lr_gen_terrain(uint64_t seed, uint32_t n_side, uint32_t octaves, float gain, float base_frequency,
+                   float height, float* verts, uint32_t* tris) {
+    if (n_side < 2 || !verts || !tris || octaves == 0 || octaves > 30)
+        return lr::fail(LR_E_INVALID, "lr_gen_terrain: need n_side >= 2, 1..30 octaves and outputs");
+    const uint32_t n = n_side;
+    const float inv = 1.0f / (float)(n - 1);
+    parallel_for(n * n, [&](uint32_t b, uint32_t e) {
+        for (uint32_t i = b; i < e; ++i) {
+            const uint32_t gx = i % n, gy = i / n;
+            const float x = gx * inv, y = gy * inv;
+            float h = 0.0f, amp = 1.0f, freq = base_frequency;
+            for (uint32_t o = 0; o < octaves; ++o) {
+                h += amp * grad_noise(seed, o, x * freq, y * freq);
+                amp *= gain;
+                freq *= 2.0f;
+            }
+            verts[3 * (size_t)i + 0] = x;
+            verts[3 * (size_t)i + 1] = y;
+            verts[3 * (size_t)i + 2] = height * h;
+        }
+    });
+    const uint32_t cells = n - 1;
+    parallel_for(cells * cells, [&](uint32_t b, uint32_t e) {
+        for (uint32_t c = b; c < e; ++c) {
+            const uint32_t cx = c % cells, cy = c / cells;
+            const uint32_t v00 = cy * n + cx, v10 = v00 + 1, v01 = v00 + n, v11 = v01 + 1;
+            uint32_t* t = tris + 6 * (size_t)c;
+            t[0] = v00; t[1] = v10; t[2] = v11;
+            t[3] = v00; t[4] = v11; t[5] = v01;
+        }
+    });
+    return LR_OK;
+}
+
 }  // extern "C"

@@ -123,3 +123,18 @@ def test_binned_spectra_of_cornell():
     green = W.GREEN.values
     np.testing.assert_array_equal(red, np.float32([0.05] * 6 + [0.6] * 4))
     np.testing.assert_array_equal(green, np.float32([0.05] * 3 + [0.6] * 2 + [0.05] * 5))
+
+
+def test_oracle_cornell_consistent_with_committed_reference():
+    """The oracle at 64 spp on the Cornell workload agrees with the committed
+    4096-spp oracle reference within Monte Carlo noise (bias < 1 %)."""
+    from pathlib import Path
+
+    ref = np.load(Path(__file__).resolve().parent / "golden" / "cornell_ref_4096spp.npz")["y"].astype(np.float64)
+    sc, cam = W.cornell_box()
+    cs = la.compile_scene(sc)
+    xyz, w = oracle.render(cs, cam, 256, 256, 16, wavelengths=4, max_depth=8, rr_depth=3, seed=5)
+    y = (xyz / w)[1]
+    bias = (y.mean() - ref.mean()) / ref.mean()
+    mrae = np.abs(y - ref).mean() / ref.mean()
+    assert abs(bias) < 1e-2 and mrae < 0.35, (bias, mrae)

@@ -164,3 +164,54 @@ def test_chunking_is_invisible(gpu_ctx):
         out.append(film.cpu().numpy())
     gs.close()
     assert np.array_equal(out[0], out[1])
+
+
+def test_terrain_small_same_seed_matches_oracle(gpu_ctx):
+    """The terrain workload's generator at 101^2 vertices and 300 spheres
+    (all three shaders, sun + sky) against the oracle, same seed."""
+    say("[gpu] small terrain vs oracle")
+    cs, cam = W.terrain(n_side=101, n_spheres=300, seed=7, tile=4)
+    gx, gw, st = gpu_render(gpu_ctx, cs, cam, 48, 27, 8, nl=1, max_depth=16, rr_depth=16, seed=11)
+    ox, ow = oracle.render(cs, cam, 48, 27, 8, wavelengths=1, max_depth=16, rr_depth=16, seed=11)
+    rel_total, frac_bad, _ = compare(gx, gw, ox, ow, "terrain-small")
+    assert rel_total < 5e-3
+    assert frac_bad < 0.05
+
+
+def test_terrain_full_size_renders(gpu_ctx):
+    """Full terrain workload (4,992,800 triangles + 10,000 spheres) at reduced
+    resolution: finite, non-negative, deterministic image."""
+    say("[gpu] full terrain 480x270 x 4 spp")
+    cs, cam = W.terrain()
+    a = gpu_render(gpu_ctx, cs, cam, 480, 270, 4, nl=1, max_depth=16, rr_depth=16, seed=1)
+    img = a[0] / a[1]
+    say(f"[gpu] terrain: mean Y {img[1].mean():.4f}, paths {a[2]['paths']}, rays {a[2]['rays']}, "
+        f"{a[2]['seconds']:.3f} s")
+    assert np.isfinite(img).all() and (img[1] >= 0).all() and img[1].mean() > 0
+
+
+def _ref_y():
+    from pathlib import Path
+
+    d = np.load(Path(__file__).resolve().parent / "golden" / "cornell_ref_4096spp.npz")
+    return d["y"].astype(np.float64)
+
+
+def test_cornell_vs_4096spp_cpu_reference(gpu_ctx):
+    """Cornell workload (256x256, 4 wavelengths, depth <= 8, RR from 3) against
+    the committed 4096-spp FP64 oracle render (seed 1000).  Measures:
+    relative mean absolute error  MRAE = mean|Y - Yref| / mean(Yref)
+    bias                          = (mean Y - mean Yref) / mean Yref.
+    Bars: at 1024 spp MRAE < 0.04 and |bias| < 2e-3; at the workload's 16 spp
+    MRAE < 0.35 (noise) and |bias| < 1e-2."""
+    ref = _ref_y()
+    sc, cam = W.cornell_box()
+    cs = la.compile_scene(sc)
+    for spp, mrae_bar, bias_bar in ((16, 0.35, 1e-2), (1024, 0.04, 2e-3)):
+        gx, gw, st = gpu_render(gpu_ctx, cs, cam, 256, 256, spp, nl=4, max_depth=8, rr_depth=3, seed=7)
+        y = (gx / gw)[1]
+        mrae = np.abs(y - ref).mean() / ref.mean()
+        bias = (y.mean() - ref.mean()) / ref.mean()
+        say(f"[gpu] cornell {spp} spp vs 4096-spp reference: MRAE {mrae:.4f}, bias {bias:+.2e}, "
+            f"{st['seconds'] * 1e3:.1f} ms")
+        assert mrae < mrae_bar and abs(bias) < bias_bar
