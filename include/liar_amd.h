@@ -296,6 +296,36 @@ int lr_scene_destroy(lr_scene* scene);
 int lr_render(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
               float* film_xyz, float* film_weight, lr_render_stats* stats, void* stream);
 
+/* ---- path-guiding network (no counterpart in LiAR) -------------------------
+ * A learned proposal for the integrand's direction sampling: an MLP
+ * (6 inputs -> 64-wide frequency encoding -> 5 hidden layers of 64, ReLU ->
+ * 40 outputs) in BF16 on MFMA with FP32 accumulation, whose outputs are an
+ * 8-lobe von Mises-Fisher mixture over incident directions.  Trained online
+ * with Adam on radiance samples the tracer produces.
+ * Inputs: device float [n][8], per row (px, py, pz, theta_o/pi, phi_o/(2 pi),
+ * (lambda - lambda_min)/(lambda_max - lambda_min), -, -), all in [0, 1].
+ * Mixture: device float [n][40], per lobe (weight, mean x, y, z, kappa).
+ * Parameters (lr_guide_param_count() floats): per layer l (0..5) Wt_l
+ * [n_out][64] row-major (n_out = 64, output layer 48 with rows 40..47 unused),
+ * then the biases (5 x 64 + 48). */
+typedef struct lr_guide lr_guide;
+int lr_guide_param_count(void);
+int lr_guide_create(lr_context* ctx, uint64_t seed, float learning_rate, lr_guide** out);
+int lr_guide_destroy(lr_guide* guide);
+int lr_guide_get_params(const lr_guide* guide, float* host_params);
+int lr_guide_set_params(lr_guide* guide, const float* host_params);
+/* n_dev: optional device count (<= n) read by the kernel. */
+int lr_guide_infer(lr_guide* guide, const float* inputs, uint32_t n, const uint32_t* n_dev,
+                   float* mixture, void* stream);
+/* One Adam step on -sum_i w_i log q(dir_i | input_i) / sum_i w_i.
+ * dirs: device float [n][3] unit vectors; weights: device float [n] >= 0.
+ * loss_out (device float, optional): the summed normalised loss.
+ * grad_out (device float [param_count], optional): the gradient.  A guide
+ * created with learning_rate 0 computes gradients without updating. */
+int lr_guide_train_step(lr_guide* guide, const float* inputs, const float* dirs,
+                        const float* weights, uint32_t n, float* loss_out, float* grad_out,
+                        void* stream);
+
 /* ---- deterministic synthetic inputs (host, counter-based Philox4x32-10) --- */
 
 /* Random triangle soup: centre c ~ U[0,1)^3, v_k = c + scale * xi_k,

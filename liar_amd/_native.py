@@ -113,6 +113,13 @@ SIGNATURES = {
     "lr_gen_random_rays": (_I, [_U64, _U32, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P]),
     "lr_philox4x32": (None, [_U64, _U32, _U32, _U32, _U32, _P]),
     "lr_gen_terrain": (_I, [_U64, _U32, _U32, _F, _F, _F, _P, _P]),
+    "lr_guide_param_count": (_I, []),
+    "lr_guide_create": (_I, [_P, _U64, _F, C.POINTER(_P)]),
+    "lr_guide_destroy": (_I, [_P]),
+    "lr_guide_get_params": (_I, [_P, _P]),
+    "lr_guide_set_params": (_I, [_P, _P]),
+    "lr_guide_infer": (_I, [_P, _P, _U32, _P, _P, _P]),
+    "lr_guide_train_step": (_I, [_P, _P, _P, _P, _U32, _P, _P, _P]),
     "lr_set_standard_observer": (_I, [_P, _P, _U32]),
     "lr_scene_create": (_I, [_P, _P, _P, _P, _U32, _P, _U32, _P, _U32, C.POINTER(_P)]),
     "lr_scene_destroy": (_I, [_P]),

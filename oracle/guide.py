@@ -1,0 +1,157 @@
+"""FP64 numpy oracle of the path-guiding MLP -- TEST INFRASTRUCTURE ONLY.
+
+Same parameter layout, encoding and mixture parameterisation as the GPU
+network (include/liar_amd.h, lr_guide_*).  With ``bf16=True`` the forward and
+backward passes round to bfloat16 at the points where the GPU stores bf16
+(weights, encoding, hidden activations, output and hidden gradients), so GPU
+and oracle differ only by FP32-vs-FP64 accumulation.  The analytic gradient
+is pinned by central finite differences in tests/test_guide_oracle.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+W, HIDDEN, OUT, OUT_PAD, LOBES, FREQ, IN = 64, 5, 40, 48, 8, 4, 6
+WT = HIDDEN * W * W + OUT_PAD * W
+NPARAMS = WT + HIDDEN * W + OUT_PAD
+
+
+def bf16(x):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    a = np.asarray(x, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    r = np.where((u & 0x7FFFFFFF) > 0x7F800000, 0x7FC00000, r)
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def unpack(p):
+    p = np.asarray(p, np.float64)
+    Wt, b, o = [], [], 0
+    for l in range(HIDDEN + 1):
+        n = W if l < HIDDEN else OUT_PAD
+        Wt.append(p[o:o + n * W].reshape(n, W))
+        o += n * W
+    for l in range(HIDDEN + 1):
+        n = W if l < HIDDEN else OUT_PAD
+        b.append(p[o:o + n])
+        o += n
+    return Wt, b
+
+
+def encode(inp):
+    v = np.asarray(inp, np.float64)[:, :IN]
+    n = v.shape[0]
+    f = np.zeros((n, W))
+    f[:, :IN] = 2 * v - 1
+    c = IN
+    for d in range(IN):
+        for k in range(FREQ):
+            a = np.float32(np.pi) * (2 ** k) * v[:, d]
+            f[:, c] = np.sin(a)
+            f[:, c + 1] = np.cos(a)
+            c += 2
+    f[:, c] = 1.0
+    return f
+
+
+def forward(p, inp, bf=False):
+    Wt, b = unpack(p)
+    if bf:
+        Wt = [bf16(w) for w in Wt]
+    a = encode(inp)
+    if bf:
+        a = bf16(a)
+    acts = [a]
+    for l in range(HIDDEN):
+        z = acts[-1] @ Wt[l].T + b[l]
+        a = np.maximum(z, 0)
+        acts.append(bf16(a) if bf else a)
+    out = acts[-1] @ Wt[HIDDEN].T + b[HIDDEN]
+    return acts, out
+
+
+def softplus(x):
+    return np.where(x > 20, x, np.log1p(np.exp(np.minimum(x, 20))))
+
+
+def mixture(out):
+    o = out[:, :OUT].reshape(-1, LOBES, 5)
+    logit = o[..., 0]
+    pi = np.exp(logit - logit.max(1, keepdims=True))
+    pi /= pi.sum(1, keepdims=True)
+    m = o[..., 1:4]
+    mu = m / (np.linalg.norm(m, axis=2, keepdims=True) + 1e-12)
+    kappa = softplus(o[..., 4]) + 0.1
+    return pi, mu, kappa
+
+
+def log_pdf(out, dirs):
+    pi, mu, kappa = mixture(out)
+    c = np.einsum("nkd,nd->nk", mu, dirs)
+    lv = np.log(kappa) - np.log(2 * np.pi * (-np.expm1(-2 * kappa))) + kappa * (c - 1)
+    lp = np.log(pi) + lv
+    mx = lp.max(1, keepdims=True)
+    return (mx + np.log(np.exp(lp - mx).sum(1, keepdims=True)))[:, 0]
+
+
+def loss(p, inp, dirs, w, bf=False):
+    _, out = forward(p, inp, bf)
+    return float(-(w / w.sum() * log_pdf(out, dirs)).sum())
+
+
+def grad(p, inp, dirs, w, bf=False):
+    """Analytic gradient of loss() w.r.t. the flat parameter vector."""
+    Wt, b = unpack(p)
+    Wtb = [bf16(x) for x in Wt] if bf else Wt
+    acts, out = forward(p, inp, bf)
+    n = out.shape[0]
+    wn = w / w.sum()
+    o = out[:, :OUT].reshape(n, LOBES, 5)
+    pi, mu, kappa = mixture(out)
+    c = np.einsum("nkd,nd->nk", mu, dirs)
+    lv = np.log(kappa) - np.log(2 * np.pi * (-np.expm1(-2 * kappa))) + kappa * (c - 1)
+    lp = np.log(pi) + lv
+    logq = log_pdf(out, dirs)[:, None]
+    resp = np.exp(lp - logq)
+    g = np.zeros((n, OUT_PAD))
+    gg = g[:, :OUT].reshape(n, LOBES, 5)
+    gg[..., 0] = -wn[:, None] * (resp - pi)
+    dlk = 1 / kappa - 2 / np.expm1(2 * kappa) + (c - 1)
+    sig = 1 / (1 + np.exp(-o[..., 4]))
+    gg[..., 4] = -wn[:, None] * resp * dlk * sig
+    nm = np.linalg.norm(o[..., 1:4], axis=2) + 1e-12
+    proj = dirs[:, None, :] - c[..., None] * mu
+    gg[..., 1:4] = (-wn[:, None] * resp * kappa / nm)[..., None] * proj
+    g[:, :OUT] = gg.reshape(n, OUT)
+    if bf:
+        g = bf16(g)
+    dW = [None] * (HIDDEN + 1)
+    db = [None] * (HIDDEN + 1)
+    dz = g
+    for l in range(HIDDEN, -1, -1):
+        dW[l] = dz.T @ acts[l]
+        db[l] = dz.sum(0)
+        if l == 0:
+            break
+        da = dz @ Wtb[l]
+        dz = np.where(acts[l] > 0, da, 0.0)
+        if bf:
+            dz = bf16(dz)
+    return np.concatenate([x.ravel() for x in dW] + [x.ravel() for x in db])
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NPARAMS)
+    Wt, _ = unpack(p)
+    o = 0
+    for l in range(HIDDEN + 1):
+        nrow = W if l < HIDDEN else OUT_PAD
+        lim = np.sqrt(6 / W) * (1 if l < HIDDEN else 0.1)
+        blk = rng.uniform(-lim, lim, (nrow, W))
+        if l == HIDDEN:
+            blk[OUT:] = 0
+        p[o:o + nrow * W] = blk.ravel()
+        o += nrow * W
+    return p

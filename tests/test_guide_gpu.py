@@ -1,0 +1,119 @@
+"""GPU guiding network (BF16 MFMA) against the FP64 oracle of oracle/guide.py.
+
+- forward: GPU mixture parameters equal the oracle's with bf16 rounding at the
+  same points (differences: FP32 vs FP64 accumulation only); a wrong MFMA
+  lane layout or layer wiring fails this;
+- gradient: lr_guide_train_step's gradient (learning rate 0) equals the
+  oracle's analytic gradient, itself pinned by finite differences;
+- training: Adam on samples of a known position-dependent vMF target lowers
+  the loss from ~log(4 pi) to near the target's entropy.
+"""
+import numpy as np
+import pytest
+
+from liar_amd.guide import Guide
+from oracle import guide as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((n, 8), np.float32)
+    x[:, :6] = rng.random((n, 6))
+    return x
+
+
+def test_forward_matches_oracle(gpu_ctx):
+    import torch
+
+    g = Guide(gpu_ctx, seed=3, learning_rate=0.0)
+    p = g.params().astype(np.float64)
+    rng = np.random.default_rng(1)
+    p[G.WT:] = rng.normal(0, 0.05, p.size - G.WT)  # nonzero biases
+    p[G.HIDDEN * G.W * G.W:G.WT] *= 8  # informative outputs
+    g.set_params(p.astype(np.float32))
+    p = g.params().astype(np.float64)
+    for n in (1, 17, 1000):
+        x = _inputs(n, n)
+        mix = g.infer(torch.from_numpy(x).cuda()).cpu().numpy().astype(np.float64)
+        _, out = G.forward(p, x.astype(np.float64), bf=True)
+        pi, mu, kappa = G.mixture(out)
+        got = mix.reshape(n, 8, 5)
+        # single-ulp bf16 rounding flips (FP32 vs FP64 accumulation) move
+        # values by ~1e-3; a wrong lane layout or wiring moves them by O(0.1)
+        e_pi, e_mu = np.abs(got[..., 0] - pi).max(), np.abs(got[..., 1:4] - mu).max()
+        e_k = (np.abs(got[..., 4] - kappa) / kappa).max()
+        print(f"[gpu] guide forward n={n}: max |d pi| {e_pi:.1e}, |d mu| {e_mu:.1e}, rel d kappa {e_k:.1e}")
+        assert e_pi < 5e-3 and e_mu < 1e-2 and e_k < 1e-2
+        assert np.abs(got[..., 0] - pi).mean() < 2e-4
+    g.close()
+
+
+def test_gradient_matches_oracle(gpu_ctx):
+    import torch
+
+    g = Guide(gpu_ctx, seed=5, learning_rate=0.0)
+    p = g.params().astype(np.float64)
+    rng = np.random.default_rng(2)
+    p[G.WT:] = rng.normal(0, 0.05, p.size - G.WT)
+    p[G.HIDDEN * G.W * G.W:G.WT] *= 6
+    g.set_params(p.astype(np.float32))
+    p = g.params().astype(np.float64)
+    n = 3000
+    x = _inputs(n, 7)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    w = (rng.random(n) + 0.2).astype(np.float32)
+    grad = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    g.train_step(torch.from_numpy(x).cuda(), torch.from_numpy(d.astype(np.float32)).cuda(),
+                 torch.from_numpy(w).cuda(), loss=loss, grad=grad)
+    torch.cuda.synchronize()
+    gg = grad.cpu().numpy().astype(np.float64)
+    go = G.grad(p, x.astype(np.float64), d, w.astype(np.float64), bf=True)
+    rel = np.linalg.norm(gg - go) / np.linalg.norm(go)
+    lo = G.loss(p, x.astype(np.float64), d, w.astype(np.float64), bf=True)
+    print(f"[gpu] guide gradient rel err {rel:.2e}, loss gpu {loss.item():.6f} oracle {lo:.6f}")
+    assert rel < 1e-2
+    assert abs(loss.item() - lo) < 1e-3 * max(1, abs(lo))
+    assert np.array_equal(g.params(), p.astype(np.float32))  # learning rate 0: no update
+    g.close()
+
+
+def _sample_vmf(mu, kappa, rng):
+    n = len(mu)
+    xi = rng.random(n)
+    wv = 1 + np.log(xi + (1 - xi) * np.exp(-2 * kappa)) / kappa
+    phi = 2 * np.pi * rng.random(n)
+    t = np.cross(mu, np.where(np.abs(mu[:, :1]) < 0.9, [[1, 0, 0]], [[0, 1, 0]]))
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(mu, t)
+    s = np.sqrt(np.maximum(0, 1 - wv ** 2))
+    return wv[:, None] * mu + s[:, None] * (np.cos(phi)[:, None] * t + np.sin(phi)[:, None] * b)
+
+
+def test_training_learns_a_vmf_target(gpu_ctx):
+    import torch
+
+    g = Guide(gpu_ctx, seed=11, learning_rate=3e-3)
+    rng = np.random.default_rng(5)
+    kappa = 20.0
+    losses = []
+    for step in range(600):
+        n = 8192
+        x = _inputs(n, 1000 + step)
+        mu = x[:, :3] - 0.5 + np.array([0, 0, 0.3])
+        mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+        d = _sample_vmf(mu.astype(np.float64), kappa, rng).astype(np.float32)
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        g.train_step(torch.from_numpy(x).cuda(), torch.from_numpy(d).cuda(),
+                     torch.ones(n, dtype=torch.float32, device="cuda"), loss=loss)
+        losses.append(loss.item())
+    entropy = -np.log(kappa / (2 * np.pi * (1 - np.exp(-2 * kappa)))) + 1 - kappa * (
+        1 / np.tanh(kappa) - 1 / kappa - (1 - 1 / kappa))
+    print(f"[gpu] guide training: loss {losses[0]:.3f} -> {np.mean(losses[-20:]):.3f} "
+          f"(uniform {np.log(4 * np.pi):.3f}, target entropy {entropy:.3f})")
+    assert losses[0] > 2.0
+    assert np.mean(losses[-20:]) < 0.6
+    g.close()

@@ -1,0 +1,60 @@
+"""Pins the FP64 guiding-network oracle (oracle/guide.py): analytic gradient
+vs central finite differences, mixture pdf normalised on the sphere, bf16
+rounding matches the IEEE round-to-nearest-even definition."""
+import numpy as np
+
+from oracle import guide as G
+
+
+def _case(seed=0, n=6):
+    rng = np.random.default_rng(seed)
+    p = G.init_params(seed + 1)
+    p[G.WT:] = rng.normal(0, 0.1, p.size - G.WT)
+    inp = rng.random((n, 8))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    w = rng.random(n) + 0.5
+    return p, inp, d, w
+
+
+def test_gradient_matches_finite_differences():
+    p, inp, d, w = _case()
+    g = G.grad(p, inp, d, w)
+    rng = np.random.default_rng(3)
+    # weights of every layer, biases, including the output layer
+    idx = np.concatenate([rng.choice(G.WT, 48, replace=False), G.WT + rng.choice(p.size - G.WT, 16, replace=False)])
+    fd = []
+    for k in idx:
+        e = np.zeros(p.size)
+        e[k] = 1e-6
+        fd.append((G.loss(p + e, inp, d, w) - G.loss(p - e, inp, d, w)) / 2e-6)
+    fd = np.array(fd)
+    assert np.max(np.abs(fd - g[idx])) < 1e-6 * max(1.0, np.abs(g[idx]).max())
+    # padded output rows 40..47 get no gradient
+    out_rows = g[G.HIDDEN * G.W * G.W:G.WT].reshape(G.OUT_PAD, G.W)
+    assert (out_rows[G.OUT:] == 0).all()
+
+
+def test_mixture_pdf_integrates_to_one():
+    p, inp, _, _ = _case(seed=4, n=3)
+    p[G.HIDDEN * G.W * G.W:G.WT] *= 20  # sharper lobes
+    _, out = G.forward(p, inp)
+    # quadrature on a fine (z, phi) grid: dA = dz dphi
+    nz, nphi = 800, 400
+    z = -1 + (np.arange(nz) + 0.5) * 2 / nz
+    phi = (np.arange(nphi) + 0.5) * 2 * np.pi / nphi
+    Z, P = np.meshgrid(z, phi, indexing="ij")
+    r = np.sqrt(1 - Z ** 2)
+    dirs = np.stack([r * np.cos(P), r * np.sin(P), Z], -1).reshape(-1, 3)
+    for row in range(3):
+        lq = G.log_pdf(np.repeat(out[row:row + 1], len(dirs), 0), dirs)
+        total = np.exp(lq).sum() * (2 / nz) * (2 * np.pi / nphi)
+        assert abs(total - 1) < 2e-3, total
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.00390625, 1.005859375, -3.0e-39, 65504.0, np.inf], np.float32)
+    r = G.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0  # tie to even (1 + 2^-8 -> 1)
+    assert r[2] == 1.0078125            # above the tie rounds up
+    assert r[4] == 65536.0 and np.isinf(r[5])
