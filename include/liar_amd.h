@@ -326,6 +326,25 @@ int lr_guide_train_step(lr_guide* guide, const float* inputs, const float* dirs,
                         const float* weights, uint32_t n, float* loss_out, float* grad_out,
                         void* stream);
 
+/* Guided rendering (no counterpart in LiAR): at diffuse / glossy vertices the
+ * integrand's one-sample MIS gains a third strategy, sampling the guide's
+ * vMF mixture, with probability `selection`; BSDF and light sampling share
+ * the rest equally, and the power heuristic over all three keeps the
+ * estimator unbiased for any network.  With `train`, the vertices of each
+ * chunk are recorded (up to max_records, 0: 2^20) and `train_steps` Adam
+ * steps of `train_batch` records (0: 65536) run after the chunk on the
+ * weights f|cos|/p_mix * L_incident. */
+typedef struct lr_guide_params {
+    float selection;
+    uint32_t train;
+    uint32_t train_steps;
+    uint32_t train_batch;
+    uint32_t max_records;
+} lr_guide_params;
+int lr_render_guided(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
+                     lr_guide* guide, const lr_guide_params* guide_params, float* film_xyz,
+                     float* film_weight, lr_render_stats* stats, void* stream);
+
 /* ---- deterministic synthetic inputs (host, counter-based Philox4x32-10) --- */
 
 /* Random triangle soup: centre c ~ U[0,1)^3, v_k = c + scale * xi_k,

@@ -83,6 +83,11 @@ class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("chunks", C.c_uint32), ("seconds", C.c_double)]
 
 
+class GuideParams(C.Structure):
+    _fields_ = [("selection", C.c_float), ("train", C.c_uint32), ("train_steps", C.c_uint32),
+                ("train_batch", C.c_uint32), ("max_records", C.c_uint32)]
+
+
 LR_NONE = 0xFFFFFFFF
 SPEC_CONSTANT, SPEC_SAMPLED, SPEC_BINNED, SPEC_BLACKBODY = 0, 1, 2, 3
 MAT_NONE, MAT_LAMBERT, MAT_MIRROR, MAT_DIELECTRIC, MAT_CONDUCTOR, MAT_COOK_TORRANCE = 0, 1, 2, 3, 4, 5
@@ -113,6 +118,8 @@ SIGNATURES = {
     "lr_gen_random_rays": (_I, [_U64, _U32, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P]),
     "lr_philox4x32": (None, [_U64, _U32, _U32, _U32, _U32, _P]),
     "lr_gen_terrain": (_I, [_U64, _U32, _U32, _F, _F, _F, _P, _P]),
+    "lr_render_guided": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, C.POINTER(GuideParams), _P, _P,
+                              C.POINTER(RenderStats), _P]),
     "lr_guide_param_count": (_I, []),
     "lr_guide_create": (_I, [_P, _U64, _F, C.POINTER(_P)]),
     "lr_guide_destroy": (_I, [_P]),

@@ -144,10 +144,96 @@ __global__ void k_fill_miss_queue(const uint32_t* count, Hits h) {
     }
 }
 
+// -- guided mode ---------------------------------------------------------------
+// The guide network proposes incident directions at diffuse / glossy
+// vertices; its mixture (8 lobes x (weight, mean xyz, kappa)) is inferred per
+// queue entry before shading.
+struct GuideArgs {
+    const float* in;   // [queue][8] network inputs (written by k_guide_inputs)
+    const float* mix;  // [queue][40] mixture (lr_guide_infer)
+    float sel;         // probability of the guide strategy
+    float lo[3], inv_ext[3];
+    float lam_lo, inv_lam_ext;
+    // training records (cap entries): inputs [8], direction [3], aux [4] =
+    // (f|cos| / p_mix, L accumulated at the vertex, throughput after scatter
+    // before roulette, path id bits)
+    uint32_t record, cap;
+    uint32_t* count;
+    float* r_in;
+    float* r_dir;
+    float* r_aux;
+};
+
+__device__ inline float vmf_mix_pdf(const float* m, V3 w) {
+    float p = 0.0f;
+    for (int k = 0; k < 8; ++k) {
+        const float kap = m[5 * k + 4];
+        const float c = m[5 * k + 1] * w.x + m[5 * k + 2] * w.y + m[5 * k + 3] * w.z;
+        p += m[5 * k] * kap / (2.0f * kPi * (-expm1f(-2.0f * kap))) * expf(kap * (c - 1.0f));
+    }
+    return p;
+}
+
+// Lobe chosen by weight, then Jakob's stable vMF sampling ("Numerically
+// stable sampling of the von Mises-Fisher distribution on S^2", 2012).
+__device__ inline V3 vmf_mix_sample(const float* m, float u0, float u1, float u2) {
+    int k = 0;
+    float acc = m[0];
+    while (k < 7 && u0 >= acc) {
+        ++k;
+        acc += m[5 * k];
+    }
+    const V3 mu = v3(m[5 * k + 1], m[5 * k + 2], m[5 * k + 3]);
+    const float kap = m[5 * k + 4];
+    const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap;
+    const float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+    const float phi = 2.0f * kPi * u2;
+    V3 t, b;
+    onb(mu, t, b);
+    return normalize((s * cosf(phi)) * t + (s * sinf(phi)) * b + w * mu);
+}
+
+__global__ void k_guide_inputs(DevScene S, Queue q, const uint32_t* count, Hits h, PathState ps, GuideArgs G,
+                               float* in) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t prim = h.prim[i];
+    if (prim != LR_MISS) {
+        const V3 o = v3(q.r[0][i], q.r[1][i], q.r[2][i]);
+        const V3 d = v3(q.r[3][i], q.r[4][i], q.r[5][i]);
+        const V3 x = o + h.t[i] * d;
+        v[0] = fminf(fmaxf((x.x - G.lo[0]) * G.inv_ext[0], 0.0f), 1.0f);
+        v[1] = fminf(fmaxf((x.y - G.lo[1]) * G.inv_ext[1], 0.0f), 1.0f);
+        v[2] = fminf(fmaxf((x.z - G.lo[2]) * G.inv_ext[2], 0.0f), 1.0f);
+        // outgoing direction (towards the previous vertex) in spherical coordinates
+        v[3] = acosf(fminf(fmaxf(-d.z, -1.0f), 1.0f)) / kPi;
+        v[4] = (atan2f(-d.y, -d.x) + kPi) / (2.0f * kPi);
+        v[5] = fminf(fmaxf((ps.lam[q.pid[i]] - G.lam_lo) * G.inv_lam_ext, 0.0f), 1.0f);
+    }
+    float4* o4 = reinterpret_cast<float4*>(in + (size_t)i * 8);
+    o4[0] = make_float4(v[0], v[1], v[2], v[3]);
+    o4[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+// Training weights of the recorded vertices: w = f|cos|/p_mix * L_i, with
+// the incident radiance estimate L_i = (L_path - L_vertex) / throughput.
+__global__ void k_guide_targets(uint32_t n, const float* __restrict__ aux, const float* __restrict__ L,
+                                float* __restrict__ w) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float fcp = aux[4 * r], lv = aux[4 * r + 1], tn = aux[4 * r + 2];
+    const uint32_t pid = __float_as_uint(aux[4 * r + 3]);
+    const float rest = L[pid] - lv;
+    float v = (tn > 0.0f && rest > 0.0f) ? fcp * rest / tn : 0.0f;
+    w[r] = isfinite(v) ? v : 0.0f;
+}
+
 // -- one generation: emission + scatter (adjoint_photon_tracer.cpp:140-292) --
+template <bool GUIDED>
 __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queue q, const uint32_t* count,
                         Hits h, const float4* __restrict__ prim_geom, PathState ps, Queue qn, uint32_t* count_next,
-                        uint64_t* rays_total) {
+                        uint64_t* rays_total, GuideArgs G) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < *count;
     bool alive = false;
@@ -220,7 +306,60 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
                 V3 wo;
                 float value = 0.0f, pdf = 0.0f;
                 bool ok = false;
-                if (u_strat < strategy) {
+                float rec_fcp = 0.0f;
+                if (GUIDED && strategy < 1.0f) {
+                    // one-sample MIS over BSDF, light and guide sampling
+                    // (power heuristic over c_i p_i), c = (b, l, g)
+                    const float* mx = G.mix + (size_t)i * 40;
+                    const float cg = G.sel, cb = 0.5f * (1.0f - cg), cl = 0.5f * (1.0f - cg);
+                    V3 dw = v3(0, 0, 1);
+                    float f = 0.0f, pb = 0.0f, pl = 0.0f, pg = 0.0f, pc = 0.0f;
+                    if (u_strat < cb) {
+                        ok = bsdf_sample(B, wi, u1, u2, u_comp, wo, f, pb);
+                        if (ok) {
+                            dw = wo.x * tt + wo.y * bb + wo.z * nf;
+                            for (uint32_t k = 0; k < S.n_lights; ++k)
+                                pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], x, dw);
+                            pg = vmf_mix_pdf(mx, dw);
+                            pc = cb * pb;
+                        }
+                    } else if (u_strat < cb + cl) {
+                        if (S.n_lights > 0) {
+                            const uint32_t k = light_pick(S, u_light);
+                            const DevLight& Lk = S.lights[k];
+                            float pdf_l = 0.0f;
+                            dw = light_sample(Lk, x, ul1, ul2, pdf_l);
+                            if (Lk.sel_pdf > 0.0f && pdf_l > 0.0f) {
+                                pl = Lk.sel_pdf * pdf_l;
+                                for (uint32_t j2 = 0; j2 < S.n_lights; ++j2)
+                                    if (j2 != k) pl += S.lights[j2].sel_pdf * light_pdf(S.lights[j2], x, dw);
+                                wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
+                                ok = bsdf_eval(B, wi, wo, f, pb) && f != 0.0f;
+                                pg = vmf_mix_pdf(mx, dw);
+                                pc = cl * pl;
+                            }
+                        }
+                    } else {
+                        const lr::u4 r2 = rng(P, gp, gen, 3u);
+                        dw = vmf_mix_sample(mx, lr::u01(r2.x), lr::u01(r2.y), lr::u01(r2.z));
+                        wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
+                        ok = bsdf_eval(B, wi, wo, f, pb) && f != 0.0f;
+                        if (ok) {
+                            for (uint32_t k = 0; k < S.n_lights; ++k)
+                                pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], x, dw);
+                            pg = vmf_mix_pdf(mx, dw);
+                            pc = cg * pg;
+                        }
+                    }
+                    if (ok && pc > 0.0f) {
+                        const float qb = cb * pb, ql = cl * pl, qg = cg * pg;
+                        value = f * (pc * pc) / (qb * qb + ql * ql + qg * qg);
+                        pdf = pc;
+                        rec_fcp = f * fabsf(wo.z) / (qb + ql + qg);
+                    } else {
+                        ok = false;
+                    }
+                } else if (u_strat < strategy) {
                     // BSDF sampling, MIS-weighted against all lights
                     ok = bsdf_sample(B, wi, u1, u2, u_comp, wo, value, pdf);
                     if (ok) {
@@ -260,6 +399,22 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
                     thr *= value * fabsf(cos_t) / pdf;
                     start = x + ((cos_t > 0.0f ? 10.0f : -10.0f) * kTolerance) * nf;
                     dir = normalize(wo.x * tt + wo.y * bb + wo.z * nf);
+                    if (GUIDED && G.record && rec_fcp > 0.0f && isfinite(thr) && thr > 0.0f) {
+                        const uint32_t slot = atomicAdd(G.count, 1u);
+                        if (slot < G.cap) {
+                            const float4* src = reinterpret_cast<const float4*>(G.in + (size_t)i * 8);
+                            float4* dst = reinterpret_cast<float4*>(G.r_in + (size_t)slot * 8);
+                            dst[0] = src[0];
+                            dst[1] = src[1];
+                            G.r_dir[3 * (size_t)slot] = dir.x;
+                            G.r_dir[3 * (size_t)slot + 1] = dir.y;
+                            G.r_dir[3 * (size_t)slot + 2] = dir.z;
+                            G.r_aux[4 * (size_t)slot] = rec_fcp;
+                            G.r_aux[4 * (size_t)slot + 1] = L;
+                            G.r_aux[4 * (size_t)slot + 2] = thr;
+                            G.r_aux[4 * (size_t)slot + 3] = __uint_as_float(pid);
+                        }
+                    }
                     const uint32_t next = gen + 1;
                     if (next >= P.max_gen) thr = 0.0f;
                     else if (next >= P.rr_depth) {
@@ -399,7 +554,10 @@ struct lr_scene {
     // wavefront workspace (grow-only)
     size_t cap = 0;
     float* d_work = nullptr;
-    uint32_t* d_counts = nullptr;  // [0..1] queue lengths, [4..5] u64 ray counter
+    uint32_t* d_counts = nullptr;  // [0..1] queue lengths, [4..5] u64 ray counter, [8] records
+    // guided mode: inputs [cap][8], mixture [cap][40], records
+    float* d_gwork = nullptr;
+    size_t gcap = 0, grec = 0;
 };
 
 namespace {
@@ -408,7 +566,7 @@ void free_scene(lr_scene* s) {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
     void* ptrs[] = {s->d_prim_mat, s->d_prim_geom, s->d_mats, s->d_spectra, s->d_table, s->d_lights,
-                    s->d_obs,      s->d_work,      s->d_counts};
+                    s->d_obs,      s->d_work,      s->d_counts, s->d_gwork};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -476,6 +634,18 @@ int build_observer(lr_scene* sc, float lo, float hi) {
     sc->obs_lo = lo;
     sc->obs_hi = hi;
     sc->obs_n = (uint32_t)n;
+    return LR_OK;
+}
+
+int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, hipStream_t s) {
+    if (sc->gcap >= paths && sc->grec >= records) return LR_OK;
+    LR_HIP_CHECK(hipStreamSynchronize(s));
+    if (sc->d_gwork) LR_HIP_CHECK(hipFree(sc->d_gwork));
+    sc->d_gwork = nullptr;
+    const size_t words = 48 * paths + 16 * records;
+    LR_HIP_CHECK(hipMalloc(&sc->d_gwork, sizeof(float) * words));
+    sc->gcap = paths;
+    sc->grec = records;
     return LR_OK;
 }
 
@@ -698,8 +868,11 @@ int lr_scene_destroy(lr_scene* sc) {
     return LR_OK;
 }
 
-int lr_render(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, float* film_xyz, float* film_w,
-              lr_render_stats* stats, void* stream) {
+}  // extern "C"
+
+static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, lr_guide* guide,
+                       const lr_guide_params* gparams, float* film_xyz, float* film_w, lr_render_stats* stats,
+                       void* stream) {
     const auto t0 = std::chrono::steady_clock::now();
     if (!sc || !cam || !prm) return lr::fail(LR_E_INVALID, "NULL scene, camera or params");
     if (!film_xyz || !film_w) return lr::fail(LR_E_INVALID, "NULL film buffer");
@@ -763,6 +936,40 @@ int lr_render(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, f
     if (cap64 >= (1ull << 31)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");
     const uint32_t cap = (uint32_t)cap64;
     if (int rc = ensure_work(sc, cap, s)) return rc;
+    const bool guided = guide != nullptr;
+    GuideArgs GA;
+    std::memset(&GA, 0, sizeof(GA));
+    float* g_in = nullptr;
+    float* g_mix = nullptr;
+    float* r_w = nullptr;
+    uint32_t rec_cap = 0;
+    if (guided) {
+        if (!gparams) return lr::fail(LR_E_INVALID, "guided render needs lr_guide_params");
+        if (!(gparams->selection >= 0.0f && gparams->selection < 1.0f))
+            return lr::fail(LR_E_INVALID, "guide selection must be in [0, 1)");
+        rec_cap = gparams->max_records ? gparams->max_records : (1u << 20);
+        if (int rc = ensure_guide_work(sc, cap, rec_cap, s)) return rc;
+        g_in = sc->d_gwork;
+        g_mix = g_in + 8 * (size_t)cap;
+        GA.in = g_in;
+        GA.mix = g_mix;
+        GA.sel = gparams->selection;
+        const lr_qbvh_info& I = sc->bvh->info;
+        for (int k = 0; k < 3; ++k) {
+            GA.lo[k] = I.scene_lo[k];
+            const float ext = I.scene_hi[k] - I.scene_lo[k];
+            GA.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+        }
+        GA.lam_lo = R.lambda_min;
+        GA.inv_lam_ext = 1.0f / (R.lambda_max - R.lambda_min);
+        GA.record = gparams->train ? 1u : 0u;
+        GA.cap = rec_cap;
+        GA.count = sc->d_counts + 8;
+        GA.r_in = g_mix + 40 * (size_t)cap;
+        GA.r_dir = GA.r_in + 8 * (size_t)rec_cap;
+        GA.r_aux = GA.r_dir + 3 * (size_t)rec_cap;
+        r_w = GA.r_aux + 4 * (size_t)rec_cap;
+    }
 
     DevScene S;
     S.nodes = sc->bvh->d_nodes;
@@ -801,6 +1008,7 @@ int lr_render(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, f
         const uint64_t path0 = pix0 * ppp;
         const dim3 blk(256), grd((n + 255) / 256);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 1u);
+        if (guided) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + 8, 1u);
         hipLaunchKernelGGL(k_camera, grd, blk, 0, s, S, C, P, path0, n, ps, q[0], counts, rays_total);
         for (uint32_t gen = 0; gen < P.max_gen; ++gen) {
             const int cur = gen & 1, nxt = cur ^ 1;
@@ -813,8 +1021,33 @@ int lr_render(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, f
             } else if (int rc = lr::trace_closest_queue(sc->bvh, &rays, n, counts + cur, &hits, s)) {
                 return rc;
             }
-            hipLaunchKernelGGL(k_shade, grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
-                               sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total);
+            if (guided) {
+                hipLaunchKernelGGL(k_guide_inputs, grd, blk, 0, s, S, q[cur], counts + cur, H, ps, GA, g_in);
+                if (int rc = lr_guide_infer(guide, g_in, n, counts + cur, g_mix, s)) return rc;
+                hipLaunchKernelGGL(k_shade<true>, grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
+                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total, GA);
+            } else {
+                hipLaunchKernelGGL(k_shade<false>, grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
+                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total, GA);
+            }
+        }
+        if (guided && gparams->train && gparams->train_steps > 0) {
+            // recorded vertices -> training weights -> Adam steps
+            uint32_t nrec = 0;
+            LR_HIP_CHECK(hipMemcpyAsync(&nrec, counts + 8, sizeof(nrec), hipMemcpyDeviceToHost, s));
+            LR_HIP_CHECK(hipStreamSynchronize(s));
+            nrec = std::min(nrec, rec_cap);
+            if (nrec > 0) {
+                hipLaunchKernelGGL(k_guide_targets, dim3((nrec + 255) / 256), blk, 0, s, nrec, GA.r_aux, ps.L, r_w);
+                const uint32_t batch = std::min(nrec, gparams->train_batch ? gparams->train_batch : 65536u);
+                for (uint32_t st = 0; st < gparams->train_steps; ++st) {
+                    const uint32_t off = (uint32_t)(((uint64_t)st * batch) % nrec);
+                    const uint32_t nb = std::min(batch, nrec - off);
+                    if (int rc = lr_guide_train_step(guide, GA.r_in + 8 * (size_t)off, GA.r_dir + 3 * (size_t)off,
+                                                     r_w + off, nb, nullptr, nullptr, s))
+                        return rc;
+                }
+            }
         }
         hipLaunchKernelGGL(k_splat, dim3((npix + 255) / 256), blk, 0, s, S, P, pix0, npix, ps, film_xyz, film_w);
         LR_HIP_CHECK(hipGetLastError());
@@ -830,6 +1063,21 @@ int lr_render(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, f
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return LR_OK;
+}
+
+
+extern "C" {
+
+int lr_render(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, float* film_xyz, float* film_w,
+              lr_render_stats* stats, void* stream) {
+    return render_impl(sc, cam, prm, nullptr, nullptr, film_xyz, film_w, stats, stream);
+}
+
+int lr_render_guided(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, lr_guide* guide,
+                     const lr_guide_params* gparams, float* film_xyz, float* film_w, lr_render_stats* stats,
+                     void* stream) {
+    if (!guide) return lr::fail(LR_E_INVALID, "NULL guide");
+    return render_impl(sc, cam, prm, guide, gparams, film_xyz, film_w, stats, stream);
 }
 
 }  // extern "C"

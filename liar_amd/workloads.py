@@ -158,3 +158,57 @@ def terrain(n_side: int = 1581, n_spheres: int = 10000, seed: int = 7, tile: int
     cam.fovAngle = math.radians(55.0)
     cam.aspectRatio = 16.0 / 9.0
     return cs, cam
+
+
+def slit_room(n_side: int = 1581, n_spheres: int = 10000, seed: int = 7, slit_fraction: float = 0.02,
+              sun_scale: float = 5e4, sky: float = 0.2):
+    """The terrain workload enclosed in a closed box room (Lambert 0.6 walls,
+    floor below the terrain, ceiling at z = 0.45) whose ceiling has one slit
+    covering `slit_fraction` of its area; the sun (0.5 degree, 5800 K) and the
+    sky outside light the room only through the slit.  Camera inside the room.
+    Returns (CompiledScene, camera)."""
+    from . import _native as N
+    from .engine import CompiledScene
+
+    cs, _ = terrain(n_side, n_spheres, seed, sun_scale=sun_scale, sky=sky)
+    x0, x1, y0, y1, z0, z1 = -0.02, 1.02, -0.02, 1.02, -0.15, 0.45
+    wall_mat = len(cs.materials)
+    cs.spectra.append(spectra.Constant(0.6))
+    NONE = N.LR_NONE
+    cs.materials.append({"type": N.MAT_LAMBERT, "reflectance": len(cs.spectra) - 1, "transmittance": NONE,
+                         "eta": NONE, "kappa": NONE, "emission": NONE, "light": NONE, "roughness": 0.0})
+    quads = []
+
+    def quad(o, a, b):
+        o, a, b = (np.asarray(x, np.float32) for x in (o, a, b))
+        quads.append(np.stack([o, o + a, o + a + b, o + b]))
+
+    W, D, H = x1 - x0, y1 - y0, z1 - z0
+    quad((x0, y0, z0), (W, 0, 0), (0, D, 0))          # floor
+    quad((x0, y0, z0), (0, 0, H), (W, 0, 0))          # y = y0 wall
+    quad((x0, y1, z0), (W, 0, 0), (0, 0, H))          # y = y1 wall
+    quad((x0, y0, z0), (0, D, 0), (0, 0, H))          # x = x0 wall
+    quad((x1, y0, z0), (0, 0, H), (0, D, 0))          # x = x1 wall
+    # ceiling with a slit along y at x in [xs, xs + w]
+    w = slit_fraction * W
+    xs = 0.5 * (x0 + x1) - 0.5 * w + 0.15
+    quad((x0, y0, z1), (xs - x0, 0, 0), (0, D, 0))
+    quad((xs + w, y0, z1), (x1 - xs - w, 0, 0), (0, D, 0))
+    v = np.concatenate(quads).astype(np.float32)
+    base = len(cs.verts)
+    t = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    tris = np.concatenate([t + base + 4 * k for k in range(len(quads))])
+    nt_old = len(cs.tris)
+    cs.verts = np.concatenate([cs.verts, v])
+    cs.tris = np.concatenate([cs.tris, tris])
+    pm = cs.prim_material
+    cs.prim_material = np.concatenate([pm[:nt_old], np.full(len(tris), wall_mat, np.uint32), pm[nt_old:]])
+    # the sun shines down through the slit onto the terrain
+    cs.lights[0]["direction"] = (-0.25, 0.1, 0.963)
+    cam = cameras.PerspectiveCamera()
+    cam.position = (0.5, 0.02, 0.3)
+    cam.lookAt((0.5, 0.7, -0.02))
+    cam.sky = (0.0, 0.0, 1.0)
+    cam.fovAngle = math.radians(60.0)
+    cam.aspectRatio = 16.0 / 9.0
+    return cs, cam
