@@ -95,6 +95,81 @@ def bench_render(args, la, oracle, torch, np):
     return out
 
 
+def bench_guided(args, la, torch, np):
+    """Learned path guiding on the slit-room workload (terrain in a closed
+    room lit through a 2 % ceiling slit; camera on the indirectly lit half).
+    Plain and guided renders accumulate 4-spp passes; every render, the
+    reference included, clamps sample luminance at 0.5 (Raster's
+    maxSampleLuminance) so sun caustics do not dominate the MSE.  Reports the
+    wall time at which the guided render (training included) first reaches
+    the plain tracer's 64-spp MSE against a 4096-spp plain reference."""
+    from liar_amd import engine as E
+    from liar_amd import workloads as W
+    from liar_amd.guide import Guide
+
+    Wd, Hd, ML = args.guided_width, args.guided_width * 9 // 16, 0.5
+    cs, cam = W.slit_room()
+    ctx = la.Context(0)
+    gs = E.GpuScene(ctx, cs)
+    camera = E.camera_struct(cam)
+
+    def film():
+        return (torch.zeros((3, Hd, Wd), dtype=torch.float32, device="cuda"),
+                torch.zeros((Hd, Wd), dtype=torch.float32, device="cuda"))
+
+    def params(spp, seed):
+        return E.render_params(Wd, Hd, spp, 1, 16, 16, 0.5, (380.0, 780.0), seed, 0, ML)
+
+    fx, fw = film()
+    t0 = time.perf_counter()
+    for k in range(8):
+        gs.render(camera, params(512, 10000 + k), fx, fw)
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    ref = (fx / fw)[1].cpu().numpy().astype(np.float64)
+
+    def run(guide, passes=32, train_passes=8):
+        fx, fw = film()
+        # warm-up pass (allocation, code load) outside the clock
+        E.guided_render(gs, guide, camera, params(1, 1), *film()) if guide else gs.render(camera, params(1, 1), *film())
+        torch.cuda.synchronize()
+        elapsed, curve = 0.0, []
+        for p in range(passes):
+            t0 = time.perf_counter()
+            if guide is not None:
+                E.guided_render(gs, guide, camera, params(4, 100 + p), fx, fw, selection=0.9, train=p < train_passes,
+                                train_steps=16, train_batch=65536, light_share=0.1)
+            else:
+                gs.render(camera, params(4, 100 + p), fx, fw)
+            torch.cuda.synchronize()
+            elapsed += time.perf_counter() - t0
+            y = (fx / fw)[1].cpu().numpy()
+            curve.append((elapsed, 4 * (p + 1), float(((y - ref) ** 2).mean())))
+        return curve
+
+    plain = run(None)
+    g = Guide(ctx, seed=1, learning_rate=2e-3)
+    guided = run(g)
+    t_plain, mse_plain = plain[15][0], plain[15][2]
+    hit = next((t for t, _, m in guided if m <= mse_plain), None)
+    out = {
+        "workload": f"slit room (terrain 4,992,800 tris + 10k spheres in a closed box, 2% ceiling slit, sun + sky), "
+                    f"{Wd}x{Hd}, depth 16, sample luminance clamp {ML}",
+        "reference": f"plain 4096 spp ({t_ref:.1f} s)",
+        "plain_64spp_mse": mse_plain,
+        "plain_64spp_seconds": round(t_plain, 4),
+        "guided_seconds_to_plain_64spp_mse": None if hit is None else round(hit, 4),
+        "time_ratio_guided_over_plain": None if hit is None else round(hit / t_plain, 3),
+        "guided_64spp_mse": guided[15][2],
+        "equal_spp_mse_ratio_plain_over_guided": round(mse_plain / guided[15][2], 3),
+        "guided_config": "selection 0.9, light share 0.1, 8 training passes x 16 Adam steps (batch 65536, lr 2e-3)",
+    }
+    g.close()
+    gs.close()
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +181,8 @@ def main() -> int:
     ap.add_argument("--count-sample", type=int, default=1 << 18,
                     help="rays used for the oracle's nodes/prims-per-ray counts and parity")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracer leg")
+    ap.add_argument("--no-guided", action="store_true", help="skip the learned-guiding leg")
+    ap.add_argument("--guided-width", type=int, default=640)
     ap.add_argument("--render-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (PMC byte-counter calibration)")
@@ -212,6 +289,7 @@ def main() -> int:
         }
 
     render = None if args.no_render else bench_render(args, la, oracle, torch, np)
+    guided = None if args.no_guided else bench_guided(args, la, torch, np)
 
     out = {
         "metric": METRIC,
@@ -263,6 +341,7 @@ def main() -> int:
             "trace_config": ctx.trace_config(),
         },
         "render": render,
+        "guided": guided,
     }
     print(json.dumps(out), flush=True)
     return 0

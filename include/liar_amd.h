@@ -262,6 +262,9 @@ typedef struct lr_render_params {
     float lambda_min, lambda_max; /* wavelength sampling domain [nm]       */
     uint64_t seed;
     uint32_t chunk_paths;  /* paths per wavefront chunk (0: 4M)              */
+    float max_sample_luminance; /* Raster::maxSampleLuminance: a sample whose
+                                   Y exceeds it is scaled down to it
+                                   (raster.cpp:567-570); 0 = no clamp       */
 } lr_render_params;
 
 typedef struct lr_render_stats {
@@ -328,9 +331,10 @@ int lr_guide_train_step(lr_guide* guide, const float* inputs, const float* dirs,
 
 /* Guided rendering (no counterpart in LiAR): at diffuse / glossy vertices the
  * integrand's one-sample MIS gains a third strategy, sampling the guide's
- * vMF mixture, with probability `selection`; BSDF and light sampling share
- * the rest equally, and the power heuristic over all three keeps the
- * estimator unbiased for any network.  With `train`, the vertices of each
+ * vMF mixture: light sampling takes `light_share` (LiAR: 1/2), and the rest
+ * is split between the guide (`selection`) and BSDF sampling
+ * (1 - selection).  The power heuristic over all three keeps the estimator
+ * unbiased for any network.  With `train`, the vertices of each
  * chunk are recorded (up to max_records, 0: 2^20) and `train_steps` Adam
  * steps of `train_batch` records (0: 65536) run after the chunk on the
  * weights f|cos|/p_mix * L_incident. */
@@ -340,6 +344,12 @@ typedef struct lr_guide_params {
     uint32_t train_steps;
     uint32_t train_batch;
     uint32_t max_records;
+    float weight_clamp; /* training weights clamped at this multiple of their
+                           mean (0: no clamp); outliers such as sun caustics
+                           otherwise dominate the loss */
+    float light_share;  /* probability of light sampling at diffuse / glossy
+                           vertices (LiAR: 0.5); the guide and BSDF share the
+                           rest as selection : 1 - selection */
 } lr_guide_params;
 int lr_render_guided(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
                      lr_guide* guide, const lr_guide_params* guide_params, float* film_xyz,

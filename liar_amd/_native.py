@@ -76,7 +76,7 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("wavelengths", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("rr_survival", C.c_float), ("lambda_min", C.c_float), ("lambda_max", C.c_float),
-                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32)]
+                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32), ("max_sample_luminance", C.c_float)]
 
 
 class RenderStats(C.Structure):
@@ -85,7 +85,8 @@ class RenderStats(C.Structure):
 
 class GuideParams(C.Structure):
     _fields_ = [("selection", C.c_float), ("train", C.c_uint32), ("train_steps", C.c_uint32),
-                ("train_batch", C.c_uint32), ("max_records", C.c_uint32)]
+                ("train_batch", C.c_uint32), ("max_records", C.c_uint32), ("weight_clamp", C.c_float),
+                ("light_share", C.c_float)]
 
 
 LR_NONE = 0xFFFFFFFF

@@ -42,6 +42,7 @@ struct DevCamera {
 struct Params {
     uint32_t width, height, spp, nl, max_gen, rr_depth;
     float rr_p;
+    float max_lum;  // 0: no clamp
     uint32_t seed_lo, seed_hi;
     uint32_t strata;  // sqrt(spp) if spp is a square, else 0
 };
@@ -151,7 +152,8 @@ __global__ void k_fill_miss_queue(const uint32_t* count, Hits h) {
 struct GuideArgs {
     const float* in;   // [queue][8] network inputs (written by k_guide_inputs)
     const float* mix;  // [queue][40] mixture (lr_guide_infer)
-    float sel;         // probability of the guide strategy
+    float sel;         // guide share of the non-light strategies
+    float light_share; // probability of light sampling
     float lo[3], inv_ext[3];
     float lam_lo, inv_lam_ext;
     // training records (cap entries): inputs [8], direction [3], aux [4] =
@@ -227,6 +229,28 @@ __global__ void k_guide_targets(uint32_t n, const float* __restrict__ aux, const
     const float rest = L[pid] - lv;
     float v = (tn > 0.0f && rest > 0.0f) ? fcp * rest / tn : 0.0f;
     w[r] = isfinite(v) ? v : 0.0f;
+}
+
+// Sum of n floats in one block (fixed order).
+__global__ void k_sum_f32(const float* __restrict__ x, uint32_t n, float* __restrict__ out) {
+    __shared__ float sh[256];
+    float acc = 0.0f;
+    for (uint32_t r = threadIdx.x; r < n; r += 256) acc += x[r];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// w = min(w, clamp * mean(w)).
+__global__ void k_clamp_weights(uint32_t n, const float* __restrict__ sum, float clamp, float* __restrict__ w) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float cap = clamp * (*sum) / (float)n;
+    w[r] = fminf(w[r], cap);
 }
 
 // -- one generation: emission + scatter (adjoint_photon_tracer.cpp:140-292) --
@@ -309,9 +333,10 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
                 float rec_fcp = 0.0f;
                 if (GUIDED && strategy < 1.0f) {
                     // one-sample MIS over BSDF, light and guide sampling
-                    // (power heuristic over c_i p_i), c = (b, l, g)
+                    // (power heuristic over c_i p_i): light sampling takes
+                    // `light_share`, the guide `sel` of the rest
                     const float* mx = G.mix + (size_t)i * 40;
-                    const float cg = G.sel, cb = 0.5f * (1.0f - cg), cl = 0.5f * (1.0f - cg);
+                    const float cl = G.light_share, cg = (1.0f - cl) * G.sel, cb = (1.0f - cl) * (1.0f - G.sel);
                     V3 dw = v3(0, 0, 1);
                     float f = 0.0f, pb = 0.0f, pl = 0.0f, pg = 0.0f, pc = 0.0f;
                     if (u_strat < cb) {
@@ -467,7 +492,8 @@ __global__ void k_splat(DevScene S, Params P, uint64_t pix0, uint32_t npix, Path
         if (L == 0.0f) continue;
         float xyz[3];
         observer_xyz(S.obs, ps.lam[i], xyz);
-        const float s = L / ps.lpdf[i];  // Spectral::xyz, spectral.cpp:212-221
+        float s = L / ps.lpdf[i];  // Spectral::xyz, spectral.cpp:212-221
+        if (P.max_lum > 0.0f && s * xyz[1] > P.max_lum) s *= P.max_lum / (s * xyz[1]);  // raster.cpp:567-570
         X += s * xyz[0];
         Y += s * xyz[1];
         Z += s * xyz[2];
@@ -922,6 +948,8 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     P.max_gen = R.max_depth ? R.max_depth : 64u;
     P.rr_depth = R.rr_depth;
     P.rr_p = R.rr_survival;
+    P.max_lum = R.max_sample_luminance;
+    if (!(P.max_lum >= 0.0f)) return lr::fail(LR_E_INVALID, "max_sample_luminance must be >= 0");
     P.seed_lo = (uint32_t)R.seed;
     P.seed_hi = (uint32_t)(R.seed >> 32);
     {
@@ -954,6 +982,9 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         GA.in = g_in;
         GA.mix = g_mix;
         GA.sel = gparams->selection;
+        GA.light_share = gparams->light_share;
+        if (!(GA.light_share >= 0.0f && GA.light_share < 1.0f))
+            return lr::fail(LR_E_INVALID, "guide light_share must be in [0, 1)");
         const lr_qbvh_info& I = sc->bvh->info;
         for (int k = 0; k < 3; ++k) {
             GA.lo[k] = I.scene_lo[k];
@@ -1039,6 +1070,12 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             nrec = std::min(nrec, rec_cap);
             if (nrec > 0) {
                 hipLaunchKernelGGL(k_guide_targets, dim3((nrec + 255) / 256), blk, 0, s, nrec, GA.r_aux, ps.L, r_w);
+                if (gparams->weight_clamp > 0.0f) {
+                    float* wsum = reinterpret_cast<float*>(counts + 12);
+                    hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, s, r_w, nrec, wsum);
+                    hipLaunchKernelGGL(k_clamp_weights, dim3((nrec + 255) / 256), blk, 0, s, nrec, wsum,
+                                       gparams->weight_clamp, r_w);
+                }
                 const uint32_t batch = std::min(nrec, gparams->train_batch ? gparams->train_batch : 65536u);
                 for (uint32_t st = 0; st < gparams->train_steps; ++st) {
                     const uint32_t off = (uint32_t)(((uint64_t)st * batch) % nrec);

@@ -217,12 +217,14 @@ class GpuScene:
 
 
 def guided_render(gs: "GpuScene", guide, camera: N.Camera, params: N.RenderParams, film_xyz, film_w,
-                  selection=0.5, train=False, train_steps=0, train_batch=65536, max_records=0, stream=None) -> dict:
+                  selection=0.5, train=False, train_steps=0, train_batch=65536, max_records=0, weight_clamp=0.0,
+                  light_share=0.5, stream=None) -> dict:
     """lr_render_guided on a GpuScene with a liar_amd.guide.Guide."""
     import torch
 
     s = torch.cuda.current_stream(film_xyz.device).cuda_stream if stream is None else stream
-    gp = N.GuideParams(float(selection), int(bool(train)), int(train_steps), int(train_batch), int(max_records))
+    gp = N.GuideParams(float(selection), int(bool(train)), int(train_steps), int(train_batch), int(max_records),
+                       float(weight_clamp), float(light_share))
     st = N.RenderStats()
     N.check(N.lib().lr_render_guided(gs._h, C.byref(camera), C.byref(params), guide.handle, C.byref(gp),
                                      C.c_void_p(film_xyz.data_ptr()), C.c_void_p(film_w.data_ptr()), C.byref(st),
@@ -231,10 +233,10 @@ def guided_render(gs: "GpuScene", guide, camera: N.Camera, params: N.RenderParam
 
 
 def render_params(width, height, spp, wavelengths=1, max_depth=0, rr_depth=8, rr_survival=0.5,
-                  lambda_range=(380.0, 780.0), seed=1, chunk_paths=0) -> N.RenderParams:
+                  lambda_range=(380.0, 780.0), seed=1, chunk_paths=0, max_sample_luminance=0.0) -> N.RenderParams:
     return N.RenderParams(int(width), int(height), int(spp), int(wavelengths), int(max_depth), int(rr_depth),
                           float(rr_survival), float(lambda_range[0]), float(lambda_range[1]), int(seed),
-                          int(chunk_paths))
+                          int(chunk_paths), float(max_sample_luminance))
 
 
 class RenderEngine:

@@ -203,12 +203,14 @@ def slit_room(n_side: int = 1581, n_spheres: int = 10000, seed: int = 7, slit_fr
     cs.tris = np.concatenate([cs.tris, tris])
     pm = cs.prim_material
     cs.prim_material = np.concatenate([pm[:nt_old], np.full(len(tris), wall_mat, np.uint32), pm[nt_old:]])
-    # the sun shines down through the slit onto the terrain
+    # the sun shines down through the slit onto a stripe of terrain near
+    # x = 0.76; the camera looks at the x < 0.45 half of the room, so every
+    # visible point is lit only indirectly (stripe, walls, ceiling)
     cs.lights[0]["direction"] = (-0.25, 0.1, 0.963)
     cam = cameras.PerspectiveCamera()
-    cam.position = (0.5, 0.02, 0.3)
-    cam.lookAt((0.5, 0.7, -0.02))
+    cam.position = (0.42, 0.02, 0.3)
+    cam.lookAt((0.15, 0.75, -0.02))
     cam.sky = (0.0, 0.0, 1.0)
-    cam.fovAngle = math.radians(60.0)
+    cam.fovAngle = math.radians(50.0)
     cam.aspectRatio = 16.0 / 9.0
     return cs, cam

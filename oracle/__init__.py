@@ -194,7 +194,7 @@ class _Params(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("wavelengths", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("rr_survival", C.c_float), ("lambda_min", C.c_float), ("lambda_max", C.c_float),
-                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32)]
+                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32), ("max_sample_luminance", C.c_float)]
 
 
 def _spec_struct(sp, keep):
@@ -217,7 +217,7 @@ def _spec_struct(sp, keep):
 
 
 def render(compiled, camera, width, height, spp, wavelengths=1, max_depth=0, rr_depth=8, rr_survival=0.5,
-           lambda_range=(380.0, 780.0), seed=1, observer=None, threads=None):
+           lambda_range=(380.0, 780.0), seed=1, observer=None, threads=None, max_sample_luminance=0.0):
     """FP64 CPU render of a compiled scene -> (xyz sums (3, h, w), weight (h, w))."""
     L = lib()
     if not hasattr(L, "_render_sig"):
@@ -252,7 +252,7 @@ def render(compiled, camera, width, height, spp, wavelengths=1, max_depth=0, rr_
     cam.sky[:] = [float(x) for x in camera.sky]
     cam.fov, cam.aspect, cam.near_limit = float(camera.fovAngle), float(camera.aspectRatio), float(camera.nearLimit)
     P = _Params(width, height, spp, wavelengths, max_depth, rr_depth, rr_survival, lambda_range[0],
-                lambda_range[1], seed, 0)
+                lambda_range[1], seed, 0, max_sample_luminance)
     if observer is None:
         from liar_amd.observer import standard_observer
         observer = standard_observer()

@@ -61,6 +61,7 @@ typedef struct {
     float rr_survival, lambda_min, lambda_max;
     uint64_t seed;
     uint32_t chunk_paths;
+    float max_sample_luminance;
 } or_params;
 
 typedef struct { double x, y, z; } V;
@@ -563,9 +564,12 @@ static void* run_rows(void* arg) {
                     if (L == 0) continue;
                     double cmf[3];
                     obs_xyz(c->S, lam, cmf);
-                    X += L / lpdf * cmf[0];
-                    Y += L / lpdf * cmf[1];
-                    Z += L / lpdf * cmf[2];
+                    double sc = L / lpdf;
+                    if (P->max_sample_luminance > 0 && sc * cmf[1] > P->max_sample_luminance)
+                        sc *= P->max_sample_luminance / (sc * cmf[1]);  /* raster.cpp:567-570 */
+                    X += sc * cmf[0];
+                    Y += sc * cmf[1];
+                    Z += sc * cmf[2];
                 }
             }
             J->xyz[pixel] += X;
