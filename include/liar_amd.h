@@ -302,14 +302,19 @@ int lr_render(lr_scene* scene, const lr_camera* camera, const lr_render_params* 
 /* ---- path-guiding network (no counterpart in LiAR) -------------------------
  * A learned proposal for the integrand's direction sampling: an MLP
  * (6 inputs -> 64-wide frequency encoding -> 5 hidden layers of 64, ReLU ->
- * 40 outputs) in BF16 on MFMA with FP32 accumulation, whose outputs are an
- * 8-lobe von Mises-Fisher mixture over incident directions.  Trained online
- * with Adam on radiance samples the tracer produces.
+ * 48 outputs) in BF16 on MFMA with FP32 accumulation.  Outputs 0..39 are an
+ * 8-lobe von Mises-Fisher mixture over incident directions, 40 and 41 the
+ * logits of the per-vertex strategy probabilities (light, guide, BSDF) =
+ * 0.025 + 0.925 (r_l, (1 - r_l) r_g, (1 - r_l)(1 - r_g)), r = sigmoid.
+ * Trained online with Adam on radiance samples the tracer produces.
  * Inputs: device float [n][8], per row (px, py, pz, theta_o/pi, phi_o/(2 pi),
- * (lambda - lambda_min)/(lambda_max - lambda_min), -, -), all in [0, 1].
- * Mixture: device float [n][40], per lobe (weight, mean x, y, z, kappa).
+ * (lambda - lambda_min)/(lambda_max - lambda_min), p_light, p_bsdf), the
+ * first six in [0, 1]; the last two are read by training only (below).
+ * Mixture: device float [n][48], per lobe (weight = softmax of the logits,
+ * unit mean, kappa = 0.1 + e^min(s, 7.6)), then
+ * (c_light, c_guide, c_bsdf, 0, 0, 0, 0, 0).
  * Parameters (lr_guide_param_count() floats): per layer l (0..5) Wt_l
- * [n_out][64] row-major (n_out = 64, output layer 48 with rows 40..47 unused),
+ * [n_out][64] row-major (n_out = 64, output layer 48 with rows 42..47 unused),
  * then the biases (5 x 64 + 48). */
 typedef struct lr_guide lr_guide;
 int lr_guide_param_count(void);
@@ -320,7 +325,10 @@ int lr_guide_set_params(lr_guide* guide, const float* host_params);
 /* n_dev: optional device count (<= n) read by the kernel. */
 int lr_guide_infer(lr_guide* guide, const float* inputs, uint32_t n, const uint32_t* n_dev,
                    float* mixture, void* stream);
-/* One Adam step on -sum_i w_i log q(dir_i | input_i) / sum_i w_i.
+/* One Adam step on sum_i w_i (-log q(dir_i | input_i) - [p_l + p_b > 0]
+ * log(c_l p_l + c_g q + c_b p_b)) / sum_i w_i, where p_l, p_b are inputs 6
+ * and 7 of row i (the light and BSDF sampling pdfs of dir_i) and q is held
+ * fixed inside the second term, which trains the strategy probabilities.
  * dirs: device float [n][3] unit vectors; weights: device float [n] >= 0.
  * loss_out (device float, optional): the summed normalised loss.
  * grad_out (device float [param_count], optional): the gradient.  A guide
@@ -333,10 +341,13 @@ int lr_guide_train_step(lr_guide* guide, const float* inputs, const float* dirs,
  * integrand's one-sample MIS gains a third strategy, sampling the guide's
  * vMF mixture: light sampling takes `light_share` (LiAR: 1/2), and the rest
  * is split between the guide (`selection`) and BSDF sampling
- * (1 - selection).  The power heuristic over all three keeps the estimator
- * unbiased for any network.  With `train`, the vertices of each
- * chunk are recorded (up to max_records, 0: 2^20) and `train_steps` Adam
- * steps of `train_batch` records (0: 65536) run after the chunk on the
+ * (1 - selection), or the network sets all three per vertex
+ * (learned_selection).  The power (or balance) heuristic over all three keeps
+ * the estimator unbiased for any network, since the probabilities depend on
+ * the vertex only and each is at least 0.025 when learned.  With `train`, the vertices of each
+ * chunk are recorded (up to max_records, 0: 2^22) and `train_steps` Adam
+ * steps, each on `train_batch` records (0: 65536) drawn uniformly with
+ * replacement from all recorded ones, run after the chunk on the
  * weights f|cos|/p_mix * L_incident. */
 typedef struct lr_guide_params {
     float selection;
@@ -350,6 +361,18 @@ typedef struct lr_guide_params {
     float light_share;  /* probability of light sampling at diffuse / glossy
                            vertices (LiAR: 0.5); the guide and BSDF share the
                            rest as selection : 1 - selection */
+    uint32_t learned_selection; /* nonzero: the network's per-vertex strategy
+                           probabilities (outputs 40, 41; each >= 0.025) replace
+                           selection / light_share, and training fits them to
+                           the recorded radiance (KL of the strategy mixture) */
+    uint32_t balance_heuristic; /* nonzero: balance heuristic (the optimal
+                           one-sample MIS weight) instead of LiAR's power heuristic */
+    uint32_t image_weighting; /* nonzero: each training record's weight is also
+                           multiplied by the path throughput before its vertex
+                           (its share of the pixel), so vertices that matter to
+                           the image dominate the fit rather than the brightest */
+    uint32_t record_depth; /* record only the vertices of the first record_depth
+                           bounces (0: all) */
 } lr_guide_params;
 int lr_render_guided(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
                      lr_guide* guide, const lr_guide_params* guide_params, float* film_xyz,

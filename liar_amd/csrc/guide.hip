@@ -5,9 +5,13 @@
 // weights with Adam.
 //
 // Kernels
-//   k_infer       fully fused forward: weights in LDS, each wave runs 16-row
-//                 tiles through all layers, activations stay in LDS; outputs
-//                 the processed mixture (weights, unit means, concentrations)
+//   k_infer       fully fused forward on v_mfma_f32_16x16x32_bf16 in the
+//                 transposed form Z^T = W^T-tiles x X^T: the accumulator of
+//                 one layer (lane = row, 4 consecutive features) is, after
+//                 bias + ReLU + bf16 packing, exactly the B operand of the
+//                 next, so activations never leave registers; weights sit in
+//                 LDS in per-lane fragment order; each lane ends holding two
+//                 whole lobes and the softmax needs two cross-lane exchanges
 //   k_train_fb    fused forward + loss + backward data path for 16-row tiles;
 //                 writes activations A_l and gradients dZ_l feature-major
 //                 (bf16) for the weight-gradient GEMMs
@@ -17,7 +21,10 @@
 //
 // MFMA 16x16x16 bf16 lane layout (wave64, lane = i + 16 j): A operand lane
 // holds A[i][4j..4j+3], B operand lane holds B[4j..4j+3][i], accumulator lane
-// holds D[4j..4j+3][i].
+// holds D[4j..4j+3][i].  16x16x32 is the same with 8 K values per lane
+// (A[i][8j..8j+7], B[8j..8j+7][i]) and the same accumulator layout; the
+// contraction is order-free, so A and B may place any K permutation in those
+// slots as long as both use the same one.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -31,13 +38,17 @@
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = 6;        // raw inputs
 constexpr int kW = 64;        // width (encoding and hidden)
 constexpr int kHidden = 5;    // hidden layers
 constexpr int kOut = 40;      // 8 lobes x (logit, mean xyz, concentration)
+constexpr int kSel = 2;       // strategy-selection logits (outputs 40, 41)
 constexpr int kOutPad = 48;   // padded to whole 16-column tiles
+constexpr int kMix = 48;      // floats per inferred row: 40 lobe values, c_light, c_guide, c_bsdf, 0 x 5
+constexpr float kSelFloor = 0.025f;  // least probability of each strategy
 constexpr int kLobes = 8;
 constexpr int kLayers = kHidden + 1;  // weight matrices
 constexpr int kFreq = 4;      // frequencies per input
@@ -53,6 +64,41 @@ constexpr int kParams = kWtElems + kBiasElems;                   // 23920
 __host__ __device__ constexpr int layer_nout(int l) { return l < kHidden ? kW : kOutPad; }
 __host__ __device__ constexpr int wt_offset(int l) { return l * kW * kW; }
 __host__ __device__ constexpr int bias_offset(int l) { return kWtElems + l * kW; }
+
+// Fragment order of the bf16 weights for k_infer.  Fragment (l, t, kb) is the
+// 16x32 A operand of layer l, output tile t (16 rows), K block kb (32 inputs):
+// 64 lanes x 8 bf16, lane (i, j) element e holding W_l[out 16t+i][in kslot].
+// K slot (kb, j, e) is input 32kb + 4j + e for e < 4 and 32kb + 16 + 4j + e - 4
+// otherwise: the features an accumulator pair (tiles 2kb, 2kb+1) leaves in
+// lane j.  The output layer's physical rows are permuted so that lane j of a
+// row ends with lobes 2j and 2j+1: physical row 16t + m holds logical output
+// 10(m / 4) + 4t + m % 4 when 4t + m % 4 < 10, and 40 + 2(m / 4) + m % 4 - 2
+// otherwise (so lane 0 also holds the selection logits 40 and 41).
+constexpr int kFrags = kHidden * 4 * 2 + 3 * 2;  // 46 fragments of 512 bf16
+static_assert(kFrags * 512 == kWtElems, "fragment count");
+__host__ __device__ constexpr int frag_id(int l, int t, int kb) { return l < kHidden ? (l * 4 + t) * 2 + kb : 40 + t * 2 + kb; }
+__host__ __device__ constexpr int kslot_feature(int kb, int j, int e) { return 32 * kb + (e < 4 ? 4 * j + e : 16 + 4 * j + e - 4); }
+// logical output of the output layer's physical row p
+__host__ __device__ constexpr int out_logical(int p) {
+    return (4 * (p >> 4) + (p & 3)) < 10 ? 10 * ((p & 15) >> 2) + 4 * (p >> 4) + (p & 3)
+                                         : kOut + 2 * ((p & 15) >> 2) + (p & 3) - 2;
+}
+// fragment-order index of logical weight Wt_l[n][k]
+__host__ __device__ inline int frag_index(int l, int n, int k) {
+    const int kb = k >> 5, c = k & 31;
+    const int j = (c & 15) >> 2, e = (c & 3) + (c >= 16 ? 4 : 0);
+    int t, i;
+    if (l < kHidden) {
+        t = n >> 4;
+        i = n & 15;
+    } else {
+        // lane group and slot of this output
+        const int jj = n < kOut ? n / 10 : (n - kOut) >> 1, sl = n < kOut ? n % 10 : 10 + ((n - kOut) & 1);
+        t = sl >> 2;
+        i = 4 * jj + (sl & 3);
+    }
+    return frag_id(l, t, kb) * 512 + (i + 16 * j) * 8 + e;
+}
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -70,8 +116,13 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
-__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
+// Lobe concentration from its raw output s: kappa = 0.1 + e^min(s, 7.6)
+// (0.1 .. ~2000).  The log scale lets a lobe sharpen to the ~0.01 sr targets
+// of small bright sources in few steps; the cap keeps e^{kappa (c - 1)}
+// consistent between sampling and evaluation in FP32.
+constexpr float kKappaLogMax = 7.6f;
+__device__ __forceinline__ float kappa_of(float s) { return 0.1f + expf(fminf(s, kKappaLogMax)); }
+__device__ __forceinline__ float dkappa_ds(float s) { return s < kKappaLogMax ? expf(s) : 0.0f; }
 
 // Frequency encoding of input row v[6] (each in [0, 1]) into 64 features:
 // f < 6 raw (2v - 1); 6 <= f < 54 sin / cos(2^k pi v_d), k < 4; f = 54 is 1;
@@ -82,8 +133,9 @@ __device__ __forceinline__ float encode(const float* v, int f) {
         const int idx = f - kIn;
         const int d = idx / (2 * kFreq);
         const int k = (idx % (2 * kFreq)) >> 1;
-        const float a = 3.14159265358979f * (float)(1 << k) * v[d];
-        return (idx & 1) ? cosf(a) : sinf(a);
+        // sin / cos(2^k pi v) = v_sin / v_cos of 2^(k-1) v revolutions (|arg| <= 4)
+        const float rev = 0.5f * (float)(1 << k) * v[d];
+        return (idx & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
     }
     return f == kIn + kIn * 2 * kFreq ? 1.0f : 0.0f;
 }
@@ -147,7 +199,7 @@ __device__ __forceinline__ void forward_hidden(const uint16_t* s_w, const float*
 }
 
 // Mixture processing of one row's 40 raw outputs (fp32) -> weights pi,
-// unit means, concentrations kappa = softplus(s) + 0.1.
+// unit means, concentrations kappa = 0.1 + e^min(s, 7.6).
 struct Lobe {
     float pi, mx, my, mz, kappa;
 };
@@ -167,7 +219,7 @@ __device__ inline void process_row(const float* o, Lobe* L) {
         L[k].mx = x / n;
         L[k].my = y / n;
         L[k].mz = z / n;
-        L[k].kappa = softplus(o[5 * k + 4]) + 0.1f;
+        L[k].kappa = kappa_of(o[5 * k + 4]);
     }
 }
 
@@ -183,80 +235,154 @@ constexpr int kBlock = 64 * kWaves;
 // ---------------------------------------------------------------------------
 // inference
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_infer(const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
-                                                  const float* __restrict__ in, uint32_t n,
-                                                  const uint32_t* __restrict__ n_dev, float* __restrict__ mix) {
+constexpr int kInferWaves = 4;
+constexpr int kInferBlock = 64 * kInferWaves;
+constexpr int kInferRT = 2;  // 16-row tiles per wave iteration (share each weight fragment)
+constexpr int kInferRows = 16 * kInferRT;
+constexpr size_t kInferSmem = sizeof(uint4) * kFrags * 64 + sizeof(float) * kBiasElems;
+
+__device__ __forceinline__ bf16x8 pack_relu(const f32x4& lo, const f32x4& hi) {
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        r[e] = (__bf16)fmaxf(lo[e], 0.0f);
+        r[e + 4] = (__bf16)fmaxf(hi[e], 0.0f);
+    }
+    return r;
+}
+
+__device__ __forceinline__ f32x4 bias4(const float* s_b, int idx) {
+    const float4 b = *reinterpret_cast<const float4*>(s_b + idx);
+    return f32x4{b.x, b.y, b.z, b.w};
+}
+
+// Strategy probabilities from the two selection logits: r_l = sigmoid(a0),
+// r_g = sigmoid(a1); (light, guide, BSDF) = floor + (1 - 3 floor) *
+// (r_l, (1 - r_l) r_g, (1 - r_l)(1 - r_g)).
+__host__ __device__ inline void selection_probs(float a0, float a1, float* c) {
+    const float rl = 1.0f / (1.0f + expf(-a0)), rg = 1.0f / (1.0f + expf(-a1));
+    const float sc = 1.0f - 3.0f * kSelFloor;
+    c[0] = kSelFloor + sc * rl;
+    c[1] = kSelFloor + sc * (1.0f - rl) * rg;
+    c[2] = kSelFloor + sc * (1.0f - rl) * (1.0f - rg);
+}
+
+// Per lane: the two lobes (logit, mean xyz, raw concentration) of one row in
+// o[0..9] (lane 0: selection logits in o[10..11]); the softmax over the row's
+// 8 logits spans lanes i, i+16, i+32, i+48.
+__device__ __forceinline__ void write_lobes(const float* o, bool valid, bool sel_lane, float* __restrict__ m) {
+    float mx = fmaxf(o[0], o[5]);
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float ea = __expf(o[0] - mx), eb = __expf(o[5] - mx);
+    float sum = ea + eb;
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+    if (!valid) return;
+    const float inv = 1.0f / sum;
+    float r[10];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float* q = o + 5 * h;
+        const float nrm = 1.0f / (sqrtf(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + 1e-12f);
+        r[5 * h + 0] = (h ? eb : ea) * inv;
+        r[5 * h + 1] = q[1] * nrm;
+        r[5 * h + 2] = q[2] * nrm;
+        r[5 * h + 3] = q[3] * nrm;
+        r[5 * h + 4] = kappa_of(q[4]);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) reinterpret_cast<float2*>(m)[c] = make_float2(r[2 * c], r[2 * c + 1]);
+    if (sel_lane) {  // lane 0 of the row: floats 40..47 of the row
+        float c[3];
+        selection_probs(o[10], o[11], c);
+        float4* s4 = reinterpret_cast<float4*>(m + kOut);
+        s4[0] = make_float4(c[0], c[1], c[2], 0.0f);
+        s4[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// mix[row][48] = lobes (pi, mu xyz, kappa) x 8, then (c_light, c_guide,
+// c_bsdf, 0 x 5), for rows < n (n from n_dev when given).  frag: the fragment-order bf16 weights; bias: FP32 biases in the
+// master layout (the output layer's are permuted here).
+__global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restrict__ frag, const float* __restrict__ bias,
+                                                       const float* __restrict__ in, uint32_t n,
+                                                       const uint32_t* __restrict__ n_dev, float* __restrict__ mix) {
     extern __shared__ uint8_t smem[];
-    uint16_t* s_w = reinterpret_cast<uint16_t*>(smem);
-    float* s_b = reinterpret_cast<float*>(smem + sizeof(uint16_t) * kLayers * kW * kWS);
-    uint8_t* wave_base = smem + sizeof(uint16_t) * kLayers * kW * kWS + sizeof(float) * kBiasElems;
+    uint4* s_f = reinterpret_cast<uint4*>(smem);
+    float* s_b = reinterpret_cast<float*>(smem + sizeof(uint4) * kFrags * 64);
     if (n_dev) n = *n_dev;
-    load_weights(wbf, bias, s_w, s_b);
+    if (blockIdx.x * (uint32_t)(kInferWaves * kInferRows) >= n) return;  // no tile for this block
+    for (int e = threadIdx.x; e < kFrags * 64; e += kInferBlock) s_f[e] = reinterpret_cast<const uint4*>(frag)[e];
+    for (int e = threadIdx.x; e < kBiasElems; e += kInferBlock) {
+        const int p = e - kHidden * kW;
+        s_b[e] = p < 0 ? bias[e] : bias[kHidden * kW + out_logical(p)];
+    }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
-    // per wave: two ping-pong activation buffers (16 x kWS bf16) + outputs (16 x 48 fp32)
-    const size_t act_bytes = sizeof(uint16_t) * kTileRows * kWS;
-    uint8_t* wb = wave_base + wave * (2 * act_bytes + sizeof(float) * kTileRows * kOutPad);
-    uint16_t* bufA = reinterpret_cast<uint16_t*>(wb);
-    uint16_t* bufB = reinterpret_cast<uint16_t*>(wb + act_bytes);
-    float* s_out = reinterpret_cast<float*>(wb + 2 * act_bytes);
-    const uint32_t tiles = (n + kTileRows - 1) / kTileRows;
-    for (uint32_t tile = blockIdx.x * kWaves + wave; tile < tiles; tile += gridDim.x * kWaves) {
-        const uint32_t row0 = tile * kTileRows;
-        // encoding: lane (i, j) -> row i, features 16 j .. 16 j + 15
-        {
-            const uint32_t row = row0 + i;
-            float v[kIn];
-            for (int d = 0; d < kIn; ++d) v[d] = row < n ? in[(size_t)row * 8 + d] : 0.0f;
-            for (int f = 16 * j; f < 16 * j + 16; ++f) bufA[i * kWS + f] = f2bf(encode(v, f));
-        }
-        wave_sync();
-        uint16_t* cur = bufA;
-        uint16_t* nxt = bufB;
-        for (int l = 0; l < kHidden; ++l) {
-            const uint16_t* wl = s_w + l * kW * kWS;
-            f32x4 acc[4];
+    const uint32_t tiles = (n + kInferRows - 1) / kInferRows;
+    for (uint32_t tile = blockIdx.x * kInferWaves + wave; tile < tiles; tile += gridDim.x * kInferWaves) {
+        // keeps the fragment reads inside the loop: hoisted, the 46 fragments
+        // would pin 184 registers and cut occupancy to one wave per SIMD
+        asm volatile("" ::: "memory");
+        bf16x8 x[kInferRT][2];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = tile_fwd(cur, wl, 16 * t, i, j);
-            wave_sync();
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    nxt[(4 * j + r) * kWS + 16 * t + i] = f2bf(fmaxf(acc[t][r] + s_b[l * kW + 16 * t + i], 0.0f));
-            wave_sync();
-            uint16_t* tmp = cur;
-            cur = nxt;
-            nxt = tmp;
-        }
-        {
-            const uint16_t* wl = s_w + kHidden * kW * kWS;
-#pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                const f32x4 acc = tile_fwd(cur, wl, 16 * t, i, j);
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    s_out[(4 * j + r) * kOutPad + 16 * t + i] = acc[r] + s_b[kHidden * kW + 16 * t + i];
-            }
-        }
-        wave_sync();
-        // lanes 0..15 turn their row's outputs into mixture parameters
-        if (j == 0) {
-            const uint32_t row = row0 + i;
+        for (int rt = 0; rt < kInferRT; ++rt) {
+            const uint32_t row = tile * kInferRows + 16 * rt + i;
+            float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             if (row < n) {
-                Lobe L[kLobes];
-                process_row(s_out + i * kOutPad, L);
-                float* m = mix + (size_t)row * kOut;
-                for (int k = 0; k < kLobes; ++k) {
-                    m[5 * k + 0] = L[k].pi;
-                    m[5 * k + 1] = L[k].mx;
-                    m[5 * k + 2] = L[k].my;
-                    m[5 * k + 3] = L[k].mz;
-                    m[5 * k + 4] = L[k].kappa;
+                const float4 a = reinterpret_cast<const float4*>(in)[2 * (size_t)row];
+                const float4 b = reinterpret_cast<const float4*>(in)[2 * (size_t)row + 1];
+                v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y;
+            }
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) x[rt][kb][e] = (__bf16)encode(v, kslot_feature(kb, j, e));
+        }
+        f32x4 acc[kInferRT][4];
+#pragma unroll
+        for (int l = 0; l < kHidden; ++l) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const f32x4 b = bias4(s_b, l * kW + 16 * t + 4 * j);
+#pragma unroll
+                for (int rt = 0; rt < kInferRT; ++rt) acc[rt][t] = b;
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb) {
+                    const bf16x8 w = __builtin_bit_cast(bf16x8, s_f[frag_id(l, t, kb) * 64 + lane]);
+#pragma unroll
+                    for (int rt = 0; rt < kInferRT; ++rt)
+                        acc[rt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, x[rt][kb], acc[rt][t], 0, 0, 0);
                 }
             }
+#pragma unroll
+            for (int rt = 0; rt < kInferRT; ++rt) {
+                x[rt][0] = pack_relu(acc[rt][0], acc[rt][1]);
+                x[rt][1] = pack_relu(acc[rt][2], acc[rt][3]);
+            }
         }
-        wave_sync();
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const f32x4 b = bias4(s_b, kHidden * kW + 16 * t + 4 * j);
+#pragma unroll
+            for (int rt = 0; rt < kInferRT; ++rt) acc[rt][t] = b;
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb) {
+                const bf16x8 w = __builtin_bit_cast(bf16x8, s_f[frag_id(kHidden, t, kb) * 64 + lane]);
+#pragma unroll
+                for (int rt = 0; rt < kInferRT; ++rt)
+                    acc[rt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, x[rt][kb], acc[rt][t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int rt = 0; rt < kInferRT; ++rt) {
+            const uint32_t row = tile * kInferRows + 16 * rt + i;
+            float o[12];
+#pragma unroll
+            for (int sl = 0; sl < 12; ++sl) o[sl] = acc[rt][sl >> 2][sl & 3];
+            write_lobes(o, row < n, j == 0, mix + (size_t)row * kMix + 10 * j);
+        }
     }
 }
 
@@ -331,6 +457,21 @@ __global__ __launch_bounds__(kBlock) void k_train_fb(const uint16_t* __restrict_
                 const float logq = lmax + logf(s);
                 const float w = weight[row] * wnorm;
                 loss = -w * logq;
+                // strategy selection: -w log(c_l p_l + c_g p_g + c_b p_b) with the
+                // record's light / BSDF pdfs (inputs 6, 7) and the guide pdf held
+                // fixed; only when the record carries them (p_l + p_b > 0)
+                const float pl = in[(size_t)row * 8 + 6], pb = in[(size_t)row * 8 + 7];
+                if (pl + pb > 0.0f) {
+                    const float pg = expf(logq);
+                    float c[3];
+                    selection_probs(o[kOut], o[kOut + 1], c);
+                    const float pmix = c[0] * pl + c[1] * pg + c[2] * pb;
+                    const float rl = 1.0f / (1.0f + expf(-o[kOut])), rg = 1.0f / (1.0f + expf(-o[kOut + 1]));
+                    const float f = -w * (1.0f - 3.0f * kSelFloor) / pmix;
+                    loss -= w * logf(pmix);
+                    g[kOut] = f * rl * (1.0f - rl) * (pl - rg * pg - (1.0f - rg) * pb);
+                    g[kOut + 1] = f * (1.0f - rl) * rg * (1.0f - rg) * (pg - pb);
+                }
                 for (int k = 0; k < kLobes; ++k) {
                     const float resp = expf(lp[k] - logq);  // responsibility
                     const float c = L[k].mx * wx + L[k].my * wy + L[k].mz * wz;
@@ -340,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_train_fb(const uint16_t* __restrict_
                     // d(-log q)/d s_k:  d log vmf / d kappa = 1/k - 2/(e^{2k} - 1) + (c - 1)
                     const float e2 = expm1f(2.0f * kap);
                     const float dlk = 1.0f / kap - 2.0f / e2 + (c - 1.0f);
-                    g[5 * k + 4] = -w * resp * dlk * sigmoid(o[5 * k + 4]);
+                    g[5 * k + 4] = -w * resp * dlk * dkappa_ds(o[5 * k + 4]);
                     // d(-log q)/d m_k = -resp * kappa * (I - mu mu^T) w / |m|
                     const float x = o[5 * k + 1], y = o[5 * k + 2], z = o[5 * k + 3];
                     const float nm = sqrtf(x * x + y * y + z * z) + 1e-12f;
@@ -419,7 +560,9 @@ __global__ __launch_bounds__(kBlock) void k_train_fb(const uint16_t* __restrict_
 
 // dWt_l[n][k] = sum_rows G_l[n][row] * A_l[k][row] over this block's row chunk;
 // bias gradient db_l[n] = sum_rows G_l[n][row].  One block per (layer, chunk);
-// wave w computes output rows n in [16 w, 16 w + 16) for all 4 k tiles.
+// wave w computes output rows n in [16 w, 16 w + 16) for all 4 k tiles.  The
+// bias sums come from the G values each lane already holds as its MFMA A
+// operand, reduced over the 4 lanes of a row at the end.
 __global__ __launch_bounds__(kBlock) void k_wgrad(const uint16_t* __restrict__ A, const uint16_t* __restrict__ G,
                                                   uint32_t n_pad, uint32_t chunk_rows, float* __restrict__ partial) {
     const int l = blockIdx.y;
@@ -442,9 +585,11 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const uint16_t* __restrict__ A
                 const bf16x4 b = ld4(Al + (size_t)(16 * t + i) * n_pad + row + 4 * j);  // B_mfma[row][k]
                 acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
             }
-            if (j == 0)
-                for (int r = 0; r < 16; ++r) bsum += bf2f(Gl[(size_t)(16 * wave + i) * n_pad + row + r]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bsum += bf2f((uint16_t)a[e]);
         }
+        bsum += __shfl_xor(bsum, 16);
+        bsum += __shfl_xor(bsum, 32);
         // acc[t][r] = dWt[16 wave + 4 j + r][16 t + i]
         for (int t = 0; t < 4; ++t)
             for (int r = 0; r < 4; ++r)
@@ -455,8 +600,9 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const uint16_t* __restrict__ A
 
 // Fixed-order sum of the chunk partials, Adam, bf16 copy.
 __global__ void k_adam(const float* __restrict__ partial, uint32_t chunks, float* __restrict__ w,
-                       float* __restrict__ m, float* __restrict__ v, uint16_t* __restrict__ wbf, float lr, float b1,
-                       float b2, float eps, float c1, float c2, float* __restrict__ grad_out) {
+                       float* __restrict__ m, float* __restrict__ v, uint16_t* __restrict__ wbf,
+                       uint16_t* __restrict__ frag, float lr, float b1, float b2, float eps, float c1, float c2,
+                       float* __restrict__ grad_out) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (uint32_t)kParams) return;
     float g = 0.0f;
@@ -469,38 +615,60 @@ __global__ void k_adam(const float* __restrict__ partial, uint32_t chunks, float
     v[p] = vv;
     const float nw = w[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
     w[p] = nw;
-    if (p < (uint32_t)kWtElems) wbf[p] = f2bf(nw);
+    if (p < (uint32_t)kWtElems) {
+        const uint16_t h = f2bf(nw);
+        wbf[p] = h;
+        const int l = min((int)p / (kW * kW), kHidden), r = (int)p - wt_offset(l);
+        frag[frag_index(l, r / kW, r % kW)] = h;
+    }
 }
 
-// 1 / sum(weights), fixed-order two-level reduction in one block.
-__global__ void k_wsum(const float* __restrict__ weight, uint32_t n, float* __restrict__ inv_wsum) {
-    __shared__ float s[256];
-    float acc = 0.0f;
-    for (uint32_t r = threadIdx.x; r < n; r += 256) acc += weight[r];
-    s[threadIdx.x] = acc;
+// Fixed-order sum of x[0..n) by one block of 1024 threads: each thread sums
+// a strided set of 4-wide groups (4 loads in flight), then a tree in LDS.
+// Elements before the first 16-byte boundary are summed on their own.
+__device__ float block_sum(const float* __restrict__ x, uint32_t n) {
+    __shared__ float s[1024];
+    const uint32_t t = threadIdx.x;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    const uint32_t head = min(n, (uint32_t)((4u - ((uint32_t)(reinterpret_cast<uintptr_t>(x) >> 2) & 3u)) & 3u));
+    if (t < head) a2 = x[t];
+    x += head;
+    n -= head;
+    const uint32_t n4 = n >> 2;
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    uint32_t g = t;
+    for (; g + 3 * 1024 < n4; g += 4 * 1024) {
+        const float4 p = x4[g], q = x4[g + 1024], r = x4[g + 2048], u = x4[g + 3072];
+        a0 += (p.x + p.y) + (p.z + p.w);
+        a1 += (q.x + q.y) + (q.z + q.w);
+        a2 += (r.x + r.y) + (r.z + r.w);
+        a3 += (u.x + u.y) + (u.z + u.w);
+    }
+    for (; g < n4; g += 1024) {
+        const float4 p = x4[g];
+        a0 += (p.x + p.y) + (p.z + p.w);
+    }
+    if (t < (n & 3u)) a1 += x[4 * n4 + t];
+    s[t] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)t < w) s[t] += s[t + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *inv_wsum = s[0] > 0.0f ? 1.0f / s[0] : 0.0f;
+    return s[0];
 }
 
-__global__ void k_sum(const float* __restrict__ x, uint32_t n, float* __restrict__ out) {
-    __shared__ float s[256];
-    float acc = 0.0f;
-    for (uint32_t r = threadIdx.x; r < n; r += 256) acc += x[r];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = s[0];
+// 1 / sum(weights)
+__global__ __launch_bounds__(1024) void k_wsum(const float* __restrict__ weight, uint32_t n, float* __restrict__ inv_wsum) {
+    const float total = block_sum(weight, n);
+    if (threadIdx.x == 0) *inv_wsum = total > 0.0f ? 1.0f / total : 0.0f;
 }
 
-constexpr size_t kInferSmem = sizeof(uint16_t) * kLayers * kW * kWS + sizeof(float) * kBiasElems +
-                              kWaves * (2 * sizeof(uint16_t) * kTileRows * kWS + sizeof(float) * kTileRows * kOutPad);
+__global__ __launch_bounds__(1024) void k_sum(const float* __restrict__ x, uint32_t n, float* __restrict__ out) {
+    const float total = block_sum(x, n);
+    if (threadIdx.x == 0) *out = total;
+}
+
 constexpr size_t kTrainSmem = sizeof(uint16_t) * kLayers * kW * kWS + sizeof(float) * kBiasElems +
                               kWaves * (8 * sizeof(uint16_t) * kTileRows * kWS + sizeof(float) * kTileRows * kOutPad);
 
@@ -511,7 +679,8 @@ struct lr_guide {
     float* w = nullptr;       // fp32 master params
     float* m = nullptr;
     float* v = nullptr;
-    uint16_t* wbf = nullptr;  // bf16 copy of the Wt part
+    uint16_t* wbf = nullptr;  // bf16 copy of the Wt part (master layout)
+    uint16_t* frag = nullptr; // bf16 copy of the Wt part in fragment order
     uint32_t step = 0;
     float lr = 1e-3f;
     // training scratch (grow-only)
@@ -529,7 +698,7 @@ namespace {
 void free_guide(lr_guide* g) {
     if (!g) return;
     (void)hipSetDevice(g->ctx->device);
-    void* ptrs[] = {g->w, g->m, g->v, g->wbf, g->A, g->G, g->partial, g->row_loss, g->scalars};
+    void* ptrs[] = {g->w, g->m, g->v, g->wbf, g->frag, g->A, g->G, g->partial, g->row_loss, g->scalars};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete g;
@@ -544,10 +713,14 @@ uint16_t host_f2bf(float f) {
 }
 
 int upload_params(lr_guide* g, const float* host) {
-    std::vector<uint16_t> bf(kWtElems);
+    std::vector<uint16_t> bf(kWtElems), fr(kWtElems, 0);
     for (int p = 0; p < kWtElems; ++p) bf[p] = host_f2bf(host[p]);
+    for (int l = 0; l < kLayers; ++l)
+        for (int n = 0; n < layer_nout(l); ++n)
+            for (int k = 0; k < kW; ++k) fr[frag_index(l, n, k)] = bf[wt_offset(l) + n * kW + k];
     LR_HIP_CHECK(hipMemcpy(g->w, host, sizeof(float) * kParams, hipMemcpyHostToDevice));
     LR_HIP_CHECK(hipMemcpy(g->wbf, bf.data(), sizeof(uint16_t) * kWtElems, hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMemcpy(g->frag, fr.data(), sizeof(uint16_t) * kWtElems, hipMemcpyHostToDevice));
     return LR_OK;
 }
 
@@ -574,6 +747,7 @@ int lr_guide_create(lr_context* ctx, uint64_t seed, float lr, lr_guide** out) {
     if (e == hipSuccess) e = hipMalloc(&g->m, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMalloc(&g->v, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMalloc(&g->wbf, sizeof(uint16_t) * kWtElems);
+    if (e == hipSuccess) e = hipMalloc(&g->frag, sizeof(uint16_t) * kWtElems);
     if (e == hipSuccess) e = hipMalloc(&g->scalars, sizeof(float) * 16);
     if (e == hipSuccess) e = hipMemset(g->m, 0, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMemset(g->v, 0, sizeof(float) * kParams);
@@ -592,7 +766,8 @@ int lr_guide_create(lr_context* ctx, uint64_t seed, float lr, lr_guide** out) {
         const int nout = layer_nout(l);
         const float lim = l < kHidden ? std::sqrt(6.0f / kW) : 0.1f * std::sqrt(6.0f / kW);
         for (int n = 0; n < nout; ++n)
-            for (int k = 0; k < kW; ++k) h[wt_offset(l) + n * kW + k] = (l == kLayers - 1 && n >= kOut) ? 0.0f : lim * uni();
+            for (int k = 0; k < kW; ++k)
+                h[wt_offset(l) + n * kW + k] = (l == kLayers - 1 && n >= kOut + kSel) ? 0.0f : lim * uni();
     }
     if (int rc = upload_params(g, h.data())) {
         free_guide(g);
@@ -629,10 +804,10 @@ int lr_guide_infer(lr_guide* g, const float* inputs, uint32_t n, const uint32_t*
     if (n == 0) return LR_OK;
     if (int rc = lr::bind(g->ctx)) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint32_t tiles = (n + kTileRows - 1) / kTileRows;
-    const uint32_t grid = std::min<uint32_t>((tiles + kWaves - 1) / kWaves, (uint32_t)g->ctx->n_cu * 2);
-    hipLaunchKernelGGL(k_infer, dim3(std::max(1u, grid)), dim3(kBlock), kInferSmem, s, g->wbf, g->w + kWtElems,
-                       inputs, n, n_dev, mixture);
+    const uint32_t tiles = (n + kInferRows - 1) / kInferRows;
+    const uint32_t grid = std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, (uint32_t)g->ctx->n_cu * 3);
+    hipLaunchKernelGGL(k_infer, dim3(std::max(1u, grid)), dim3(kInferBlock), kInferSmem, s, g->frag,
+                       g->w + kWtElems, inputs, n, n_dev, mixture);
     LR_HIP_CHECK(hipGetLastError());
     return LR_OK;
 }
@@ -644,7 +819,10 @@ int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, con
     if (int rc = lr::bind(g->ctx)) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t n_pad = (n + 15) & ~15u;
-    const uint32_t chunk_rows = 1024;
+    // 16 .. 64 row chunks of >= 256 rows: enough blocks to spread a small
+    // batch over the chip, few partials for the Adam reduction
+    const uint32_t want = std::min<uint32_t>(64, std::max<uint32_t>(16, n_pad / 1024));
+    const uint32_t chunk_rows = std::max<uint32_t>(256, ((n_pad + want - 1) / want + 15) & ~15u);
     const uint32_t chunks = (n_pad + chunk_rows - 1) / chunk_rows;
     if (g->cap_rows < n_pad || g->cap_chunks < chunks) {
         LR_HIP_CHECK(hipStreamSynchronize(s));
@@ -661,7 +839,7 @@ int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, con
         g->cap_rows = n_pad;
         g->cap_chunks = chunks;
     }
-    hipLaunchKernelGGL(k_wsum, dim3(1), dim3(256), 0, s, weights, n, g->scalars);
+    hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, s, weights, n, g->scalars);
     const uint32_t tiles = n_pad / kTileRows;
     const uint32_t grid = std::min<uint32_t>((tiles + kWaves - 1) / kWaves, (uint32_t)g->ctx->n_cu * 2);
     hipLaunchKernelGGL(k_train_fb, dim3(std::max(1u, grid)), dim3(kBlock), kTrainSmem, s, g->wbf, g->w + kWtElems,
@@ -673,8 +851,8 @@ int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, con
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, (float)g->step), c2 = 1.0f - std::pow(b2, (float)g->step);
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, g->partial, chunks, g->w, g->m, g->v,
-                       g->wbf, g->lr, b1, b2, 1e-8f, c1, c2, grad_out);
-    if (loss_out) hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, g->row_loss, n, loss_out);
+                       g->wbf, g->frag, g->lr, b1, b2, 1e-8f, c1, c2, grad_out);
+    if (loss_out) hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, g->row_loss, n, loss_out);
     LR_HIP_CHECK(hipGetLastError());
     return LR_OK;
 }

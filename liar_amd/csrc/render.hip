@@ -151,14 +151,19 @@ __global__ void k_fill_miss_queue(const uint32_t* count, Hits h) {
 // queue entry before shading.
 struct GuideArgs {
     const float* in;   // [queue][8] network inputs (written by k_guide_inputs)
-    const float* mix;  // [queue][40] mixture (lr_guide_infer)
+    const float* mix;  // [queue][48] mixture + strategy probabilities (lr_guide_infer)
     float sel;         // guide share of the non-light strategies
     float light_share; // probability of light sampling
+    uint32_t learned;  // strategy probabilities from the network (mix[40..42]) instead of sel / light_share
+    uint32_t balance;  // balance heuristic instead of the power heuristic
+    uint32_t image_weight;  // training weights times the throughput before the vertex
+    uint32_t record_depth;  // record vertices of generations < record_depth only
     float lo[3], inv_ext[3];
     float lam_lo, inv_lam_ext;
-    // training records (cap entries): inputs [8], direction [3], aux [4] =
-    // (f|cos| / p_mix, L accumulated at the vertex, throughput after scatter
-    // before roulette, path id bits)
+    // training records (cap entries): inputs [8] (network inputs 0..5, then
+    // the light and BSDF sampling pdfs of the taken direction), direction [3],
+    // aux [4] = (f|cos| / p_mix, L accumulated at the vertex, throughput after
+    // scatter before roulette, path id bits)
     uint32_t record, cap;
     uint32_t* count;
     float* r_in;
@@ -220,37 +225,83 @@ __global__ void k_guide_inputs(DevScene S, Queue q, const uint32_t* count, Hits 
 
 // Training weights of the recorded vertices: w = f|cos|/p_mix * L_i, with
 // the incident radiance estimate L_i = (L_path - L_vertex) / throughput.
-__global__ void k_guide_targets(uint32_t n, const float* __restrict__ aux, const float* __restrict__ L,
-                                float* __restrict__ w) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const float fcp = aux[4 * r], lv = aux[4 * r + 1], tn = aux[4 * r + 2];
-    const uint32_t pid = __float_as_uint(aux[4 * r + 3]);
-    const float rest = L[pid] - lv;
-    float v = (tn > 0.0f && rest > 0.0f) ? fcp * rest / tn : 0.0f;
-    w[r] = isfinite(v) ? v : 0.0f;
+// The record count is read on the device (capped at cap); fixed grid.
+__global__ void k_guide_targets(uint32_t cap, const uint32_t* __restrict__ count, const float* __restrict__ aux,
+                                const float* __restrict__ L, float* __restrict__ w) {
+    const uint32_t n = min(*count, cap);
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const float fcp = aux[4 * r], lv = aux[4 * r + 1], tn = aux[4 * r + 2];
+        const uint32_t pid = __float_as_uint(aux[4 * r + 3]);
+        const float rest = L[pid] - lv;
+        const float v = (tn > 0.0f && rest > 0.0f) ? fcp * rest / tn : 0.0f;
+        w[r] = isfinite(v) ? v : 0.0f;
+    }
 }
 
-// Sum of n floats in one block (fixed order).
-__global__ void k_sum_f32(const float* __restrict__ x, uint32_t n, float* __restrict__ out) {
-    __shared__ float sh[256];
-    float acc = 0.0f;
-    for (uint32_t r = threadIdx.x; r < n; r += 256) acc += x[r];
-    sh[threadIdx.x] = acc;
+// Sum of the first min(*count, cap) floats of x by one block of 1024
+// threads (fixed order): strided partial sums, then a tree in LDS.
+__global__ __launch_bounds__(1024) void k_sum_f32(const float* __restrict__ x, uint32_t cap,
+                                                  const uint32_t* __restrict__ count, float* __restrict__ out) {
+    __shared__ float sh[1024];
+    const uint32_t n = min(*count, cap);
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    uint32_t r = threadIdx.x;
+    for (; r + 3 * 1024 < n; r += 4 * 1024) {
+        a0 += x[r];
+        a1 += x[r + 1024];
+        a2 += x[r + 2048];
+        a3 += x[r + 3072];
+    }
+    for (; r < n; r += 1024) a0 += x[r];
+    sh[threadIdx.x] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 512; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-// w = min(w, clamp * mean(w)).
-__global__ void k_clamp_weights(uint32_t n, const float* __restrict__ sum, float clamp, float* __restrict__ w) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const float cap = clamp * (*sum) / (float)n;
-    w[r] = fminf(w[r], cap);
+// w = min(w, clamp * mean(w)) over the recorded vertices.
+__global__ void k_clamp_weights(uint32_t cap, const uint32_t* __restrict__ count, const float* __restrict__ sum,
+                                float clamp, float* __restrict__ w) {
+    const uint32_t n = min(*count, cap);
+    if (n == 0) return;
+    const float lim = clamp * (*sum) / (float)n;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+        w[r] = fminf(w[r], lim);
+}
+
+// One training batch of `batch` rows drawn uniformly with replacement from
+// the min(*count, cap) records (Philox keyed by seed, counter (chunk, step,
+// row)), so that every bounce's vertices are represented whatever the order
+// the records were appended in.  No records: zero weights.
+__global__ void k_gather_batch(uint32_t cap, const uint32_t* __restrict__ count, const float* __restrict__ r_in,
+                               const float* __restrict__ r_dir, const float* __restrict__ r_w, uint64_t seed,
+                               uint32_t chunk, uint32_t step, uint32_t batch, float* __restrict__ b_in,
+                               float* __restrict__ b_dir, float* __restrict__ b_w) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= batch) return;
+    const uint32_t n = min(*count, cap);
+    float4* o4 = reinterpret_cast<float4*>(b_in + 8 * (size_t)row);
+    if (n == 0) {
+        o4[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        o4[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b_dir[3 * (size_t)row] = 0.0f;
+        b_dir[3 * (size_t)row + 1] = 0.0f;
+        b_dir[3 * (size_t)row + 2] = 1.0f;
+        b_w[row] = 0.0f;
+        return;
+    }
+    const lr::u4 h = lr::philox4x32((uint32_t)seed ^ 0x5bd1e995u, (uint32_t)(seed >> 32), lr::u4{row, step, chunk, 0x6e1du});
+    const uint32_t r = (uint32_t)(((uint64_t)h.x * n) >> 32);
+    const float4* i4 = reinterpret_cast<const float4*>(r_in + 8 * (size_t)r);
+    o4[0] = i4[0];
+    o4[1] = i4[1];
+    b_dir[3 * (size_t)row] = r_dir[3 * (size_t)r];
+    b_dir[3 * (size_t)row + 1] = r_dir[3 * (size_t)r + 1];
+    b_dir[3 * (size_t)row + 2] = r_dir[3 * (size_t)r + 2];
+    b_w[row] = r_w[r];
 }
 
 // -- one generation: emission + scatter (adjoint_photon_tracer.cpp:140-292) --
@@ -269,6 +320,7 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
         const V3 d = v3(q.r[3][i], q.r[4][i], q.r[5][i]);
         const float tnear = q.r[6][i];
         float thr = ps.thr[pid];
+        const float thr_in = thr;
         float L = ps.L[pid];
         const float lam = ps.lam[pid];
         const uint32_t prim = h.prim[i];
@@ -330,13 +382,23 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
                 V3 wo;
                 float value = 0.0f, pdf = 0.0f;
                 bool ok = false;
-                float rec_fcp = 0.0f;
+                float rec_fcp = 0.0f, rec_pl = 0.0f, rec_pb = 0.0f;
                 if (GUIDED && strategy < 1.0f) {
                     // one-sample MIS over BSDF, light and guide sampling
-                    // (power heuristic over c_i p_i): light sampling takes
-                    // `light_share`, the guide `sel` of the rest
-                    const float* mx = G.mix + (size_t)i * 40;
-                    const float cl = G.light_share, cg = (1.0f - cl) * G.sel, cb = (1.0f - cl) * (1.0f - G.sel);
+                    // (power or balance heuristic over c_i p_i): the strategy
+                    // probabilities are the network's (learned) or light
+                    // sampling takes `light_share` and the guide `sel` of the rest
+                    const float* mx = G.mix + (size_t)i * 48;
+                    float cl, cg, cb;
+                    if (G.learned) {
+                        cl = mx[40];
+                        cg = mx[41];
+                        cb = mx[42];
+                    } else {
+                        cl = G.light_share;
+                        cg = (1.0f - cl) * G.sel;
+                        cb = (1.0f - cl) * (1.0f - G.sel);
+                    }
                     V3 dw = v3(0, 0, 1);
                     float f = 0.0f, pb = 0.0f, pl = 0.0f, pg = 0.0f, pc = 0.0f;
                     if (u_strat < cb) {
@@ -378,9 +440,11 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
                     }
                     if (ok && pc > 0.0f) {
                         const float qb = cb * pb, ql = cl * pl, qg = cg * pg;
-                        value = f * (pc * pc) / (qb * qb + ql * ql + qg * qg);
+                        value = G.balance ? f * pc / (qb + ql + qg) : f * (pc * pc) / (qb * qb + ql * ql + qg * qg);
                         pdf = pc;
                         rec_fcp = f * fabsf(wo.z) / (qb + ql + qg);
+                        rec_pl = pl;
+                        rec_pb = pb;
                     } else {
                         ok = false;
                     }
@@ -424,17 +488,18 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
                     thr *= value * fabsf(cos_t) / pdf;
                     start = x + ((cos_t > 0.0f ? 10.0f : -10.0f) * kTolerance) * nf;
                     dir = normalize(wo.x * tt + wo.y * bb + wo.z * nf);
-                    if (GUIDED && G.record && rec_fcp > 0.0f && isfinite(thr) && thr > 0.0f) {
+                    if (GUIDED && G.record && gen < G.record_depth && rec_fcp > 0.0f && isfinite(thr) && thr > 0.0f) {
                         const uint32_t slot = atomicAdd(G.count, 1u);
                         if (slot < G.cap) {
                             const float4* src = reinterpret_cast<const float4*>(G.in + (size_t)i * 8);
                             float4* dst = reinterpret_cast<float4*>(G.r_in + (size_t)slot * 8);
+                            const float4 tail = src[1];
                             dst[0] = src[0];
-                            dst[1] = src[1];
+                            dst[1] = make_float4(tail.x, tail.y, rec_pl, rec_pb);
                             G.r_dir[3 * (size_t)slot] = dir.x;
                             G.r_dir[3 * (size_t)slot + 1] = dir.y;
                             G.r_dir[3 * (size_t)slot + 2] = dir.z;
-                            G.r_aux[4 * (size_t)slot] = rec_fcp;
+                            G.r_aux[4 * (size_t)slot] = G.image_weight ? rec_fcp * thr_in : rec_fcp;
                             G.r_aux[4 * (size_t)slot + 1] = L;
                             G.r_aux[4 * (size_t)slot + 2] = thr;
                             G.r_aux[4 * (size_t)slot + 3] = __uint_as_float(pid);
@@ -583,7 +648,7 @@ struct lr_scene {
     uint32_t* d_counts = nullptr;  // [0..1] queue lengths, [4..5] u64 ray counter, [8] records
     // guided mode: inputs [cap][8], mixture [cap][40], records
     float* d_gwork = nullptr;
-    size_t gcap = 0, grec = 0;
+    size_t gcap = 0, grec = 0, gbatch = 0;
 };
 
 namespace {
@@ -663,15 +728,22 @@ int build_observer(lr_scene* sc, float lo, float hi) {
     return LR_OK;
 }
 
-int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, hipStream_t s) {
-    if (sc->gcap >= paths && sc->grec >= records) return LR_OK;
+// Guided-mode work memory: per path inputs [8] + mixture [48]; per record
+// inputs [8], direction [3], aux [4], weight [1]; per batch row inputs [8],
+// direction [3], weight [1].
+int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, size_t batch, hipStream_t s) {
+    if (sc->gcap >= paths && sc->grec >= records && sc->gbatch >= batch) return LR_OK;
     LR_HIP_CHECK(hipStreamSynchronize(s));
     if (sc->d_gwork) LR_HIP_CHECK(hipFree(sc->d_gwork));
     sc->d_gwork = nullptr;
-    const size_t words = 48 * paths + 16 * records;
+    paths = std::max(paths, sc->gcap);
+    records = std::max(records, sc->grec);
+    batch = std::max(batch, sc->gbatch);
+    const size_t words = 56 * paths + 16 * records + 12 * batch;
     LR_HIP_CHECK(hipMalloc(&sc->d_gwork, sizeof(float) * words));
     sc->gcap = paths;
     sc->grec = records;
+    sc->gbatch = batch;
     return LR_OK;
 }
 
@@ -970,13 +1042,15 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     float* g_in = nullptr;
     float* g_mix = nullptr;
     float* r_w = nullptr;
-    uint32_t rec_cap = 0;
+    float *b_in = nullptr, *b_dir = nullptr, *b_w = nullptr;
+    uint32_t rec_cap = 0, batch = 0;
     if (guided) {
         if (!gparams) return lr::fail(LR_E_INVALID, "guided render needs lr_guide_params");
         if (!(gparams->selection >= 0.0f && gparams->selection < 1.0f))
             return lr::fail(LR_E_INVALID, "guide selection must be in [0, 1)");
-        rec_cap = gparams->max_records ? gparams->max_records : (1u << 20);
-        if (int rc = ensure_guide_work(sc, cap, rec_cap, s)) return rc;
+        rec_cap = gparams->max_records ? gparams->max_records : (1u << 22);
+        batch = gparams->train_batch ? gparams->train_batch : 65536u;
+        if (int rc = ensure_guide_work(sc, cap, rec_cap, batch, s)) return rc;
         g_in = sc->d_gwork;
         g_mix = g_in + 8 * (size_t)cap;
         GA.in = g_in;
@@ -985,6 +1059,10 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         GA.light_share = gparams->light_share;
         if (!(GA.light_share >= 0.0f && GA.light_share < 1.0f))
             return lr::fail(LR_E_INVALID, "guide light_share must be in [0, 1)");
+        GA.learned = gparams->learned_selection ? 1u : 0u;
+        GA.balance = gparams->balance_heuristic ? 1u : 0u;
+        GA.image_weight = gparams->image_weighting ? 1u : 0u;
+        GA.record_depth = gparams->record_depth ? gparams->record_depth : 0xFFFFFFFFu;
         const lr_qbvh_info& I = sc->bvh->info;
         for (int k = 0; k < 3; ++k) {
             GA.lo[k] = I.scene_lo[k];
@@ -996,10 +1074,13 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         GA.record = gparams->train ? 1u : 0u;
         GA.cap = rec_cap;
         GA.count = sc->d_counts + 8;
-        GA.r_in = g_mix + 40 * (size_t)cap;
+        GA.r_in = g_mix + 48 * (size_t)cap;
         GA.r_dir = GA.r_in + 8 * (size_t)rec_cap;
         GA.r_aux = GA.r_dir + 3 * (size_t)rec_cap;
         r_w = GA.r_aux + 4 * (size_t)rec_cap;
+        b_in = r_w + rec_cap;
+        b_dir = b_in + 8 * (size_t)batch;
+        b_w = b_dir + 3 * (size_t)batch;
     }
 
     DevScene S;
@@ -1063,27 +1144,20 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             }
         }
         if (guided && gparams->train && gparams->train_steps > 0) {
-            // recorded vertices -> training weights -> Adam steps
-            uint32_t nrec = 0;
-            LR_HIP_CHECK(hipMemcpyAsync(&nrec, counts + 8, sizeof(nrec), hipMemcpyDeviceToHost, s));
-            LR_HIP_CHECK(hipStreamSynchronize(s));
-            nrec = std::min(nrec, rec_cap);
-            if (nrec > 0) {
-                hipLaunchKernelGGL(k_guide_targets, dim3((nrec + 255) / 256), blk, 0, s, nrec, GA.r_aux, ps.L, r_w);
-                if (gparams->weight_clamp > 0.0f) {
-                    float* wsum = reinterpret_cast<float*>(counts + 12);
-                    hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, s, r_w, nrec, wsum);
-                    hipLaunchKernelGGL(k_clamp_weights, dim3((nrec + 255) / 256), blk, 0, s, nrec, wsum,
-                                       gparams->weight_clamp, r_w);
-                }
-                const uint32_t batch = std::min(nrec, gparams->train_batch ? gparams->train_batch : 65536u);
-                for (uint32_t st = 0; st < gparams->train_steps; ++st) {
-                    const uint32_t off = (uint32_t)(((uint64_t)st * batch) % nrec);
-                    const uint32_t nb = std::min(batch, nrec - off);
-                    if (int rc = lr_guide_train_step(guide, GA.r_in + 8 * (size_t)off, GA.r_dir + 3 * (size_t)off,
-                                                     r_w + off, nb, nullptr, nullptr, s))
-                        return rc;
-                }
+            // recorded vertices -> training weights -> Adam steps on batches drawn
+            // from all of them; the record count stays on the device
+            const uint32_t* nrec = counts + 8;
+            hipLaunchKernelGGL(k_guide_targets, dim3(1024), blk, 0, s, rec_cap, nrec, GA.r_aux, ps.L, r_w);
+            if (gparams->weight_clamp > 0.0f) {
+                float* wsum = reinterpret_cast<float*>(counts + 12);
+                hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(1024), 0, s, r_w, rec_cap, nrec, wsum);
+                hipLaunchKernelGGL(k_clamp_weights, dim3(1024), blk, 0, s, rec_cap, nrec, wsum, gparams->weight_clamp,
+                                   r_w);
+            }
+            for (uint32_t st = 0; st < gparams->train_steps; ++st) {
+                hipLaunchKernelGGL(k_gather_batch, dim3((batch + 255) / 256), blk, 0, s, rec_cap, nrec, GA.r_in, GA.r_dir,
+                                   r_w, R.seed, chunks, st, batch, b_in, b_dir, b_w);
+                if (int rc = lr_guide_train_step(guide, b_in, b_dir, b_w, batch, nullptr, nullptr, s)) return rc;
             }
         }
         hipLaunchKernelGGL(k_splat, dim3((npix + 255) / 256), blk, 0, s, S, P, pix0, npix, ps, film_xyz, film_w);

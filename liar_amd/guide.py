@@ -49,13 +49,14 @@ class Guide:
         return torch.cuda.current_stream(t.device).cuda_stream if stream is None else stream
 
     def infer(self, inputs, mixture=None, stream=None):
-        """inputs: (n, 8) float32 CUDA -> mixture (n, 40)."""
+        """inputs: (n, 8) float32 CUDA -> (n, 48): 8 lobes x (weight, mean
+        xyz, kappa), then the strategy probabilities (light, guide, BSDF)."""
         import torch
 
         n = inputs.shape[0]
         assert inputs.shape[1] == 8 and inputs.is_contiguous()
         if mixture is None:
-            mixture = torch.empty((n, 40), dtype=torch.float32, device=inputs.device)
+            mixture = torch.empty((n, 48), dtype=torch.float32, device=inputs.device)
         N.check(N.lib().lr_guide_infer(self._h, C.c_void_p(inputs.data_ptr()), n, None,
                                        C.c_void_p(mixture.data_ptr()), C.c_void_p(self._stream(inputs, stream))),
                 "lr_guide_infer")

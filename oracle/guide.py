@@ -1,7 +1,9 @@
 """FP64 numpy oracle of the path-guiding MLP -- TEST INFRASTRUCTURE ONLY.
 
 Same parameter layout, encoding and mixture parameterisation as the GPU
-network (include/liar_amd.h, lr_guide_*).  With ``bf16=True`` the forward and
+network (include/liar_amd.h, lr_guide_*).  Outputs 0..39 are the 8 vMF lobes,
+40 and 41 the strategy-selection logits (light / guide / BSDF probabilities
+of guided one-sample MIS), 42..47 padding.  With ``bf16=True`` the forward and
 backward passes round to bfloat16 at the points where the GPU stores bf16
 (weights, encoding, hidden activations, output and hidden gradients), so GPU
 and oracle differ only by FP32-vs-FP64 accumulation.  The analytic gradient
@@ -12,6 +14,7 @@ from __future__ import annotations
 import numpy as np
 
 W, HIDDEN, OUT, OUT_PAD, LOBES, FREQ, IN = 64, 5, 40, 48, 8, 4, 6
+SEL, SEL_FLOOR = 2, 0.025
 WT = HIDDEN * W * W + OUT_PAD * W
 NPARAMS = WT + HIDDEN * W + OUT_PAD
 
@@ -71,8 +74,12 @@ def forward(p, inp, bf=False):
     return acts, out
 
 
-def softplus(x):
-    return np.where(x > 20, x, np.log1p(np.exp(np.minimum(x, 20))))
+KAPPA_LOG_MAX = 7.6
+
+
+def kappa_of(s):
+    """Lobe concentration 0.1 + e^min(s, 7.6)."""
+    return 0.1 + np.exp(np.minimum(s, KAPPA_LOG_MAX))
 
 
 def mixture(out):
@@ -82,7 +89,7 @@ def mixture(out):
     pi /= pi.sum(1, keepdims=True)
     m = o[..., 1:4]
     mu = m / (np.linalg.norm(m, axis=2, keepdims=True) + 1e-12)
-    kappa = softplus(o[..., 4]) + 0.1
+    kappa = kappa_of(o[..., 4])
     return pi, mu, kappa
 
 
@@ -95,9 +102,34 @@ def log_pdf(out, dirs):
     return (mx + np.log(np.exp(lp - mx).sum(1, keepdims=True)))[:, 0]
 
 
-def loss(p, inp, dirs, w, bf=False):
+def sigmoid(x):
+    return 1 / (1 + np.exp(-x))
+
+
+def selection(out):
+    """(c_light, c_guide, c_bsdf) per row: floor + (1 - 3 floor) * (r_l,
+    (1 - r_l) r_g, (1 - r_l)(1 - r_g)), r = sigmoid of outputs 40, 41."""
+    rl, rg = sigmoid(out[:, OUT]), sigmoid(out[:, OUT + 1])
+    ct = np.stack([rl, (1 - rl) * rg, (1 - rl) * (1 - rg)], 1)
+    return SEL_FLOOR + (1 - 3 * SEL_FLOOR) * ct
+
+
+def loss(p, inp, dirs, w, bf=False, pg=None):
+    """sum_r w_r / sum(w) * (-log q(d_r) - [p_l + p_b > 0] log(c_l p_l + c_g
+    p_g + c_b p_b)), with p_l, p_b = inp[:, 6], inp[:, 7].  The guide pdf p_g
+    in the selection term is a constant of the optimisation (pass the value
+    at the point of differentiation as ``pg``; default: the current q)."""
     _, out = forward(p, inp, bf)
-    return float(-(w / w.sum() * log_pdf(out, dirs)).sum())
+    wn = w / w.sum()
+    lq = log_pdf(out, dirs)
+    inp = np.asarray(inp, np.float64)
+    pl, pb = inp[:, 6], inp[:, 7]
+    on = (pl + pb) > 0
+    c = selection(out)
+    pgv = np.exp(lq) if pg is None else pg
+    pmix = c[:, 0] * pl + c[:, 1] * pgv + c[:, 2] * pb
+    sel = np.where(on, np.log(np.where(on, pmix, 1.0)), 0.0)
+    return float(-(wn * lq).sum() - (wn * sel).sum())
 
 
 def grad(p, inp, dirs, w, bf=False):
@@ -118,12 +150,22 @@ def grad(p, inp, dirs, w, bf=False):
     gg = g[:, :OUT].reshape(n, LOBES, 5)
     gg[..., 0] = -wn[:, None] * (resp - pi)
     dlk = 1 / kappa - 2 / np.expm1(2 * kappa) + (c - 1)
-    sig = 1 / (1 + np.exp(-o[..., 4]))
-    gg[..., 4] = -wn[:, None] * resp * dlk * sig
+    dk = np.where(o[..., 4] < KAPPA_LOG_MAX, np.exp(np.minimum(o[..., 4], KAPPA_LOG_MAX)), 0.0)
+    gg[..., 4] = -wn[:, None] * resp * dlk * dk
     nm = np.linalg.norm(o[..., 1:4], axis=2) + 1e-12
     proj = dirs[:, None, :] - c[..., None] * mu
     gg[..., 1:4] = (-wn[:, None] * resp * kappa / nm)[..., None] * proj
     g[:, :OUT] = gg.reshape(n, OUT)
+    inp = np.asarray(inp, np.float64)
+    pl, pb = inp[:, 6], inp[:, 7]
+    on = (pl + pb) > 0
+    pg = np.exp(logq[:, 0])
+    c = selection(out)
+    pmix = np.where(on, c[:, 0] * pl + c[:, 1] * pg + c[:, 2] * pb, 1.0)
+    rl, rg = sigmoid(out[:, OUT]), sigmoid(out[:, OUT + 1])
+    f = np.where(on, -wn * (1 - 3 * SEL_FLOOR) / pmix, 0.0)
+    g[:, OUT] = f * rl * (1 - rl) * (pl - rg * pg - (1 - rg) * pb)
+    g[:, OUT + 1] = f * (1 - rl) * rg * (1 - rg) * (pg - pb)
     if bf:
         g = bf16(g)
     dW = [None] * (HIDDEN + 1)
@@ -151,7 +193,7 @@ def init_params(seed=0):
         lim = np.sqrt(6 / W) * (1 if l < HIDDEN else 0.1)
         blk = rng.uniform(-lim, lim, (nrow, W))
         if l == HIDDEN:
-            blk[OUT:] = 0
+            blk[OUT + SEL:] = 0
         p[o:o + nrow * W] = blk.ravel()
         o += nrow * W
     return p

@@ -39,7 +39,9 @@ def test_forward_matches_oracle(gpu_ctx):
         mix = g.infer(torch.from_numpy(x).cuda()).cpu().numpy().astype(np.float64)
         _, out = G.forward(p, x.astype(np.float64), bf=True)
         pi, mu, kappa = G.mixture(out)
-        got = mix.reshape(n, 8, 5)
+        got = mix[:, :40].reshape(n, 8, 5)
+        e_c = np.abs(mix[:, 40:43] - G.selection(out)).max()
+        assert e_c < 5e-3 and (mix[:, 43:] == 0).all()
         # single-ulp bf16 rounding flips (FP32 vs FP64 accumulation) move
         # values by ~1e-3; a wrong lane layout or wiring moves them by O(0.1)
         e_pi, e_mu = np.abs(got[..., 0] - pi).max(), np.abs(got[..., 1:4] - mu).max()
@@ -65,6 +67,10 @@ def test_gradient_matches_oracle(gpu_ctx):
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     w = (rng.random(n) + 0.2).astype(np.float32)
+    # light / BSDF pdfs of the records (zero for a third: no selection term there)
+    x[:, 6] = np.where(rng.random(n) < 0.5, rng.random(n) * 4, 0.0)
+    x[:, 7] = np.where(np.arange(n) % 3 == 0, 0.0, rng.random(n) * 0.6)
+    x[np.arange(n) % 3 == 0, 6] = 0.0
     grad = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
     g.train_step(torch.from_numpy(x).cuda(), torch.from_numpy(d.astype(np.float32)).cuda(),

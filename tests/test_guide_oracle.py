@@ -23,16 +23,54 @@ def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(3)
     # weights of every layer, biases, including the output layer
     idx = np.concatenate([rng.choice(G.WT, 48, replace=False), G.WT + rng.choice(p.size - G.WT, 16, replace=False)])
+    # the selection rows (outputs 40, 41) and their biases, always
+    out0 = G.HIDDEN * G.W * G.W
+    idx = np.concatenate([idx, out0 + G.OUT * G.W + rng.choice(2 * G.W, 8, replace=False),
+                          [G.WT + G.HIDDEN * G.W + G.OUT, G.WT + G.HIDDEN * G.W + G.OUT + 1]])
+    # the guide pdf inside the selection term is held at its value here
+    _, out = G.forward(p, inp)
+    pg = np.exp(G.log_pdf(out, d))
     fd = []
     for k in idx:
         e = np.zeros(p.size)
         e[k] = 1e-6
-        fd.append((G.loss(p + e, inp, d, w) - G.loss(p - e, inp, d, w)) / 2e-6)
+        fd.append((G.loss(p + e, inp, d, w, pg=pg) - G.loss(p - e, inp, d, w, pg=pg)) / 2e-6)
     fd = np.array(fd)
     assert np.max(np.abs(fd - g[idx])) < 1e-6 * max(1.0, np.abs(g[idx]).max())
-    # padded output rows 40..47 get no gradient
-    out_rows = g[G.HIDDEN * G.W * G.W:G.WT].reshape(G.OUT_PAD, G.W)
-    assert (out_rows[G.OUT:] == 0).all()
+    sel_rows = g[out0 + G.OUT * G.W:out0 + (G.OUT + 2) * G.W]
+    assert np.abs(sel_rows).max() > 1e-4  # the selection term is exercised
+    # padded output rows 42..47 get no gradient
+    out_rows = g[out0:G.WT].reshape(G.OUT_PAD, G.W)
+    assert (out_rows[G.OUT + G.SEL:] == 0).all()
+    # rows without light / BSDF pdfs (inputs 6, 7 zero) train no selection
+    inp0 = inp.copy()
+    inp0[:, 6:8] = 0
+    g0 = G.grad(p, inp0, d, w)
+    assert (g0[out0 + G.OUT * G.W:out0 + (G.OUT + 2) * G.W] == 0).all()
+
+
+def test_selection_optimum_matches_closed_form():
+    # With the target concentrated where only one strategy has density, the
+    # selection loss drives that strategy's probability to its maximum
+    # 1 - 2 floor.  Brute force over the two logits on a grid.
+    rng = np.random.default_rng(9)
+    n = 400
+    pl = np.where(rng.random(n) < 0.8, 5.0, 0.0)   # light covers 80 % of the records
+    pb = np.full(n, 0.3)                           # BSDF covers all, thinly
+    pg = np.full(n, 0.01)
+    best = None
+    for a0 in np.linspace(-6, 6, 61):
+        for a1 in np.linspace(-6, 6, 61):
+            out = np.zeros((1, G.OUT_PAD))
+            out[0, G.OUT], out[0, G.OUT + 1] = a0, a1
+            c = G.selection(out)[0]
+            val = -np.log(c[0] * pl + c[1] * pg + c[2] * pb).mean()
+            if best is None or val < best[0]:
+                best = (val, c)
+    c = best[1]
+    assert abs(c.sum() - 1) < 1e-12 and (c >= G.SEL_FLOOR - 1e-12).all()
+    # light share close to the 0.8 of records only it explains well; BSDF takes the rest
+    assert 0.7 < c[0] < 0.9 and c[2] > c[1]
 
 
 def test_mixture_pdf_integrates_to_one():

@@ -51,26 +51,42 @@ def test_guided_furnace_unbiased_with_untrained_guide(gpu_ctx):
     gs.close()
 
 
-def test_guided_cornell_unbiased(gpu_ctx):
+@pytest.mark.parametrize("learned", [False, True])
+def test_guided_cornell_unbiased(gpu_ctx, learned):
     ref = np.load(Path(__file__).resolve().parent / "golden" / "cornell_ref_4096spp.npz")["y"].astype(np.float64)
     sc, cam = W.cornell_box()
     gs = E.GpuScene(gpu_ctx, la.compile_scene(sc))
     g = Guide(gpu_ctx, seed=2, learning_rate=2e-3)
-    fx, fw = _film(256, 256)
-    for p in range(16):  # 16 passes x 64 spp, training during the first 8
-        E.guided_render(gs, g, E.camera_struct(cam), E.render_params(256, 256, 64, 4, 8, 3, 0.5, (380, 780), 50 + p),
-                        fx, fw, selection=0.5, train=p < 8, train_steps=8, train_batch=32768)
-    y = _y(fx, fw)
+    blocks = lambda a: a.reshape(8, 32, 8, 32).mean(axis=(1, 3))
+    passes, yg, yp = 16, [], []
+    for p in range(passes):  # 16 passes x 64 spp each, the guide training during the first 8
+        prm = E.render_params(256, 256, 64, 4, 8, 3, 0.5, (380, 780), 50 + p)
+        fx, fw = _film(256, 256)
+        E.guided_render(gs, g, E.camera_struct(cam), prm, fx, fw, selection=0.5, train=p < 8, train_steps=8,
+                        train_batch=32768, learned_selection=learned, balance_heuristic=learned)
+        yg.append(_y(fx, fw))
+        prm.seed = 5000 + p
+        fx, fw = _film(256, 256)
+        gs.render(E.camera_struct(cam), prm, fx, fw)
+        yp.append(_y(fx, fw))
+    y = np.mean(yg, axis=0)
     bias = (y.mean() - ref.mean()) / ref.mean()
     mrae = np.abs(y - ref).mean() / ref.mean()
-    # 32x32-pixel block means: a bias confined to one region (a wrong pdf for
-    # one strategy at some geometry) shows here, where per-pixel noise averages out
-    blocks = lambda a: a.reshape(8, 32, 8, 32).mean(axis=(1, 3))
-    block_err = np.abs(blocks(y) - blocks(ref)) / np.maximum(blocks(ref), 1e-3 * ref.mean())
-    print(f"[gpu] guided cornell 1024 spp: bias {bias:+.2e}, MRAE {mrae:.4f}, worst 32x32 block {block_err.max():.4f}")
+    # 32x32-pixel block means of the guided render against an independent plain
+    # render of the same spp, in units of their standard error: every pass is
+    # an unbiased estimate (the guide changes only between passes), so the
+    # spread of the per-pass block means gives the error of their average.  A
+    # bias confined to one region (a wrong pdf for one strategy at some
+    # geometry) shows as a large z where per-pixel noise averages out.
+    bg, bp = np.stack([blocks(v) for v in yg]), np.stack([blocks(v) for v in yp])
+    se = np.sqrt(bg.var(axis=0, ddof=1) / passes + bp.var(axis=0, ddof=1) / passes)
+    z = np.abs(bg.mean(0) - bp.mean(0)) / np.maximum(se, 1e-12)
+    print(f"[gpu] guided cornell 1024 spp (learned selection {learned}): bias {bias:+.2e}, MRAE {mrae:.4f}, "
+          f"block |z| vs plain: max {z.max():.2f} mean {z.mean():.2f}")
     # per-pixel MRAE is noise: plain 1024 spp scores ~0.04 against this reference; guided
-    # sampling at selection 0.5 spends half the BSDF share on a network that is still learning
-    assert abs(bias) < 3e-3 and mrae < 0.07 and block_err.max() < 0.03
+    # sampling spends part of the samples on a network that is still learning
+    assert abs(bias) < 3e-3 and mrae < 0.07
+    assert z.max() < 5.0 and z.mean() < 1.3
     g.close()
     gs.close()
 
