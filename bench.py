@@ -96,18 +96,21 @@ def bench_render(args, la, oracle, torch, np):
 
 
 def bench_guided(args, la, torch, np):
-    """Learned path guiding on the slit-room workload (terrain in a closed
-    room lit through a 2 % ceiling slit; camera on the indirectly lit half).
-    Plain and guided renders accumulate 4-spp passes; every render, the
-    reference included, clamps sample luminance at 0.5 (Raster's
-    maxSampleLuminance) so sun caustics do not dominate the MSE.  Reports the
-    wall time at which the guided render (training included) first reaches
-    the plain tracer's 64-spp MSE against a 4096-spp plain reference."""
+    """Learned path guiding on the slit-room workload: the terrain workload
+    (4,992,800 triangles + 10k spheres, Lambert / GGX gold / dielectric) in a
+    closed box room lit by the sun and sky only through a 2 % ceiling slit;
+    camera on the indirectly lit half; 1920x1080, depth 16.  Plain and guided
+    renders accumulate 4-spp passes against a plain 4096-spp reference; every
+    render, the reference included, clamps sample luminance at 0.5 (Raster's
+    maxSampleLuminance) so sun caustics do not swamp the MSE.  The guided clock
+    includes its online training (the first 2 passes).  Reports the wall time
+    at which the guided render first reaches the plain tracer's 64-spp MSE."""
     from liar_amd import engine as E
     from liar_amd import workloads as W
     from liar_amd.guide import Guide
 
     Wd, Hd, ML = args.guided_width, args.guided_width * 9 // 16, 0.5
+    log(f"[bench] guided: building the slit room ({Wd}x{Hd})")
     cs, cam = W.slit_room()
     ctx = la.Context(0)
     gs = E.GpuScene(ctx, cs)
@@ -121,48 +124,62 @@ def bench_guided(args, la, torch, np):
         return E.render_params(Wd, Hd, spp, 1, 16, 16, 0.5, (380.0, 780.0), seed, 0, ML)
 
     fx, fw = film()
+    ref_spp = args.guided_ref_spp
+    log(f"[bench] guided: {ref_spp}-spp plain reference")
     t0 = time.perf_counter()
-    for k in range(8):
+    for k in range(ref_spp // 512):
         gs.render(camera, params(512, 10000 + k), fx, fw)
     torch.cuda.synchronize()
     t_ref = time.perf_counter() - t0
-    ref = (fx / fw)[1].cpu().numpy().astype(np.float64)
+    ref = (fx / fw)[1].clone()
+    del fx, fw
+    gcfg = dict(selection=0.95, light_share=0.05, train_steps=128, train_batch=4096, weight_clamp=20.0,
+                record_depth=1)
+    train_passes = 2
 
-    def run(guide, passes=32, train_passes=8):
+    def run(guide, passes=32):
         fx, fw = film()
         # warm-up pass (allocation, code load) outside the clock
-        E.guided_render(gs, guide, camera, params(1, 1), *film()) if guide else gs.render(camera, params(1, 1), *film())
+        if guide is None:
+            gs.render(camera, params(1, 1), *film())
+        else:
+            E.guided_render(gs, guide, camera, params(1, 1), *film(), **gcfg)
         torch.cuda.synchronize()
         elapsed, curve = 0.0, []
         for p in range(passes):
             t0 = time.perf_counter()
             if guide is not None:
-                E.guided_render(gs, guide, camera, params(4, 100 + p), fx, fw, selection=0.9, train=p < train_passes,
-                                train_steps=16, train_batch=65536, light_share=0.1)
+                E.guided_render(gs, guide, camera, params(4, 100 + p), fx, fw, train=p < train_passes, **gcfg)
             else:
                 gs.render(camera, params(4, 100 + p), fx, fw)
             torch.cuda.synchronize()
             elapsed += time.perf_counter() - t0
-            y = (fx / fw)[1].cpu().numpy()
-            curve.append((elapsed, 4 * (p + 1), float(((y - ref) ** 2).mean())))
+            curve.append((elapsed, 4 * (p + 1), float(((fx[1] / fw - ref) ** 2).mean())))
         return curve
 
+    log("[bench] guided: plain curve")
     plain = run(None)
-    g = Guide(ctx, seed=1, learning_rate=2e-3)
+    log("[bench] guided: guided curve")
+    g = Guide(ctx, seed=1, learning_rate=1e-3)
     guided = run(g)
     t_plain, mse_plain = plain[15][0], plain[15][2]
     hit = next((t for t, _, m in guided if m <= mse_plain), None)
     out = {
         "workload": f"slit room (terrain 4,992,800 tris + 10k spheres in a closed box, 2% ceiling slit, sun + sky), "
-                    f"{Wd}x{Hd}, depth 16, sample luminance clamp {ML}",
-        "reference": f"plain 4096 spp ({t_ref:.1f} s)",
+                    f"{Wd}x{Hd}, 1 wavelength per sample, depth 16, sample luminance clamp {ML}",
+        "reference": f"plain {ref_spp} spp ({t_ref:.1f} s)",
         "plain_64spp_mse": mse_plain,
         "plain_64spp_seconds": round(t_plain, 4),
         "guided_seconds_to_plain_64spp_mse": None if hit is None else round(hit, 4),
         "time_ratio_guided_over_plain": None if hit is None else round(hit / t_plain, 3),
         "guided_64spp_mse": guided[15][2],
+        "guided_64spp_seconds": round(guided[15][0], 4),
         "equal_spp_mse_ratio_plain_over_guided": round(mse_plain / guided[15][2], 3),
-        "guided_config": "selection 0.9, light share 0.1, 8 training passes x 16 Adam steps (batch 65536, lr 2e-3)",
+        "guided_config": "one-sample MIS: guide 0.95 x 0.95, BSDF 0.05 x 0.95, light 0.05; "
+                         f"online training in the first {train_passes} passes: 128 Adam steps (lr 1e-3, batch 4096) "
+                         "per 4M-path chunk on first-bounce records, weights clamped at 20x their mean",
+        "curves": {"plain": [(round(t, 4), spp, m) for t, spp, m in plain[::4]],
+                   "guided": [(round(t, 4), spp, m) for t, spp, m in guided[::4]]},
     }
     g.close()
     gs.close()
@@ -182,7 +199,9 @@ def main() -> int:
                     help="rays used for the oracle's nodes/prims-per-ray counts and parity")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracer leg")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guiding leg")
-    ap.add_argument("--guided-width", type=int, default=640)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default traversal configuration")
+    ap.add_argument("--guided-width", type=int, default=1920)
+    ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--render-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (PMC byte-counter calibration)")
@@ -240,10 +259,27 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
+    default_cfg = ctx.trace_config()
+
+    def tune(r, any_hit):
+        # traversal block size, LDS stack depth, residency and refill threshold
+        # from the autotuner, on the rays about to be timed (outside the timed region)
+        if args.no_tune:
+            return None
+        res = bvh.autotune(r, any_hit=any_hit, repeats=2)
+        d = next((ms for c, ms in res["table"] if c == default_cfg), None)
+        return {"config": res["best"], "ms": round(res["best_ms"], 4),
+                "default_config_ms": None if d is None else round(d, 4), "configs_timed": len(res["table"])}
+
+    log("[bench] autotuning closest-hit traversal")
+    tuned_c = tune(rays, False)
     log(f"[bench] closest-hit: {args.warmup} warmup + {args.steps} timed launches of {N_RAYS} rays")
     wall_c, kern_c = timed(lambda: bvh.trace_closest(rays, prim, tuv), args.steps, args.warmup)
+    log("[bench] autotuning any-hit traversal")
+    tuned_a = tune(rays_any, True)
     log("[bench] any-hit")
     wall_a, kern_a = timed(lambda: bvh.trace_any(rays_any, occ), args.steps, args.warmup)
+    ctx.set_trace_config(**(tuned_c["config"] if tuned_c else default_cfg))
 
     mrays = N_RAYS * args.steps / wall_c / 1e6
     mrays_any = N_RAYS * args.steps / wall_a / 1e6
@@ -338,7 +374,8 @@ def main() -> int:
             "parity_mismatches_any": mism_any,
             "hit_rate_closest": round(float((ref[0] != oracle.MISS).mean()), 4),
             "qbvh_build_s": round(info["build_seconds"], 3),
-            "trace_config": ctx.trace_config(),
+            "autotune_closest": tuned_c,
+            "autotune_any": tuned_a,
         },
         "render": render,
         "guided": guided,

@@ -175,6 +175,25 @@ typedef struct lr_trace_config {
 int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg);
 int lr_get_trace_config(const lr_context* ctx, lr_trace_config* cfg);
 
+/* Traversal autotuner (no counterpart in LiAR, whose CPU tracer has no launch
+ * parameters).  Times the caller's own batch under every instantiated
+ * (block, lds_stack) kernel x blocks_per_cu in {2, 4, 8, 16} x refill_min in
+ * {8, 16, 32, 48}: one warm-up launch, then `repeats` (0: 3) launches between
+ * HIP events on `stream`.  Tunes closest hit when hits is non-NULL, else any
+ * hit into occluded.  The fastest configuration becomes the context's
+ * (lr_set_trace_config) and is returned in *best; entries (optional, up to
+ * max_entries) receive every configuration with its mean milliseconds per
+ * launch.  Synchronises the stream; other traces on the same context must not
+ * run meanwhile.  Results are those of the last timed launch. */
+typedef struct lr_tune_entry {
+    lr_trace_config cfg;
+    float ms;
+} lr_tune_entry;
+int lr_autotune_trace(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits_soa* hits,
+                      uint8_t* occluded, uint32_t repeats, lr_tune_entry* entries,
+                      uint32_t max_entries, uint32_t* n_entries, lr_trace_config* best,
+                      void* stream);
+
 /* ---- spectral path tracing (wavefront) ------------------------------------ */
 
 #define LR_NONE 0xFFFFFFFFu

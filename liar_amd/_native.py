@@ -49,6 +49,10 @@ class TraceConfig(C.Structure):
                 ("refill_min", C.c_uint32)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("cfg", TraceConfig), ("ms", C.c_float)]
+
+
 class Spectrum(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("count", C.c_uint32), ("w", C.c_void_p), ("v", C.c_void_p),
                 ("value", C.c_float), ("lo", C.c_float), ("hi", C.c_float), ("temperature", C.c_float),
@@ -116,6 +120,8 @@ SIGNATURES = {
     "lr_trace_any": (_I, [_P, C.POINTER(RaysSoa), _U32, _P, _P]),
     "lr_set_trace_config": (_I, [_P, C.POINTER(TraceConfig)]),
     "lr_get_trace_config": (_I, [_P, C.POINTER(TraceConfig)]),
+    "lr_autotune_trace": (_I, [_P, C.POINTER(RaysSoa), _U32, _P, _P, _U32, _P, _U32, C.POINTER(_U32),
+                               C.POINTER(TraceConfig), _P]),
     "lr_gen_random_triangles": (_I, [_U64, _U32, _F, _P, _P]),
     "lr_gen_random_rays": (_I, [_U64, _U32, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P]),
     "lr_philox4x32": (None, [_U64, _U32, _U32, _U32, _U32, _P]),

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -511,6 +512,80 @@ int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg) {
 int lr_get_trace_config(const lr_context* ctx, lr_trace_config* cfg) {
     if (!ctx || !cfg) return lr::fail(LR_E_INVALID, "NULL argument");
     *cfg = ctx->cfg;
+    return LR_OK;
+}
+
+int lr_autotune_trace(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits_soa* hits, uint8_t* occluded,
+                      uint32_t repeats, lr_tune_entry* entries, uint32_t max_entries, uint32_t* n_entries,
+                      lr_trace_config* best, void* stream) {
+    if (!bvh || !bvh->ctx) return lr::fail(LR_E_INVALID, "NULL or host-only qbvh");
+    if (int rc = check_rays(rays, n)) return rc;
+    const bool any = hits == nullptr;
+    if (any && !occluded) return lr::fail(LR_E_INVALID, "need hits (closest) or occluded (any)");
+    if (!any && (!hits->prim || !hits->t || !hits->u || !hits->v))
+        return lr::fail(LR_E_INVALID, "a hit array pointer is NULL");
+    if (entries && max_entries && !n_entries) return lr::fail(LR_E_INVALID, "entries given without n_entries");
+    if (n == 0 || bvh->info.n_nodes == 0) return lr::fail(LR_E_INVALID, "nothing to tune: no rays or empty scene");
+    if (int rc = lr::bind(bvh->ctx)) return rc;
+    lr_context* ctx = bvh->ctx;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t reps = repeats ? repeats : 3;
+    lr_trace_config saved;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        saved = ctx->cfg;
+    }
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar};
+    const HitsArg ha = any ? HitsArg{nullptr, nullptr, nullptr, nullptr} : HitsArg{hits->prim, hits->t, hits->u, hits->v};
+    hipEvent_t e0, e1;
+    LR_HIP_CHECK(hipEventCreate(&e0));
+    LR_HIP_CHECK(hipEventCreate(&e1));
+    static const uint32_t kBpc[] = {2, 4, 8, 16};
+    static const uint32_t kRefill[] = {8, 16, 32, 48};
+    lr_trace_config win = saved;
+    float win_ms = INFINITY;
+    uint32_t count = 0;
+    int rc = LR_OK;
+    for (const auto& k : kKernels) {
+        for (uint32_t bpc : kBpc) {
+            for (uint32_t rf : kRefill) {
+                const lr_trace_config c{k.block, bpc, k.lds_stack, rf};
+                {
+                    std::lock_guard<std::mutex> g(ctx->mu);
+                    ctx->cfg = c;
+                }
+                Launch L;
+                if ((rc = prepare(bvh, n, any, s, L))) break;
+                if ((rc = launch(bvh, L, ra, n, ha, any ? occluded : nullptr, s))) break;  // warm-up
+                hipError_t he = hipEventRecord(e0, s);
+                for (uint32_t r = 0; r < reps && !rc; ++r) rc = launch(bvh, L, ra, n, ha, any ? occluded : nullptr, s);
+                if (he == hipSuccess) he = hipEventRecord(e1, s);
+                if (he == hipSuccess) he = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+                if (he != hipSuccess) rc = lr::fail(LR_E_DEVICE, std::string("lr_autotune_trace: ") + hipGetErrorString(he));
+                if (rc) break;
+                ms /= (float)reps;
+                if (entries && count < max_entries) entries[count] = lr_tune_entry{c, ms};
+                ++count;
+                if (ms < win_ms) {
+                    win_ms = ms;
+                    win = c;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->cfg = rc ? saved : win;
+    }
+    if (rc) return rc;
+    if (n_entries) *n_entries = std::min(count, max_entries);
+    if (best) *best = win;
     return LR_OK;
 }
 

@@ -175,6 +175,37 @@ class Qbvh:
                                      C.c_void_p(s)), "lr_trace_any")
         return occluded
 
+    def autotune(self, rays, any_hit=False, repeats=3, stream=None) -> dict:
+        """lr_autotune_trace on an (8, n) CUDA ray tensor: times every traversal
+        configuration on these rays, installs the fastest in the context and
+        returns {"best": config, "best_ms": ms, "table": [(config, ms), ...]}."""
+        import torch
+
+        n = rays.shape[1]
+        rs = self._rays_struct(rays)
+        s = torch.cuda.current_stream(rays.device).cuda_stream if stream is None else stream
+        cap = 1024
+        entries = (N.TuneEntry * cap)()
+        count = C.c_uint32(0)
+        best = N.TraceConfig()
+        if any_hit:
+            occ = torch.empty(n, dtype=torch.uint8, device=rays.device)
+            hs_p, occ_p = None, C.c_void_p(occ.data_ptr())
+        else:
+            prim = torch.empty(n, dtype=torch.int32, device=rays.device)
+            tuv = torch.empty((3, n), dtype=torch.float32, device=rays.device)
+            hs = N.HitsSoa(prim.data_ptr(), tuv.data_ptr(), tuv.data_ptr() + 4 * n, tuv.data_ptr() + 8 * n)
+            hs_p, occ_p = C.byref(hs), None
+        N.check(N.lib().lr_autotune_trace(self.handle, C.byref(rs), n, hs_p, occ_p, int(repeats), entries, cap,
+                                          C.byref(count), C.byref(best), C.c_void_p(s)), "lr_autotune_trace")
+
+        def cfg(c):
+            return {"block": c.block, "blocks_per_cu": c.blocks_per_cu, "lds_stack": c.lds_stack,
+                    "refill_min": c.refill_min}
+
+        table = [(cfg(entries[k].cfg), float(entries[k].ms)) for k in range(count.value)]
+        return {"best": cfg(best), "best_ms": min(ms for _, ms in table), "table": table}
+
 
 # -- deterministic synthetic inputs -------------------------------------------
 

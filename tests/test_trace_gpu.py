@@ -114,6 +114,33 @@ def test_launch_configs_agree(gpu_ctx, block, lds):
         gpu_ctx.set_trace_config(**old)
 
 
+def test_autotuner_installs_fastest_config_and_keeps_hits(gpu_ctx):
+    """lr_autotune_trace times all 9 kernels x 4 blocks_per_cu x 4 refill
+    thresholds, installs the fastest, and traversal under the tuned
+    configuration still matches the oracle bit-exactly (closest and any)."""
+    say("[gpu] autotuner")
+    v, t, s = scene(100000, 1000, 4)
+    b = la.Qbvh(gpu_ctx, v, t, s)
+    r = la.random_rays(1 << 17, seed=8)
+    nodes, prims = b.export()
+    old = gpu_ctx.trace_config()
+    try:
+        res = b.autotune(to_dev(r), repeats=2)
+        assert len(res["table"]) == 9 * 4 * 4
+        assert all(ms > 0 for _, ms in res["table"])
+        assert res["best_ms"] == min(ms for _, ms in res["table"])
+        assert gpu_ctx.trace_config() == res["best"]
+        say(f"[gpu] tuned closest {res['best']} {res['best_ms']:.3f} ms; "
+            f"slowest {max(ms for _, ms in res['table']):.3f} ms")
+        assert_hits_equal(oracle.qbvh_closest(nodes, prims, r)[:4], gpu_closest(b, r), "tuned")
+        r2 = la.random_rays(1 << 17, seed=9, tfar_lo=0.01, tfar_hi=2.0)
+        res_any = b.autotune(to_dev(r2), any_hit=True, repeats=2)
+        assert gpu_ctx.trace_config() == res_any["best"]
+        np.testing.assert_array_equal(oracle.qbvh_any(nodes, prims, r2)[0], gpu_any(b, r2))
+    finally:
+        gpu_ctx.set_trace_config(**old)
+
+
 def test_full_size_microbenchmark_parity(gpu_ctx):
     """1M random triangles, 4,194,304 rays: closest and any-hit bit-exact
     against the oracle QBVH traversal."""
