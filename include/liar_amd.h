@@ -284,6 +284,12 @@ typedef struct lr_render_params {
     float max_sample_luminance; /* Raster::maxSampleLuminance: a sample whose
                                    Y exceeds it is scaled down to it
                                    (raster.cpp:567-570); 0 = no clamp       */
+    uint32_t graph;        /* nonzero: the call's launches (every chunk and
+                              bounce, guide inference and training) are
+                              captured into a HIP graph, kept per scene and
+                              updated in place on later calls, and launched
+                              as one; the caller's stream is ordered before
+                              and after it by events                         */
 } lr_render_params;
 
 typedef struct lr_render_stats {
@@ -317,6 +323,32 @@ int lr_scene_destroy(lr_scene* scene);
  * count.  Deterministic for a given seed.  stats may be NULL. */
 int lr_render(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
               float* film_xyz, float* film_weight, lr_render_stats* stats, void* stream);
+
+/* ---- batches: many scenes in one wavefront (no counterpart in LiAR) -------
+ * A batch holds scenes built with lr_scene_create (non-empty geometry each);
+ * their trees are concatenated into one node array (indices rebased, each
+ * ray starting at its scene's root), so that one sequence of launches renders
+ * them all: chunks of up to 4M paths are packed from whole pixels of the
+ * scenes, shortest scene first, and every bounce's queue mixes the paths of
+ * all scenes in its chunk.  Every path draws the random numbers it would
+ * draw in lr_render of its scene alone, so each film equals that render's.
+ * The batch keeps its own copy of the trees: the scenes must outlive it. */
+typedef struct lr_batch lr_batch;
+typedef struct lr_batch_item {
+    lr_camera camera;
+    lr_render_params params;  /* graph is ignored; the batch's chunk size is
+                                 the smallest nonzero chunk_paths (0: 4M)     */
+    float* film_xyz;          /* device, 3 * width * height (accumulated)    */
+    float* film_weight;       /* device, width * height                      */
+} lr_batch_item;
+#define LR_BATCH_GRAPH 1u     /* capture the call's launches into a HIP graph */
+int lr_batch_create(lr_context* ctx, lr_scene* const* scenes, uint32_t n, lr_batch** out);
+int lr_batch_destroy(lr_batch* batch);
+/* items: one per scene, in the batch's scene order.  scene_ms (optional, n
+ * floats): milliseconds from the start of the call's first launch to the end
+ * of the last chunk holding the scene's pixels (device wall clock). */
+int lr_render_batch(lr_batch* batch, const lr_batch_item* items, uint32_t flags, float* scene_ms,
+                    lr_render_stats* stats, void* stream);
 
 /* ---- path-guiding network (no counterpart in LiAR) -------------------------
  * A learned proposal for the integrand's direction sampling: an MLP

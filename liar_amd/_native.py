@@ -80,7 +80,15 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("wavelengths", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("rr_survival", C.c_float), ("lambda_min", C.c_float), ("lambda_max", C.c_float),
-                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32), ("max_sample_luminance", C.c_float)]
+                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32), ("max_sample_luminance", C.c_float),
+                ("graph", C.c_uint32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("camera", Camera), ("params", RenderParams), ("film_xyz", C.c_void_p), ("film_weight", C.c_void_p)]
+
+
+LR_BATCH_GRAPH = 1
 
 
 class RenderStats(C.Structure):
@@ -128,6 +136,9 @@ SIGNATURES = {
     "lr_gen_terrain": (_I, [_U64, _U32, _U32, _F, _F, _F, _P, _P]),
     "lr_render_guided": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, C.POINTER(GuideParams), _P, _P,
                               C.POINTER(RenderStats), _P]),
+    "lr_batch_create": (_I, [_P, _P, _U32, C.POINTER(_P)]),
+    "lr_batch_destroy": (_I, [_P]),
+    "lr_render_batch": (_I, [_P, _P, _U32, _P, C.POINTER(RenderStats), _P]),
     "lr_guide_param_count": (_I, []),
     "lr_guide_create": (_I, [_P, _U64, _F, C.POINTER(_P)]),
     "lr_guide_destroy": (_I, [_P]),

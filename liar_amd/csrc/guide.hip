@@ -812,19 +812,24 @@ int lr_guide_infer(lr_guide* g, const float* inputs, uint32_t n, const uint32_t*
     return LR_OK;
 }
 
-int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, const float* weights, uint32_t n,
-                        float* loss_out, float* grad_out, void* stream) {
-    if (!g || (n && (!inputs || !dirs || !weights))) return lr::fail(LR_E_INVALID, "NULL argument");
-    if (n == 0) return LR_OK;
-    if (int rc = lr::bind(g->ctx)) return rc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint32_t n_pad = (n + 15) & ~15u;
-    // 16 .. 64 row chunks of >= 256 rows: enough blocks to spread a small
-    // batch over the chip, few partials for the Adam reduction
+}  // extern "C"
+
+namespace {
+
+// Row chunking of the weight-gradient pass: 16 .. 64 chunks of >= 256 rows,
+// enough blocks to spread a small batch over the chip, few partials for the
+// Adam reduction.
+void wgrad_chunks(uint32_t n_pad, uint32_t& chunk_rows, uint32_t& chunks) {
     const uint32_t want = std::min<uint32_t>(64, std::max<uint32_t>(16, n_pad / 1024));
-    const uint32_t chunk_rows = std::max<uint32_t>(256, ((n_pad + want - 1) / want + 15) & ~15u);
-    const uint32_t chunks = (n_pad + chunk_rows - 1) / chunk_rows;
-    if (g->cap_rows < n_pad || g->cap_chunks < chunks) {
+    chunk_rows = std::max<uint32_t>(256, ((n_pad + want - 1) / want + 15) & ~15u);
+    chunks = (n_pad + chunk_rows - 1) / chunk_rows;
+}
+
+int grow_training(lr_guide* g, uint32_t n_pad, uint32_t chunks, hipStream_t s) {
+    if (g->cap_rows >= n_pad && g->cap_chunks >= chunks) return LR_OK;
+    n_pad = std::max<uint32_t>(n_pad, (uint32_t)g->cap_rows);
+    chunks = std::max(chunks, g->cap_chunks);
+    {
         LR_HIP_CHECK(hipStreamSynchronize(s));
         if (g->A) LR_HIP_CHECK(hipFree(g->A));
         if (g->G) LR_HIP_CHECK(hipFree(g->G));
@@ -839,6 +844,31 @@ int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, con
         g->cap_rows = n_pad;
         g->cap_chunks = chunks;
     }
+    return LR_OK;
+}
+
+}  // namespace
+
+int lr_guide_reserve(lr_guide* g, uint32_t n, hipStream_t s) {
+    if (!g || n == 0) return LR_OK;
+    const uint32_t n_pad = (n + 15) & ~15u;
+    uint32_t chunk_rows, chunks;
+    wgrad_chunks(n_pad, chunk_rows, chunks);
+    return grow_training(g, n_pad, chunks, s);
+}
+
+extern "C" {
+
+int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, const float* weights,
+                                   uint32_t n, float* loss_out, float* grad_out, void* stream) {
+    if (!g || (n && (!inputs || !dirs || !weights))) return lr::fail(LR_E_INVALID, "NULL argument");
+    if (n == 0) return LR_OK;
+    if (int rc = lr::bind(g->ctx)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t n_pad = (n + 15) & ~15u;
+    uint32_t chunk_rows, chunks;
+    wgrad_chunks(n_pad, chunk_rows, chunks);
+    if (int rc = grow_training(g, n_pad, chunks, s)) return rc;
     hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, s, weights, n, g->scalars);
     const uint32_t tiles = n_pad / kTileRows;
     const uint32_t grid = std::min<uint32_t>((tiles + kWaves - 1) / kWaves, (uint32_t)g->ctx->n_cu * 2);

@@ -25,9 +25,15 @@ struct lr_context {
     lr_trace_config cfg{256, 8, 8, 32};
     std::mutex mu;
     std::unordered_map<hipStream_t, lr_scratch> spill;
+    // resident workgroups per CU of each traversal kernel at its block size
+    // (queried once: launches under graph capture make no runtime queries)
+    std::unordered_map<const void*, int> occupancy;
 
     // Returns a device buffer of at least `bytes` for `stream` (locked).
     int scratch(hipStream_t stream, size_t bytes, void** out);
+    // Frees the buffer of `stream` (still valid: called before the stream
+    // is destroyed) and forgets the stream.
+    void release(hipStream_t stream);
 };
 
 struct lr_qbvh {
@@ -43,7 +49,17 @@ namespace lr {
 int bind(const lr_context* ctx);
 
 // Closest-hit over a device ray queue whose length (<= n_max) is read from
-// device memory `n_dev` by the kernel; no host synchronisation.
+// device memory `n_dev` by the kernel; no host synchronisation.  root
+// (optional, per ray): the node each ray's traversal starts at, for node
+// arrays that hold several trees.
 int trace_closest_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, const uint32_t* n_dev,
-                        lr_hits_soa* hits, hipStream_t stream);
+                        lr_hits_soa* hits, hipStream_t stream, const uint32_t* root = nullptr);
+// Grows the per-stream traversal scratch for queues of up to n_max rays, so
+// that later trace_closest_queue calls (e.g. under graph capture) allocate
+// nothing.
+int trace_reserve(lr_qbvh* bvh, uint32_t n_max, hipStream_t stream);
 }
+
+// Grows a guide's training scratch for batches of up to n rows (no
+// allocation in later lr_guide_train_step calls of at most n rows).
+int lr_guide_reserve(lr_guide* g, uint32_t n, hipStream_t stream);

@@ -50,7 +50,46 @@ struct Params {
 struct Queue {
     float* r[8];  // ox oy oz dx dy dz tnear tfar
     uint32_t* pid;
+    uint32_t* root;  // batch mode: root node of the path's scene in the shared node array
 };
+
+// -- batch mode: one wavefront over many scenes ------------------------------
+// A chunk is a run of segments, each a range of whole pixels of one scene;
+// chunk-local path i of segment k is scene path pixel0 * ppp + i - path_begin.
+struct Segment {
+    uint32_t scene;
+    uint32_t path_begin;  // first chunk-local path of the segment
+    uint32_t pix_begin;   // first chunk-local pixel of the segment
+    uint32_t pixel0;      // first pixel (scene-local linear index)
+};
+
+struct SceneFilm {
+    float* xyz;
+    float* w;
+};
+
+struct BatchArgs {
+    const Segment* seg;  // this chunk's segments, ascending path_begin / pix_begin
+    uint32_t n_seg;
+    const DevScene* scenes;
+    const Params* params;
+    const DevCamera* cams;
+    const float4* const* geoms;  // per scene: prim_geom
+    const uint32_t* roots;
+    const SceneFilm* films;
+};
+
+// Last segment whose path_begin (by_pixel: pix_begin) is <= x.
+__device__ inline uint32_t find_segment(const BatchArgs& B, uint32_t x, bool by_pixel) {
+    uint32_t lo = 0, hi = B.n_seg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t key = by_pixel ? B.seg[mid].pix_begin : B.seg[mid].path_begin;
+        if (key <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 struct PathState {
     float* thr;
@@ -69,11 +108,24 @@ __device__ inline lr::u4 rng(const Params& P, uint64_t path, uint32_t gen, uint3
 }
 
 // -- generation 0: camera rays (perspective_camera.cpp:406-434) -------------
-__global__ void k_camera(DevScene S, DevCamera C, Params P, uint64_t path0, uint32_t n, PathState ps, Queue q,
-                         uint32_t* count, uint64_t* rays_total) {
+template <bool BATCH>
+__global__ void k_camera(DevScene S0, DevCamera C0, Params P0, uint64_t path0, uint32_t n, PathState ps, Queue q,
+                         uint32_t* count, uint64_t* rays_total, BatchArgs B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint64_t p = path0 + i;
+    DevScene S = S0;
+    DevCamera C = C0;
+    Params P = P0;
+    uint64_t p = path0 + i;
+    uint32_t root = 0;
+    if (BATCH) {
+        const Segment sg = B.seg[find_segment(B, i, false)];
+        S = B.scenes[sg.scene];
+        C = B.cams[sg.scene];
+        P = B.params[sg.scene];
+        p = (uint64_t)sg.pixel0 * (P.spp * P.nl) + (i - sg.path_begin);
+        root = B.roots[sg.scene];
+    }
     const uint32_t l = (uint32_t)(p % P.nl);
     const uint64_t sidx = p / P.nl;
     const uint32_t s = (uint32_t)(sidx % P.spp);
@@ -129,6 +181,7 @@ __global__ void k_camera(DevScene S, DevCamera C, Params P, uint64_t path0, uint
             q.r[6][pos] = fmaxf(kTolerance, ray_factor * C.near_limit);
             q.r[7][pos] = INFINITY;
             q.pid[pos] = i;
+            if (BATCH) q.root[pos] = root;
         }
     }
 }
@@ -305,17 +358,29 @@ __global__ void k_gather_batch(uint32_t cap, const uint32_t* __restrict__ count,
 }
 
 // -- one generation: emission + scatter (adjoint_photon_tracer.cpp:140-292) --
-template <bool GUIDED>
-__global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queue q, const uint32_t* count,
-                        Hits h, const float4* __restrict__ prim_geom, PathState ps, Queue qn, uint32_t* count_next,
-                        uint64_t* rays_total, GuideArgs G) {
+template <bool GUIDED, bool BATCH>
+__global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Queue q, const uint32_t* count,
+                        Hits h, const float4* __restrict__ prim_geom0, PathState ps, Queue qn, uint32_t* count_next,
+                        uint64_t* rays_total, GuideArgs G, BatchArgs B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < *count;
     bool alive = false;
     V3 start = v3(0, 0, 0), dir = v3(0, 0, 1);
-    uint32_t pid = 0;
+    uint32_t pid = 0, root = 0;
     if (valid) {
         pid = q.pid[i];
+        DevScene S = S0;
+        Params P = P0;
+        const float4* __restrict__ prim_geom = prim_geom0;
+        uint64_t gp = path0 + pid;  // the path's index in its scene (random streams)
+        if (BATCH) {
+            const Segment sg = B.seg[find_segment(B, pid, false)];
+            S = B.scenes[sg.scene];
+            P = B.params[sg.scene];
+            prim_geom = B.geoms[sg.scene];
+            gp = (uint64_t)sg.pixel0 * (P.spp * P.nl) + (pid - sg.path_begin);
+            root = q.root[i];
+        }
         const V3 o = v3(q.r[0][i], q.r[1][i], q.r[2][i]);
         const V3 d = v3(q.r[3][i], q.r[4][i], q.r[5][i]);
         const float tnear = q.r[6][i];
@@ -372,7 +437,6 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
                 onb(nf, tt, bb);
                 const V3 wiw = -d;
                 const V3 wi = v3(dot(wiw, tt), dot(wiw, bb), fmaxf(dot(wiw, nf), 1.1920929e-7f));
-                const uint64_t gp = path0 + pid;
                 const lr::u4 r0 = rng(P, gp, gen, 1u);
                 const lr::u4 r1 = rng(P, gp, gen, 2u);
                 const float u_strat = lr::u01(r0.x), u1 = lr::u01(r0.y), u2 = lr::u01(r0.z);
@@ -540,19 +604,36 @@ __global__ void k_shade(DevScene S, Params P, uint32_t gen, uint64_t path0, Queu
             qn.r[6][pos] = kTolerance;
             qn.r[7][pos] = INFINITY;
             qn.pid[pos] = pid;
+            if (BATCH) qn.root[pos] = root;
         }
     }
 }
 
 // -- film: Raster box filter, fixed-order per-pixel sums (raster.cpp:547-606)
-__global__ void k_splat(DevScene S, Params P, uint64_t pix0, uint32_t npix, PathState ps, float* film_xyz,
-                        float* film_w) {
+template <bool BATCH>
+__global__ void k_splat(DevScene S0, Params P0, uint64_t pix0, uint32_t npix, PathState ps, float* film_xyz0,
+                        float* film_w0, BatchArgs B) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= npix) return;
+    DevScene S = S0;
+    Params P = P0;
+    float* film_xyz = film_xyz0;
+    float* film_w = film_w0;
+    uint64_t px = pix0 + j;
+    uint32_t first = j * (P.spp * P.nl);  // chunk-local index of the pixel's first path
+    if (BATCH) {
+        const Segment sg = B.seg[find_segment(B, j, true)];
+        S = B.scenes[sg.scene];
+        P = B.params[sg.scene];
+        film_xyz = B.films[sg.scene].xyz;
+        film_w = B.films[sg.scene].w;
+        px = sg.pixel0 + (j - sg.pix_begin);
+        first = sg.path_begin + (j - sg.pix_begin) * (P.spp * P.nl);
+    }
     const uint32_t ppp = P.spp * P.nl;
     float X = 0.0f, Y = 0.0f, Z = 0.0f;
     for (uint32_t k = 0; k < ppp; ++k) {
-        const uint32_t i = j * ppp + k;
+        const uint32_t i = first + k;
         const float L = ps.L[i];
         if (L == 0.0f) continue;
         float xyz[3];
@@ -563,7 +644,6 @@ __global__ void k_splat(DevScene S, Params P, uint64_t pix0, uint32_t npix, Path
         Y += s * xyz[1];
         Z += s * xyz[2];
     }
-    const uint64_t px = pix0 + j;
     const uint64_t np = (uint64_t)P.width * P.height;
     film_xyz[px] += X;
     film_xyz[np + px] += Y;
@@ -649,6 +729,10 @@ struct lr_scene {
     // guided mode: inputs [cap][8], mixture [cap][40], records
     float* d_gwork = nullptr;
     size_t gcap = 0, grec = 0, gbatch = 0;
+    // graph mode: capture stream, ordering events, instantiated graph
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -656,6 +740,14 @@ namespace {
 void free_scene(lr_scene* s) {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
+    if (s->cstream) {
+        (void)hipStreamSynchronize(s->cstream);
+        s->ctx->release(s->cstream);
+    }
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->ev_in) (void)hipEventDestroy(s->ev_in);
+    if (s->ev_out) (void)hipEventDestroy(s->ev_out);
+    if (s->cstream) (void)hipStreamDestroy(s->cstream);
     void* ptrs[] = {s->d_prim_mat, s->d_prim_geom, s->d_mats, s->d_spectra, s->d_table, s->d_lights,
                     s->d_obs,      s->d_work,      s->d_counts, s->d_gwork};
     for (void* p : ptrs)
@@ -968,6 +1060,95 @@ int lr_scene_destroy(lr_scene* sc) {
 
 }  // extern "C"
 
+// Camera frame of an lr_camera (perspective_camera.cpp:126-128, 259, 466-468).
+static int make_camera(const lr_camera* cam, DevCamera& C) {
+    if (!(cam->fov > 0.0f && cam->fov < 3.14159f) || !(cam->aspect > 0.0f))
+        return lr::fail(LR_E_INVALID, "camera fov must be in (0, pi) and aspect > 0");
+    double d[3] = {cam->direction[0], cam->direction[1], cam->direction[2]};
+    double sk[3] = {cam->sky[0], cam->sky[1], cam->sky[2]};
+    auto nrm = [](double* v) {
+        const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (l > 0)
+            for (int k = 0; k < 3; ++k) v[k] /= l;
+        return l;
+    };
+    if (!(nrm(d) > 0)) return lr::fail(LR_E_INVALID, "camera direction is zero");
+    double r[3] = {d[1] * sk[2] - d[2] * sk[1], d[2] * sk[0] - d[0] * sk[2], d[0] * sk[1] - d[1] * sk[0]};
+    if (!(nrm(r) > 0)) return lr::fail(LR_E_INVALID, "camera sky is parallel to its direction");
+    double dn[3] = {d[1] * r[2] - d[2] * r[1], d[2] * r[0] - d[0] * r[2], d[0] * r[1] - d[1] * r[0]};
+    nrm(dn);
+    const double width = 0.036, height = width / cam->aspect;
+    const double focal = (width / 2) / std::tan(cam->fov / 2);
+    for (int k = 0; k < 3; ++k) {
+        C.pos[k] = cam->position[k];
+        C.dir[k] = (float)(d[k] * focal);
+        C.right[k] = (float)(r[k] * width);
+        C.down[k] = (float)(dn[k] * height);
+        C.dir_n[k] = (float)d[k];
+    }
+    C.near_limit = cam->near_limit;
+    return LR_OK;
+}
+
+// Validated kernel parameters of an lr_render_params.
+static int make_params(const lr_render_params& R, Params& P) {
+    if (!R.width || !R.height || !R.spp || !R.wavelengths)
+        return lr::fail(LR_E_INVALID, "width, height, spp and wavelengths must be > 0");
+    if (!(R.rr_survival > 0.0f && R.rr_survival <= 1.0f)) return lr::fail(LR_E_INVALID, "rr_survival must be in (0, 1]");
+    if (!(R.lambda_max > R.lambda_min)) return lr::fail(LR_E_INVALID, "lambda_max must exceed lambda_min");
+    if (!(R.max_sample_luminance >= 0.0f)) return lr::fail(LR_E_INVALID, "max_sample_luminance must be >= 0");
+    P.width = R.width;
+    P.height = R.height;
+    P.spp = R.spp;
+    P.nl = R.wavelengths;
+    P.max_gen = R.max_depth ? R.max_depth : 64u;
+    P.rr_depth = R.rr_depth;
+    P.rr_p = R.rr_survival;
+    P.max_lum = R.max_sample_luminance;
+    P.seed_lo = (uint32_t)R.seed;
+    P.seed_hi = (uint32_t)(R.seed >> 32);
+    const uint32_t q = (uint32_t)std::lround(std::sqrt((double)R.spp));
+    P.strata = q * q == R.spp ? q : 0u;
+    return LR_OK;
+}
+
+// Device view of a scene (its observer table must be built for the call's domain).
+static DevScene dev_scene(const lr_scene* sc) {
+    DevScene S;
+    S.nodes = sc->bvh->d_nodes;
+    S.prims = sc->bvh->d_prims;
+    S.prim_mat = sc->d_prim_mat;
+    S.mats = sc->d_mats;
+    S.spectra = sc->d_spectra;
+    S.table = sc->d_table;
+    S.lights = sc->d_lights;
+    S.n_lights = sc->n_lights;
+    S.n_triangles = sc->n_triangles;
+    S.obs.n = sc->obs_n;
+    S.obs.w = sc->d_obs;
+    S.obs.cdf = sc->d_obs + sc->obs_n;
+    S.obs.xyz = sc->d_obs + 2 * sc->obs_n;
+    S.obs.dxyz = sc->d_obs + 5 * sc->obs_n;
+    return S;
+}
+
+// Launches a captured graph through a cached executable: updated in place
+// when the topology allows it, re-instantiated otherwise.
+static int launch_graph(hipGraphExec_t& exec, hipGraph_t graph, hipStream_t s) {
+    if (exec) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(exec, graph, &err_node, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            LR_HIP_CHECK(hipGraphExecDestroy(exec));
+            exec = nullptr;
+        }
+    }
+    if (!exec) LR_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    LR_HIP_CHECK(hipGraphLaunch(exec, s));
+    return LR_OK;
+}
+
 static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, lr_guide* guide,
                        const lr_guide_params* gparams, float* film_xyz, float* film_w, lr_render_stats* stats,
                        void* stream) {
@@ -977,57 +1158,14 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     const lr_render_params& R = *prm;
     if (!R.width || !R.height || !R.spp || !R.wavelengths)
         return lr::fail(LR_E_INVALID, "width, height, spp and wavelengths must be > 0");
-    if (!(R.rr_survival > 0.0f && R.rr_survival <= 1.0f)) return lr::fail(LR_E_INVALID, "rr_survival must be in (0, 1]");
-    if (!(R.lambda_max > R.lambda_min)) return lr::fail(LR_E_INVALID, "lambda_max must exceed lambda_min");
-    if (!(cam->fov > 0.0f && cam->fov < 3.14159f) || !(cam->aspect > 0.0f))
-        return lr::fail(LR_E_INVALID, "camera fov must be in (0, pi) and aspect > 0");
     if (int rc = lr::bind(sc->ctx)) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (int rc = build_observer(sc, R.lambda_min, R.lambda_max)) return rc;
 
-    // camera frame (perspective_camera.cpp:126-128, 259, 466-468)
     DevCamera C;
-    {
-        double d[3] = {cam->direction[0], cam->direction[1], cam->direction[2]};
-        double sk[3] = {cam->sky[0], cam->sky[1], cam->sky[2]};
-        auto nrm = [](double* v) {
-            const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-            if (l > 0)
-                for (int k = 0; k < 3; ++k) v[k] /= l;
-            return l;
-        };
-        if (!(nrm(d) > 0)) return lr::fail(LR_E_INVALID, "camera direction is zero");
-        double r[3] = {d[1] * sk[2] - d[2] * sk[1], d[2] * sk[0] - d[0] * sk[2], d[0] * sk[1] - d[1] * sk[0]};
-        if (!(nrm(r) > 0)) return lr::fail(LR_E_INVALID, "camera sky is parallel to its direction");
-        double dn[3] = {d[1] * r[2] - d[2] * r[1], d[2] * r[0] - d[0] * r[2], d[0] * r[1] - d[1] * r[0]};
-        nrm(dn);
-        const double width = 0.036, height = width / cam->aspect;
-        const double focal = (width / 2) / std::tan(cam->fov / 2);
-        for (int k = 0; k < 3; ++k) {
-            C.pos[k] = cam->position[k];
-            C.dir[k] = (float)(d[k] * focal);
-            C.right[k] = (float)(r[k] * width);
-            C.down[k] = (float)(dn[k] * height);
-            C.dir_n[k] = (float)d[k];
-        }
-        C.near_limit = cam->near_limit;
-    }
+    if (int rc = make_camera(cam, C)) return rc;
     Params P;
-    P.width = R.width;
-    P.height = R.height;
-    P.spp = R.spp;
-    P.nl = R.wavelengths;
-    P.max_gen = R.max_depth ? R.max_depth : 64u;
-    P.rr_depth = R.rr_depth;
-    P.rr_p = R.rr_survival;
-    P.max_lum = R.max_sample_luminance;
-    if (!(P.max_lum >= 0.0f)) return lr::fail(LR_E_INVALID, "max_sample_luminance must be >= 0");
-    P.seed_lo = (uint32_t)R.seed;
-    P.seed_hi = (uint32_t)(R.seed >> 32);
-    {
-        const uint32_t q = (uint32_t)std::lround(std::sqrt((double)R.spp));
-        P.strata = q * q == R.spp ? q : 0u;
-    }
+    if (int rc = make_params(R, P)) return rc;
     const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
     const uint64_t n_pix = (uint64_t)R.width * R.height;
     const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 22);
@@ -1083,21 +1221,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         b_w = b_dir + 3 * (size_t)batch;
     }
 
-    DevScene S;
-    S.nodes = sc->bvh->d_nodes;
-    S.prims = sc->bvh->d_prims;
-    S.prim_mat = sc->d_prim_mat;
-    S.mats = sc->d_mats;
-    S.spectra = sc->d_spectra;
-    S.table = sc->d_table;
-    S.lights = sc->d_lights;
-    S.n_lights = sc->n_lights;
-    S.n_triangles = sc->n_triangles;
-    S.obs.n = sc->obs_n;
-    S.obs.w = sc->d_obs;
-    S.obs.cdf = sc->d_obs + sc->obs_n;
-    S.obs.xyz = sc->d_obs + 2 * sc->obs_n;
-    S.obs.dxyz = sc->d_obs + 5 * sc->obs_n;
+    const DevScene S = dev_scene(sc);
 
     float* w = sc->d_work;
     PathState ps{w, w + cap, w + 2 * (size_t)cap, w + 3 * (size_t)cap};
@@ -1106,14 +1230,39 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         float* base = w + (4 + 9 * b) * (size_t)cap;
         for (int k = 0; k < 8; ++k) q[b].r[k] = base + k * (size_t)cap;
         q[b].pid = reinterpret_cast<uint32_t*>(base + 8 * (size_t)cap);
+        q[b].root = nullptr;
     }
     float* hb = w + 22 * (size_t)cap;
     Hits H{reinterpret_cast<uint32_t*>(hb), hb + cap, hb + 2 * (size_t)cap, hb + 3 * (size_t)cap};
     uint32_t* counts = sc->d_counts;
     uint64_t* rays_total = reinterpret_cast<uint64_t*>(sc->d_counts + 4);
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 16u);
+    BatchArgs NB;
+    std::memset(&NB, 0, sizeof(NB));
+    const bool training = guided && gparams->train && gparams->train_steps > 0;
+
+    // graph mode: every launch below is captured on the scene's own stream
+    // (the caller's may be the legacy null stream, which cannot capture)
+    const hipStream_t caller = s;
+    if (R.graph) {
+        if (!sc->cstream) {
+            LR_HIP_CHECK(hipStreamCreateWithFlags(&sc->cstream, hipStreamNonBlocking));
+            LR_HIP_CHECK(hipEventCreateWithFlags(&sc->ev_in, hipEventDisableTiming));
+            LR_HIP_CHECK(hipEventCreateWithFlags(&sc->ev_out, hipEventDisableTiming));
+        }
+        s = sc->cstream;
+        // everything the captured launches use is allocated and queried first
+        if (sc->bvh->info.n_nodes)
+            if (int rc = lr::trace_reserve(sc->bvh, cap, s)) return rc;
+        if (training)
+            if (int rc = lr_guide_reserve(guide, batch, s)) return rc;
+        LR_HIP_CHECK(hipEventRecord(sc->ev_in, caller));
+        LR_HIP_CHECK(hipStreamWaitEvent(s, sc->ev_in, 0));
+        LR_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    }
 
     uint32_t chunks = 0;
+    auto body = [&]() -> int {
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 16u);
     for (uint64_t pix0 = 0; pix0 < n_pix; pix0 += pix_per_chunk) {
         const uint32_t npix = (uint32_t)std::min<uint64_t>(pix_per_chunk, n_pix - pix0);
         const uint32_t n = (uint32_t)(npix * ppp);
@@ -1121,7 +1270,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         const dim3 blk(256), grd((n + 255) / 256);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 1u);
         if (guided) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + 8, 1u);
-        hipLaunchKernelGGL(k_camera, grd, blk, 0, s, S, C, P, path0, n, ps, q[0], counts, rays_total);
+        hipLaunchKernelGGL(k_camera<false>, grd, blk, 0, s, S, C, P, path0, n, ps, q[0], counts, rays_total, NB);
         for (uint32_t gen = 0; gen < P.max_gen; ++gen) {
             const int cur = gen & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + nxt, 1u);
@@ -1136,14 +1285,14 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             if (guided) {
                 hipLaunchKernelGGL(k_guide_inputs, grd, blk, 0, s, S, q[cur], counts + cur, H, ps, GA, g_in);
                 if (int rc = lr_guide_infer(guide, g_in, n, counts + cur, g_mix, s)) return rc;
-                hipLaunchKernelGGL(k_shade<true>, grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
-                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total, GA);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
+                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total, GA, NB);
             } else {
-                hipLaunchKernelGGL(k_shade<false>, grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
-                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total, GA);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, false>), grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
+                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total, GA, NB);
             }
         }
-        if (guided && gparams->train && gparams->train_steps > 0) {
+        if (training) {
             // recorded vertices -> training weights -> Adam steps on batches drawn
             // from all of them; the record count stays on the device
             const uint32_t* nrec = counts + 8;
@@ -1160,10 +1309,27 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
                 if (int rc = lr_guide_train_step(guide, b_in, b_dir, b_w, batch, nullptr, nullptr, s)) return rc;
             }
         }
-        hipLaunchKernelGGL(k_splat, dim3((npix + 255) / 256), blk, 0, s, S, P, pix0, npix, ps, film_xyz, film_w);
+        hipLaunchKernelGGL(k_splat<false>, dim3((npix + 255) / 256), blk, 0, s, S, P, pix0, npix, ps, film_xyz, film_w,
+                           NB);
         LR_HIP_CHECK(hipGetLastError());
         ++chunks;
     }
+    return LR_OK;
+    };
+    int rc = body();
+    if (R.graph) {
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (!rc && ec != hipSuccess) rc = lr::fail(LR_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        if (!rc) rc = launch_graph(sc->gexec, graph, s);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!rc) {
+            LR_HIP_CHECK(hipEventRecord(sc->ev_out, s));
+            LR_HIP_CHECK(hipStreamWaitEvent(caller, sc->ev_out, 0));
+        }
+        s = caller;
+    }
+    if (rc) return rc;
     if (stats) {
         uint64_t rays = 0;
         LR_HIP_CHECK(hipMemcpyAsync(&rays, rays_total, sizeof(rays), hipMemcpyDeviceToHost, s));
@@ -1176,8 +1342,340 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     return LR_OK;
 }
 
+// -- batches: many scenes in one wavefront -------------------------------------
+
+__global__ void k_stamp(uint64_t* out) {
+    if (threadIdx.x == 0) out[0] = wall_clock64();
+}
+
+struct lr_batch {
+    lr_context* ctx = nullptr;
+    std::vector<lr_scene*> scenes;
+    lr_qbvh tree;                 // every scene's nodes / prims in one array, indices rebased
+    std::vector<uint32_t> roots;  // root node of each scene
+    uint32_t* d_roots = nullptr;
+    // per-call tables (grow-only)
+    DevScene* d_scenes = nullptr;
+    Params* d_params = nullptr;
+    DevCamera* d_cams = nullptr;
+    const float4** d_geoms = nullptr;
+    SceneFilm* d_films = nullptr;
+    Segment* d_seg = nullptr;
+    size_t seg_cap = 0;
+    uint64_t* d_stamps = nullptr;
+    size_t stamp_cap = 0;
+    // wavefront workspace (grow-only): path state 4, two queues 2 x 10, hits 4
+    size_t cap = 0;
+    float* d_work = nullptr;
+    uint32_t* d_counts = nullptr;
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+void free_batch(lr_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->cstream) {
+        (void)hipStreamSynchronize(b->cstream);
+        b->ctx->release(b->cstream);
+    }
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+    if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+    if (b->cstream) (void)hipStreamDestroy(b->cstream);
+    void* ptrs[] = {b->tree.d_nodes, b->tree.d_prims, b->d_roots, b->d_scenes, b->d_params, b->d_cams,
+                    (void*)b->d_geoms, b->d_films, b->d_seg, b->d_stamps, b->d_work, b->d_counts};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+template <class T>
+int grow(T** p, size_t& cap, size_t n) {
+    if (cap >= n) return LR_OK;
+    if (*p) LR_HIP_CHECK(hipFree((void*)*p));
+    *p = nullptr;
+    LR_HIP_CHECK(hipMalloc((void**)p, sizeof(T) * n));
+    cap = n;
+    return LR_OK;
+}
+
+struct ChunkPlan {
+    uint32_t seg_begin, n_seg, n_paths, n_pix, max_gen;
+};
+
+}  // namespace
 
 extern "C" {
+
+int lr_batch_create(lr_context* ctx, lr_scene* const* scenes, uint32_t n, lr_batch** out) {
+    if (!ctx || !scenes || !out || n == 0) return lr::fail(LR_E_INVALID, "need a context, scenes and out");
+    *out = nullptr;
+    uint64_t nodes = 0, prims = 0;
+    uint32_t max_stack = 1, max_depth = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const lr_scene* sc = scenes[k];
+        if (!sc) return lr::fail(LR_E_INVALID, "scene " + std::to_string(k) + " is NULL");
+        if (sc->ctx != ctx) return lr::fail(LR_E_INVALID, "scene " + std::to_string(k) + " belongs to another context");
+        if (sc->bvh->host.nodes.empty())
+            return lr::fail(LR_E_INVALID, "scene " + std::to_string(k) + " has no geometry (batches need a tree per scene)");
+        nodes += sc->bvh->host.nodes.size();
+        prims += sc->bvh->host.prims.size();
+        max_stack = std::max(max_stack, sc->bvh->info.max_stack);
+        max_depth = std::max(max_depth, sc->bvh->info.max_depth);
+    }
+    if (nodes >= (1ull << 31) || prims >= (1ull << 29))
+        return lr::fail(LR_E_INVALID, "batch too large for the node encoding (2^31 nodes, 2^29 primitives)");
+    if (int rc = lr::bind(ctx)) return rc;
+    lr_batch* b = new (std::nothrow) lr_batch();
+    if (!b) return lr::fail(LR_E_NOMEM, "out of host memory");
+    b->ctx = ctx;
+    b->scenes.assign(scenes, scenes + n);
+    // concatenate the trees: inner children += node base, leaf first prim += prim base
+    std::vector<lr_node> hn;
+    std::vector<lr_prim> hp;
+    hn.reserve(nodes);
+    hp.reserve(prims);
+    for (uint32_t k = 0; k < n; ++k) {
+        const lr::BuiltQbvh& H = scenes[k]->bvh->host;
+        const uint32_t nb = (uint32_t)hn.size(), pb = (uint32_t)hp.size();
+        b->roots.push_back(nb);
+        for (const lr_node& nd : H.nodes) {
+            lr_node m = nd;
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t e = nd.child[c];
+                if (e == LR_CHILD_EMPTY) continue;
+                if (e & 0x80000000u) {
+                    const uint32_t first = ((e >> 2) & 0x1FFFFFFFu) + pb;
+                    m.child[c] = 0x80000000u | (first << 2) | (e & 3u);
+                } else {
+                    m.child[c] = e + nb;
+                }
+            }
+            hn.push_back(m);
+        }
+        hp.insert(hp.end(), H.prims.begin(), H.prims.end());
+    }
+    lr_qbvh& T = b->tree;
+    T.ctx = ctx;
+    T.info.n_nodes = (uint32_t)hn.size();
+    T.info.n_prims = (uint32_t)hp.size();
+    T.info.max_depth = max_depth;
+    T.info.max_stack = max_stack;
+    hipError_t e = hipMalloc(&T.d_nodes, sizeof(lr_node) * hn.size());
+    if (e == hipSuccess) e = hipMalloc(&T.d_prims, sizeof(lr_prim) * hp.size());
+    if (e == hipSuccess) e = hipMalloc(&b->d_roots, sizeof(uint32_t) * n);
+    if (e == hipSuccess) e = hipMemcpy(T.d_nodes, hn.data(), sizeof(lr_node) * hn.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(T.d_prims, hp.data(), sizeof(lr_prim) * hp.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_roots, b->roots.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->d_scenes, sizeof(DevScene) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_params, sizeof(Params) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_cams, sizeof(DevCamera) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_geoms, sizeof(float4*) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_films, sizeof(SceneFilm) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_counts, 64 * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        free_batch(b);
+        return lr::fail(LR_E_DEVICE, std::string("lr_batch_create: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return LR_OK;
+}
+
+int lr_batch_destroy(lr_batch* b) {
+    free_batch(b);
+    return LR_OK;
+}
+
+int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, float* scene_ms, lr_render_stats* stats,
+                    void* stream) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!b || !items) return lr::fail(LR_E_INVALID, "NULL batch or items");
+    if (int rc = lr::bind(b->ctx)) return rc;
+    const uint32_t n = (uint32_t)b->scenes.size();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<DevScene> hs(n);
+    std::vector<Params> hprm(n);
+    std::vector<DevCamera> hcam(n);
+    std::vector<const float4*> hgeo(n);
+    std::vector<SceneFilm> hfilm(n);
+    std::vector<uint64_t> paths(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        const lr_batch_item& it = items[k];
+        if (!it.film_xyz || !it.film_weight) return lr::fail(LR_E_INVALID, "item " + std::to_string(k) + ": NULL film");
+        if (int rc = make_camera(&it.camera, hcam[k])) return rc;
+        if (int rc = make_params(it.params, hprm[k])) return rc;
+        lr_scene* sc = b->scenes[k];
+        if (int rc = build_observer(sc, it.params.lambda_min, it.params.lambda_max)) return rc;
+        hs[k] = dev_scene(sc);
+        hgeo[k] = sc->d_prim_geom;
+        hfilm[k] = SceneFilm{it.film_xyz, it.film_weight};
+        paths[k] = (uint64_t)it.params.width * it.params.height * it.params.spp * it.params.wavelengths;
+    }
+    // shortest scene first: the mean and median completion times are least
+    std::vector<uint32_t> order(n);
+    for (uint32_t k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return paths[a] < paths[c]; });
+    // chunks of whole pixels, up to 4M paths each (or the smallest nonzero
+    // chunk_paths of the items), spanning scenes
+    uint64_t cap = 1u << 22;
+    for (uint32_t k = 0; k < n; ++k)
+        if (items[k].params.chunk_paths) cap = std::min<uint64_t>(cap, items[k].params.chunk_paths);
+    std::vector<Segment> segs;
+    std::vector<ChunkPlan> chunks;
+    std::vector<uint32_t> last_chunk(n, 0);
+    ChunkPlan cur{0, 0, 0, 0, 0};
+    for (uint32_t k : order) {
+        const Params& P = hprm[k];
+        const uint64_t ppp = (uint64_t)P.spp * P.nl;
+        if (ppp > cap) return lr::fail(LR_E_INVALID, "item " + std::to_string(k) + ": spp * wavelengths exceeds a chunk");
+        const uint64_t npix = (uint64_t)P.width * P.height;
+        uint64_t done = 0;
+        while (done < npix) {
+            uint64_t fit = (cap - cur.n_paths) / ppp;
+            if (fit == 0) {
+                chunks.push_back(cur);
+                cur = ChunkPlan{(uint32_t)segs.size(), 0, 0, 0, 0};
+                fit = cap / ppp;
+            }
+            const uint64_t take = std::min(fit, npix - done);
+            segs.push_back(Segment{k, cur.n_paths, cur.n_pix, (uint32_t)done});
+            cur.n_seg += 1;
+            cur.n_paths += (uint32_t)(take * ppp);
+            cur.n_pix += (uint32_t)take;
+            cur.max_gen = std::max(cur.max_gen, P.max_gen);
+            done += take;
+            last_chunk[k] = (uint32_t)chunks.size();
+        }
+    }
+    if (cur.n_seg) chunks.push_back(cur);
+    // the previous call may still read the tables: wait for it before rewriting them
+    if (b->cstream) LR_HIP_CHECK(hipStreamSynchronize(b->cstream));
+    LR_HIP_CHECK(hipStreamSynchronize(s));
+    if (int rc = grow(&b->d_seg, b->seg_cap, segs.size())) return rc;
+    if (int rc = grow(&b->d_stamps, b->stamp_cap, chunks.size() + 1)) return rc;
+    LR_HIP_CHECK(hipMemcpy(b->d_seg, segs.data(), sizeof(Segment) * segs.size(), hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMemcpy(b->d_scenes, hs.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMemcpy(b->d_params, hprm.data(), sizeof(Params) * n, hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMemcpy(b->d_cams, hcam.data(), sizeof(DevCamera) * n, hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMemcpy((void*)b->d_geoms, hgeo.data(), sizeof(float4*) * n, hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMemcpy(b->d_films, hfilm.data(), sizeof(SceneFilm) * n, hipMemcpyHostToDevice));
+    if (b->cap < cap) {
+        if (b->d_work) LR_HIP_CHECK(hipFree(b->d_work));
+        b->d_work = nullptr;
+        LR_HIP_CHECK(hipMalloc(&b->d_work, sizeof(float) * 28 * cap));
+        b->cap = cap;
+    }
+    float* w = b->d_work;
+    PathState ps{w, w + cap, w + 2 * cap, w + 3 * cap};
+    Queue q[2];
+    for (int k = 0; k < 2; ++k) {
+        float* base = w + (4 + 10 * k) * cap;
+        for (int r = 0; r < 8; ++r) q[k].r[r] = base + r * cap;
+        q[k].pid = reinterpret_cast<uint32_t*>(base + 8 * cap);
+        q[k].root = reinterpret_cast<uint32_t*>(base + 9 * cap);
+    }
+    float* hb = w + 24 * cap;
+    Hits H{reinterpret_cast<uint32_t*>(hb), hb + cap, hb + 2 * cap, hb + 3 * cap};
+    uint32_t* counts = b->d_counts;
+    uint64_t* rays_total = reinterpret_cast<uint64_t*>(b->d_counts + 4);
+    BatchArgs B;
+    B.scenes = b->d_scenes;
+    B.params = b->d_params;
+    B.cams = b->d_cams;
+    B.geoms = b->d_geoms;
+    B.roots = b->d_roots;
+    B.films = b->d_films;
+    GuideArgs NG;
+    std::memset(&NG, 0, sizeof(NG));
+    const DevScene S0 = hs[0];
+    const Params P0 = hprm[0];
+    const DevCamera C0 = hcam[0];
+
+    const bool graph = (flags & LR_BATCH_GRAPH) != 0;
+    const hipStream_t caller = s;
+    if (graph) {
+        if (!b->cstream) {
+            LR_HIP_CHECK(hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking));
+            LR_HIP_CHECK(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
+            LR_HIP_CHECK(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
+        }
+        s = b->cstream;
+    }
+    if (int rc = lr::trace_reserve(&b->tree, (uint32_t)cap, s)) return rc;
+    if (graph) {
+        LR_HIP_CHECK(hipEventRecord(b->ev_in, caller));
+        LR_HIP_CHECK(hipStreamWaitEvent(s, b->ev_in, 0));
+        LR_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    }
+    auto body = [&]() -> int {
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 16u);
+        hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, b->d_stamps);
+        for (size_t c = 0; c < chunks.size(); ++c) {
+            const ChunkPlan& ch = chunks[c];
+            B.seg = b->d_seg + ch.seg_begin;
+            B.n_seg = ch.n_seg;
+            const dim3 blk(256), grd((ch.n_paths + 255) / 256);
+            hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 1u);
+            hipLaunchKernelGGL(k_camera<true>, grd, blk, 0, s, S0, C0, P0, (uint64_t)0, ch.n_paths, ps, q[0], counts,
+                               rays_total, B);
+            for (uint32_t gen = 0; gen < ch.max_gen; ++gen) {
+                const int cu = gen & 1, nx = cu ^ 1;
+                hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + nx, 1u);
+                lr_rays_soa rays{q[cu].r[0], q[cu].r[1], q[cu].r[2], q[cu].r[3],
+                                 q[cu].r[4], q[cu].r[5], q[cu].r[6], q[cu].r[7]};
+                lr_hits_soa hits{H.prim, H.t, H.u, H.v};
+                if (int rc = lr::trace_closest_queue(&b->tree, &rays, ch.n_paths, counts + cu, &hits, s, q[cu].root))
+                    return rc;
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, true>), grd, blk, 0, s, S0, P0, gen, (uint64_t)0,
+                                   q[cu], counts + cu, H, hgeo[0], ps, q[nx], counts + nx, rays_total, NG, B);
+            }
+            hipLaunchKernelGGL(k_splat<true>, dim3((ch.n_pix + 255) / 256), blk, 0, s, S0, P0, (uint64_t)0, ch.n_pix,
+                               ps, (float*)nullptr, (float*)nullptr, B);
+            hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, b->d_stamps + c + 1);
+            LR_HIP_CHECK(hipGetLastError());
+        }
+        return LR_OK;
+    };
+    int rc = body();
+    if (graph) {
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (!rc && ec != hipSuccess) rc = lr::fail(LR_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        if (!rc) rc = launch_graph(b->gexec, g, s);
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) {
+            LR_HIP_CHECK(hipEventRecord(b->ev_out, s));
+            LR_HIP_CHECK(hipStreamWaitEvent(caller, b->ev_out, 0));
+        }
+        s = caller;
+    }
+    if (rc) return rc;
+    if (scene_ms || stats) {
+        std::vector<uint64_t> st(chunks.size() + 1);
+        uint64_t rays = 0;
+        LR_HIP_CHECK(hipMemcpyAsync(st.data(), b->d_stamps, sizeof(uint64_t) * st.size(), hipMemcpyDeviceToHost, s));
+        LR_HIP_CHECK(hipMemcpyAsync(&rays, rays_total, sizeof(rays), hipMemcpyDeviceToHost, s));
+        LR_HIP_CHECK(hipStreamSynchronize(s));
+        int khz = 0;
+        LR_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->ctx->device));
+        if (scene_ms)
+            for (uint32_t k = 0; k < n; ++k)
+                scene_ms[k] = khz > 0 ? (float)((double)(st[last_chunk[k] + 1] - st[0]) / (double)khz) : 0.0f;
+        if (stats) {
+            uint64_t total = 0;
+            for (uint64_t p : paths) total += p;
+            stats->paths = total;
+            stats->rays = rays;
+            stats->chunks = (uint32_t)chunks.size();
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    }
+    return LR_OK;
+}
 
 int lr_render(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, float* film_xyz, float* film_w,
               lr_render_stats* stats, void* stream) {

@@ -33,6 +33,7 @@ constexpr float kCullSlack = 1.0f + 1.0f / 65536.0f;
 
 struct RaysArg {
     const float *ox, *oy, *oz, *dx, *dy, *dz, *tnear, *tfar;
+    const uint32_t* root;  // per-ray root node (several trees in one node array), or NULL: node 0
 };
 struct HitsArg {
     uint32_t* prim;
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
                         tri_u = tri_v = 0.0f;
                         tri_id = sph_id = LR_MISS;
                         doomed = false;
-                        cur = 0;
+                        cur = rays.root ? rays.root[my] : 0u;
                         st.sp = 0;
                     }
                 }
@@ -371,8 +372,17 @@ int prepare(lr_qbvh* bvh, uint32_t n, bool any, hipStream_t stream, Launch& L) {
     L.block = cfg.block;
     L.refill_min = cfg.refill_min;
     int occ = 0;
-    LR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(L.fn),
-                                                              (int)cfg.block, 0));
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto it = ctx->occupancy.find(reinterpret_cast<const void*>(L.fn));
+        if (it != ctx->occupancy.end()) occ = it->second;
+    }
+    if (occ == 0) {
+        LR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(L.fn),
+                                                                  (int)cfg.block, 0));
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->occupancy[reinterpret_cast<const void*>(L.fn)] = occ;
+    }
     occ = std::max(1, std::min(occ, (int)cfg.blocks_per_cu));
     const uint64_t want = ((uint64_t)n + cfg.block - 1) / cfg.block;
     const uint64_t maxg = (uint64_t)ctx->n_cu * (uint64_t)occ;
@@ -419,13 +429,19 @@ int bind(const lr_context* ctx) {
 namespace lr {
 
 int trace_closest_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, const uint32_t* n_dev,
-                        lr_hits_soa* hits, hipStream_t s) {
+                        lr_hits_soa* hits, hipStream_t s, const uint32_t* root) {
     if (n_max == 0 || bvh->info.n_nodes == 0) return LR_OK;  // callers treat an empty scene separately
-    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar};
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar, root};
     const HitsArg ha{hits->prim, hits->t, hits->u, hits->v};
     Launch L;
     if (int rc = prepare(bvh, n_max, false, s, L)) return rc;
     return launch(bvh, L, ra, n_max, ha, nullptr, s, n_dev);
+}
+
+int trace_reserve(lr_qbvh* bvh, uint32_t n_max, hipStream_t s) {
+    if (n_max == 0 || bvh->info.n_nodes == 0) return LR_OK;
+    Launch L;
+    return prepare(bvh, n_max, false, s, L);
 }
 
 }  // namespace lr
@@ -445,6 +461,17 @@ int lr_context::scratch(hipStream_t stream, size_t bytes, void** out) {
     }
     *out = s.ptr;
     return LR_OK;
+}
+
+void lr_context::release(hipStream_t stream) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = spill.find(stream);
+    if (it == spill.end()) return;
+    if (it->second.ptr) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(it->second.ptr);
+    }
+    spill.erase(it);
 }
 
 extern "C" {
@@ -477,12 +504,11 @@ int lr_context_create(int device, lr_context** out) {
 int lr_context_destroy(lr_context* ctx) {
     if (!ctx) return LR_OK;
     (void)hipSetDevice(ctx->device);
-    for (auto& kv : ctx->spill) {
-        if (kv.second.ptr) {
-            (void)hipStreamSynchronize(kv.first);
-            (void)hipFree(kv.second.ptr);
-        }
-    }
+    // streams in the map may be gone by now (the caller's, or a scene's):
+    // wait for the whole device instead of for each of them
+    (void)hipDeviceSynchronize();
+    for (auto& kv : ctx->spill)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     delete ctx;
     return LR_OK;
 }
@@ -535,7 +561,7 @@ int lr_autotune_trace(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits
         std::lock_guard<std::mutex> g(ctx->mu);
         saved = ctx->cfg;
     }
-    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar};
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar, nullptr};
     const HitsArg ha = any ? HitsArg{nullptr, nullptr, nullptr, nullptr} : HitsArg{hits->prim, hits->t, hits->u, hits->v};
     hipEvent_t e0, e1;
     LR_HIP_CHECK(hipEventCreate(&e0));
@@ -666,7 +692,7 @@ int lr_trace_closest(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits_
     if (n == 0) return LR_OK;
     if (int rc = lr::bind(bvh->ctx)) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar};
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar, nullptr};
     const HitsArg ha{hits->prim, hits->t, hits->u, hits->v};
     if (bvh->info.n_nodes == 0) {
         hipLaunchKernelGGL(k_fill_miss, dim3((n + 255) / 256), dim3(256), 0, s, n, ha);
@@ -686,7 +712,7 @@ int lr_trace_any(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, uint8_t* occ
     if (n == 0) return LR_OK;
     if (int rc = lr::bind(bvh->ctx)) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar};
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar, nullptr};
     if (bvh->info.n_nodes == 0) {
         hipLaunchKernelGGL(k_fill_u8, dim3((n + 255) / 256), dim3(256), 0, s, n, occluded, (uint8_t)0);
         LR_HIP_CHECK(hipGetLastError());

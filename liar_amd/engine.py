@@ -216,6 +216,52 @@ class GpuScene:
         return {"paths": st.paths, "rays": st.rays, "chunks": st.chunks, "seconds": st.seconds}
 
 
+class Batch:
+    """Many GpuScenes rendered by one wavefront (lr_batch_*): every bounce's
+    queue mixes the paths of all scenes in a chunk; each film equals the
+    scene's own lr_render with the same parameters."""
+
+    def __init__(self, ctx, scenes):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        hs = (C.c_void_p * len(self.scenes))(*[g._h.value for g in self.scenes])
+        h = C.c_void_p()
+        N.check(N.lib().lr_batch_create(ctx.handle, hs, len(self.scenes), C.byref(h)), "lr_batch_create")
+        self._h = h
+
+    def render(self, items, graph=False, stream=None) -> dict:
+        """items: one (camera, params, film_xyz, film_w) per scene.  Returns the
+        stats and per-scene completion times ``scene_ms`` (from the call's
+        first launch, device clock)."""
+        import torch
+
+        assert len(items) == len(self.scenes)
+        arr = (N.BatchItem * len(items))()
+        for k, (cam, prm, fx, fw) in enumerate(items):
+            arr[k].camera = cam
+            arr[k].params = prm
+            arr[k].film_xyz = fx.data_ptr()
+            arr[k].film_weight = fw.data_ptr()
+        ms = (C.c_float * len(items))()
+        st = N.RenderStats()
+        s = torch.cuda.current_stream(items[0][2].device).cuda_stream if stream is None else stream
+        N.check(N.lib().lr_render_batch(self._h, arr, N.LR_BATCH_GRAPH if graph else 0, ms, C.byref(st),
+                                        C.c_void_p(s)), "lr_render_batch")
+        return {"paths": st.paths, "rays": st.rays, "chunks": st.chunks, "seconds": st.seconds,
+                "scene_ms": [float(x) for x in ms]}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            N.lib().lr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def guided_render(gs: "GpuScene", guide, camera: N.Camera, params: N.RenderParams, film_xyz, film_w,
                   selection=0.5, train=False, train_steps=0, train_batch=65536, max_records=0, weight_clamp=0.0,
                   light_share=0.5, learned_selection=False, balance_heuristic=False, image_weighting=False,
@@ -237,10 +283,11 @@ def guided_render(gs: "GpuScene", guide, camera: N.Camera, params: N.RenderParam
 
 
 def render_params(width, height, spp, wavelengths=1, max_depth=0, rr_depth=8, rr_survival=0.5,
-                  lambda_range=(380.0, 780.0), seed=1, chunk_paths=0, max_sample_luminance=0.0) -> N.RenderParams:
+                  lambda_range=(380.0, 780.0), seed=1, chunk_paths=0, max_sample_luminance=0.0,
+                  graph=False) -> N.RenderParams:
     return N.RenderParams(int(width), int(height), int(spp), int(wavelengths), int(max_depth), int(rr_depth),
                           float(rr_survival), float(lambda_range[0]), float(lambda_range[1]), int(seed),
-                          int(chunk_paths), float(max_sample_luminance))
+                          int(chunk_paths), float(max_sample_luminance), int(bool(graph)))
 
 
 class RenderEngine:

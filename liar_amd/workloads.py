@@ -214,3 +214,60 @@ def slit_room(n_side: int = 1581, n_spheres: int = 10000, seed: int = 7, slit_fr
     cam.fovAngle = math.radians(50.0)
     cam.aspectRatio = 16.0 / 9.0
     return cs, cam
+
+
+def random_cube(n_tris: int, seed: int = 2, scale: float = 0.01, sun_scale: float = 5e4, sky: float = 0.2):
+    """The intersection microbenchmark's geometry as a rendered scene: n_tris
+    triangles in the unit cube (each a random offset of `scale` from a uniform
+    centre), split 70/30 over Lambert 0.6 and Cook-Torrance GGX gold
+    (roughness 0.3), under the terrain workload's sun and sky.  Camera outside
+    the cube.  Returns (CompiledScene, camera)."""
+    from . import _native as N
+    from .engine import CompiledScene
+    from .qbvh import random_triangles
+
+    v, t = random_triangles(int(n_tris), seed=seed, scale=scale)
+    rng = np.random.default_rng(seed)
+    n_au, k_au = gold_nk()
+    table = [spectra.Constant(0.6), n_au, k_au, spectra.Constant(1.0), spectra.BlackBody(5800.0, sun_scale),
+             spectra.Constant(sky)]
+    NONE = N.LR_NONE
+    mats = [{"type": N.MAT_LAMBERT, "reflectance": 0, "transmittance": NONE, "eta": NONE, "kappa": NONE,
+             "emission": NONE, "light": NONE, "roughness": 0.0},
+            {"type": N.MAT_COOK_TORRANCE, "reflectance": 3, "transmittance": NONE, "eta": 1, "kappa": 2,
+             "emission": NONE, "light": NONE, "roughness": 0.3}]
+    pm = np.where(rng.random(len(t)) < 0.7, 0, 1).astype(np.uint32)
+    lights = [{"type": N.LIGHT_SUN, "radiance": 4, "direction": (0.35, 0.45, 0.82), "angle": float(np.radians(0.5))},
+              {"type": N.LIGHT_SKY, "radiance": 5}]
+    cs = CompiledScene(v.reshape(-1, 3), t.reshape(-1, 3), np.zeros((0, 4), np.float32), pm, mats, table, lights)
+    cam = cameras.PerspectiveCamera()
+    cam.position = (0.5, -1.1, 1.1)
+    cam.lookAt((0.5, 0.5, 0.45))
+    cam.sky = (0.0, 0.0, 1.0)
+    cam.fovAngle = math.radians(45.0)
+    cam.aspectRatio = 1.0
+    return cs, cam
+
+
+def batched_scenes(n_scenes: int = 64, seed: int = 11, min_tris: int = 10_000, max_tris: int = 2_000_000,
+                   sizes=(128, 256, 512, 1024), spps=(4, 16, 64), depth=(4, 16)):
+    """The batched uneven workload: `n_scenes` independent scenes with triangle
+    counts log-uniform in [min_tris, max_tris] (random-cube or terrain
+    generator, alternately), square images with sides drawn from `sizes`, spp
+    from `spps` and max depth uniform in `depth`.  Returns a list of dicts
+    (scene, camera, width, height, spp, max_depth, seed, n_tris)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_scenes):
+        nt = int(round(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris)))))
+        if k % 2 == 0:
+            cs, cam = random_cube(nt, seed=seed * 1000 + k)
+        else:
+            n_side = max(2, int(round(np.sqrt(nt / 2.0))) + 1)
+            cs, cam = terrain(n_side, n_spheres=min(10000, nt // 200), seed=seed * 1000 + k)
+        side = int(rng.choice(sizes))
+        cam.aspectRatio = 1.0
+        out.append({"scene": cs, "camera": cam, "width": side, "height": side, "spp": int(rng.choice(spps)),
+                    "max_depth": int(rng.integers(depth[0], depth[1] + 1)), "seed": 100 + k,
+                    "n_tris": int(len(cs.tris))})
+    return out
