@@ -61,6 +61,11 @@ def bench_render(args, la, oracle, torch, np):
     camera = E.camera_struct(cam)
     film = torch.zeros((3, H_, W_), dtype=torch.float32, device="cuda")
     wgt = torch.zeros((H_, W_), dtype=torch.float32, device="cuda")
+    tuned = None
+    if not args.no_tune:
+        from liar_amd.autotune import tune_render
+
+        tuned = tune_render(gs, camera, E.render_params(W_, H_, SPP, NL, 8, 3, 0.5, (380.0, 780.0), 99))
     stats = []
     for k in range(args.warmup + args.render_steps):
         film.zero_()
@@ -79,6 +84,7 @@ def bench_render(args, la, oracle, torch, np):
         "camera_samples_per_s": round(paths / NL / secs, 1),
         "mrays_per_s": round(rays / secs / 1e6, 2),
         "rays_per_path": round(rays / paths, 3),
+        "trace_config": None if tuned is None else tuned["best"],
     }
     if not args.no_cpu:
         cw, ch = 64, 64
@@ -123,6 +129,12 @@ def bench_guided(args, la, torch, np):
     def params(spp, seed):
         return E.render_params(Wd, Hd, spp, 1, 16, 16, 0.5, (380.0, 780.0), seed, 0, ML)
 
+    tuned = None
+    if not args.no_tune:
+        from liar_amd.autotune import tune_render
+
+        log("[bench] guided: autotuning traversal on the plain render")
+        tuned = tune_render(gs, camera, params(1, 77), repeats=1)
     fx, fw = film()
     ref_spp = args.guided_ref_spp
     log(f"[bench] guided: {ref_spp}-spp plain reference")
@@ -178,6 +190,7 @@ def bench_guided(args, la, torch, np):
         "guided_config": "one-sample MIS: guide 0.95 x 0.95, BSDF 0.05 x 0.95, light 0.05; "
                          f"online training in the first {train_passes} passes: 128 Adam steps (lr 1e-3, batch 4096) "
                          "per 4M-path chunk on first-bounce records, weights clamped at 20x their mean",
+        "trace_config": None if tuned is None else tuned["best"],
         "curves": {"plain": [(round(t, 4), spp, m) for t, spp, m in plain[::4]],
                    "guided": [(round(t, 4), spp, m) for t, spp, m in guided[::4]]},
     }
@@ -185,6 +198,188 @@ def bench_guided(args, la, torch, np):
     gs.close()
     ctx.close()
     return out
+
+
+def bench_terrain(args, la, torch, np):
+    """The plain path tracer at scale: the terrain workload (1581x1581 fBm
+    heightfield, 4,992,800 triangles + 10k spheres; Lambert / GGX gold /
+    dielectric; 0.5-degree sun + sky) at 1920x1080, 64 spp, depth 16, one
+    wavelength per sample; traversal autotuned on a 480x270 4-spp render."""
+    from liar_amd import engine as E
+    from liar_amd import workloads as W
+
+    log("[bench] terrain: building")
+    cs, cam = W.terrain()
+    ctx = la.Context(0)
+    gs = E.GpuScene(ctx, cs)
+    camera = E.camera_struct(cam)
+    tuned = None
+    if not args.no_tune:
+        from liar_amd.autotune import tune_render
+
+        log("[bench] terrain: autotuning traversal")
+        tuned = tune_render(gs, camera, E.render_params(480, 270, 4, 1, 16, 16, 0.5, (380.0, 780.0), 5), repeats=2)
+    Wd, Hd, SPP = 1920, 1080, 64
+    fx = torch.zeros((3, Hd, Wd), dtype=torch.float32, device="cuda")
+    fw = torch.zeros((Hd, Wd), dtype=torch.float32, device="cuda")
+    log(f"[bench] terrain: {Wd}x{Hd} x {SPP} spp frames")
+    stats = []
+    for k in range(1 + args.terrain_frames):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = gs.render(camera, E.render_params(Wd, Hd, SPP, 1, 16, 16, 0.5, (380.0, 780.0), 1 + k), fx, fw)
+        torch.cuda.synchronize()
+        if k >= 1:
+            stats.append((time.perf_counter() - t0, st))
+    secs = float(np.mean([t for t, _ in stats]))
+    paths = stats[0][1]["paths"]
+    rays = float(np.mean([st["rays"] for _, st in stats]))
+    out = {
+        "workload": "terrain 4,992,800 tris + 10k spheres, 1920x1080, 64 spp, 1 wavelength per sample, depth 16",
+        "ms_per_frame": round(secs * 1e3, 2),
+        "samples_per_s": round(paths / secs, 1),
+        "mrays_per_s": round(rays / secs / 1e6, 1),
+        "rays_per_path": round(rays / paths, 3),
+        "trace_config": None if tuned is None else tuned["best"],
+        "autotune_default_ms": None if tuned is None or tuned["default_ms"] is None else round(tuned["default_ms"], 3),
+        "autotune_best_ms": None if tuned is None else round(tuned["best_ms"], 3),
+    }
+    gs.close()
+    ctx.close()
+    return out
+
+
+def bench_batch(args, la, torch, np):
+    """The batched uneven workload: 64 independent scenes (triangle counts
+    log-uniform in 10k..2M, random-cube and terrain generators alternately;
+    square images of 128..1024; 4/16/64 spp; max depth 4..16) rendered by one
+    lr_render_batch call, its launches captured in a HIP graph and the
+    traversal configuration autotuned on the batch's camera rays.  Reports
+    rays/s and the per-scene completion latency p50 / p99 (device clock, from
+    the call's first launch), next to the same scenes rendered one by one."""
+    from liar_amd import engine as E
+    from liar_amd import workloads as W
+
+    log("[bench] batch: generating and building 64 scenes")
+    specs = W.batched_scenes(64)
+    ctx = la.Context(0)
+    t0 = time.perf_counter()
+    gss = [E.GpuScene(ctx, s["scene"]) for s in specs]
+    t_build = time.perf_counter() - t0
+    batch = E.Batch(ctx, gss)
+
+    def items():
+        out = []
+        for s in specs:
+            prm = E.render_params(s["width"], s["height"], s["spp"], 1, s["max_depth"], 3, 0.5, (380.0, 780.0),
+                                  s["seed"])
+            out.append((E.camera_struct(s["camera"]), prm,
+                        torch.zeros((3, s["height"], s["width"]), device="cuda"),
+                        torch.zeros((s["height"], s["width"]), device="cuda")))
+        return out
+
+    # autotune the traversal on the largest scene's camera rays (one bounce)
+    big = max(range(len(specs)), key=lambda k: specs[k]["n_tris"])
+    cam = specs[big]["camera"]
+    n_tune = 1 << 20
+    tune_rays = _camera_rays(np, torch, cam, n_tune)
+    res = gss[big].qbvh.autotune(tune_rays, repeats=2)
+    ctx.set_trace_config(**res["best"])
+    batch.render(items(), graph=True)  # warm-up: code load, workspace, graph instantiation
+    torch.cuda.synchronize()
+    it = items()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = batch.render(it, graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = np.array(r["scene_ms"])
+    it = items()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (c, prm, fx, fw), gs in zip(it, gss):
+        gs.render(c, prm, fx, fw)
+    torch.cuda.synchronize()
+    wall_seq = time.perf_counter() - t0
+    out = {
+        "workload": "64 scenes, triangles log-uniform 10k..2M (random cube / terrain), images 128^2..1024^2, "
+                    "spp 4/16/64, depth 4..16, one wavelength per sample",
+        "triangles_total": int(sum(s["n_tris"] for s in specs)),
+        "paths": int(r["paths"]),
+        "rays": int(r["rays"]),
+        "chunks": int(r["chunks"]),
+        "seconds": round(wall, 4),
+        "mrays_per_s": round(r["rays"] / wall / 1e6, 1),
+        "scene_latency_ms_p50": round(float(np.percentile(ms, 50)), 2),
+        "scene_latency_ms_p99": round(float(np.percentile(ms, 99)), 2),
+        "scene_latency_ms_max": round(float(ms.max()), 2),
+        "sequential_seconds": round(wall_seq, 4),
+        "sequential_mrays_per_s": round(r["rays"] / wall_seq / 1e6, 1),
+        "trace_config": res["best"],
+        "build_seconds": round(t_build, 2),
+    }
+    batch.close()
+    for gs in gss:
+        gs.close()
+    ctx.close()
+    return out
+
+
+def _camera_rays(np, torch, cam, n):
+    """(8, n) primary rays of a PerspectiveCamera over a square grid."""
+    side = int(np.sqrt(n))
+    d = np.asarray(cam.direction, np.float64)
+    d /= np.linalg.norm(d)
+    sky = np.asarray(cam.sky, np.float64)
+    r = np.cross(d, sky)
+    r /= np.linalg.norm(r)
+    u = np.cross(r, d)
+    t = np.tan(cam.fovAngle / 2)
+    ys, xs = np.meshgrid((np.arange(side) + 0.5) / side * 2 - 1, (np.arange(side) + 0.5) / side * 2 - 1,
+                         indexing="ij")
+    dirs = d[None, :] + t * (xs.ravel()[:, None] * r[None, :] - ys.ravel()[:, None] * u[None, :])
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    m = side * side
+    rays = np.zeros((8, m), np.float32)
+    rays[0:3] = np.asarray(cam.position, np.float32)[:, None]
+    rays[3:6] = dirs.T
+    rays[6] = 1e-6
+    rays[7] = np.inf
+    return torch.from_numpy(rays).cuda()
+
+
+def profile_workload(args, la, torch):
+    """A short fixed run for profilers (rocprofv3 --kernel-trace / --pmc):
+    the slit room at 640x360, 2 plain 4-spp passes, then 2 guided passes with
+    online training and 2 without.  Prints one line per pass, no JSON."""
+    from liar_amd import engine as E
+    from liar_amd import workloads as W
+    from liar_amd.guide import Guide
+
+    Wd, Hd = 640, 360
+    cs, cam = W.slit_room()
+    ctx = la.Context(0)
+    gs = E.GpuScene(ctx, cs)
+    camera = E.camera_struct(cam)
+    fx = torch.zeros((3, Hd, Wd), device="cuda")
+    fw = torch.zeros((Hd, Wd), device="cuda")
+    g = Guide(ctx, seed=1, learning_rate=1e-3)
+    gcfg = dict(selection=0.95, light_share=0.05, train_steps=128, train_batch=4096, weight_clamp=20.0,
+                record_depth=1)
+    for p in range(6):
+        prm = E.render_params(Wd, Hd, 4, 1, 16, 16, 0.5, (380.0, 780.0), 100 + p, 0, 0.5)
+        t0 = time.perf_counter()
+        if p < 2:
+            st = gs.render(camera, prm, fx, fw)
+        else:
+            st = E.guided_render(gs, g, camera, prm, fx, fw, train=p < 4, **gcfg)
+        torch.cuda.synchronize()
+        kind = "plain" if p < 2 else ("guided+train" if p < 4 else "guided")
+        log(f"[profile] pass {p} {kind}: {1e3 * (time.perf_counter() - t0):.2f} ms, {st['rays']} rays")
+    g.close()
+    gs.close()
+    ctx.close()
+    return 0
 
 
 def main() -> int:
@@ -200,6 +395,11 @@ def main() -> int:
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracer leg")
     ap.add_argument("--no-guided", action="store_true", help="skip the learned-guiding leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default traversal configuration")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg")
+    ap.add_argument("--no-terrain", action="store_true", help="skip the terrain path-tracer leg")
+    ap.add_argument("--terrain-frames", type=int, default=2)
+    ap.add_argument("--profile-workload", action="store_true",
+                    help="run only a short slit-room plain + guided sequence for profilers")
     ap.add_argument("--guided-width", type=int, default=1920)
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--render-steps", type=int, default=5)
@@ -219,6 +419,8 @@ def main() -> int:
     import oracle
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.profile_workload:
+        return profile_workload(args, la, torch)
     dev = torch.device("cuda:0")
     ctx = la.Context(0)
 
@@ -325,7 +527,9 @@ def main() -> int:
         }
 
     render = None if args.no_render else bench_render(args, la, oracle, torch, np)
+    terrain = None if args.no_terrain else bench_terrain(args, la, torch, np)
     guided = None if args.no_guided else bench_guided(args, la, torch, np)
+    batch = None if args.no_batch else bench_batch(args, la, torch, np)
 
     out = {
         "metric": METRIC,
@@ -378,7 +582,9 @@ def main() -> int:
             "autotune_any": tuned_a,
         },
         "render": render,
+        "terrain": terrain,
         "guided": guided,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -107,17 +107,54 @@ __device__ inline lr::u4 rng(const Params& P, uint64_t path, uint32_t gen, uint3
     return lr::philox4x32(P.seed_lo, P.seed_hi, lr::u4{(uint32_t)path, (uint32_t)(path >> 32), gen, purpose});
 }
 
+// Slot of this lane's entry in a queue that the whole workgroup appends to:
+// wave counts by ballot, one atomic per workgroup (a single counter address
+// serialises its atomics, so fewer of them matter).  Every thread of the
+// block must call it; blockDim.x is a multiple of 64, at most 1024.
+__device__ inline uint32_t block_append(bool alive, uint32_t* count) {
+    __shared__ uint32_t s_cnt[16];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint64_t mask = __ballot(alive);
+    if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t c = s_cnt[w];
+            s_cnt[w] = tot;
+            tot += c;
+        }
+        s_base = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    return s_base + s_cnt[wave] +
+           __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Start of a generation: queue `cur` is about to be traced (its rays are
+// counted), queue `nxt` starts empty.
+__global__ void k_next_gen(uint32_t* counts, uint32_t cur, uint32_t nxt, uint64_t* rays_total) {
+    if (threadIdx.x == 0) {
+        *rays_total += counts[cur];
+        counts[nxt] = 0u;
+    }
+}
+
 // -- generation 0: camera rays (perspective_camera.cpp:406-434) -------------
 template <bool BATCH>
 __global__ void k_camera(DevScene S0, DevCamera C0, Params P0, uint64_t path0, uint32_t n, PathState ps, Queue q,
-                         uint32_t* count, uint64_t* rays_total, BatchArgs B) {
+                         uint32_t* count, BatchArgs B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    DevScene S = S0;
+    bool alive = false;
+    V3 d = v3(0, 0, 1);
+    float ray_factor = 1.0f;
     DevCamera C = C0;
+    uint32_t root = 0;
+    if (i < n) {
+    DevScene S = S0;
     Params P = P0;
     uint64_t p = path0 + i;
-    uint32_t root = 0;
     if (BATCH) {
         const Segment sg = B.seg[find_segment(B, i, false)];
         S = B.scenes[sg.scene];
@@ -140,9 +177,9 @@ __global__ void k_camera(DevScene S0, DevCamera C0, Params P0, uint64_t path0, u
     }
     const float X = ((float)px + sx) / (float)P.width;
     const float Y = ((float)py + sy) / (float)P.height;
-    V3 d = ld3(C.dir) + (X - 0.5f) * ld3(C.right) + (Y - 0.5f) * ld3(C.down);
+    d = ld3(C.dir) + (X - 0.5f) * ld3(C.right) + (Y - 0.5f) * ld3(C.down);
     d = normalize(d);
-    const float ray_factor = 1.0f / dot(d, ld3(C.dir_n));
+    ray_factor = 1.0f / dot(d, ld3(C.dir_n));
     // stratified wavelength (stratifier.cpp:191-196 with nl strata per sample)
     const float uw = ((float)l + lr::u01(rp.x)) / (float)P.nl;
     float lpdf;
@@ -158,20 +195,11 @@ __global__ void k_camera(DevScene S0, DevCamera C0, Params P0, uint64_t path0, u
     ps.L[i] = 0.0f;
     ps.lam[i] = lam;
     ps.lpdf[i] = lpdf;
-    const bool alive = thr > 0.0f;
-    const uint64_t mask = __ballot(alive);
-    if (mask) {
-        const uint32_t lane = threadIdx.x & 63u;
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) {
-            base = atomicAdd(count, (uint32_t)__popcll(mask));
-            atomicAdd((unsigned long long*)rays_total, (unsigned long long)__popcll(mask));
-        }
-        base = __shfl(base, leader);
+    alive = thr > 0.0f;
+    }
+    const uint32_t pos = block_append(alive, count);
+    {
         if (alive) {
-            const uint32_t pos =
-                base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
             q.r[0][pos] = C.pos[0];
             q.r[1][pos] = C.pos[1];
             q.r[2][pos] = C.pos[2];
@@ -361,7 +389,7 @@ __global__ void k_gather_batch(uint32_t cap, const uint32_t* __restrict__ count,
 template <bool GUIDED, bool BATCH>
 __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Queue q, const uint32_t* count,
                         Hits h, const float4* __restrict__ prim_geom0, PathState ps, Queue qn, uint32_t* count_next,
-                        uint64_t* rays_total, GuideArgs G, BatchArgs B) {
+                        GuideArgs G, BatchArgs B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < *count;
     bool alive = false;
@@ -582,19 +610,9 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
         ps.L[pid] = L;
         ps.thr[pid] = thr;
     }
-    const uint64_t mask = __ballot(alive);
-    if (mask) {
-        const uint32_t lane = threadIdx.x & 63u;
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) {
-            base = atomicAdd(count_next, (uint32_t)__popcll(mask));
-            atomicAdd((unsigned long long*)rays_total, (unsigned long long)__popcll(mask));
-        }
-        base = __shfl(base, leader);
+    const uint32_t pos = block_append(alive, count_next);
+    {
         if (alive) {
-            const uint32_t pos =
-                base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
             qn.r[0][pos] = start.x;
             qn.r[1][pos] = start.y;
             qn.r[2][pos] = start.z;
@@ -1270,10 +1288,10 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         const dim3 blk(256), grd((n + 255) / 256);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 1u);
         if (guided) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + 8, 1u);
-        hipLaunchKernelGGL(k_camera<false>, grd, blk, 0, s, S, C, P, path0, n, ps, q[0], counts, rays_total, NB);
+        hipLaunchKernelGGL(k_camera<false>, grd, blk, 0, s, S, C, P, path0, n, ps, q[0], counts, NB);
         for (uint32_t gen = 0; gen < P.max_gen; ++gen) {
             const int cur = gen & 1, nxt = cur ^ 1;
-            hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + nxt, 1u);
+            hipLaunchKernelGGL(k_next_gen, dim3(1), dim3(64), 0, s, counts, (uint32_t)cur, (uint32_t)nxt, rays_total);
             lr_rays_soa rays{q[cur].r[0], q[cur].r[1], q[cur].r[2], q[cur].r[3],
                              q[cur].r[4], q[cur].r[5], q[cur].r[6], q[cur].r[7]};
             lr_hits_soa hits{H.prim, H.t, H.u, H.v};
@@ -1286,10 +1304,10 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
                 hipLaunchKernelGGL(k_guide_inputs, grd, blk, 0, s, S, q[cur], counts + cur, H, ps, GA, g_in);
                 if (int rc = lr_guide_infer(guide, g_in, n, counts + cur, g_mix, s)) return rc;
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
-                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total, GA, NB);
+                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, GA, NB);
             } else {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, false>), grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
-                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, rays_total, GA, NB);
+                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, GA, NB);
             }
         }
         if (training) {
@@ -1621,17 +1639,17 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
             const dim3 blk(256), grd((ch.n_paths + 255) / 256);
             hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 1u);
             hipLaunchKernelGGL(k_camera<true>, grd, blk, 0, s, S0, C0, P0, (uint64_t)0, ch.n_paths, ps, q[0], counts,
-                               rays_total, B);
+                               B);
             for (uint32_t gen = 0; gen < ch.max_gen; ++gen) {
                 const int cu = gen & 1, nx = cu ^ 1;
-                hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + nx, 1u);
+                hipLaunchKernelGGL(k_next_gen, dim3(1), dim3(64), 0, s, counts, (uint32_t)cu, (uint32_t)nx, rays_total);
                 lr_rays_soa rays{q[cu].r[0], q[cu].r[1], q[cu].r[2], q[cu].r[3],
                                  q[cu].r[4], q[cu].r[5], q[cu].r[6], q[cu].r[7]};
                 lr_hits_soa hits{H.prim, H.t, H.u, H.v};
                 if (int rc = lr::trace_closest_queue(&b->tree, &rays, ch.n_paths, counts + cu, &hits, s, q[cu].root))
                     return rc;
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, true>), grd, blk, 0, s, S0, P0, gen, (uint64_t)0,
-                                   q[cu], counts + cu, H, hgeo[0], ps, q[nx], counts + nx, rays_total, NG, B);
+                                   q[cu], counts + cu, H, hgeo[0], ps, q[nx], counts + nx, NG, B);
             }
             hipLaunchKernelGGL(k_splat<true>, dim3((ch.n_pix + 255) / 256), blk, 0, s, S0, P0, (uint64_t)0, ch.n_pix,
                                ps, (float*)nullptr, (float*)nullptr, B);
