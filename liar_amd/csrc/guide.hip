@@ -12,7 +12,8 @@
 //                 next, so activations never leave registers; weights sit in
 //                 LDS in per-lane fragment order; each lane ends holding two
 //                 whole lobes and the softmax needs two cross-lane exchanges
-//   k_train_fb    fused forward + loss + backward data path for 16-row tiles;
+//   k_train_fb    fused forward + loss + backward data path, register
+//                 resident like k_infer (backward on transposed fragments);
 //                 writes activations A_l and gradients dZ_l feature-major
 //                 (bf16) for the weight-gradient GEMMs
 //   k_wgrad       dW_l = dZ_l^T A_{l-1} on MFMA, split over row chunks
@@ -52,8 +53,6 @@ constexpr float kSelFloor = 0.025f;  // least probability of each strategy
 constexpr int kLobes = 8;
 constexpr int kLayers = kHidden + 1;  // weight matrices
 constexpr int kFreq = 4;      // frequencies per input
-constexpr int kRowPad = 8;    // LDS row padding (bf16 elements)
-constexpr int kWS = kW + kRowPad;  // LDS row stride of Wt and activations
 
 // Parameter layout (FP32 master, and the bf16 copy): per layer l, Wt_l
 // [nout][64] row-major (nout = 64, or 48 for the output layer), then biases.
@@ -83,21 +82,36 @@ __host__ __device__ constexpr int out_logical(int p) {
     return (4 * (p >> 4) + (p & 3)) < 10 ? 10 * ((p & 15) >> 2) + 4 * (p >> 4) + (p & 3)
                                          : kOut + 2 * ((p & 15) >> 2) + (p & 3) - 2;
 }
+// physical row of the output layer's logical output n (inverse of out_logical)
+__host__ __device__ inline int out_physical(int n) {
+    const int jj = n < kOut ? n / 10 : (n - kOut) >> 1, sl = n < kOut ? n % 10 : 10 + ((n - kOut) & 1);
+    return 16 * (sl >> 2) + 4 * jj + (sl & 3);
+}
+// K slot (kb, j, e) of feature / physical row c: the inverse of kslot_feature
+__host__ __device__ inline void kslot_of(int c, int& kb, int& j, int& e) {
+    kb = c >> 5;
+    const int r = c & 31;
+    j = (r & 15) >> 2;
+    e = (r & 3) + (r >= 16 ? 4 : 0);
+}
 // fragment-order index of logical weight Wt_l[n][k]
 __host__ __device__ inline int frag_index(int l, int n, int k) {
-    const int kb = k >> 5, c = k & 31;
-    const int j = (c & 15) >> 2, e = (c & 3) + (c >= 16 ? 4 : 0);
-    int t, i;
-    if (l < kHidden) {
-        t = n >> 4;
-        i = n & 15;
-    } else {
-        // lane group and slot of this output
-        const int jj = n < kOut ? n / 10 : (n - kOut) >> 1, sl = n < kOut ? n % 10 : 10 + ((n - kOut) & 1);
-        t = sl >> 2;
-        i = 4 * jj + (sl & 3);
-    }
-    return frag_id(l, t, kb) * 512 + (i + 16 * j) * 8 + e;
+    int kb, j, e;
+    kslot_of(k, kb, j, e);
+    const int p = l < kHidden ? n : out_physical(n);
+    return frag_id(l, p >> 4, kb) * 512 + ((p & 15) + 16 * j) * 8 + e;
+}
+
+// Transposed fragments for the backward pass.  Fragment (l, t, kb), l = 1..5,
+// is the A operand of dH_l^T = W_l^T-tiles x dZ_l^T: lane (i, j) element e holds
+// Wt_l[n][k] with k = 16 t + i (an input of layer l) and n the (physical, for
+// the output layer) output in K slot (kb, j, e); outputs past 48 are zeros.
+constexpr int kFragsT = kHidden * 4 * 2;  // 40 fragments of 512 bf16
+__host__ __device__ constexpr int fragT_id(int l, int t, int kb) { return ((l - 1) * 4 + t) * 2 + kb; }
+__host__ __device__ inline int fragT_index(int l, int n, int k) {
+    int kb, j, e;
+    kslot_of(l < kHidden ? n : out_physical(n), kb, j, e);
+    return fragT_id(l, k >> 4, kb) * 512 + ((k & 15) + 16 * j) * 8 + e;
 }
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -107,14 +121,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-// Orders a wave's LDS writes before its later reads by other lanes of the
-// same wave (memory fence at wavefront scope + no code motion across).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Lobe concentration from its raw output s: kappa = 0.1 + e^min(s, 7.6)
 // (0.1 .. ~2000).  The log scale lets a lobe sharpen to the ~0.01 sr targets
@@ -140,19 +146,6 @@ __device__ __forceinline__ float encode(const float* v, int f) {
     return f == kIn + kIn * 2 * kFreq ? 1.0f : 0.0f;
 }
 
-// Loads every layer's bf16 Wt into LDS (padded rows) and biases (fp32).
-__device__ void load_weights(const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
-                             uint16_t* s_w, float* s_b) {
-    for (int l = 0; l < kLayers; ++l) {
-        const int nout = layer_nout(l);
-        for (int e = threadIdx.x; e < nout * kW; e += blockDim.x) {
-            const int n = e / kW, k = e % kW;
-            s_w[l * kW * kWS + n * kWS + k] = wbf[wt_offset(l) + e];
-        }
-    }
-    for (int e = threadIdx.x; e < kBiasElems; e += blockDim.x) s_b[e] = bias[e];
-}
-
 __device__ __forceinline__ bf16x4 ld4(const uint16_t* p) {
     const uint2 u = *reinterpret_cast<const uint2*>(p);  // 8-byte aligned
     bf16x4 r;
@@ -161,66 +154,6 @@ __device__ __forceinline__ bf16x4 ld4(const uint16_t* p) {
     r[2] = (short)(u.y & 0xffffu);
     r[3] = (short)(u.y >> 16);
     return r;
-}
-
-// out tile (16 rows x 16 cols at n0) = act[16 x 64] * W_l[:, n0..n0+15]
-__device__ __forceinline__ f32x4 tile_fwd(const uint16_t* act, const uint16_t* wl, int n0, int i, int j) {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int kb = 0; kb < kW / 16; ++kb) {
-        const bf16x4 a = ld4(act + i * kWS + kb * 16 + 4 * j);
-        const bf16x4 b = ld4(wl + (n0 + i) * kWS + kb * 16 + 4 * j);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    return acc;
-}
-
-// Runs the hidden layers for one 16-row tile.  act_in holds the encoding;
-// acts[l] (l = 0..4) receive post-ReLU hidden activations (bf16).
-__device__ __forceinline__ void forward_hidden(const uint16_t* s_w, const float* s_b, uint16_t* const* acts,
-                                               int i, int j) {
-    for (int l = 0; l < kHidden; ++l) {
-        const uint16_t* in = acts[l];
-        uint16_t* out = acts[l + 1];
-        const uint16_t* wl = s_w + l * kW * kWS;
-        f32x4 acc[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = tile_fwd(in, wl, 16 * t, i, j);
-        wave_sync();
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float z = acc[t][r] + s_b[l * kW + 16 * t + i];
-                out[(4 * j + r) * kWS + 16 * t + i] = f2bf(fmaxf(z, 0.0f));
-            }
-        wave_sync();
-    }
-}
-
-// Mixture processing of one row's 40 raw outputs (fp32) -> weights pi,
-// unit means, concentrations kappa = 0.1 + e^min(s, 7.6).
-struct Lobe {
-    float pi, mx, my, mz, kappa;
-};
-
-__device__ inline void process_row(const float* o, Lobe* L) {
-    float mx = -INFINITY;
-    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, o[5 * k]);
-    float sum = 0.0f;
-    for (int k = 0; k < kLobes; ++k) {
-        L[k].pi = expf(o[5 * k] - mx);
-        sum += L[k].pi;
-    }
-    for (int k = 0; k < kLobes; ++k) {
-        L[k].pi /= sum;
-        const float x = o[5 * k + 1], y = o[5 * k + 2], z = o[5 * k + 3];
-        const float n = sqrtf(x * x + y * y + z * z) + 1e-12f;
-        L[k].mx = x / n;
-        L[k].my = y / n;
-        L[k].mz = z / n;
-        L[k].kappa = kappa_of(o[5 * k + 4]);
-    }
 }
 
 // log vMF density at cosine c = mu . w:  log(k / (2 pi (1 - e^{-2k}))) + k (c - 1)
@@ -391,170 +324,210 @@ __global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restric
 // ---------------------------------------------------------------------------
 // Global scratch (feature-major, bf16): A[l][64][n_pad] for l = 0..5 (0 is
 // the encoding, 1..5 hidden outputs) and G[l][nout_l][n_pad] for l = 0..5
-// (dZ of layer l).  n_pad is a multiple of 16.
-__global__ __launch_bounds__(kBlock) void k_train_fb(const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
-                                                     const float* __restrict__ in, const float* __restrict__ dirs,
-                                                     const float* __restrict__ weight, uint32_t n, uint32_t n_pad,
-                                                     const float* __restrict__ inv_wsum, uint16_t* __restrict__ A,
-                                                     uint16_t* __restrict__ G, float* __restrict__ row_loss) {
+// (dZ of layer l, the output layer's rows in logical order).  n_pad is a
+// multiple of 16.
+//
+// Same register-resident form as k_infer, one 16-row tile per wave
+// iteration: the forward keeps every layer's packed activations in
+// registers; each lane computes the loss gradient of its two lobes (lane 0
+// of a row also that of the selection logits); the backward pass runs
+// dH^T = W^T-tiles x dZ^T on the transposed fragments, and the ReLU mask
+// comes from the activation registers.  A and dZ go to global memory for the
+// weight-gradient GEMMs.
+constexpr int kTrainWaves = 4;
+constexpr int kTrainBlock = 64 * kTrainWaves;
+constexpr size_t kTrainSmem = sizeof(uint4) * (kFrags + kFragsT) * 64 + sizeof(float) * kBiasElems;
+
+// Bits of element e of a bf16 vector (by a whole-vector bit cast: a bit cast
+// of one element of the __bf16 vector stored the wrong elements).
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ uint16_t bf_bits(const bf16x8& v, int e) { return __builtin_bit_cast(u16x8, v)[e]; }
+
+__device__ __forceinline__ bf16x8 pack8(const float* v) {
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
+    return r;
+}
+
+__global__ __launch_bounds__(kTrainBlock) void k_train_fb(const uint16_t* __restrict__ frag,
+                                                          const uint16_t* __restrict__ fragT,
+                                                          const float* __restrict__ bias, const float* __restrict__ in,
+                                                          const float* __restrict__ dirs,
+                                                          const float* __restrict__ weight, uint32_t n, uint32_t n_pad,
+                                                          const float* __restrict__ inv_wsum, uint16_t* __restrict__ A,
+                                                          uint16_t* __restrict__ G, float* __restrict__ row_loss) {
     extern __shared__ uint8_t smem[];
-    uint16_t* s_w = reinterpret_cast<uint16_t*>(smem);
-    float* s_b = reinterpret_cast<float*>(smem + sizeof(uint16_t) * kLayers * kW * kWS);
-    uint8_t* wave_base = smem + sizeof(uint16_t) * kLayers * kW * kWS + sizeof(float) * kBiasElems;
-    load_weights(wbf, bias, s_w, s_b);
+    uint4* s_f = reinterpret_cast<uint4*>(smem);
+    uint4* s_t = s_f + kFrags * 64;
+    float* s_b = reinterpret_cast<float*>(s_t + kFragsT * 64);
+    for (int e = threadIdx.x; e < kFrags * 64; e += kTrainBlock) s_f[e] = reinterpret_cast<const uint4*>(frag)[e];
+    for (int e = threadIdx.x; e < kFragsT * 64; e += kTrainBlock) s_t[e] = reinterpret_cast<const uint4*>(fragT)[e];
+    for (int e = threadIdx.x; e < kBiasElems; e += kTrainBlock) {
+        const int p = e - kHidden * kW;
+        s_b[e] = p < 0 ? bias[e] : bias[kHidden * kW + out_logical(p)];
+    }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
-    const size_t act_bytes = sizeof(uint16_t) * kTileRows * kWS;
-    // per wave: 6 activation buffers, 2 gradient buffers, fp32 outputs
-    uint8_t* wb = wave_base + wave * (8 * act_bytes + sizeof(float) * kTileRows * kOutPad);
-    uint16_t* acts[kLayers];
-    for (int l = 0; l < kLayers; ++l) acts[l] = reinterpret_cast<uint16_t*>(wb + l * act_bytes);
-    uint16_t* gbuf[2] = {reinterpret_cast<uint16_t*>(wb + 6 * act_bytes),
-                         reinterpret_cast<uint16_t*>(wb + 7 * act_bytes)};
-    float* s_out = reinterpret_cast<float*>(wb + 8 * act_bytes);
     const float wnorm = *inv_wsum;
     const uint32_t tiles = n_pad / kTileRows;
-    for (uint32_t tile = blockIdx.x * kWaves + wave; tile < tiles; tile += gridDim.x * kWaves) {
-        const uint32_t row0 = tile * kTileRows;
-        {
-            const uint32_t row = row0 + i;
-            float v[kIn];
-            for (int d = 0; d < kIn; ++d) v[d] = row < n ? in[(size_t)row * 8 + d] : 0.0f;
-            for (int f = 16 * j; f < 16 * j + 16; ++f) acts[0][i * kWS + f] = f2bf(encode(v, f));
+    for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < tiles; tile += gridDim.x * kTrainWaves) {
+        asm volatile("" ::: "memory");  // fragment reads stay inside the loop (as in k_infer)
+        const uint32_t row = tile * kTileRows + i;
+        const bool valid = row < n;
+        float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (valid) {
+            const float4 a = reinterpret_cast<const float4*>(in)[2 * (size_t)row];
+            const float4 b = reinterpret_cast<const float4*>(in)[2 * (size_t)row + 1];
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
         }
-        wave_sync();
-        forward_hidden(s_w, s_b, acts, i, j);
-        {
-            const uint16_t* wl = s_w + kHidden * kW * kWS;
+        // forward: x[0] the encoding, x[l] the output of hidden layer l - 1
+        bf16x8 x[kLayers][2];
 #pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                const f32x4 acc = tile_fwd(acts[kHidden], wl, 16 * t, i, j);
+        for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    s_out[(4 * j + r) * kOutPad + 16 * t + i] = acc[r] + s_b[kHidden * kW + 16 * t + i];
+            for (int e = 0; e < 8; ++e) x[0][kb][e] = (__bf16)encode(v, kslot_feature(kb, j, e));
+        f32x4 acc[4];
+#pragma unroll
+        for (int l = 0; l < kHidden; ++l) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                acc[t] = bias4(s_b, l * kW + 16 * t + 4 * j);
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, s_f[frag_id(l, t, kb) * 64 + lane]), x[l][kb], acc[t], 0, 0, 0);
             }
+            x[l + 1][0] = pack_relu(acc[0], acc[1]);
+            x[l + 1][1] = pack_relu(acc[2], acc[3]);
         }
-        wave_sync();
-        // loss and its gradient w.r.t. the 40 raw outputs (lanes 0..15, one row each)
-        if (j == 0) {
-            const uint32_t row = row0 + i;
-            float g[kOutPad];
-            for (int c = 0; c < kOutPad; ++c) g[c] = 0.0f;
-            float loss = 0.0f;
-            if (row < n) {
-                const float* o = s_out + i * kOutPad;
-                Lobe L[kLobes];
-                process_row(o, L);
-                const float wx = dirs[3 * (size_t)row], wy = dirs[3 * (size_t)row + 1], wz = dirs[3 * (size_t)row + 2];
-                float lp[kLobes], lmax = -INFINITY;
-                for (int k = 0; k < kLobes; ++k) {
-                    const float c = L[k].mx * wx + L[k].my * wy + L[k].mz * wz;
-                    lp[k] = logf(L[k].pi) + log_vmf(L[k].kappa, c);
-                    lmax = fmaxf(lmax, lp[k]);
-                }
-                float s = 0.0f;
-                for (int k = 0; k < kLobes; ++k) s += expf(lp[k] - lmax);
-                const float logq = lmax + logf(s);
-                const float w = weight[row] * wnorm;
-                loss = -w * logq;
-                // strategy selection: -w log(c_l p_l + c_g p_g + c_b p_b) with the
-                // record's light / BSDF pdfs (inputs 6, 7) and the guide pdf held
-                // fixed; only when the record carries them (p_l + p_b > 0)
-                const float pl = in[(size_t)row * 8 + 6], pb = in[(size_t)row * 8 + 7];
-                if (pl + pb > 0.0f) {
-                    const float pg = expf(logq);
-                    float c[3];
-                    selection_probs(o[kOut], o[kOut + 1], c);
-                    const float pmix = c[0] * pl + c[1] * pg + c[2] * pb;
-                    const float rl = 1.0f / (1.0f + expf(-o[kOut])), rg = 1.0f / (1.0f + expf(-o[kOut + 1]));
-                    const float f = -w * (1.0f - 3.0f * kSelFloor) / pmix;
-                    loss -= w * logf(pmix);
-                    g[kOut] = f * rl * (1.0f - rl) * (pl - rg * pg - (1.0f - rg) * pb);
-                    g[kOut + 1] = f * (1.0f - rl) * rg * (1.0f - rg) * (pg - pb);
-                }
-                for (int k = 0; k < kLobes; ++k) {
-                    const float resp = expf(lp[k] - logq);  // responsibility
-                    const float c = L[k].mx * wx + L[k].my * wy + L[k].mz * wz;
-                    const float kap = L[k].kappa;
-                    // d(-log q)/d logit_k
-                    g[5 * k + 0] = -w * (resp - L[k].pi);
-                    // d(-log q)/d s_k:  d log vmf / d kappa = 1/k - 2/(e^{2k} - 1) + (c - 1)
-                    const float e2 = expm1f(2.0f * kap);
-                    const float dlk = 1.0f / kap - 2.0f / e2 + (c - 1.0f);
-                    g[5 * k + 4] = -w * resp * dlk * dkappa_ds(o[5 * k + 4]);
-                    // d(-log q)/d m_k = -resp * kappa * (I - mu mu^T) w / |m|
-                    const float x = o[5 * k + 1], y = o[5 * k + 2], z = o[5 * k + 3];
-                    const float nm = sqrtf(x * x + y * y + z * z) + 1e-12f;
-                    const float px = wx - c * L[k].mx, py = wy - c * L[k].my, pz = wz - c * L[k].mz;
-                    const float f = -w * resp * kap / nm;
-                    g[5 * k + 1] = f * px;
-                    g[5 * k + 2] = f * py;
-                    g[5 * k + 3] = f * pz;
-                }
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            acc[t] = bias4(s_b, kHidden * kW + 16 * t + 4 * j);
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(bf16x8, s_f[frag_id(kHidden, t, kb) * 64 + lane]), x[kHidden][kb], acc[t], 0,
+                    0, 0);
+        }
+        float o[12];
+#pragma unroll
+        for (int sl = 0; sl < 12; ++sl) o[sl] = acc[sl >> 2][sl & 3];
+
+        // loss and its gradient: this lane's lobes 2j, 2j + 1 (slots 0..4, 5..9)
+        const float wx = valid ? dirs[3 * (size_t)row] : 0.0f;
+        const float wy = valid ? dirs[3 * (size_t)row + 1] : 0.0f;
+        const float wz = valid ? dirs[3 * (size_t)row + 2] : 1.0f;
+        float mx = fmaxf(o[0], o[5]);
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        const float ea = expf(o[0] - mx), eb = expf(o[5] - mx);
+        float esum = ea + eb;
+        esum += __shfl_xor(esum, 16);
+        esum += __shfl_xor(esum, 32);
+        float pi[2] = {ea / esum, eb / esum}, kap[2], cc[2], nm[2], mu[2][3], lp[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float* q = o + 5 * h;
+            nm[h] = sqrtf(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + 1e-12f;
+            mu[h][0] = q[1] / nm[h];
+            mu[h][1] = q[2] / nm[h];
+            mu[h][2] = q[3] / nm[h];
+            kap[h] = kappa_of(q[4]);
+            cc[h] = mu[h][0] * wx + mu[h][1] * wy + mu[h][2] * wz;
+            lp[h] = logf(pi[h]) + log_vmf(kap[h], cc[h]);
+        }
+        float lmax = fmaxf(lp[0], lp[1]);
+        lmax = fmaxf(lmax, __shfl_xor(lmax, 16));
+        lmax = fmaxf(lmax, __shfl_xor(lmax, 32));
+        float ls = expf(lp[0] - lmax) + expf(lp[1] - lmax);
+        ls += __shfl_xor(ls, 16);
+        ls += __shfl_xor(ls, 32);
+        const float logq = lmax + logf(ls);
+        const float w = valid ? weight[row] * wnorm : 0.0f;
+        float g[12];
+#pragma unroll
+        for (int sl = 0; sl < 12; ++sl) g[sl] = 0.0f;
+        float loss = -w * logq;
+        if (valid) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float resp = expf(lp[h] - logq);  // responsibility
+                g[5 * h + 0] = -w * (resp - pi[h]);
+                // d log vmf / d kappa = 1/k - 2/(e^{2k} - 1) + (c - 1)
+                const float dlk = 1.0f / kap[h] - 2.0f / expm1f(2.0f * kap[h]) + (cc[h] - 1.0f);
+                g[5 * h + 4] = -w * resp * dlk * dkappa_ds(o[5 * h + 4]);
+                // d(-log q)/d m = -resp * kappa * (I - mu mu^T) w / |m|
+                const float f = -w * resp * kap[h] / nm[h];
+                g[5 * h + 1] = f * (wx - cc[h] * mu[h][0]);
+                g[5 * h + 2] = f * (wy - cc[h] * mu[h][1]);
+                g[5 * h + 3] = f * (wz - cc[h] * mu[h][2]);
             }
-            if (row < n) row_loss[row] = loss;
-            for (int c = 0; c < kOutPad; ++c) gbuf[0][i * kWS + c] = f2bf(g[c]);
+            // strategy selection (lane 0 of the row): -w log(c_l p_l + c_g q + c_b p_b),
+            // q held fixed, only when the record carries p_l, p_b (inputs 6, 7)
+            const float pl = v[6], pb = v[7];
+            if (j == 0 && pl + pb > 0.0f) {
+                const float pg = expf(logq);
+                float c[3];
+                selection_probs(o[10], o[11], c);
+                const float pmix = c[0] * pl + c[1] * pg + c[2] * pb;
+                const float rl = 1.0f / (1.0f + expf(-o[10])), rg = 1.0f / (1.0f + expf(-o[11]));
+                const float f = -w * (1.0f - 3.0f * kSelFloor) / pmix;
+                loss -= w * logf(pmix);
+                g[10] = f * rl * (1.0f - rl) * (pl - rg * pg - (1.0f - rg) * pb);
+                g[11] = f * (1.0f - rl) * rg * (1.0f - rg) * (pg - pb);
+            }
+            if (j == 0) row_loss[row] = loss;
         }
-        wave_sync();
-        // dZ of the output layer -> global G[5] (feature-major)
+        // activations -> A[l] (feature-major)
+#pragma unroll
+        for (int l = 0; l < kLayers; ++l) {
+            uint16_t* Al = A + (size_t)l * kW * n_pad + row;
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    Al[(size_t)kslot_feature(kb, j, e) * n_pad] = bf_bits(x[l][kb], e);
+        }
+        // output-layer dZ (bf16) -> G[5] in logical row order, and as the B operand
+        float gb[16];
+#pragma unroll
+        for (int sl = 0; sl < 12; ++sl) gb[sl] = (float)(__bf16)g[sl];
+#pragma unroll
+        for (int sl = 12; sl < 16; ++sl) gb[sl] = 0.0f;
         {
-            uint16_t* Gl = G + (size_t)kHidden * kW * n_pad;
-            for (int c = j; c < kOutPad; c += 4) {
-                Gl[(size_t)c * n_pad + row0 + i] = gbuf[0][i * kWS + c];
-            }
+            uint16_t* G5 = G + (size_t)kHidden * kW * n_pad + row;
+#pragma unroll
+            for (int sl = 0; sl < 12; ++sl)
+                G5[(size_t)out_logical(16 * (sl >> 2) + 4 * j + (sl & 3)) * n_pad] =
+                    __builtin_bit_cast(uint16_t, (__bf16)gb[sl]);
         }
-        // backward data path: dA_{l} = dZ_{l+1} W_{l+1}^T, dZ_l = dA_l * (A_{l+1} > 0)
-        int gcur = 0;
-        for (int l = kHidden - 1; l >= 0; --l) {
-            const uint16_t* wl = s_w + (l + 1) * kW * kWS;  // Wt_{l+1} [nout][64]
-            const int kin = layer_nout(l + 1);              // contraction over layer l+1 outputs
-            uint16_t* gin = gbuf[gcur];
-            uint16_t* gout = gbuf[gcur ^ 1];
-            f32x4 acc[4];
+        bf16x8 dz[2] = {pack8(gb), pack8(gb + 8)};
+        // backward: dH_l = W_l^T dZ_l, dZ_{l-1} = dH_l * (x[l] > 0)
+#pragma unroll
+        for (int l = kLayers - 1; l >= 1; --l) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-                for (int kb = 0; kb < kin / 16; ++kb) {
-                    const bf16x4 a = ld4(gin + i * kWS + kb * 16 + 4 * j);
-                    bf16x4 b;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) b[r] = (short)wl[(kb * 16 + 4 * j + r) * kWS + 16 * t + i];
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-                }
+                for (int kb = 0; kb < 2; ++kb)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, s_t[fragT_id(l, t, kb) * 64 + lane]), dz[kb], acc[t], 0, 0, 0);
             }
-            wave_sync();
-            uint16_t* Gl = G + (size_t)l * kW * n_pad;
+            float d[16];
+            uint16_t* Gl = G + (size_t)(l - 1) * kW * n_pad + row;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                uint16_t h[4];
+            for (int t = 0; t < 4; ++t)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = 4 * j + r, col = 16 * t + i;
-                    const bool on = bf2f(acts[l + 1][row * kWS + col]) > 0.0f;
-                    h[r] = f2bf(on ? acc[t][r] : 0.0f);
-                    gout[row * kWS + col] = h[r];
+                    const float hv = (float)x[l][t >> 1][(t & 1) * 4 + r];
+                    const __bf16 b = (__bf16)(hv > 0.0f ? acc[t][r] : 0.0f);
+                    d[4 * t + r] = (float)b;
+                    Gl[(size_t)(16 * t + 4 * j + r) * n_pad] = __builtin_bit_cast(uint16_t, b);
                 }
-                uint2 pk;
-                pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-                pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-                *reinterpret_cast<uint2*>(Gl + (size_t)(16 * t + i) * n_pad + row0 + 4 * j) = pk;
-            }
-            wave_sync();
-            gcur ^= 1;
+            dz[0] = pack8(d);
+            dz[1] = pack8(d + 8);
         }
-        // activations A_0..A_5 -> global (feature-major)
-        for (int l = 0; l < kLayers; ++l) {
-            uint16_t* Al = A + (size_t)l * kW * n_pad;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int col = 16 * t + i;
-                uint2 pk;
-                pk.x = (uint32_t)acts[l][(4 * j + 0) * kWS + col] | ((uint32_t)acts[l][(4 * j + 1) * kWS + col] << 16);
-                pk.y = (uint32_t)acts[l][(4 * j + 2) * kWS + col] | ((uint32_t)acts[l][(4 * j + 3) * kWS + col] << 16);
-                *reinterpret_cast<uint2*>(Al + (size_t)col * n_pad + row0 + 4 * j) = pk;
-            }
-        }
-        wave_sync();
     }
 }
 
@@ -600,9 +573,8 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const uint16_t* __restrict__ A
 
 // Fixed-order sum of the chunk partials, Adam, bf16 copy.
 __global__ void k_adam(const float* __restrict__ partial, uint32_t chunks, float* __restrict__ w,
-                       float* __restrict__ m, float* __restrict__ v, uint16_t* __restrict__ wbf,
-                       uint16_t* __restrict__ frag, float lr, float b1, float b2, float eps, float c1, float c2,
-                       float* __restrict__ grad_out) {
+                       float* __restrict__ m, float* __restrict__ v, uint16_t* __restrict__ frag, uint16_t* __restrict__ fragT, float lr, float b1, float b2,
+                       float eps, float c1, float c2, float* __restrict__ grad_out) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (uint32_t)kParams) return;
     float g = 0.0f;
@@ -617,9 +589,9 @@ __global__ void k_adam(const float* __restrict__ partial, uint32_t chunks, float
     w[p] = nw;
     if (p < (uint32_t)kWtElems) {
         const uint16_t h = f2bf(nw);
-        wbf[p] = h;
         const int l = min((int)p / (kW * kW), kHidden), r = (int)p - wt_offset(l);
         frag[frag_index(l, r / kW, r % kW)] = h;
+        if (l > 0) fragT[fragT_index(l, r / kW, r % kW)] = h;
     }
 }
 
@@ -669,8 +641,6 @@ __global__ __launch_bounds__(1024) void k_sum(const float* __restrict__ x, uint3
     if (threadIdx.x == 0) *out = total;
 }
 
-constexpr size_t kTrainSmem = sizeof(uint16_t) * kLayers * kW * kWS + sizeof(float) * kBiasElems +
-                              kWaves * (8 * sizeof(uint16_t) * kTileRows * kWS + sizeof(float) * kTileRows * kOutPad);
 
 }  // namespace
 
@@ -679,8 +649,8 @@ struct lr_guide {
     float* w = nullptr;       // fp32 master params
     float* m = nullptr;
     float* v = nullptr;
-    uint16_t* wbf = nullptr;  // bf16 copy of the Wt part (master layout)
     uint16_t* frag = nullptr; // bf16 copy of the Wt part in fragment order
+    uint16_t* fragT = nullptr; // bf16 copy of layers 1..5 in transposed fragment order
     uint32_t step = 0;
     float lr = 1e-3f;
     // training scratch (grow-only)
@@ -698,7 +668,7 @@ namespace {
 void free_guide(lr_guide* g) {
     if (!g) return;
     (void)hipSetDevice(g->ctx->device);
-    void* ptrs[] = {g->w, g->m, g->v, g->wbf, g->frag, g->A, g->G, g->partial, g->row_loss, g->scalars};
+    void* ptrs[] = {g->w, g->m, g->v, g->frag, g->fragT, g->A, g->G, g->partial, g->row_loss, g->scalars};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete g;
@@ -713,14 +683,17 @@ uint16_t host_f2bf(float f) {
 }
 
 int upload_params(lr_guide* g, const float* host) {
-    std::vector<uint16_t> bf(kWtElems), fr(kWtElems, 0);
+    std::vector<uint16_t> bf(kWtElems), fr(kWtElems, 0), ft(kFragsT * 512, 0);
     for (int p = 0; p < kWtElems; ++p) bf[p] = host_f2bf(host[p]);
     for (int l = 0; l < kLayers; ++l)
         for (int n = 0; n < layer_nout(l); ++n)
-            for (int k = 0; k < kW; ++k) fr[frag_index(l, n, k)] = bf[wt_offset(l) + n * kW + k];
+            for (int k = 0; k < kW; ++k) {
+                fr[frag_index(l, n, k)] = bf[wt_offset(l) + n * kW + k];
+                if (l > 0) ft[fragT_index(l, n, k)] = bf[wt_offset(l) + n * kW + k];
+            }
     LR_HIP_CHECK(hipMemcpy(g->w, host, sizeof(float) * kParams, hipMemcpyHostToDevice));
-    LR_HIP_CHECK(hipMemcpy(g->wbf, bf.data(), sizeof(uint16_t) * kWtElems, hipMemcpyHostToDevice));
     LR_HIP_CHECK(hipMemcpy(g->frag, fr.data(), sizeof(uint16_t) * kWtElems, hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMemcpy(g->fragT, ft.data(), sizeof(uint16_t) * ft.size(), hipMemcpyHostToDevice));
     return LR_OK;
 }
 
@@ -746,8 +719,8 @@ int lr_guide_create(lr_context* ctx, uint64_t seed, float lr, lr_guide** out) {
     hipError_t e = hipMalloc(&g->w, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMalloc(&g->m, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMalloc(&g->v, sizeof(float) * kParams);
-    if (e == hipSuccess) e = hipMalloc(&g->wbf, sizeof(uint16_t) * kWtElems);
     if (e == hipSuccess) e = hipMalloc(&g->frag, sizeof(uint16_t) * kWtElems);
+    if (e == hipSuccess) e = hipMalloc(&g->fragT, sizeof(uint16_t) * kFragsT * 512);
     if (e == hipSuccess) e = hipMalloc(&g->scalars, sizeof(float) * 16);
     if (e == hipSuccess) e = hipMemset(g->m, 0, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMemset(g->v, 0, sizeof(float) * kParams);
@@ -871,9 +844,10 @@ int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, con
     if (int rc = grow_training(g, n_pad, chunks, s)) return rc;
     hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, s, weights, n, g->scalars);
     const uint32_t tiles = n_pad / kTileRows;
-    const uint32_t grid = std::min<uint32_t>((tiles + kWaves - 1) / kWaves, (uint32_t)g->ctx->n_cu * 2);
-    hipLaunchKernelGGL(k_train_fb, dim3(std::max(1u, grid)), dim3(kBlock), kTrainSmem, s, g->wbf, g->w + kWtElems,
-                       inputs, dirs, weights, n, n_pad, g->scalars, g->A, g->G, g->row_loss);
+    // one workgroup per CU fits the 88 KB of fragments in LDS
+    const uint32_t grid = std::min<uint32_t>((tiles + kTrainWaves - 1) / kTrainWaves, (uint32_t)g->ctx->n_cu);
+    hipLaunchKernelGGL(k_train_fb, dim3(std::max(1u, grid)), dim3(kTrainBlock), kTrainSmem, s, g->frag, g->fragT,
+                       g->w + kWtElems, inputs, dirs, weights, n, n_pad, g->scalars, g->A, g->G, g->row_loss);
     // rows n .. n_pad-1 are written by k_train_fb with zero gradients, so
     // they add nothing to the weight gradients
     hipLaunchKernelGGL(k_wgrad, dim3(chunks, kLayers), dim3(kBlock), 0, s, g->A, g->G, n_pad, chunk_rows, g->partial);
@@ -881,7 +855,7 @@ int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, con
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, (float)g->step), c2 = 1.0f - std::pow(b2, (float)g->step);
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, g->partial, chunks, g->w, g->m, g->v,
-                       g->wbf, g->frag, g->lr, b1, b2, 1e-8f, c1, c2, grad_out);
+                       g->frag, g->fragT, g->lr, b1, b2, 1e-8f, c1, c2, grad_out);
     if (loss_out) hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, g->row_loss, n, loss_out);
     LR_HIP_CHECK(hipGetLastError());
     return LR_OK;
