@@ -429,6 +429,25 @@ int lr_render_guided(lr_scene* scene, const lr_camera* camera, const lr_render_p
                      lr_guide* guide, const lr_guide_params* guide_params, float* film_xyz,
                      float* film_weight, lr_render_stats* stats, void* stream);
 
+/* ---- scene import --------------------------------------------------------
+ * PLY meshes for TriangleMesh::loadPly (src/liar/scenery/triangle_mesh.cpp:183-281):
+ * ascii and binary (either endianness), any numeric property types; reads
+ * vertex x y z [nx ny nz] [u v] and face vertex_indices (or vertex_index).
+ * Triangles as LiAR makes them: a polygon keeps only its last fan triangle
+ * (i0, i_n-2, i_n-1) (LiAR stores the fan triangle after its loop), or every
+ * fan triangle with LR_PLY_FAN_ALL.  Arrays are allocated by the library
+ * (NULL when absent) and released with lr_mesh_data_free. */
+typedef struct lr_mesh_data {
+    uint32_t n_verts, n_tris;
+    float* verts;     /* 3 * n_verts                   */
+    float* normals;   /* 3 * n_verts, or NULL           */
+    float* uvs;       /* 2 * n_verts, or NULL           */
+    uint32_t* tris;   /* 3 * n_tris vertex indices       */
+} lr_mesh_data;
+#define LR_PLY_FAN_ALL 1u
+int lr_load_ply(const char* path, uint32_t flags, lr_mesh_data* out);
+void lr_mesh_data_free(lr_mesh_data* mesh);
+
 /* ---- deterministic synthetic inputs (host, counter-based Philox4x32-10) --- */
 
 /* Random triangle soup: centre c ~ U[0,1)^3, v_k = c + scale * xi_k,

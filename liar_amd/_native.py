@@ -84,6 +84,14 @@ class RenderParams(C.Structure):
                 ("graph", C.c_uint32)]
 
 
+class MeshData(C.Structure):
+    _fields_ = [("n_verts", C.c_uint32), ("n_tris", C.c_uint32), ("verts", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)), ("tris", C.POINTER(C.c_uint32))]
+
+
+LR_PLY_FAN_ALL = 1
+
+
 class BatchItem(C.Structure):
     _fields_ = [("camera", Camera), ("params", RenderParams), ("film_xyz", C.c_void_p), ("film_weight", C.c_void_p)]
 
@@ -136,6 +144,8 @@ SIGNATURES = {
     "lr_gen_terrain": (_I, [_U64, _U32, _U32, _F, _F, _F, _P, _P]),
     "lr_render_guided": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, C.POINTER(GuideParams), _P, _P,
                               C.POINTER(RenderStats), _P]),
+    "lr_load_ply": (_I, [C.c_char_p, _U32, C.POINTER(MeshData)]),
+    "lr_mesh_data_free": (None, [C.POINTER(MeshData)]),
     "lr_batch_create": (_I, [_P, _P, _U32, C.POINTER(_P)]),
     "lr_batch_destroy": (_I, [_P]),
     "lr_render_batch": (_I, [_P, _P, _U32, _P, C.POINTER(RenderStats), _P]),

@@ -24,16 +24,47 @@ class TriangleMesh(SceneObject):
     def __init__(self, vertices, normals=(), uvs=(), triangles=()):
         super().__init__()
         self._vertices = np.asarray(vertices, dtype=np.float32).reshape(-1, 3)
-        self._normals = list(normals)
-        self._uvs = list(uvs)
-        idx = []
-        for tri in triangles:
-            if len(tri) != 3:
-                raise ValueError("a triangle needs three corners")
-            idx.append([c[0] if isinstance(c, (tuple, list)) else c for c in tri])
-        self._triangles = np.asarray(idx, dtype=np.uint32).reshape(-1, 3)
+        self._normals = normals if isinstance(normals, np.ndarray) else list(normals)
+        self._uvs = uvs if isinstance(uvs, np.ndarray) else list(uvs)
+        if isinstance(triangles, np.ndarray) and triangles.ndim == 2 and triangles.shape[1] == 3:
+            self._triangles = np.ascontiguousarray(triangles, dtype=np.uint32)
+        else:
+            idx = []
+            for tri in triangles:
+                if len(tri) != 3:
+                    raise ValueError("a triangle needs three corners")
+                idx.append([c[0] if isinstance(c, (tuple, list)) else c for c in tri])
+            self._triangles = np.asarray(idx, dtype=np.uint32).reshape(-1, 3)
         if len(self._triangles) and self._triangles.max() >= len(self._vertices):
             raise ValueError("triangle references a missing vertex")
+
+    @staticmethod
+    def loadPly(path, fan_all=False):
+        """``TriangleMesh.loadPly(path)`` (src/liar/scenery/triangle_mesh.cpp:183-281)
+        through the library's PLY reader (lr_load_ply).  Like LiAR, a polygon
+        face keeps only its last fan triangle; ``fan_all=True`` keeps all."""
+        import ctypes as C
+
+        from . import _native as N
+
+        m = N.MeshData()
+        N.check(N.lib().lr_load_ply(str(path).encode(), N.LR_PLY_FAN_ALL if fan_all else 0, C.byref(m)),
+                "lr_load_ply")
+        try:
+            nv, nt = m.n_verts, m.n_tris
+
+            def arr(ptr, n, dtype):
+                if not ptr or n == 0:
+                    return np.zeros((0,), dtype)
+                return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+            verts = arr(m.verts, 3 * nv, np.float32).reshape(-1, 3)
+            normals = arr(m.normals, 3 * nv, np.float32).reshape(-1, 3)
+            uvs = arr(m.uvs, 2 * nv, np.float32).reshape(-1, 2)
+            tris = arr(m.tris, 3 * nt, np.uint32).reshape(-1, 3)
+        finally:
+            N.lib().lr_mesh_data_free(C.byref(m))
+        return TriangleMesh(verts, normals, uvs, tris)
 
     def vertices(self):
         return self._vertices
