@@ -258,6 +258,10 @@ typedef struct lr_light {
                                       front face normal = cross(dx, dy)     */
     float direction[3];     /* sun: unit vector towards the sun             */
     float angle;            /* sun: apparent diameter [rad]                 */
+    uint32_t samples;       /* DirectLighting emission samples per camera
+                               sample (LightArea / LightSky
+                               numberOfEmissionSamples); 0: LiAR's default,
+                               9 for a parallelogram, 1 for sky and sun     */
 } lr_light;
 
 /* PerspectiveCamera (src/liar/cameras/perspective_camera.cpp:406-434),
@@ -323,6 +327,34 @@ int lr_scene_destroy(lr_scene* scene);
  * count.  Deterministic for a given seed.  stats may be NULL. */
 int lr_render(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
               float* film_xyz, float* film_weight, lr_render_stats* stats, void* stream);
+
+/* ---- DirectLighting (src/liar/tracers/direct_lighting.cpp:42-377) --------
+ * Every surface hit adds its emission (camera rays, and objects that are not
+ * lights), a light estimate with shadow rays (RayTracer::
+ * estimateLightContribution, src/liar/kernel/ray_tracer.cpp:188-268) and
+ * the radiance of its specular (and, with trace_glossy, glossy) reflection
+ * and transmission directions, traced recursively.  High quality (camera
+ * rays and specular continuations of high quality): every light with its
+ * own number of samples (lr_light.samples), each a light sample and a BSDF
+ * sample combined by the power heuristic; otherwise secondary_light_samples
+ * lights picked by power.  Rays that escape add nothing.  Deterministic for
+ * a seed; the film is accumulated as by lr_render. */
+typedef struct lr_direct_params {
+    uint32_t max_generation;           /* RayTracer::maxRayGeneration: rays of
+                                          generation 0 (camera) .. this are cast
+                                          (LiAR default 8, at most 23)        */
+    uint32_t secondary_light_samples;  /* numSecondaryLightSamples (0: 1, <= 8) */
+    uint32_t trace_glossy;             /* traceGlossy (LiAR default true)    */
+} lr_direct_params;
+
+/* params->max_depth, rr_depth, rr_survival and graph are ignored;
+ * chunk_paths 0 means 128k camera paths per chunk (each chunk keeps a branch
+ * queue of 4x its paths and a shadow queue of as many rays as its hits can
+ * make: a scene whose branches overflow returns LR_E_INVALID and asks for a
+ * smaller chunk).  dparams NULL: LiAR's defaults (8, 1, true). */
+int lr_render_direct(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
+                     const lr_direct_params* dparams, float* film_xyz, float* film_weight,
+                     lr_render_stats* stats, void* stream);
 
 /* ---- batches: many scenes in one wavefront (no counterpart in LiAR) -------
  * A batch holds scenes built with lr_scene_create (non-empty geometry each);

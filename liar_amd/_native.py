@@ -68,7 +68,7 @@ class Material(C.Structure):
 class Light(C.Structure):
     _fields_ = [("type", C.c_uint32), ("radiance", C.c_uint32), ("double_sided", C.c_uint32),
                 ("origin", C.c_float * 3), ("dx", C.c_float * 3), ("dy", C.c_float * 3),
-                ("direction", C.c_float * 3), ("angle", C.c_float)]
+                ("direction", C.c_float * 3), ("angle", C.c_float), ("samples", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -101,6 +101,11 @@ LR_BATCH_GRAPH = 1
 
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("chunks", C.c_uint32), ("seconds", C.c_double)]
+
+
+class DirectParams(C.Structure):
+    _fields_ = [("max_generation", C.c_uint32), ("secondary_light_samples", C.c_uint32),
+                ("trace_glossy", C.c_uint32)]
 
 
 class GuideParams(C.Structure):
@@ -160,6 +165,8 @@ SIGNATURES = {
     "lr_scene_create": (_I, [_P, _P, _P, _P, _U32, _P, _U32, _P, _U32, C.POINTER(_P)]),
     "lr_scene_destroy": (_I, [_P]),
     "lr_render": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, _P, C.POINTER(RenderStats), _P]),
+    "lr_render_direct": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(DirectParams), _P, _P,
+                              C.POINTER(RenderStats), _P]),
 }
 
 _lib = None

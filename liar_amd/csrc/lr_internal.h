@@ -54,6 +54,9 @@ int bind(const lr_context* ctx);
 // arrays that hold several trees.
 int trace_closest_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, const uint32_t* n_dev,
                         lr_hits_soa* hits, hipStream_t stream, const uint32_t* root = nullptr);
+// Any-hit over a device ray queue of length *n_dev (<= n_max): occluded[i] = 0 / 1.
+int trace_any_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, const uint32_t* n_dev, uint8_t* occluded,
+                    hipStream_t stream);
 // Grows the per-stream traversal scratch for queues of up to n_max rays, so
 // that later trace_closest_queue calls (e.g. under graph capture) allocate
 // nothing.

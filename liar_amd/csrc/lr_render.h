@@ -26,7 +26,7 @@ struct DevSpectrum {
 };
 
 struct DevLight {
-    uint32_t type, radiance, double_sided, pad;
+    uint32_t type, radiance, double_sided, samples;  // samples: DirectLighting emission samples
     float o[3], area;
     float dx[3], cos_max;   // cos_max: sun cone
     float dy[3], inv_solid;  // inv_solid: 1 / sun cone solid angle
@@ -295,8 +295,10 @@ __device__ inline bool bsdf_eval(const BsdfParams& P, V3 wi, V3 wo, float& value
 }
 
 // Bsdf::sample.  u1, u2: 2D sample; uc: component sample.
+// allow: bit 0 reflection, bit 1 transmission (the caps a caller permits;
+// only the dielectric has both, and chooses among the permitted ones).
 __device__ inline bool bsdf_sample(const BsdfParams& P, V3 wi, float u1, float u2, float uc, V3& wo, float& value,
-                                   float& pdf) {
+                                   float& pdf, uint32_t allow = 3u) {
     switch (P.type) {
     case LR_MAT_LAMBERT: {  // lambert.cpp:116-124
         wo = cosine_hemisphere(u1, u2, pdf);
@@ -325,8 +327,8 @@ __device__ inline bool bsdf_sample(const BsdfParams& P, V3 wi, float u1, float u
         const float r_orth = (ior * cos_i - cos_t) / (ior * cos_i + cos_t);
         const float r_par = (cos_i - ior * cos_t) / (cos_i + ior * cos_t);
         const float fr = 0.5f * (r_orth * r_orth + r_par * r_par);
-        const float pow_r = fabsf(P.refl) * fr;
-        const float pow_t = fabsf(P.trans) * (1.0f - fr);
+        const float pow_r = (allow & 1u) ? fabsf(P.refl) * fr : 0.0f;
+        const float pow_t = (allow & 2u) ? fabsf(P.trans) * (1.0f - fr) : 0.0f;
         if (pow_r + pow_t == 0.0f) return false;
         const float prob_r = pow_r / (pow_r + pow_t);
         if (uc < prob_r) {

@@ -751,6 +751,11 @@ struct lr_scene {
     hipStream_t cstream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // DirectLighting: two branch queues, hits and the shadow-ray queue
+    float* d_dlwork = nullptr;
+    uint8_t* d_dlocc = nullptr;
+    size_t dl_qcap = 0, dl_scap = 0;
+    uint32_t light_samples = 0;  // sum of the lights' DirectLighting samples
 };
 
 namespace {
@@ -767,7 +772,7 @@ void free_scene(lr_scene* s) {
     if (s->ev_out) (void)hipEventDestroy(s->ev_out);
     if (s->cstream) (void)hipStreamDestroy(s->cstream);
     void* ptrs[] = {s->d_prim_mat, s->d_prim_geom, s->d_mats, s->d_spectra, s->d_table, s->d_lights,
-                    s->d_obs,      s->d_work,      s->d_counts, s->d_gwork};
+                    s->d_obs,      s->d_work,      s->d_counts, s->d_gwork, s->d_dlwork, s->d_dlocc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -994,6 +999,8 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
         d.type = L.type;
         d.radiance = L.radiance;
         d.double_sided = L.double_sided ? 1u : 0u;
+        d.samples = L.samples ? L.samples : (L.type == LR_LIGHT_PARALLELOGRAM ? 9u : 1u);
+        if (d.samples > 64) return lr::fail(LR_E_INVALID, "light " + std::to_string(i) + ": at most 64 samples");
         if (L.radiance >= n_spectra) return lr::fail(LR_E_INVALID, "light " + std::to_string(i) + " has no radiance");
         const double lum = host_luminance(spectra[L.radiance]);
         if (L.type == LR_LIGHT_PARALLELOGRAM) {
@@ -1056,6 +1063,7 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
     sc->bvh = bvh;
     sc->n_triangles = bvh->info.n_triangles;
     sc->n_lights = n_lights;
+    for (const DevLight& d : dl) sc->light_samples += d.samples;
     int rc = lr::bind(ctx);
     if (!rc) rc = upload(pm, &sc->d_prim_mat);
     if (!rc) rc = upload(geom, &sc->d_prim_geom);
@@ -1705,6 +1713,475 @@ int lr_render_guided(lr_scene* sc, const lr_camera* cam, const lr_render_params*
                      void* stream) {
     if (!guide) return lr::fail(LR_E_INVALID, "NULL guide");
     return render_impl(sc, cam, prm, guide, gparams, film_xyz, film_w, stats, stream);
+}
+
+}  // extern "C"
+
+
+// ============================================================================
+// DirectLighting (src/liar/tracers/direct_lighting.cpp:119-377) as a
+// wavefront.  Every surface hit adds its emission, a light estimate
+// (RayTracer::estimateLightContribution, src/liar/kernel/ray_tracer.cpp:188-268)
+// whose samples become shadow rays, and continues along its specular (and
+// glossy) reflection and transmission directions.  A dielectric continues
+// both ways, so a camera sample's rays form a tree: every queue entry carries
+// the weight of its branch, and radiance reaches the camera sample's
+// accumulator by atomics (so films are reproducible up to the rounding of
+// the order of those additions).
+// ============================================================================
+namespace {
+
+struct DLQueue {
+    float* r[8];     // ox oy oz dx dy dz tnear tfar
+    uint32_t* pid;   // camera path (chunk-local)
+    float* w;        // branch weight: product of value |cos| / pdf of its continuations
+    uint32_t* code;  // bit 31: high quality; bits 0..22: one bit per continuation (0 reflection, 1 transmission)
+};
+
+struct ShadowQueue {
+    float* r[8];
+    uint32_t* pid;
+    float* c;  // radiance added to the camera path when the ray is unoccluded
+};
+
+struct DLArgs {
+    uint32_t max_gen;    // generations 0 (camera) .. max_gen are cast
+    uint32_t secondary;  // light samples of a hit that is not high quality
+    uint32_t glossy;     // continue along glossy reflections
+    uint32_t slots;      // shadow-ray candidates per hit (uniform loop bound)
+    uint32_t qcap, scap;
+    uint32_t* qover;  // branch entries that did not fit (the host grows the queues and repeats the chunk)
+    uint32_t* sover;  // shadow rays that did not fit
+};
+
+constexpr uint32_t kHQ = 0x80000000u;
+
+// Power heuristic a^2 / (a^2 + b^2) for a > 0, written so that large pdfs
+// do not overflow in FP32 (ray_tracer.cpp:180-183).
+__device__ inline float power_heuristic(float a, float b) {
+    const float r = b / a;
+    return 1.0f / (1.0f + r * r);
+}
+
+__device__ inline bool bsdf_is_specular(uint32_t t) {
+    return t == LR_MAT_MIRROR || t == LR_MAT_CONDUCTOR || t == LR_MAT_DIELECTRIC;
+}
+
+// Random numbers of a tree node: (generation, branch) replaces the generation
+// of the path tracer's key, so every node of every tree has its own streams.
+__device__ inline lr::u4 dl_rng(const Params& P, uint64_t gp, uint32_t gen, uint32_t br, uint32_t purpose) {
+    return rng(P, gp, gen | (br << 8), purpose);
+}
+
+struct DLHit {
+    BsdfParams B;
+    V3 wi, tt, bb, nf, start;
+    float w, lam;
+    uint64_t gp;
+    uint32_t gen, br;
+    bool hq;
+};
+
+// Shadow-ray candidate j of a hit.  High quality: light k's 2 n_k
+// candidates follow those of lights 0..k-1, even ones light samples, odd
+// ones BSDF samples (n_k = lr_light.samples).  Otherwise candidates 2s and
+// 2s+1 are the light and BSDF sample of secondary sample s, whose light is
+// picked by power and whose estimate is divided by (samples x pick pdf)
+// (direct_lighting.cpp:259-310).  Returns false when the candidate makes no
+// ray; c is its radiance if unoccluded, [kTolerance, far) its extent.
+__device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DLArgs& A, const DLHit& H, uint32_t j,
+                                    float& c, V3& dir, float& far) {
+    uint32_t k = 0, local = 0;
+    float nl = 1.0f, scale = 1.0f;
+    if (H.hq) {
+        uint32_t base = 0;
+        bool found = false;
+        for (uint32_t m = 0; m < S.n_lights && !found; ++m) {
+            const uint32_t n2 = 2u * S.lights[m].samples;
+            if (j < base + n2) {
+                k = m;
+                local = j - base;
+                found = true;
+            }
+            base += n2;
+        }
+        if (!found) return false;
+        nl = (float)S.lights[k].samples;
+    } else {
+        if (j >= 2u * A.secondary || S.n_lights == 0) return false;
+        const lr::u4 rs = dl_rng(P, H.gp, H.gen, H.br, 10u + (j >> 1));
+        k = light_pick(S, lr::u01(rs.x));
+        const float psel = S.lights[k].sel_pdf;
+        if (!(psel > 0.0f)) return false;
+        local = j & 1u;
+        scale = (float)A.secondary * psel;
+    }
+    const DevLight& Lk = S.lights[k];
+    const float Le = spectrum_eval(S, Lk.radiance, H.lam, false);
+    if (Le == 0.0f) return false;
+    const lr::u4 r = dl_rng(P, H.gp, H.gen, H.br, 100u + j);
+    float f = 0.0f, pb = 0.0f, pl = 0.0f;
+    if ((local & 1u) == 0u) {
+        // light sample, weighted against BSDF sampling (ray_tracer.cpp:211-231)
+        dir = light_sample(Lk, H.start, lr::u01(r.x), lr::u01(r.y), pl);
+        if (!(pl > 0.0f)) return false;
+        const V3 wo = v3(dot(dir, H.tt), dot(dir, H.bb), dot(dir, H.nf));
+        if (!bsdf_eval(H.B, H.wi, wo, f, pb) || f == 0.0f) return false;
+        c = H.w * f * Le * power_heuristic(nl * pl, nl * pb) * fabsf(wo.z) / (nl * pl * scale);
+    } else {
+        // BSDF sample that reaches the light (ray_tracer.cpp:234-264)
+        V3 wo;
+        if (!bsdf_sample(H.B, H.wi, lr::u01(r.x), lr::u01(r.y), lr::u01(r.z), wo, f, pb) || !(pb > 0.0f) ||
+            f == 0.0f)
+            return false;
+        dir = normalize(wo.x * H.tt + wo.y * H.bb + wo.z * H.nf);
+        pl = light_pdf(Lk, H.start, dir);
+        if (!(pl > 0.0f)) return false;
+        const float wt = bsdf_is_specular(H.B.type) ? 1.0f : power_heuristic(nl * pb, nl * pl);
+        c = H.w * f * Le * wt * fabsf(wo.z) / (nl * pb * scale);
+    }
+    far = INFINITY;
+    if (Lk.type == LR_LIGHT_PARALLELOGRAM) {
+        // ends 1e-4 (relative) short of the light's plane, so that the
+        // light's own triangles never occlude it in FP32
+        const V3 n = ld3(Lk.n);
+        const float t = dot(ld3(Lk.o) - H.start, n) / dot(dir, n);
+        if (!(t > 0.0f)) return false;
+        far = t * (1.0f - 1e-4f);
+    }
+    return c != 0.0f && isfinite(c);
+}
+
+// Camera entries: the path's throughput, high quality, branch 0.
+__global__ void k_dl_init(const uint32_t* count, PathState ps, DLQueue d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    d.w[i] = ps.thr[d.pid[i]];
+    d.code[i] = kHQ;
+}
+
+// Start of a generation: queue `cur` is about to be traced, the next queue
+// and the shadow queue start empty.
+__global__ void k_dl_next(uint32_t* counts, uint32_t cur, uint32_t nxt, uint32_t qcap, uint64_t* rays) {
+    if (threadIdx.x == 0) {
+        *rays += min(counts[cur], qcap);
+        counts[nxt] = 0u;
+        counts[2] = 0u;
+    }
+}
+
+__global__ void k_dl_fill_miss(const uint32_t* count, uint32_t cap, Hits h) {
+    const uint32_t n = min(*count, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        h.prim[i] = LR_MISS;
+        h.t[i] = INFINITY;
+    }
+}
+
+// One generation's hits: emission, shadow-ray candidates, continuations.
+// Grid-stride over the queue in whole workgroups (every thread of a block
+// runs the same number of appends).
+__global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t gen, Hits h,
+                                                  const float4* __restrict__ prim_geom, DLQueue q,
+                                                  const uint32_t* count, PathState ps, DLQueue qn,
+                                                  uint32_t* count_next, ShadowQueue sq, uint32_t* scount,
+                                                  uint64_t path0, DLArgs A) {
+    const uint32_t n = min(*count, A.qcap);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        DLHit H;
+        bool surf = false;
+        uint32_t pid = 0;
+        V3 x = v3(0, 0, 0);
+        if (i < n && h.prim[i] != LR_MISS) {  // escaped rays add nothing (alpha = 0)
+            pid = q.pid[i];
+            const uint32_t prim = h.prim[i];
+            const V3 o = v3(q.r[0][i], q.r[1][i], q.r[2][i]);
+            const V3 d = v3(q.r[3][i], q.r[4][i], q.r[5][i]);
+            const float tnear = q.r[6][i];
+            H.w = q.w[i];
+            const uint32_t code = q.code[i];
+            H.hq = (code & kHQ) != 0u;
+            H.br = code & ~kHQ;
+            H.gen = gen;
+            H.gp = path0 + pid;
+            H.lam = ps.lam[pid];
+            x = o + h.t[i] * d;
+            const float4 g = prim_geom[prim];
+            V3 ng;
+            bool leaving;
+            if (prim < S.n_triangles) {
+                ng = v3(g.x, g.y, g.z);
+                leaving = dot(d, ng) > 0.0f;
+            } else {
+                const V3 c = v3(g.x, g.y, g.z);
+                ng = (1.0f / g.w) * (x - c);
+                const V3 s0 = (o + tnear * d) - c;
+                leaving = dot(s0, s0) < g.w * g.w;
+            }
+            H.nf = dot(ng, d) < 0.0f ? ng : -ng;
+            const DevMaterial m = S.mats[S.prim_mat[prim]];
+            // shader emission: camera rays and objects that are not lights
+            // (direct_lighting.cpp:248-255)
+            if ((gen == 0 || m.light == LR_NONE) && m.emission != LR_NONE) {
+                const float Le = spectrum_eval(S, m.emission, H.lam, false);
+                if (Le != 0.0f) atomicAdd(&ps.L[pid], H.w * Le);
+            }
+            if (m.type != LR_MAT_NONE) {
+                surf = true;
+                H.B.type = m.type;
+                H.B.refl = spectrum_eval(S, m.refl, H.lam, true);
+                H.B.trans = spectrum_eval(S, m.trans, H.lam, true);
+                H.B.eta = fmaxf(spectrum_eval(S, m.eta, H.lam, false), 1e-9f);
+                H.B.kappa = fmaxf(spectrum_eval(S, m.kappa, H.lam, false), 1e-9f);
+                H.B.ior = leaving ? H.B.eta / H.B.kappa : H.B.kappa / H.B.eta;
+                const float rgh = fmaxf(m.roughness, 1e-3f);
+                H.B.alpha = rgh * rgh;
+                onb(H.nf, H.tt, H.bb);
+                const V3 wiw = -d;
+                H.wi = v3(dot(wiw, H.tt), dot(wiw, H.bb), fmaxf(dot(wiw, H.nf), 1.1920929e-7f));
+                H.start = x + (10.0f * kTolerance) * H.nf;
+            }
+        }
+        for (uint32_t j = 0; j < A.slots; ++j) {
+            float c = 0.0f, far = 0.0f;
+            V3 dir = v3(0, 0, 1);
+            const bool alive = surf && dl_candidate(S, P, A, H, j, c, dir, far);
+            const uint32_t pos = block_append(alive, scount);
+            if (alive) {
+                if (pos < A.scap) {
+                    sq.r[0][pos] = H.start.x;
+                    sq.r[1][pos] = H.start.y;
+                    sq.r[2][pos] = H.start.z;
+                    sq.r[3][pos] = dir.x;
+                    sq.r[4][pos] = dir.y;
+                    sq.r[5][pos] = dir.z;
+                    sq.r[6][pos] = kTolerance;
+                    sq.r[7][pos] = far;
+                    sq.pid[pos] = pid;
+                    sq.c[pos] = c;
+                } else {
+                    atomicAdd(A.sover, 1u);
+                }
+            }
+        }
+        // specular / glossy continuations (direct_lighting.cpp:314-377):
+        // reflection with the reflection caps, transmission (dielectric only)
+        // with the transmission caps; high quality only along specular ones
+        const bool glossy = surf && (H.B.type == LR_MAT_COOK_TORRANCE || H.B.type == LR_MAT_COOK_TORRANCE_BECKMANN);
+        const bool cont = surf && gen < A.max_gen && (bsdf_is_specular(H.B.type) || (glossy && A.glossy));
+        for (uint32_t side = 0; side < 2; ++side) {
+            bool alive = false;
+            V3 dir = v3(0, 0, 1), org = v3(0, 0, 0);
+            float wc = 0.0f;
+            if (cont && (side == 0 || H.B.type == LR_MAT_DIELECTRIC)) {
+                const lr::u4 r = dl_rng(P, H.gp, gen, H.br, 5u + side);
+                V3 wo;
+                float f = 0.0f, pdf = 0.0f;
+                if (bsdf_sample(H.B, H.wi, lr::u01(r.x), lr::u01(r.y), lr::u01(r.z), wo, f, pdf, side ? 2u : 1u) &&
+                    pdf > 0.0f && f != 0.0f && (side ? wo.z < 0.0f : wo.z > 0.0f)) {
+                    wc = H.w * f * fabsf(wo.z) / pdf;
+                    if (wc > 0.0f && isfinite(wc)) {
+                        alive = true;
+                        dir = normalize(wo.x * H.tt + wo.y * H.bb + wo.z * H.nf);
+                        org = x + ((side ? -10.0f : 10.0f) * kTolerance) * H.nf;
+                    }
+                }
+            }
+            const uint32_t pos = block_append(alive, count_next);
+            if (alive) {
+                if (pos < A.qcap) {
+                    qn.r[0][pos] = org.x;
+                    qn.r[1][pos] = org.y;
+                    qn.r[2][pos] = org.z;
+                    qn.r[3][pos] = dir.x;
+                    qn.r[4][pos] = dir.y;
+                    qn.r[5][pos] = dir.z;
+                    qn.r[6][pos] = kTolerance;
+                    qn.r[7][pos] = INFINITY;
+                    qn.pid[pos] = pid;
+                    qn.w[pos] = wc;
+                    qn.code[pos] = ((H.hq && bsdf_is_specular(H.B.type)) ? kHQ : 0u) | ((H.br << 1) | side);
+                } else {
+                    atomicAdd(A.qover, 1u);
+                }
+            }
+        }
+    }
+}
+
+// Unoccluded shadow rays add their radiance to their camera path.
+__global__ void k_dl_resolve(const uint32_t* scount, uint32_t scap, ShadowQueue sq, const uint8_t* occluded,
+                             PathState ps, uint64_t* rays) {
+    const uint32_t n = min(*scount, scap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *rays += n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (!occluded || !occluded[i]) atomicAdd(&ps.L[sq.pid[i]], sq.c[i]);
+}
+
+// Grows the DirectLighting workspace of a scene: two branch queues (11 words
+// per entry), their hits (4 words) and the shadow queue (10 words per ray +
+// an occlusion byte).
+int ensure_dl_work(lr_scene* sc, size_t qcap, size_t scap, hipStream_t s) {
+    if (sc->dl_qcap >= qcap && sc->dl_scap >= scap) return LR_OK;
+    LR_HIP_CHECK(hipStreamSynchronize(s));
+    if (sc->d_dlwork) LR_HIP_CHECK(hipFree(sc->d_dlwork));
+    if (sc->d_dlocc) LR_HIP_CHECK(hipFree(sc->d_dlocc));
+    sc->d_dlwork = nullptr;
+    sc->d_dlocc = nullptr;
+    sc->dl_qcap = sc->dl_scap = 0;
+    qcap = std::max(qcap, sc->dl_qcap);
+    scap = std::max(scap, sc->dl_scap);
+    LR_HIP_CHECK(hipMalloc(&sc->d_dlwork, sizeof(float) * (26 * qcap + 10 * scap)));
+    LR_HIP_CHECK(hipMalloc(&sc->d_dlocc, scap));
+    sc->dl_qcap = qcap;
+    sc->dl_scap = scap;
+    return LR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params* prm, const lr_direct_params* dprm,
+                     float* film_xyz, float* film_w, lr_render_stats* stats, void* stream) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!sc || !cam || !prm) return lr::fail(LR_E_INVALID, "NULL scene, camera or params");
+    if (!film_xyz || !film_w) return lr::fail(LR_E_INVALID, "NULL film buffer");
+    if (int rc = lr::bind(sc->ctx)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const lr_render_params& R = *prm;
+    DevCamera C;
+    if (int rc = make_camera(cam, C)) return rc;
+    Params P;
+    if (int rc = make_params(R, P)) return rc;
+    P.max_gen = 1;              // the camera kernel makes generation 0 only
+    P.rr_depth = 0xFFFFFFFFu;   // no roulette
+    DLArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.max_gen = dprm ? dprm->max_generation : 8u;
+    A.secondary = dprm && dprm->secondary_light_samples ? dprm->secondary_light_samples : 1u;
+    A.glossy = dprm ? (dprm->trace_glossy ? 1u : 0u) : 1u;
+    if (A.max_gen > 23) return lr::fail(LR_E_INVALID, "lr_render_direct: max_generation must be <= 23");
+    if (A.secondary > 8) return lr::fail(LR_E_INVALID, "lr_render_direct: secondary_light_samples must be <= 8");
+    if (int rc = build_observer(sc, R.lambda_min, R.lambda_max)) return rc;
+    const DevScene S = dev_scene(sc);
+    A.slots = sc->n_lights ? std::max(2 * sc->light_samples, 2 * A.secondary) : 0u;
+    if (A.slots > 256) return lr::fail(LR_E_INVALID, "lr_render_direct: more than 128 light samples per hit");
+
+    const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
+    const uint64_t n_pix = (uint64_t)R.width * R.height;
+    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 17);
+    const uint64_t pix_per_chunk = std::max<uint64_t>(1, chunk_req / ppp);
+    const uint64_t cap64 = pix_per_chunk * ppp;
+    if (cap64 >= (1ull << 26)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");
+    const uint32_t cap = (uint32_t)cap64;
+    if (int rc = ensure_work(sc, cap, s)) return rc;  // camera path state
+    size_t qcap = std::max<size_t>(sc->dl_qcap, 2 * (size_t)cap);
+    size_t scap = std::max<size_t>(sc->dl_scap, (size_t)std::max(A.slots, 1u) * cap);
+    uint32_t* counts = sc->d_counts;  // [0..1] branch queues, [2] shadow rays, [3] qover, [4..5] rays, [6] sover
+    uint64_t* rays_dev = reinterpret_cast<uint64_t*>(counts + 4);
+    float* pw = sc->d_work;
+    const PathState ps{pw, pw + cap, pw + 2 * (size_t)cap, pw + 3 * (size_t)cap};
+    BatchArgs NB;
+    std::memset(&NB, 0, sizeof(NB));
+    const dim3 blk(256);
+    const uint32_t grid_max = (uint32_t)std::max(1, sc->ctx->n_cu) * 16u;
+    uint64_t rays = 0;
+    uint32_t chunks = 0;
+    for (uint64_t pix0 = 0; pix0 < n_pix; pix0 += pix_per_chunk) {
+        const uint32_t npix = (uint32_t)std::min<uint64_t>(pix_per_chunk, n_pix - pix0);
+        const uint32_t n = (uint32_t)(npix * ppp);
+        const uint64_t path0 = pix0 * ppp;
+        for (;;) {  // repeated with larger queues when a branch or shadow queue overflowed
+            if (int rc = ensure_dl_work(sc, qcap, scap, s)) return rc;
+            qcap = sc->dl_qcap;
+            scap = sc->dl_scap;
+            A.qcap = (uint32_t)qcap;
+            A.scap = (uint32_t)scap;
+            A.qover = counts + 3;
+            A.sover = counts + 6;
+            DLQueue q[2];
+            float* b = sc->d_dlwork;
+            for (int k = 0; k < 2; ++k) {
+                float* base = b + (size_t)k * 11 * qcap;
+                for (int r = 0; r < 8; ++r) q[k].r[r] = base + (size_t)r * qcap;
+                q[k].pid = reinterpret_cast<uint32_t*>(base + 8 * qcap);
+                q[k].w = base + 9 * qcap;
+                q[k].code = reinterpret_cast<uint32_t*>(base + 10 * qcap);
+            }
+            float* hb = b + 22 * qcap;
+            const Hits H{reinterpret_cast<uint32_t*>(hb), hb + qcap, hb + 2 * qcap, hb + 3 * qcap};
+            ShadowQueue sq;
+            {
+                float* base = hb + 4 * qcap;
+                for (int r = 0; r < 8; ++r) sq.r[r] = base + (size_t)r * scap;
+                sq.pid = reinterpret_cast<uint32_t*>(base + 8 * scap);
+                sq.c = base + 9 * scap;
+            }
+            Queue cq;  // the camera kernel writes the first branch queue's rays
+            for (int r = 0; r < 8; ++r) cq.r[r] = q[0].r[r];
+            cq.pid = q[0].pid;
+            cq.root = nullptr;
+            hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 8u);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_camera<false>), dim3((n + 255) / 256), blk, 0, s, S, C, P, path0, n,
+                               ps, cq, counts, NB);
+            hipLaunchKernelGGL(k_dl_init, dim3((n + 255) / 256), blk, 0, s, counts, ps, q[0]);
+            for (uint32_t gen = 0; gen <= A.max_gen; ++gen) {
+                const int cur = gen & 1, nxt = cur ^ 1;
+                const uint32_t glimit = gen == 0 ? n : A.qcap;  // entries this generation can hold
+                const uint32_t grid = std::min<uint32_t>((glimit + 255) / 256, grid_max);
+                hipLaunchKernelGGL(k_dl_next, dim3(1), dim3(64), 0, s, counts, (uint32_t)cur, (uint32_t)nxt, A.qcap,
+                                   rays_dev);
+                lr_rays_soa rs{q[cur].r[0], q[cur].r[1], q[cur].r[2], q[cur].r[3],
+                               q[cur].r[4], q[cur].r[5], q[cur].r[6], q[cur].r[7]};
+                lr_hits_soa hs{H.prim, H.t, H.u, H.v};
+                if (sc->bvh->info.n_nodes == 0) {
+                    hipLaunchKernelGGL(k_dl_fill_miss, dim3(grid), blk, 0, s, counts + cur, A.qcap, H);
+                } else if (int rc = lr::trace_closest_queue(sc->bvh, &rs, glimit, counts + cur, &hs, s)) {
+                    return rc;
+                }
+                hipLaunchKernelGGL(k_dl_shade, dim3(grid), blk, 0, s, S, P, gen, H, sc->d_prim_geom, q[cur],
+                                   counts + cur, ps, q[nxt], counts + nxt, sq, counts + 2, path0, A);
+                const uint8_t* occ = nullptr;
+                if (A.slots && sc->bvh->info.n_nodes) {
+                    lr_rays_soa ss{sq.r[0], sq.r[1], sq.r[2], sq.r[3], sq.r[4], sq.r[5], sq.r[6], sq.r[7]};
+                    if (int rc = lr::trace_any_queue(sc->bvh, &ss, A.scap, counts + 2, sc->d_dlocc, s)) return rc;
+                    occ = sc->d_dlocc;
+                }
+                if (A.slots)
+                    hipLaunchKernelGGL(k_dl_resolve, dim3(grid_max), blk, 0, s, counts + 2, A.scap, sq, occ, ps,
+                                       rays_dev);
+            }
+            LR_HIP_CHECK(hipGetLastError());
+            uint32_t host[8];
+            LR_HIP_CHECK(hipMemcpyAsync(host, counts, sizeof(host), hipMemcpyDeviceToHost, s));
+            LR_HIP_CHECK(hipStreamSynchronize(s));
+            if (host[3] == 0 && host[6] == 0) {
+                uint64_t r64;
+                std::memcpy(&r64, host + 4, sizeof(r64));
+                rays += r64;
+                break;
+            }
+            // the chunk's radiance is discarded (the camera kernel resets it) and the chunk repeated
+            if (host[3]) qcap *= 2;
+            if (host[6]) scap *= 2;
+            if (qcap > (1ull << 30) || scap > (1ull << 30))
+                return lr::fail(LR_E_INVALID, "lr_render_direct: ray tree too large; use a smaller chunk_paths");
+        }
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_splat<false>), dim3((npix + 255) / 256), blk, 0, s, S, P, pix0, npix, ps,
+                           film_xyz, film_w, NB);
+        LR_HIP_CHECK(hipGetLastError());
+        ++chunks;
+    }
+    if (stats) {
+        LR_HIP_CHECK(hipStreamSynchronize(s));
+        stats->paths = n_pix * ppp;
+        stats->rays = rays;
+        stats->chunks = chunks;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return LR_OK;
 }
 
 }  // extern "C"

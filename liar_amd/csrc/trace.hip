@@ -123,7 +123,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
                                                  uint2* __restrict__ spill, int cap,
                                                  uint32_t* __restrict__ counter, uint32_t refill_min,
                                                  const uint32_t* __restrict__ n_dev) {
-    if (n_dev) n = *n_dev;  // queue length produced on the device
+    if (n_dev) n = min(*n_dev, n);  // queue length produced on the device (n: its capacity)
     __shared__ uint32_t s_e[SHORT * BLOCK];
     __shared__ float s_t[SHORT * BLOCK];
     Stack<BLOCK, SHORT> st;
@@ -436,6 +436,16 @@ int trace_closest_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, c
     Launch L;
     if (int rc = prepare(bvh, n_max, false, s, L)) return rc;
     return launch(bvh, L, ra, n_max, ha, nullptr, s, n_dev);
+}
+
+int trace_any_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, const uint32_t* n_dev, uint8_t* occluded,
+                    hipStream_t s) {
+    if (n_max == 0 || bvh->info.n_nodes == 0) return LR_OK;
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar, nullptr};
+    const HitsArg none{nullptr, nullptr, nullptr, nullptr};
+    Launch L;
+    if (int rc = prepare(bvh, n_max, true, s, L)) return rc;
+    return launch(bvh, L, ra, n_max, none, occluded, s, n_dev);
 }
 
 int trace_reserve(lr_qbvh* bvh, uint32_t n_max, hipStream_t s) {

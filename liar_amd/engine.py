@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native as N
-from . import cameras, scenery, shaders, spectra, textures
+from . import cameras, scenery, shaders, spectra, textures, tracers
 from .observer import ensure_standard_observer
 
 
@@ -132,14 +132,17 @@ def compile_scene(scene) -> CompiledScene:
             lights.append({"type": N.LIGHT_PARALLELOGRAM,
                            "radiance": _spectrum_of(obj.radiance, table, spec_index),
                            "double_sided": bool(obj.isDoubleSided),
-                           "origin": surf.support, "dx": surf.sizeU, "dy": surf.sizeV})
+                           "origin": surf.support, "dx": surf.sizeU, "dy": surf.sizeV,
+                           "samples": int(obj.numberOfEmissionSamples)})
             s = obj.shader if obj.shader is not None else surf.shader
             add_mesh(*surf._mesh(), material(s, li))
         elif isinstance(obj, scenery.LightSky):
-            lights.append({"type": N.LIGHT_SKY, "radiance": _spectrum_of(obj.radiance, table, spec_index)})
+            lights.append({"type": N.LIGHT_SKY, "radiance": _spectrum_of(obj.radiance, table, spec_index),
+                           "samples": int(obj.numberOfEmissionSamples)})
         elif isinstance(obj, scenery.LightSun):
             lights.append({"type": N.LIGHT_SUN, "radiance": _spectrum_of(obj.radiance, table, spec_index),
-                           "direction": obj.direction, "angle": obj.angle})
+                           "direction": obj.direction, "angle": obj.angle,
+                           "samples": int(obj.numberOfEmissionSamples)})
         else:
             raise NotImplementedError(f"scene object {type(obj).__name__} is not supported")
 
@@ -187,6 +190,7 @@ class GpuScene:
                 if name in L:
                     getattr(ls[i], name)[:] = [float(x) for x in L[name]]
             ls[i].angle = float(L.get("angle", 0.0))
+            ls[i].samples = int(L.get("samples", 0))
         pm = np.ascontiguousarray(cs.prim_material, np.uint32)
         h = C.c_void_p()
         N.check(N.lib().lr_scene_create(ctx.handle, self.qbvh.handle, N.ptr(pm), mats, len(cs.materials), specs,
@@ -213,6 +217,21 @@ class GpuScene:
         st = N.RenderStats()
         N.check(N.lib().lr_render(self._h, C.byref(camera), C.byref(params), C.c_void_p(film_xyz.data_ptr()),
                                   C.c_void_p(film_w.data_ptr()), C.byref(st), C.c_void_p(s)), "lr_render")
+        return {"paths": st.paths, "rays": st.rays, "chunks": st.chunks, "seconds": st.seconds}
+
+
+    def render_direct(self, camera: N.Camera, params: N.RenderParams, film_xyz, film_w, max_generation=8,
+                      secondary_light_samples=1, trace_glossy=True, stream=None) -> dict:
+        """lr_render_direct: LiAR's DirectLighting integrand (``params``'
+        depth and roulette fields are ignored)."""
+        import torch
+
+        s = torch.cuda.current_stream(film_xyz.device).cuda_stream if stream is None else stream
+        dp = N.DirectParams(int(max_generation), int(secondary_light_samples), int(bool(trace_glossy)))
+        st = N.RenderStats()
+        N.check(N.lib().lr_render_direct(self._h, C.byref(camera), C.byref(params), C.byref(dp),
+                                         C.c_void_p(film_xyz.data_ptr()), C.c_void_p(film_w.data_ptr()),
+                                         C.byref(st), C.c_void_p(s)), "lr_render_direct")
         return {"paths": st.paths, "rays": st.rays, "chunks": st.chunks, "seconds": st.seconds}
 
 
@@ -320,16 +339,22 @@ class RenderEngine:
         if tuple(self.target.resolution) != (w, h):
             raise RuntimeError("sampler and target resolutions differ")
         tr = self.tracer
-        rr_start = tr.maxRayGeneration if tr.russianRouletteStart is None else tr.russianRouletteStart
+        rr_start = getattr(tr, "russianRouletteStart", None)
+        rr_start = tr.maxRayGeneration if rr_start is None else rr_start
         lr = self.wavelengthRange or (float(standard_observer()[0][0]), float(standard_observer()[0][-1]))
         params = render_params(w, h, self.sampler.samplesPerPixel, self.wavelengthsPerSample,
-                               tr.maxDepth or 0, rr_start, tr.survivalProbability, lr, self.seed)
+                               getattr(tr, "maxDepth", None) or 0, rr_start, getattr(tr, "survivalProbability", 0.5),
+                               lr, self.seed)
         ctx = Context(self.device)
         gs = GpuScene(ctx, compile_scene(self.scene))
         dev = torch.device("cuda", self.device)
         film = torch.zeros((3, h, w), dtype=torch.float32, device=dev)
         wgt = torch.zeros((h, w), dtype=torch.float32, device=dev)
-        self.lastStats = gs.render(camera_struct(self.camera), params, film, wgt)
+        if isinstance(tr, tracers.DirectLighting):
+            self.lastStats = gs.render_direct(camera_struct(self.camera), params, film, wgt, tr.maxRayGeneration,
+                                              tr.numSecondaryLightSamples, tr.traceGlossy)
+        else:
+            self.lastStats = gs.render(camera_struct(self.camera), params, film, wgt)
         torch.cuda.synchronize(dev)
         self.target.accumulate(film.cpu().numpy().astype(np.float64), wgt.cpu().numpy().astype(np.float64))
         if hasattr(self.target, "save"):

@@ -131,7 +131,7 @@ class LightArea(SceneObject):
         self.surface = surface
         self.radiance = as_texture(1.0)
         self.isDoubleSided = False
-        self.numberOfEmissionSamples = 1
+        self.numberOfEmissionSamples = 9  # light_area.cpp:50 (DirectLighting)
 
 
 class LightSky(SceneObject):
@@ -140,6 +140,7 @@ class LightSky(SceneObject):
     def __init__(self, radiance=None):
         super().__init__()
         self.radiance = as_texture(1.0 if radiance is None else radiance)
+        self.numberOfEmissionSamples = 1  # light_sky.cpp:395 (DirectLighting)
 
 
 class LightSun(SceneObject):
@@ -152,3 +153,4 @@ class LightSun(SceneObject):
         self.direction = tuple(float(x) for x in direction)
         self.angle = float(angle)
         self.radiance = as_texture(1.0 if radiance is None else radiance)
+        self.numberOfEmissionSamples = 1

@@ -84,6 +84,32 @@ def furnace_sphere(shader, sky=1.0):
     return scene, cam
 
 
+def lit_floor(shader, sky=None, area=None, cam_height=1.0, fov_deg=4.0, half=50.0):
+    """A square floor (z = 0, side 2 half) seen straight down from
+    (0, 0, cam_height), lit by a constant sky (``sky`` = radiance) and/or a
+    parallelogram light ``area`` = dict(origin, dx, dy, radiance, samples),
+    whose front face (cross(dx, dy)) should face the floor.  The camera's
+    footprint is the square |x|, |y| <= cam_height tan(fov / 2)."""
+    parts = [_quad((-half, -half, 0.0), (2 * half, 0.0, 0.0), (0.0, 2 * half, 0.0), shader)]
+    if sky is not None:
+        parts.append(scenery.LightSky(textures.Constant(sky)))
+    if area is not None:
+        surf = scenery.Parallelogram(area["origin"], area["dx"], area["dy"])
+        surf.shader = shaders.Unshaded(textures.Constant(area["radiance"]))
+        light = scenery.LightArea(surf)
+        light.radiance = textures.Constant(area["radiance"])
+        if "samples" in area:
+            light.numberOfEmissionSamples = int(area["samples"])
+        parts.append(light)
+    cam = cameras.PerspectiveCamera()
+    cam.position = (0.0, 0.0, float(cam_height))
+    cam.lookAt((0.0, 0.0, 0.0))
+    cam.sky = (0.0, 1.0, 0.0)
+    cam.fovAngle = math.radians(fov_deg)
+    cam.aspectRatio = 1.0
+    return scenery.List(parts), cam
+
+
 # Gold (n, k): Hagemann, Gudat & Kunz 1974 (liar_amd/data/Au.spd).
 def gold_nk():
     rows = [l.split() for l in (__import__("pathlib").Path(__file__).resolve().parent / "data" / "Au.spd")

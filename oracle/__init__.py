@@ -182,7 +182,7 @@ class _Mat(C.Structure):
 class _Light(C.Structure):
     _fields_ = [("type", C.c_uint32), ("radiance", C.c_uint32), ("double_sided", C.c_uint32),
                 ("origin", C.c_float * 3), ("dx", C.c_float * 3), ("dy", C.c_float * 3),
-                ("direction", C.c_float * 3), ("angle", C.c_float)]
+                ("direction", C.c_float * 3), ("angle", C.c_float), ("samples", C.c_uint32)]
 
 
 class _Cam(C.Structure):
@@ -218,14 +218,33 @@ def _spec_struct(sp, keep):
 
 def render(compiled, camera, width, height, spp, wavelengths=1, max_depth=0, rr_depth=8, rr_survival=0.5,
            lambda_range=(380.0, 780.0), seed=1, observer=None, threads=None, max_sample_luminance=0.0):
-    """FP64 CPU render of a compiled scene -> (xyz sums (3, h, w), weight (h, w))."""
+    """FP64 CPU render (AdjointPhotonTracer) of a compiled scene -> (xyz sums
+    (3, h, w), weight (h, w))."""
+    return _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_depth, rr_survival,
+                   lambda_range, seed, observer, threads, max_sample_luminance, None)
+
+
+def render_direct(compiled, camera, width, height, spp, wavelengths=1, max_generation=8,
+                  secondary_light_samples=1, trace_glossy=True, lambda_range=(380.0, 780.0), seed=1,
+                  observer=None, threads=None, max_sample_luminance=0.0):
+    """FP64 CPU render (DirectLighting) of a compiled scene -> (xyz sums
+    (3, h, w), weight (h, w))."""
+    cfg = np.array([max_generation, secondary_light_samples, int(bool(trace_glossy))], np.uint32)
+    return _render(compiled, camera, width, height, spp, wavelengths, 0, 8, 0.5, lambda_range, seed, observer,
+                   threads, max_sample_luminance, cfg)
+
+
+def _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_depth, rr_survival, lambda_range, seed,
+            observer, threads, max_sample_luminance, dl_cfg):
     L = lib()
     if not hasattr(L, "_render_sig"):
-        L.or_render.argtypes = [C.c_void_p] * 2 + [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
-                                                  C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
-                                                  C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
-                                                  C.c_void_p, C.c_int]
+        head = [C.c_void_p] * 2 + [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                   C.c_void_p, C.c_void_p]
+        L.or_render.argtypes = head + [C.c_void_p, C.c_void_p, C.c_int]
         L.or_render.restype = C.c_int
+        L.or_render_direct.argtypes = head + [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.or_render_direct.restype = C.c_int
         L._render_sig = True
     keep = []
     cs = compiled
@@ -246,6 +265,7 @@ def render(compiled, camera, width, height, spp, wavelengths=1, max_depth=0, rr_
             if name in Ld:
                 getattr(lights[i], name)[:] = [float(x) for x in Ld[name]]
         lights[i].angle = float(Ld.get("angle", 0.0))
+        lights[i].samples = int(Ld.get("samples", 0))
     cam = _Cam()
     cam.position[:] = [float(x) for x in camera.position]
     cam.direction[:] = [float(x) for x in camera.direction]
@@ -260,14 +280,19 @@ def render(compiled, camera, width, height, spp, wavelengths=1, max_depth=0, rr_
     oxyz = np.ascontiguousarray(observer[1], np.float32)
     xyz = np.zeros((3, height, width), np.float64)
     wgt = np.zeros((height, width), np.float64)
-    rc = L.or_render(V.ctypes.data, T.ctypes.data, len(T), S.ctypes.data if len(S) else None, len(S),
-                     PM.ctypes.data, mats, specs, len(cs.spectra), lights, len(cs.lights), ow.ctypes.data,
-                     oxyz.ctypes.data, len(ow), C.byref(cam), C.byref(P), xyz.ctypes.data, wgt.ctypes.data,
-                     threads or default_threads())
+    args = [V.ctypes.data, T.ctypes.data, len(T), S.ctypes.data if len(S) else None, len(S), PM.ctypes.data, mats,
+            specs, len(cs.spectra), lights, len(cs.lights), ow.ctypes.data, oxyz.ctypes.data, len(ow),
+            C.byref(cam), C.byref(P)]
+    tail = [xyz.ctypes.data, wgt.ctypes.data, threads or default_threads()]
+    if dl_cfg is None:
+        rc = L.or_render(*args, *tail)
+    else:
+        rc = L.or_render_direct(*args, dl_cfg.ctypes.data, *tail)
     assert rc == 0, "oracle render failed"
     return xyz, wgt
 
 
 def set_planted_mistake(k: int) -> None:
-    """Test hook of the oracle renderer: 1 drops the MIS weights."""
+    """Test hook of the oracle renderer: 1 drops the path tracer's MIS
+    weights, 2 those of the DirectLighting light estimate."""
     lib().or_set_planted_mistake(int(k))

@@ -15,6 +15,11 @@
  * Intersections are brute force over all primitives in FP64 with LiAR's
  * per-object interval rules.
  *
+ * DirectLighting (src/liar/tracers/direct_lighting.cpp:119-377 with
+ * RayTracer::estimateLightContribution, src/liar/kernel/ray_tracer.cpp:188-268)
+ * is restated as a recursion over the ray tree with brute-force FP64 shadow
+ * rays (or_render_direct).
+ *
  * The random numbers are the same Philox4x32-10 streams, keyed the same way,
  * as the GPU renderer; sampling warps are the published maps named in
  * liar_amd/csrc/lr_render.h (Lass's own maps are parity unpinned).
@@ -54,7 +59,11 @@ static double u01(uint32_t x) { return (double)(x >> 8) * (1.0 / 16777216.0); }
 /* ---- scene description (layouts of include/liar_amd.h) ---------------- */
 typedef struct { uint32_t kind, count; const float *w, *v; float value, lo, hi, temperature, scale; } or_spectrum;
 typedef struct { uint32_t type, refl, trans, eta, kappa, emission, light; float roughness; } or_material;
-typedef struct { uint32_t type, radiance, double_sided; float origin[3], dx[3], dy[3], direction[3], angle; } or_light;
+typedef struct {
+    uint32_t type, radiance, double_sided;
+    float origin[3], dx[3], dy[3], direction[3], angle;
+    uint32_t samples; /* DirectLighting emission samples (0: 9 parallelogram, 1 otherwise) */
+} or_light;
 typedef struct { float position[3], direction[3], sky[3], fov, aspect, near_limit; } or_camera;
 typedef struct {
     uint32_t width, height, spp, wavelengths, max_depth, rr_depth;
@@ -84,6 +93,7 @@ typedef struct {
     /* derived */
     double* light_sel; double* light_cdf; double* light_area; V* light_n; V* light_du; V* light_dv;
     double* light_cos; double* light_invsolid;
+    uint32_t* light_ns; /* DirectLighting samples per light */
     double* spec_reflscale;
     /* observer restricted to the domain */
     int on; double *ow, *ocdf, *oxyz;
@@ -344,7 +354,10 @@ static int bsdf_eval(const Bsdf* B, V wi, V wo, double* val, double* pdf) {
     }
     return 0;
 }
-static int bsdf_sample(const Bsdf* B, V wi, double u1, double u2, double uc, V* wo, double* val, double* pdf) {
+/* allow: bit 0 reflection, bit 1 transmission (the caps the caller permits;
+ * only the dielectric has both and chooses among the permitted ones). */
+static int bsdf_sample(const Bsdf* B, V wi, double u1, double u2, double uc, V* wo, double* val, double* pdf,
+                       unsigned allow) {
     switch (B->type) {
     case 1: {  /* concentric disk + Malley */
         double a = 2 * u1 - 1, b = 2 * u2 - 1, r, phi;
@@ -373,7 +386,7 @@ static int bsdf_sample(const Bsdf* B, V wi, double u1, double u2, double uc, V* 
         double st2 = ior * ior * (1 - ci * ci), ct = sqrt(fmax(1 - st2, 0));
         double ro = (ior * ci - ct) / (ior * ci + ct), rp = (ci - ior * ct) / (ci + ior * ct);
         double fr = (ro * ro + rp * rp) / 2;
-        double pr = fabs(B->refl) * fr, pt = fabs(B->trans) * (1 - fr);
+        double pr = (allow & 1u) ? fabs(B->refl) * fr : 0, pt = (allow & 2u) ? fabs(B->trans) * (1 - fr) : 0;
         if (pr + pt == 0) return 0;
         double prob = pr / (pr + pt);
         if (uc < prob) { *wo = v(-wi.x, -wi.y, wi.z); *val = B->refl * fr / ci; *pdf = prob; }
@@ -400,7 +413,8 @@ static int bsdf_sample(const Bsdf* B, V wi, double u1, double u2, double uc, V* 
 }
 
 /* ---- one path -------------------------------------------------------------- */
-typedef struct { const Scene* S; const or_params* P; uint32_t k0, k1; } Ctx;
+typedef struct { uint32_t max_gen, secondary, glossy; } DLCfg;
+typedef struct { const Scene* S; const or_params* P; uint32_t k0, k1; const DLCfg* dl; } Ctx;
 
 static void rng(const Ctx* c, uint64_t path, uint32_t gen, uint32_t purpose, uint32_t out[4]) {
     uint32_t ctr[4] = {(uint32_t)path, (uint32_t)(path >> 32), gen, purpose};
@@ -468,7 +482,7 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
         double val = 0, pdf = 0;
         int ok = 0;
         if (us < strategy) {
-            ok = bsdf_sample(&B, wi, u1, u2, uc, &wo, &val, &pdf);
+            ok = bsdf_sample(&B, wi, u1, u2, uc, &wo, &val, &pdf, 3u);
             if (ok) {
                 pdf *= strategy;
                 double pl = 0;
@@ -515,6 +529,178 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
     }
 }
 
+
+/* ---- DirectLighting (src/liar/tracers/direct_lighting.cpp:119-377) ---------- */
+
+/* Scene::isIntersecting over the List{TriangleMesh, Spheres}: any triangle
+ * with tn < t < tf (triangle_mesh.cpp:321-330), any sphere whose nearest
+ * root beyond tn is in range (sphere.cpp:104-127). */
+static int occluded(const Scene* S, V o, V d, double tn, double tf) {
+    for (uint32_t k = 0; k < S->nt; ++k) {
+        double t;
+        if (tri_hit(S, k, o, d, &t) && t > tn && t < tf) return 1;
+    }
+    for (uint32_t j = 0; j < S->ns; ++j) {
+        double t;
+        if (sph_hit(S, j, o, d, tn, &t) && in_range(t, tn, tf)) return 1;
+    }
+    return 0;
+}
+
+static double sq_heuristic(double a, double b) { return a * a / (a * a + b * b); }  /* ray_tracer.cpp:180-183 */
+
+static int is_specular(uint32_t type) { return type == 2 || type == 3 || type == 4; }
+
+typedef struct {
+    Bsdf B;
+    V wi, tt, bb, nf, start;
+    double lam;
+    uint64_t p;
+    uint32_t key;  /* generation | branch << 8: the node's random-number key */
+    int hq;
+} DLHit;
+
+/* Candidate j of a hit's light estimate (RayTracer::estimateLightContribution,
+ * src/liar/kernel/ray_tracer.cpp:188-268, split into one light or BSDF
+ * sample per candidate): high quality takes light k's 2 n_k candidates after
+ * those of lights 0..k-1, even = light sample, odd = BSDF sample; otherwise
+ * candidates 2s, 2s+1 belong to secondary sample s whose light is picked by
+ * power and whose estimate is divided by (samples x pick pdf)
+ * (direct_lighting.cpp:259-310).  Shadow rays to a parallelogram end 1e-4
+ * (relative) short of its plane. */
+static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
+    const Scene* S = c->S;
+    uint32_t k = 0, local = 0;
+    double nl = 1, scale = 1;
+    if (H->hq) {
+        uint32_t base = 0;
+        int found = 0;
+        for (uint32_t m = 0; m < S->n_lights && !found; ++m) {
+            uint32_t n2 = 2 * S->light_ns[m];
+            if (j < base + n2) { k = m; local = j - base; found = 1; }
+            base += n2;
+        }
+        if (!found) return 0;
+        nl = S->light_ns[k];
+    } else {
+        if (j >= 2 * c->dl->secondary || S->n_lights == 0) return 0;
+        uint32_t rs[4];
+        rng(c, H->p, H->key, 10u + (j >> 1), rs);
+        k = light_pick(S, u01(rs[0]));
+        double psel = S->light_sel[k];
+        if (!(psel > 0)) return 0;
+        local = j & 1u;
+        scale = c->dl->secondary * psel;
+    }
+    const or_light* L = &S->lights[k];
+    double Le = spec_eval(S, L->radiance, H->lam, 0);
+    if (Le == 0) return 0;
+    uint32_t r[4];
+    rng(c, H->p, H->key, 100u + j, r);
+    double f = 0, pb = 0, pl = 0, contrib;
+    V dir;
+    if ((local & 1u) == 0) {
+        dir = light_sample(S, k, H->start, u01(r[0]), u01(r[1]), &pl);
+        if (!(pl > 0)) return 0;
+        V wo = v(dot(dir, H->tt), dot(dir, H->bb), dot(dir, H->nf));
+        if (!bsdf_eval(&H->B, H->wi, wo, &f, &pb) || f == 0) return 0;
+        double wt = g_plant == 2 ? 1 : sq_heuristic(nl * pl, nl * pb);
+        contrib = f * Le * wt * fabs(wo.z) / (nl * pl * scale);
+    } else {
+        V wo;
+        if (!bsdf_sample(&H->B, H->wi, u01(r[0]), u01(r[1]), u01(r[2]), &wo, &f, &pb, 3u) || !(pb > 0) || f == 0)
+            return 0;
+        dir = nrm(add(add(mul(wo.x, H->tt), mul(wo.y, H->bb)), mul(wo.z, H->nf)));
+        pl = light_pdf(S, k, H->start, dir);
+        if (!(pl > 0)) return 0;
+        double wt = (is_specular(H->B.type) || g_plant == 2) ? 1 : sq_heuristic(nl * pb, nl * pl);
+        contrib = f * Le * wt * fabs(wo.z) / (nl * pb * scale);
+    }
+    double far = INFINITY;
+    if (L->type == 0) {
+        V n = S->light_n[k];
+        double t = dot(sub(f3(L->origin), H->start), n) / dot(dir, n);
+        if (!(t > 0)) return 0;
+        far = t * (1 - 1e-4);
+    }
+    if (!(contrib != 0) || !isfinite(contrib)) return 0;
+    return occluded(S, H->start, dir, TOL, far) ? 0 : contrib;
+}
+
+/* DirectLighting::castRay / doShadeSurface, recursively: radiance along
+ * (o, d) for the tree node (gen, br) of path p. */
+static double dl_cast(const Ctx* c, uint64_t p, V o, V d, double tn, double lam, uint32_t gen, uint32_t br,
+                      int hq) {
+    const Scene* S = c->S;
+    const DLCfg* A = c->dl;
+    double t;
+    uint32_t prim = intersect(S, o, d, tn, &t);
+    if (prim == RMISS) return 0;  /* alpha = 0: escaped rays add nothing */
+    V x = add(o, mul(t, d));
+    V ng;
+    int leaving;
+    if (prim < S->nt) {
+        const float* a = S->verts + 3 * (size_t)S->tris[3 * prim];
+        const float* b = S->verts + 3 * (size_t)S->tris[3 * prim + 1];
+        const float* cc = S->verts + 3 * (size_t)S->tris[3 * prim + 2];
+        ng = nrm(cross(sub(f3(b), f3(a)), sub(f3(cc), f3(a))));
+        leaving = dot(d, ng) > 0;
+    } else {
+        const float* s = S->spheres + 4 * (size_t)(prim - S->nt);
+        ng = mul(1.0 / s[3], sub(x, f3(s)));
+        V s0 = sub(add(o, mul(tn, d)), f3(s));
+        leaving = dot(s0, s0) < (double)s[3] * s[3];
+    }
+    V nf = dot(ng, d) < 0 ? ng : mul(-1, ng);
+    const or_material* m = &S->mats[S->prim_mat[prim]];
+    double L = 0;
+    if ((gen == 0 || m->light == RNONE) && m->emission != RNONE) L += spec_eval(S, m->emission, lam, 0);
+    if (m->type == 0) return L;
+    DLHit H;
+    H.B.type = m->type;
+    H.B.refl = spec_eval(S, m->refl, lam, 1);
+    H.B.trans = spec_eval(S, m->trans, lam, 1);
+    H.B.eta = fmax(spec_eval(S, m->eta, lam, 0), 1e-9);
+    H.B.kappa = fmax(spec_eval(S, m->kappa, lam, 0), 1e-9);
+    H.B.ior = leaving ? H.B.eta / H.B.kappa : H.B.kappa / H.B.eta;
+    double rg = fmax(m->roughness, 1e-3);
+    H.B.alpha = rg * rg;
+    H.nf = nf;
+    onb(nf, &H.tt, &H.bb);
+    V wiw = mul(-1, d);
+    H.wi = v(dot(wiw, H.tt), dot(wiw, H.bb), fmax(dot(wiw, nf), 1.1920929e-7));
+    H.start = add(x, mul(10 * TOL, nf));
+    H.lam = lam;
+    H.p = p;
+    H.key = gen | (br << 8);
+    H.hq = hq;
+    uint32_t slots = 0;
+    if (S->n_lights) {
+        for (uint32_t k = 0; k < S->n_lights; ++k) slots += 2 * S->light_ns[k];
+        if (2 * A->secondary > slots) slots = 2 * A->secondary;
+    }
+    for (uint32_t j = 0; j < slots; ++j) L += dl_candidate(c, &H, j);
+    /* specular / glossy continuations (direct_lighting.cpp:314-377) */
+    int glossy = m->type == 5 || m->type == 6;
+    if (gen < A->max_gen && (is_specular(m->type) || (glossy && A->glossy))) {
+        for (uint32_t side = 0; side < 2; ++side) {
+            if (side == 1 && m->type != 3) break;
+            uint32_t r[4];
+            rng(c, p, H.key, 5u + side, r);
+            V wo;
+            double f = 0, pdf = 0;
+            if (!bsdf_sample(&H.B, H.wi, u01(r[0]), u01(r[1]), u01(r[2]), &wo, &f, &pdf, side ? 2u : 1u)) continue;
+            if (!(pdf > 0) || f == 0 || (side ? !(wo.z < 0) : !(wo.z > 0))) continue;
+            double w = f * fabs(wo.z) / pdf;
+            if (!(w > 0) || !isfinite(w)) continue;
+            V dir = nrm(add(add(mul(wo.x, H.tt), mul(wo.y, H.bb)), mul(wo.z, nf)));
+            V org = add(x, mul((side ? -10.0 : 10.0) * TOL, nf));
+            L += w * dl_cast(c, p, org, dir, TOL, lam, gen + 1, (br << 1) | side, hq && is_specular(m->type));
+        }
+    }
+    return L;
+}
+
 typedef struct {
     const Ctx* c;
     V pos, dir, right, down, dirn;
@@ -556,11 +742,12 @@ static void* run_rows(void* arg) {
                     double lam = obs_sample(c->S, uw, &lpdf);
                     if (!(lpdf > 0) || max_gen == 0) continue;
                     double thr = 1;
-                    if (P->rr_depth == 0) {
+                    if (!c->dl && P->rr_depth == 0) {
                         if (u01(rp[1]) < P->rr_survival) thr /= P->rr_survival;
                         else continue;
                     }
-                    double L = trace_path(c, p, J->pos, d, tn, lam, max_gen, thr);
+                    double L = c->dl ? dl_cast(c, p, J->pos, d, tn, lam, 0, 0, 1)
+                                     : trace_path(c, p, J->pos, d, tn, lam, max_gen, thr);
                     if (L == 0) continue;
                     double cmf[3];
                     obs_xyz(c->S, lam, cmf);
@@ -590,12 +777,11 @@ static double lum(const Scene* S, uint32_t i, const float* w, const float* xyz, 
     return den > 0 ? num / den : 0;
 }
 
-/* Renders into xyz (3*W*H doubles, += sums) and weight (W*H, += counts). */
-int or_render(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres, uint32_t ns,
-              const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
-              uint32_t n_spectra, const or_light* lights, uint32_t n_lights, const float* obs_w,
-              const float* obs_xyz, uint32_t obs_n, const or_camera* cam, const or_params* P, double* xyz,
-              double* weight, int nthreads) {
+static int render_impl(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres, uint32_t ns,
+                       const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
+                       uint32_t n_spectra, const or_light* lights, uint32_t n_lights, const float* obs_w,
+                       const float* obs_xyz, uint32_t obs_n, const or_camera* cam, const or_params* P, double* xyz,
+                       double* weight, int nthreads, const DLCfg* dl) {
     Scene S;
     memset(&S, 0, sizeof S);
     S.verts = verts; S.tris = tris; S.nt = nt; S.spheres = spheres; S.ns = ns; S.prim_mat = prim_mat;
@@ -629,6 +815,9 @@ int or_render(const float* verts, const uint32_t* tris, uint32_t nt, const float
     S.light_area = calloc(nl, sizeof(double)); S.light_n = calloc(nl, sizeof(V));
     S.light_du = calloc(nl, sizeof(V)); S.light_dv = calloc(nl, sizeof(V));
     S.light_cos = calloc(nl, sizeof(double)); S.light_invsolid = calloc(nl, sizeof(double));
+    S.light_ns = calloc(nl, sizeof(uint32_t));
+    for (uint32_t k = 0; k < n_lights; ++k)
+        S.light_ns[k] = lights[k].samples ? lights[k].samples : (lights[k].type == 0 ? 9u : 1u);
     double total = 0;
     double* power = calloc(nl, sizeof(double));
     for (uint32_t k = 0; k < n_lights; ++k) {
@@ -664,7 +853,7 @@ int or_render(const float* verts, const uint32_t* tris, uint32_t nt, const float
     if (n_lights) S.light_cdf[n_lights - 1] = 1;
     free(power);
 
-    Ctx c = {&S, P, (uint32_t)P->seed, (uint32_t)(P->seed >> 32)};
+    Ctx c = {&S, P, (uint32_t)P->seed, (uint32_t)(P->seed >> 32), dl};
     Job proto;
     memset(&proto, 0, sizeof proto);
     proto.c = &c;
@@ -704,6 +893,29 @@ int or_render(const float* verts, const uint32_t* tris, uint32_t nt, const float
     for (int k = 0; k < used; ++k) pthread_join(th[k], NULL);
     free(S.ow); free(S.ocdf); free(S.oxyz); free(S.spec_reflscale);
     free(S.light_sel); free(S.light_cdf); free(S.light_area); free(S.light_n); free(S.light_du);
-    free(S.light_dv); free(S.light_cos); free(S.light_invsolid);
+    free(S.light_dv); free(S.light_cos); free(S.light_invsolid); free(S.light_ns);
     return 0;
+}
+
+/* AdjointPhotonTracer render into xyz (3*W*H doubles, += sums) and weight
+ * (W*H, += counts). */
+int or_render(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres, uint32_t ns,
+              const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
+              uint32_t n_spectra, const or_light* lights, uint32_t n_lights, const float* obs_w,
+              const float* obs_xyz, uint32_t obs_n, const or_camera* cam, const or_params* P, double* xyz,
+              double* weight, int nthreads) {
+    return render_impl(verts, tris, nt, spheres, ns, prim_mat, mats, spectra, n_spectra, lights, n_lights, obs_w,
+                       obs_xyz, obs_n, cam, P, xyz, weight, nthreads, NULL);
+}
+
+/* DirectLighting render (cfg: max generation, secondary light samples,
+ * trace glossy); P's depth and roulette fields are ignored. */
+int or_render_direct(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres, uint32_t ns,
+                     const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
+                     uint32_t n_spectra, const or_light* lights, uint32_t n_lights, const float* obs_w,
+                     const float* obs_xyz, uint32_t obs_n, const or_camera* cam, const or_params* P,
+                     const uint32_t* cfg, double* xyz, double* weight, int nthreads) {
+    DLCfg dl = {cfg[0], cfg[1] ? cfg[1] : 1u, cfg[2]};
+    return render_impl(verts, tris, nt, spheres, ns, prim_mat, mats, spectra, n_spectra, lights, n_lights, obs_w,
+                       obs_xyz, obs_n, cam, P, xyz, weight, nthreads, &dl);
 }
