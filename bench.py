@@ -101,6 +101,57 @@ def bench_render(args, la, oracle, torch, np):
     return out
 
 
+def bench_direct(args, la, oracle, torch, np):
+    """DirectLighting leg: the Cornell-box workload (256x256, 16 spp, 4
+    stratified wavelengths) with LiAR's DirectLighting defaults (generations
+    <= 8, 9 emission samples on the area light, one secondary sample, glossy
+    continuations); CPU baseline = the oracle's DirectLighting on a crop."""
+    from liar_amd import engine as E
+    from liar_amd import workloads as W
+
+    W_, H_, SPP, NL = 256, 256, 16, 4
+    sc, cam = W.cornell_box()
+    cs = la.compile_scene(sc)
+    ctx = la.Context(0)
+    gs = E.GpuScene(ctx, cs)
+    camera = E.camera_struct(cam)
+    film = torch.zeros((3, H_, W_), dtype=torch.float32, device="cuda")
+    wgt = torch.zeros((H_, W_), dtype=torch.float32, device="cuda")
+    stats = []
+    for k in range(args.warmup + args.render_steps):
+        film.zero_()
+        wgt.zero_()
+        torch.cuda.synchronize()
+        st = gs.render_direct(camera, E.render_params(W_, H_, SPP, NL, 0, 8, 0.5, (380.0, 780.0), 1 + k), film, wgt)
+        if k >= args.warmup:
+            stats.append(st)
+    secs = float(np.mean([s["seconds"] for s in stats]))
+    paths = stats[0]["paths"]
+    rays = float(np.mean([s["rays"] for s in stats]))
+    out = {
+        "workload": "Cornell box 36 tris, 256x256, 16 spp x 4 wavelengths, DirectLighting (generations <= 8, "
+                    "9 light samples + 9 BSDF samples per camera hit, shadow rays)",
+        "ms_per_frame": round(secs * 1e3, 3),
+        "camera_samples_per_s": round(paths / NL / secs, 1),
+        "paths_per_s": round(paths / secs, 1),
+        "mrays_per_s": round(rays / secs / 1e6, 2),
+        "rays_per_path": round(rays / paths, 3),
+    }
+    if not args.no_cpu:
+        cw, ch = 128, 128
+        threads = min(oracle.default_threads(), 16)
+        t0 = time.perf_counter()
+        oracle.render_direct(cs, cam, cw, ch, SPP, wavelengths=NL, seed=1, threads=threads)
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cw * ch * SPP * NL / t_cpu, 1), "unit": "paths/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{cw}x{ch} pixels of the same frame geometry (FP64 oracle, brute-force "
+                                         f"shadow rays), {t_cpu:.2f} s"}
+    gs.close()
+    ctx.close()
+    return out
+
+
 def bench_guided(args, la, torch, np):
     """Learned path guiding on the slit-room workload: the terrain workload
     (4,992,800 triangles + 10k spheres, Lambert / GGX gold / dielectric) in a
@@ -397,6 +448,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the default traversal configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg")
     ap.add_argument("--no-terrain", action="store_true", help="skip the terrain path-tracer leg")
+    ap.add_argument("--no-direct", action="store_true", help="skip the DirectLighting leg")
     ap.add_argument("--terrain-frames", type=int, default=2)
     ap.add_argument("--profile-workload", action="store_true",
                     help="run only a short slit-room plain + guided sequence for profilers")
@@ -527,6 +579,7 @@ def main() -> int:
         }
 
     render = None if args.no_render else bench_render(args, la, oracle, torch, np)
+    direct = None if args.no_direct else bench_direct(args, la, oracle, torch, np)
     terrain = None if args.no_terrain else bench_terrain(args, la, torch, np)
     guided = None if args.no_guided else bench_guided(args, la, torch, np)
     batch = None if args.no_batch else bench_batch(args, la, torch, np)
@@ -582,6 +635,7 @@ def main() -> int:
             "autotune_any": tuned_a,
         },
         "render": render,
+        "direct": direct,
         "terrain": terrain,
         "guided": guided,
         "batch": batch,

@@ -348,7 +348,7 @@ typedef struct lr_direct_params {
 } lr_direct_params;
 
 /* params->max_depth, rr_depth, rr_survival and graph are ignored;
- * chunk_paths 0 means 128k camera paths per chunk (each chunk keeps a branch
+ * chunk_paths 0 means 1M camera paths per chunk (each chunk keeps a branch
  * queue of 4x its paths and a shadow queue of as many rays as its hits can
  * make: a scene whose branches overflow returns LR_E_INVALID and asks for a
  * smaller chunk).  dparams NULL: LiAR's defaults (8, 1, true). */

@@ -756,6 +756,7 @@ struct lr_scene {
     uint8_t* d_dlocc = nullptr;
     size_t dl_qcap = 0, dl_scap = 0;
     uint32_t light_samples = 0;  // sum of the lights' DirectLighting samples
+    bool has_specular = false, has_glossy = false;  // materials that continue DirectLighting rays
 };
 
 namespace {
@@ -1064,6 +1065,10 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
     sc->n_triangles = bvh->info.n_triangles;
     sc->n_lights = n_lights;
     for (const DevLight& d : dl) sc->light_samples += d.samples;
+    for (const DevMaterial& m : dmat) {
+        if (m.type == LR_MAT_MIRROR || m.type == LR_MAT_CONDUCTOR || m.type == LR_MAT_DIELECTRIC) sc->has_specular = true;
+        if (m.type == LR_MAT_COOK_TORRANCE || m.type == LR_MAT_COOK_TORRANCE_BECKMANN) sc->has_glossy = true;
+    }
     int rc = lr::bind(ctx);
     if (!rc) rc = upload(pm, &sc->d_prim_mat);
     if (!rc) rc = upload(geom, &sc->d_prim_geom);
@@ -2071,7 +2076,7 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
 
     const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
     const uint64_t n_pix = (uint64_t)R.width * R.height;
-    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 17);
+    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 20);
     const uint64_t pix_per_chunk = std::max<uint64_t>(1, chunk_req / ppp);
     const uint64_t cap64 = pix_per_chunk * ppp;
     if (cap64 >= (1ull << 26)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");
@@ -2127,7 +2132,10 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_camera<false>), dim3((n + 255) / 256), blk, 0, s, S, C, P, path0, n,
                                ps, cq, counts, NB);
             hipLaunchKernelGGL(k_dl_init, dim3((n + 255) / 256), blk, 0, s, counts, ps, q[0]);
-            for (uint32_t gen = 0; gen <= A.max_gen; ++gen) {
+            // no specular (or traced glossy) material: no hit continues, so
+            // generations after the camera's are empty and not launched
+            const uint32_t last_gen = (sc->has_specular || (A.glossy && sc->has_glossy)) ? A.max_gen : 0u;
+            for (uint32_t gen = 0; gen <= last_gen; ++gen) {
                 const int cur = gen & 1, nxt = cur ^ 1;
                 const uint32_t glimit = gen == 0 ? n : A.qcap;  // entries this generation can hold
                 const uint32_t grid = std::min<uint32_t>((glimit + 255) / 256, grid_max);

@@ -142,6 +142,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     uint32_t ray = kNone;
     bool no_more = false;  // wave-uniform
+    // The first fill of every wave is its own slice of [0, grid * BLOCK)
+    // (no atomic: one atomic per wave on one address costs ~10 ns each,
+    // serialised, which dominated short and empty launches); later refills
+    // take from the counter, which numbers the rays after those slices.
+    bool first = true;
+    const uint32_t static_total = gridDim.x * BLOCK;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
     float tnear = 0, tfar = 0, far_lim = 0, near_lim = 0;
     float tri_t = 0, tri_u = 0, tri_v = 0, sph_t = 0;
@@ -158,8 +164,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
                 const uint32_t cnt = (uint32_t)__popcll(need);
                 const int leader = __ffsll((unsigned long long)need) - 1;
                 uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(counter, cnt);
-                base = __shfl(base, leader);
+                if (first) {
+                    base = blockIdx.x * BLOCK + (threadIdx.x & ~63u);  // all 64 lanes are idle
+                } else {
+                    if ((int)lane == leader) base = atomicAdd(counter, cnt);
+                    base = static_total + __shfl(base, leader);
+                }
                 if (ray == kNone) {
                     const uint32_t my = base + __builtin_amdgcn_mbcnt_hi(
                                                    (uint32_t)(need >> 32),
@@ -188,7 +198,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
                         st.sp = 0;
                     }
                 }
-                if ((uint64_t)base + cnt >= n) no_more = true;
+                if (first) {
+                    if (static_total >= n) no_more = true;
+                    first = false;
+                } else if ((uint64_t)base + cnt >= n) {
+                    no_more = true;
+                }
             }
         }
         if (__ballot(ray != kNone) == 0) break;
