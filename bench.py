@@ -152,6 +152,51 @@ def bench_direct(args, la, oracle, torch, np):
     return out
 
 
+def bench_film(args, la, torch, np):
+    """Film output leg: lr_tonemap of a 1920x1080 film (automatic exposure,
+    Reinhard 2002 RGB, sRGB) to float and 8-bit RGBA, timed with HIP events
+    on the launch stream.  Algorithmic bytes per pixel: 16 read (X, Y, Z,
+    weight) + 16 + 4 written; the exposure pass re-reads Y and weight (8)."""
+    import ctypes as C
+
+    from liar_amd import _native as N
+
+    w, h = 1920, 1080
+    n = w * h
+    g = torch.Generator(device="cuda").manual_seed(5)
+    wgt = torch.randint(1, 64, (h, w), device="cuda", generator=g).float()
+    xyz = torch.rand((3, h, w), device="cuda", generator=g) * wgt
+    rgba = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    rgba8 = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+    ctx = la.Context(0)
+    prm = N.TonemapParams(la.sRGB.struct(), 4, 1, 0.0, 0.0, 0.184)
+    s = torch.cuda.current_stream()
+
+    def run():
+        N.check(N.lib().lr_tonemap(ctx.handle, C.c_void_p(xyz.data_ptr()), C.c_void_p(wgt.data_ptr()), None, w, h,
+                                   C.byref(prm), C.c_void_p(rgba.data_ptr()), C.c_void_p(rgba8.data_ptr()), None,
+                                   C.c_void_p(s.cuda_stream)), "lr_tonemap")
+
+    for _ in range(3):
+        run()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    torch.cuda.synchronize()
+    a.record(s)
+    for _ in range(reps):
+        run()
+    b.record(s)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    algo = n * (16 + 16 + 4 + 8)
+    ctx.close()
+    return {"workload": "lr_tonemap 1920x1080 film, Reinhard2002RGB, automatic exposure, float + 8-bit RGBA",
+            "ms_per_film": round(ms, 4), "gpixels_per_s": round(n / ms / 1e6, 3),
+            "algorithmic_bytes": algo, "achieved_gbs": round(algo / (ms * 1e-3) / 1e9, 1),
+            "frac_hbm": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "launches": "k_film_stats, k_exposure, k_tonemap"}
+
+
 def bench_guided(args, la, torch, np):
     """Learned path guiding on the slit-room workload: the terrain workload
     (4,992,800 triangles + 10k spheres, Lambert / GGX gold / dielectric) in a
@@ -449,6 +494,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg")
     ap.add_argument("--no-terrain", action="store_true", help="skip the terrain path-tracer leg")
     ap.add_argument("--no-direct", action="store_true", help="skip the DirectLighting leg")
+    ap.add_argument("--no-film", action="store_true", help="skip the film output (tone mapping) leg")
     ap.add_argument("--terrain-frames", type=int, default=2)
     ap.add_argument("--profile-workload", action="store_true",
                     help="run only a short slit-room plain + guided sequence for profilers")
@@ -580,6 +626,7 @@ def main() -> int:
 
     render = None if args.no_render else bench_render(args, la, oracle, torch, np)
     direct = None if args.no_direct else bench_direct(args, la, oracle, torch, np)
+    film = None if args.no_film else bench_film(args, la, torch, np)
     terrain = None if args.no_terrain else bench_terrain(args, la, torch, np)
     guided = None if args.no_guided else bench_guided(args, la, torch, np)
     batch = None if args.no_batch else bench_batch(args, la, torch, np)
@@ -636,6 +683,7 @@ def main() -> int:
         },
         "render": render,
         "direct": direct,
+        "film": film,
         "terrain": terrain,
         "guided": guided,
         "batch": batch,

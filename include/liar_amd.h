@@ -356,6 +356,76 @@ int lr_render_direct(lr_scene* scene, const lr_camera* camera, const lr_render_p
                      const lr_direct_params* dparams, float* film_xyz, float* film_weight,
                      lr_render_stats* stats, void* stream);
 
+/* ---- film output -----------------------------------------------------------
+ * RgbSpace (src/liar/kernel/rgb_space.cpp:73-79, 335-386; rgb_space.h:124-140):
+ * primaries and white point as CIE xy chromaticities and a gamma.  Linear RGB
+ * maps to XYZ relative to CIE illuminant E (the renderer's white is a flat
+ * spectrum) through the primaries' matrix and a Bradford adaptation from the
+ * white point to E.  LiAR's sRGB: red (0.64, 0.33), green (0.30, 0.60), blue
+ * (0.15, 0.06), white (0.3127, 0.3290), gamma 2.2. */
+typedef struct lr_rgb_space {
+    float red[2], green[2], blue[2], white[2];
+    float gamma;
+} lr_rgb_space;
+
+/* Row-major 3x3 matrices of a space (either pointer may be NULL). */
+int lr_rgb_space_matrices(const lr_rgb_space* space, float* rgb_to_xyz, float* xyz_to_rgb);
+
+/* Raster::ToneMapping (src/liar/output/raster.cpp:30-40), same order. */
+#define LR_TONEMAP_LINEAR 0u
+#define LR_TONEMAP_COMPRESS_Y 1u
+#define LR_TONEMAP_COMPRESS_RGB 2u
+#define LR_TONEMAP_REINHARD2002_Y 3u
+#define LR_TONEMAP_REINHARD2002_RGB 4u
+#define LR_TONEMAP_EXPONENTIAL_Y 5u
+#define LR_TONEMAP_EXPONENTIAL_RGB 6u
+#define LR_TONEMAP_DUIKER_Y 7u
+#define LR_TONEMAP_DUIKER_RGB 8u
+
+typedef struct lr_tonemap_params {
+    lr_rgb_space space;              /* destination space (the image writer's)  */
+    uint32_t tone_mapping;           /* LR_TONEMAP_*                           */
+    uint32_t auto_exposure;          /* Raster.autoExposure (LiAR default on)  */
+    float exposure_stops;            /* used when auto_exposure == 0           */
+    float exposure_correction_stops; /* added to either                        */
+    float middle_grey;               /* Raster.middleGrey (LiAR: 0.184)        */
+} lr_tonemap_params;
+
+typedef struct lr_tonemap_info {
+    float exposure_stops;     /* stops used (automatic: rounded to 1/3)        */
+    float gain;               /* 2^(stops + correction)                        */
+    float average_luminance;  /* log-average weighted Y of the covered pixels  */
+    float max_luminance;      /* largest weighted Y of the final film          */
+} lr_tonemap_info;
+
+/* Raster::tonemap (src/liar/output/raster.cpp:306-494) of a film written by
+ * lr_render / lr_render_direct / lr_render_guided: film_xyz (device, planes
+ * X, Y, Z of width*height floats), film_weight (device), film_alpha (device
+ * sum of sample alphas, or NULL: alpha 1 wherever the weight is positive).
+ * Writes display RGBA, rows top to bottom: rgba (device, 4 floats per pixel,
+ * 16-byte aligned) and/or rgba8 (device, 4 bytes per pixel, quantised as
+ * LiAR's PNG writer does: clamp(255 x, 0, 255) truncated).  NaN pixels become
+ * pink.  Automatic exposure is computed on the device; info (host, may be
+ * NULL) returns it and then synchronises the stream.  LiAR's
+ * maxSceneLuminance is a running maximum over every splat of the render;
+ * here it is the maximum over the final film. */
+int lr_tonemap(lr_context* ctx, const float* film_xyz, const float* film_weight, const float* film_alpha,
+               uint32_t width, uint32_t height, const lr_tonemap_params* params, float* rgba,
+               uint8_t* rgba8, lr_tonemap_info* info, void* stream);
+
+/* Image codecs (src/liar/kernel/image_codec.cpp:38-45, src/liar/codecs/):
+ * writes host RGBA floats (rows top to bottom, values as lr_tonemap made
+ * them for this format's space) to `path`, chosen by its extension:
+ *   .png        8-bit RGBA, zlib-deflated (lodepng.cpp:248-260 quantisation)
+ *   .exr        half RGBA, uncompressed scanlines, chromaticities attribute
+ *               (openexr.cpp:95-112; linear space)
+ *   .hdr .rgbe .pic  Radiance RGBE, flat scanlines (linear space)
+ *   .pfm        32-bit float RGB, bottom row first (linear space)
+ *   .tga .targa 8-bit BGRA, uncompressed, top-left origin
+ * space (may be NULL: sRGB) fills the EXR chromaticities. */
+int lr_image_write(const char* path, const float* rgba, uint32_t width, uint32_t height,
+                   const lr_rgb_space* space);
+
 /* ---- batches: many scenes in one wavefront (no counterpart in LiAR) -------
  * A batch holds scenes built with lr_scene_create (non-empty geometry each);
  * their trees are concatenated into one node array (indices rebased, each

@@ -11,5 +11,6 @@ from . import cameras, output, samplers, scenery, shaders, spectra, textures, tr
 from ._native import LIB_PATH, LiarError  # noqa: F401
 from .engine import RenderEngine, compile_scene  # noqa: F401
 from .qbvh import Context, Qbvh, philox4x32, random_rays, random_triangles  # noqa: F401
+from .rgb_space import CIEXYZ, RgbSpace, sRGB  # noqa: F401
 
 __version__ = "0.1.0"

@@ -108,6 +108,21 @@ class DirectParams(C.Structure):
                 ("trace_glossy", C.c_uint32)]
 
 
+class RgbSpace(C.Structure):
+    _fields_ = [("red", C.c_float * 2), ("green", C.c_float * 2), ("blue", C.c_float * 2), ("white", C.c_float * 2),
+                ("gamma", C.c_float)]
+
+
+class TonemapParams(C.Structure):
+    _fields_ = [("space", RgbSpace), ("tone_mapping", C.c_uint32), ("auto_exposure", C.c_uint32),
+                ("exposure_stops", C.c_float), ("exposure_correction_stops", C.c_float), ("middle_grey", C.c_float)]
+
+
+class TonemapInfo(C.Structure):
+    _fields_ = [("exposure_stops", C.c_float), ("gain", C.c_float), ("average_luminance", C.c_float),
+                ("max_luminance", C.c_float)]
+
+
 class GuideParams(C.Structure):
     _fields_ = [("selection", C.c_float), ("train", C.c_uint32), ("train_steps", C.c_uint32),
                 ("train_batch", C.c_uint32), ("max_records", C.c_uint32), ("weight_clamp", C.c_float),
@@ -167,6 +182,9 @@ SIGNATURES = {
     "lr_render": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, _P, C.POINTER(RenderStats), _P]),
     "lr_render_direct": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(DirectParams), _P, _P,
                               C.POINTER(RenderStats), _P]),
+    "lr_rgb_space_matrices": (_I, [C.POINTER(RgbSpace), _P, _P]),
+    "lr_tonemap": (_I, [_P, _P, _P, _P, _U32, _U32, C.POINTER(TonemapParams), _P, _P, C.POINTER(TonemapInfo), _P]),
+    "lr_image_write": (_I, [C.c_char_p, _P, _U32, _U32, C.POINTER(RgbSpace)]),
 }
 
 _lib = None
@@ -179,6 +197,14 @@ def lib() -> C.CDLL:
         if not LIB_PATH.exists():
             raise ImportError(
                 f"native library {LIB_PATH} is missing; build it with `python -m liar_amd.build`")
+        # torch's HIP runtime must be the process's HIP runtime: device
+        # pointers and streams pass between torch and this library, so torch
+        # (which ships its own runtime) is loaded first and this library then
+        # binds to the same one
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -82,7 +82,7 @@ def build(verbose: bool = False) -> Path:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
-          "-lpthread"])
+          "-lpthread", "-lz"])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -342,9 +342,10 @@ class RenderEngine:
         rr_start = getattr(tr, "russianRouletteStart", None)
         rr_start = tr.maxRayGeneration if rr_start is None else rr_start
         lr = self.wavelengthRange or (float(standard_observer()[0][0]), float(standard_observer()[0][-1]))
+        msl = float(getattr(self.target, "maxSampleLuminance", math.inf))
         params = render_params(w, h, self.sampler.samplesPerPixel, self.wavelengthsPerSample,
                                getattr(tr, "maxDepth", None) or 0, rr_start, getattr(tr, "survivalProbability", 0.5),
-                               lr, self.seed)
+                               lr, self.seed, max_sample_luminance=msl if math.isfinite(msl) else 0.0)
         ctx = Context(self.device)
         gs = GpuScene(ctx, compile_scene(self.scene))
         dev = torch.device("cuda", self.device)
