@@ -158,6 +158,54 @@ int lr_trace_closest(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n,
 int lr_trace_any(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n,
                  uint8_t* occluded, void* stream);
 
+/* ---- instancing: two-level trees ------------------------------------------
+ * A top-level tree over instances, each an lr_qbvh (the instanced geometry,
+ * built once) placed by a local-to-world affine transform.  Replaces
+ * Transformation::doIntersect / doIsIntersecting
+ * (src/liar/scenery/transformation.cpp:117-131): the ray goes to the
+ * instance's local space, origin W o and direction normalise(W d) with W the
+ * inverse transform, and its bounds scale by s = |W d|; the instance is then
+ * intersected with the single-level rules above in local space, and its hit
+ * distance is t_local / s in world units.  Instances combine as a List
+ * (src/liar/scenery/list.cpp:106-150): nearest world t, equal t broken by
+ * the smaller instance index, then the smaller primitive id.  The local ray
+ * is computed in FP32 as ((W00 ox + W01 oy) + W02 oz) + W03 per row, s =
+ * sqrt((x x + y y) + z z), d_local = W d / s. */
+typedef struct lr_instance {
+    uint32_t geometry;        /* index into the geometry (lr_qbvh) array     */
+    float local_to_world[12]; /* row-major 3x4 affine transform              */
+} lr_instance;
+
+typedef struct lr_tlas lr_tlas;
+
+typedef struct lr_tlas_info {
+    uint32_t n_instances;     /* instances with geometry (empty ones dropped) */
+    uint32_t n_nodes;         /* top-level nodes                              */
+    uint32_t max_depth;       /* top-level depth                              */
+    uint32_t max_stack;       /* two-level traversal stack bound              */
+    uint64_t n_geometry_nodes, n_geometry_prims;
+    double build_seconds;
+    float scene_lo[3], scene_hi[3];
+} lr_tlas_info;
+
+/* geometry: n_geometry lr_qbvh of ctx (each may be shared by many
+ * instances; they must outlive the tlas only until this call returns: their
+ * trees are copied).  The local-to-world transforms must be invertible. */
+int lr_tlas_create(lr_context* ctx, lr_qbvh* const* geometry, uint32_t n_geometry,
+                   const lr_instance* instances, uint32_t n_instances, lr_tlas** out);
+int lr_tlas_destroy(lr_tlas* tlas);
+int lr_tlas_get_info(const lr_tlas* tlas, lr_tlas_info* info);
+
+/* Closest hit over the instances: hits->prim is the primitive id within the
+ * instance's geometry, hits->t the world distance, u / v as for
+ * lr_trace_closest; instance (device, n) receives the instance index
+ * (LR_MISS on a miss). */
+int lr_trace_closest_instanced(lr_tlas* tlas, const lr_rays_soa* rays, uint32_t n, lr_hits_soa* hits,
+                               uint32_t* instance, void* stream);
+/* Any hit over the instances (occluded[i] = 1 or 0). */
+int lr_trace_any_instanced(lr_tlas* tlas, const lr_rays_soa* rays, uint32_t n, uint8_t* occluded,
+                           void* stream);
+
 /* Traversal launch parameters.  0 keeps the current value.
  *   block          threads per workgroup (64, 128 or 256)
  *   blocks_per_cu  cap on resident workgroups per CU (the grid is the

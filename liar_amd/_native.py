@@ -108,6 +108,16 @@ class DirectParams(C.Structure):
                 ("trace_glossy", C.c_uint32)]
 
 
+class Instance(C.Structure):
+    _fields_ = [("geometry", C.c_uint32), ("local_to_world", C.c_float * 12)]
+
+
+class TlasInfo(C.Structure):
+    _fields_ = [("n_instances", C.c_uint32), ("n_nodes", C.c_uint32), ("max_depth", C.c_uint32),
+                ("max_stack", C.c_uint32), ("n_geometry_nodes", C.c_uint64), ("n_geometry_prims", C.c_uint64),
+                ("build_seconds", C.c_double), ("scene_lo", C.c_float * 3), ("scene_hi", C.c_float * 3)]
+
+
 class RgbSpace(C.Structure):
     _fields_ = [("red", C.c_float * 2), ("green", C.c_float * 2), ("blue", C.c_float * 2), ("white", C.c_float * 2),
                 ("gamma", C.c_float)]
@@ -182,6 +192,11 @@ SIGNATURES = {
     "lr_render": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, _P, C.POINTER(RenderStats), _P]),
     "lr_render_direct": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(DirectParams), _P, _P,
                               C.POINTER(RenderStats), _P]),
+    "lr_tlas_create": (_I, [_P, _P, _U32, _P, _U32, C.POINTER(_P)]),
+    "lr_tlas_destroy": (_I, [_P]),
+    "lr_tlas_get_info": (_I, [_P, C.POINTER(TlasInfo)]),
+    "lr_trace_closest_instanced": (_I, [_P, C.POINTER(RaysSoa), _U32, C.POINTER(HitsSoa), _P, _P]),
+    "lr_trace_any_instanced": (_I, [_P, C.POINTER(RaysSoa), _U32, _P, _P]),
     "lr_rgb_space_matrices": (_I, [C.POINTER(RgbSpace), _P, _P]),
     "lr_tonemap": (_I, [_P, _P, _P, _P, _U32, _U32, C.POINTER(TonemapParams), _P, _P, C.POINTER(TonemapInfo), _P]),
     "lr_image_write": (_I, [C.c_char_p, _P, _U32, _U32, C.POINTER(RgbSpace)]),

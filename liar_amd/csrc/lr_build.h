@@ -22,4 +22,8 @@ struct BuiltQbvh {
 int build_qbvh(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, const float* spheres,
                uint32_t ns, BuiltQbvh& out, std::string& err);
 
+// The same SAH build over boxes (n x {lo[3], hi[3]}): fills out.nodes (leaves
+// reference box slots in leaf order) and order[slot] = input box index.
+int build_boxes(const float* boxes, uint32_t n, BuiltQbvh& out, std::vector<uint32_t>& order, std::string& err);
+
 }  // namespace lr

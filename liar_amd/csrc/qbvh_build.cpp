@@ -258,6 +258,8 @@ struct Builder {
     }
 };
 
+void build_nodes(std::vector<Ref>& refs, const Box& scene, BuiltQbvh& out);
+
 }  // namespace
 
 int build_qbvh(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, const float* spheres,
@@ -335,6 +337,81 @@ int build_qbvh(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t n
         out.scene_hi[k] = scene.hi[k];
     }
 
+    build_nodes(refs, scene, out);
+
+    out.prims.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t p = refs[i].prim;
+        lr_prim& q = out.prims[i];
+        std::memset(&q, 0, sizeof(q));
+        if (p < nt) {
+            const float* v0 = verts + 3 * (size_t)tris[3 * p + 0];
+            const float* v1 = verts + 3 * (size_t)tris[3 * p + 1];
+            const float* v2 = verts + 3 * (size_t)tris[3 * p + 2];
+            for (int k = 0; k < 3; ++k) {
+                q.a[k] = v0[k];
+                q.b[k] = v1[k] - v0[k];
+                q.c[k] = v2[k] - v0[k];
+            }
+            q.a[3] = as_f32(p);
+            q.b[3] = as_f32(LR_PRIM_TRIANGLE);
+        } else {
+            const float* s = spheres + 4 * (size_t)(p - nt);
+            q.a[0] = s[0];
+            q.a[1] = s[1];
+            q.a[2] = s[2];
+            q.a[3] = as_f32(p);
+            q.b[0] = s[3];
+            q.b[1] = s[3] * s[3];
+            q.b[3] = as_f32(LR_PRIM_SPHERE);
+        }
+    }
+    out.build_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return 0;
+}
+
+int build_boxes(const float* boxes, uint32_t n, BuiltQbvh& out, std::vector<uint32_t>& order, std::string& err) {
+    const auto t_start = std::chrono::steady_clock::now();
+    out = BuiltQbvh();
+    order.clear();
+    if (n >= (1u << 29)) {
+        err = "too many boxes (limit 2^29 - 1)";
+        return -1;
+    }
+    if (n == 0) return 0;
+    std::vector<Ref> refs(n);
+    Box scene;
+    scene.reset();
+    for (uint32_t i = 0; i < n; ++i) {
+        Ref& r = refs[i];
+        for (int k = 0; k < 3; ++k) {
+            r.box.lo[k] = boxes[6 * (size_t)i + k];
+            r.box.hi[k] = boxes[6 * (size_t)i + 3 + k];
+            if (!std::isfinite(r.box.lo[k]) || !std::isfinite(r.box.hi[k]) || !(r.box.lo[k] <= r.box.hi[k])) {
+                err = "box " + std::to_string(i) + " is not finite or not ordered";
+                return -1;
+            }
+            r.c[k] = 0.5f * (r.box.lo[k] + r.box.hi[k]);
+        }
+        r.prim = i;
+        scene.grow(r.box);
+    }
+    for (int k = 0; k < 3; ++k) {
+        out.scene_lo[k] = scene.lo[k];
+        out.scene_hi[k] = scene.hi[k];
+    }
+    build_nodes(refs, scene, out);
+    order.resize(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = refs[i].prim;
+    out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return 0;
+}
+
+namespace {
+
+void build_nodes(std::vector<Ref>& refs, const Box& scene, BuiltQbvh& out) {
+    const uint32_t n = (uint32_t)refs.size();
     // Every inner node has >= 2 children and every leaf >= 1 prim, so there
     // are at most n - 1 inner nodes (1 when n == 1).
     const uint32_t capacity = std::max(n, 1u);
@@ -397,37 +474,8 @@ int build_qbvh(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t n
         }
     }
     out.sah_cost = sah;
-
-    out.prims.resize(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t p = refs[i].prim;
-        lr_prim& q = out.prims[i];
-        std::memset(&q, 0, sizeof(q));
-        if (p < nt) {
-            const float* v0 = verts + 3 * (size_t)tris[3 * p + 0];
-            const float* v1 = verts + 3 * (size_t)tris[3 * p + 1];
-            const float* v2 = verts + 3 * (size_t)tris[3 * p + 2];
-            for (int k = 0; k < 3; ++k) {
-                q.a[k] = v0[k];
-                q.b[k] = v1[k] - v0[k];
-                q.c[k] = v2[k] - v0[k];
-            }
-            q.a[3] = as_f32(p);
-            q.b[3] = as_f32(LR_PRIM_TRIANGLE);
-        } else {
-            const float* s = spheres + 4 * (size_t)(p - nt);
-            q.a[0] = s[0];
-            q.a[1] = s[1];
-            q.a[2] = s[2];
-            q.a[3] = as_f32(p);
-            q.b[0] = s[3];
-            q.b[1] = s[3] * s[3];
-            q.b[3] = as_f32(LR_PRIM_SPHERE);
-        }
-    }
-    out.build_seconds =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-    return 0;
 }
+
+}  // namespace
 
 }  // namespace lr

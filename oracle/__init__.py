@@ -95,6 +95,48 @@ def brute_any(verts, tris, spheres, rays, threads: int | None = None):
     return occ
 
 
+class _Geom(C.Structure):
+    _fields_ = [("verts", C.c_void_p), ("tris", C.c_void_p), ("nt", C.c_uint32), ("spheres", C.c_void_p),
+                ("ns", C.c_uint32)]
+
+
+def inst_brute(geoms, inst_geom, local_to_world, rays, any_hit=False, threads: int | None = None):
+    """Instances by brute force (LiAR's Transformation of each geometry inside
+    a List): geoms = [(verts, tris, spheres)], inst_geom (m,), local_to_world
+    (m, 3, 4).  Returns (prim, t, u, v, inst) or occluded (uint8)."""
+    L = lib()
+    if not hasattr(L, "_inst_sig"):
+        P = C.c_void_p
+        L.or_inst_brute.argtypes = [P, C.c_uint32, P, P, C.c_uint32, P, C.c_uint32, C.c_int, P, P, P, P, P, P,
+                                    C.c_int]
+        L.or_inst_brute.restype = C.c_int
+        L._inst_sig = True
+    keep = []
+    arr = (_Geom * max(1, len(geoms)))()
+    for k, (v, t, s) in enumerate(geoms):
+        v, t, s = _scene(v, t, s)
+        keep += [v, t, s]
+        arr[k] = _Geom(_p(v), _p(t), len(t), _p(s) if len(s) else None, len(s))
+    ig = np.ascontiguousarray(inst_geom, np.uint32).ravel()
+    m = np.ascontiguousarray(local_to_world, np.float32).reshape(-1, 12)
+    assert len(ig) == len(m)
+    rays, ra = _rays_arg(rays)
+    n = rays.shape[1]
+    prim = np.empty(n, np.uint32)
+    inst = np.empty(n, np.uint32)
+    t, u, v = (np.empty(n, np.float32) for _ in range(3))
+    occ = np.empty(n, np.uint8)
+    rc = L.or_inst_brute(arr, len(geoms), _p(ig), _p(m), len(ig), ra, n, int(bool(any_hit)), _p(prim), _p(t), _p(u),
+                         _p(v), _p(inst), _p(occ), threads or default_threads())
+    assert rc == 0, rc
+    return occ if any_hit else (prim, t, u, v, inst)
+
+
+def set_inst_mistake(k: int) -> None:
+    """Test hook of the instanced oracle: 1 forgets t_local / s."""
+    lib().or_set_inst_mistake(int(k))
+
+
 def brute_closest_f64(verts, tris, spheres, rays, threads: int | None = None):
     """Same candidates, all arithmetic in FP64 (error reference)."""
     verts, tris, spheres = _scene(verts, tris, spheres)

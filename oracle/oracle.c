@@ -18,6 +18,11 @@
  *     (LiAR keeps the first found, object_trees.h:243-244, which depends on
  *     traversal order; the id rule makes the answer order-independent).
  *
+ *   - Transformation::doIntersect / doIsIntersecting: the ray in local space
+ *     (origin W o, direction W d / |W d|, bounds times |W d|), world t =
+ *     t_local / |W d| (src/liar/scenery/transformation.cpp:117-131); the
+ *     instances combine as a List.  FP32, in the library's operation order.
+ *
  * Lass (lass 1.11.0-250+g078d9309, conanfile.py:39), which holds LiAR's
  * triangle/sphere/box arithmetic and its QBVH, is not vendored in the
  * reference, so the FP32 expressions below are this project's statement of
@@ -403,7 +408,44 @@ static int qbvh_any_one(const or_node* nodes, const or_prim* prims, uint32_t n_n
 /* threaded drivers (exported)                                              */
 /* ------------------------------------------------------------------------ */
 
-enum { JOB_BRUTE_CLOSEST, JOB_BRUTE_ANY, JOB_BRUTE_F64, JOB_QBVH_CLOSEST, JOB_QBVH_ANY };
+enum { JOB_BRUTE_CLOSEST, JOB_BRUTE_ANY, JOB_BRUTE_F64, JOB_QBVH_CLOSEST, JOB_QBVH_ANY, JOB_INST_CLOSEST,
+       JOB_INST_ANY };
+
+/* ---- instances (src/liar/scenery/transformation.cpp:117-131) -------------- */
+
+/* World -> local of a row-major 3x4 local-to-world affine, in FP64 and then
+ * rounded to FP32 (the library does the same); 0 when singular. */
+static int affine_inverse_f32(const float* m, float* w) {
+    const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+    const double A = e * i - f * h, B = f * g - d * i, Cc = d * h - e * g;
+    const double det = a * A + b * B + c * Cc;
+    if (!(fabs(det) > 0.0) || !isfinite(det)) return 0;
+    const double inv[9] = {A / det, (c * h - b * i) / det, (b * f - c * e) / det,
+                           B / det, (a * i - c * g) / det, (c * d - a * f) / det,
+                           Cc / det, (b * g - a * h) / det, (a * e - b * d) / det};
+    const double t[3] = {m[3], m[7], m[11]};
+    for (int r = 0; r < 3; ++r) {
+        w[4 * r + 0] = (float)inv[3 * r + 0];
+        w[4 * r + 1] = (float)inv[3 * r + 1];
+        w[4 * r + 2] = (float)inv[3 * r + 2];
+        w[4 * r + 3] = (float)(-(inv[3 * r] * t[0] + inv[3 * r + 1] * t[1] + inv[3 * r + 2] * t[2]));
+    }
+    return 1;
+}
+
+/* The local ray in FP32: origin ((W00 ox + W01 oy) + W02 oz) + W03 per row,
+ * direction W d / s with s = sqrt((x x + y y) + z z). */
+static float local_ray(const float* w, const float o[3], const float d[3], float lo[3], float ld[3]) {
+    for (int r = 0; r < 3; ++r) lo[r] = ((w[4 * r] * o[0] + w[4 * r + 1] * o[1]) + w[4 * r + 2] * o[2]) + w[4 * r + 3];
+    float x = (w[0] * d[0] + w[1] * d[1]) + w[2] * d[2];
+    float y = (w[4] * d[0] + w[5] * d[1]) + w[6] * d[2];
+    float z = (w[8] * d[0] + w[9] * d[1]) + w[10] * d[2];
+    float s = sqrtf((x * x + y * y) + z * z);
+    ld[0] = x / s;
+    ld[1] = y / s;
+    ld[2] = z / s;
+    return s;
+}
 
 typedef struct {
     int kind;
@@ -419,7 +461,52 @@ typedef struct {
     uint8_t* occ;
     or_counts counts;
     int status;
+    /* instances */
+    const or_scene* geoms;
+    const uint32_t* inst_geom;
+    const float* inst_w;  /* n_inst x 12, world -> local */
+    uint32_t n_inst;
+    uint32_t* inst_out;
 } or_job;
+
+static int g_inst_plant = 0;
+/* Test hook: 1 forgets the division of the local distance by the scale. */
+void or_set_inst_mistake(int k) { g_inst_plant = k; }
+
+/* List over the instances: each intersected in its local space with the
+ * single-level rules, world t = t_local / s, nearest t then smaller
+ * instance index then smaller primitive id. */
+static void inst_closest_one(const or_job* j, const float o[3], const float d[3], float tn, float tf, uint32_t* prim,
+                             float* th, float* uh, float* vh, uint32_t* inst) {
+    float bt = INFINITY, bu = 0.0f, bv = 0.0f;
+    uint32_t bp = OR_MISS, bi = OR_MISS;
+    for (uint32_t k = 0; k < j->n_inst; ++k) {
+        const or_scene* g = &j->geoms[j->inst_geom[k]];
+        if (g->nt + g->ns == 0) continue;
+        float lo[3], ld[3];
+        const float s = local_ray(j->inst_w + 12 * (size_t)k, o, d, lo, ld);
+        uint32_t p;
+        float t, u, v;
+        brute_closest_one(g, lo, ld, tn * s, tf * s, &p, &t, &u, &v);
+        if (p == OR_MISS) continue;
+        const float tw = g_inst_plant == 1 ? t : t / s;
+        if (tw < bt || (tw == bt && (k < bi || (k == bi && p < bp)))) {
+            bt = tw; bu = u; bv = v; bp = p; bi = k;
+        }
+    }
+    *prim = bp; *th = bp == OR_MISS ? INFINITY : bt; *uh = bu; *vh = bv; *inst = bi;
+}
+
+static int inst_any_one(const or_job* j, const float o[3], const float d[3], float tn, float tf) {
+    for (uint32_t k = 0; k < j->n_inst; ++k) {
+        const or_scene* g = &j->geoms[j->inst_geom[k]];
+        if (g->nt + g->ns == 0) continue;
+        float lo[3], ld[3];
+        const float s = local_ray(j->inst_w + 12 * (size_t)k, o, d, lo, ld);
+        if (brute_any_one(g, lo, ld, tn * s, tf * s)) return 1;
+    }
+    return 0;
+}
 
 static void* run_job(void* arg) {
     or_job* j = (or_job*)arg;
@@ -442,6 +529,12 @@ static void* run_job(void* arg) {
             if (qbvh_closest_one(j->nodes, j->prims, j->n_nodes, o, d, tn, tf, j->prim + i, j->t + i,
                                  j->u + i, j->v + i, &j->counts))
                 j->status = -1;
+            break;
+        case JOB_INST_CLOSEST:
+            inst_closest_one(j, o, d, tn, tf, j->prim + i, j->t + i, j->u + i, j->v + i, j->inst_out + i);
+            break;
+        case JOB_INST_ANY:
+            j->occ[i] = (uint8_t)inst_any_one(j, o, d, tn, tf);
             break;
         case JOB_QBVH_ANY: {
             int r = qbvh_any_one(j->nodes, j->prims, j->n_nodes, o, d, tn, tf, &j->counts);
@@ -501,6 +594,33 @@ int or_brute_closest(const float* verts, const uint32_t* tris, uint32_t nt, cons
     j.rays = mk_rays(rays);
     j.prim = prim; j.t = t; j.u = u; j.v = v;
     return run_threads(j, n, nthreads, NULL, NULL);
+}
+
+/* Instanced brute force.  geoms: n_geom (verts, tris, nt, spheres, ns)
+ * descriptions (layout of or_scene); inst_geom / local_to_world: per
+ * instance.  any != 0: occluded only.  Returns -2 for a singular transform. */
+int or_inst_brute(const or_scene* geoms, uint32_t n_geom, const uint32_t* inst_geom, const float* local_to_world,
+                  uint32_t n_inst, const float* const rays[8], uint32_t n, int any, uint32_t* prim, float* t,
+                  float* u, float* v, uint32_t* inst, uint8_t* occ, int nthreads) {
+    float* w = (float*)malloc(sizeof(float) * 12 * (size_t)(n_inst ? n_inst : 1));
+    for (uint32_t k = 0; k < n_inst; ++k) {
+        if (inst_geom[k] >= n_geom || !affine_inverse_f32(local_to_world + 12 * (size_t)k, w + 12 * (size_t)k)) {
+            free(w);
+            return -2;
+        }
+    }
+    or_job j;
+    memset(&j, 0, sizeof(j));
+    j.kind = any ? JOB_INST_ANY : JOB_INST_CLOSEST;
+    j.geoms = geoms;
+    j.inst_geom = inst_geom;
+    j.inst_w = w;
+    j.n_inst = n_inst;
+    j.rays = mk_rays(rays);
+    j.prim = prim; j.t = t; j.u = u; j.v = v; j.inst_out = inst; j.occ = occ;
+    int rc = run_threads(j, n, nthreads, NULL, NULL);
+    free(w);
+    return rc;
 }
 
 int or_brute_any(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres,
