@@ -197,6 +197,70 @@ def bench_film(args, la, torch, np):
             "launches": "k_film_stats, k_exposure, k_tonemap"}
 
 
+def bench_instanced(args, la, torch, np):
+    """Instancing leg: 2,000 placements (random rotation, scale 0.5-2 per axis,
+    translation in a 60-unit cube) of one 20,000-triangle object -- 40M
+    instanced triangles in a 2-level tree -- hit by 4,194,304 rays from a
+    surrounding shell towards random interior points; closest and any hit
+    (lr_trace_*_instanced), HIP events on the launch stream."""
+    from liar_amd.instancing import Tlas, random_rotation
+
+    rng = np.random.default_rng(21)
+    nt = 20000
+    c = rng.uniform(-0.5, 0.5, (nt, 3))
+    verts = (np.repeat(c, 3, axis=0) + rng.uniform(-0.01, 0.01, (3 * nt, 3))).astype(np.float32)
+    tris = np.arange(3 * nt, dtype=np.uint32).reshape(-1, 3)
+    irng = np.random.default_rng(22)
+    inst = []
+    for _ in range(2000):
+        R = random_rotation(irng)
+        S = np.diag(irng.uniform(0.5, 2.0, 3))
+        inst.append((0, np.concatenate([R @ S, irng.uniform(-30, 30, (3, 1))], 1)))
+    ctx = la.Context(0)
+    q = la.Qbvh(ctx, verts, tris)
+    t0 = time.perf_counter()
+    tl = Tlas(ctx, [q], inst)
+    build_s = time.perf_counter() - t0
+    n = N_RAYS
+    o = rng.normal(size=(n, 3))
+    o = 90.0 * o / np.linalg.norm(o, axis=1, keepdims=True)
+    d = rng.uniform(-30, 30, (n, 3)) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros((8, n), np.float32)
+    r[0:3], r[3:6], r[7] = o.T, d.T, np.inf
+    rays = torch.from_numpy(r).cuda()
+    prim = torch.empty(n, dtype=torch.int32, device="cuda")
+    tuv = torch.empty((3, n), dtype=torch.float32, device="cuda")
+    ii = torch.empty(n, dtype=torch.int32, device="cuda")
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream()
+
+    def timed(fn, reps=5):
+        fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(s)
+        for _ in range(reps):
+            fn()
+        b.record(s)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    ms_c = timed(lambda: tl.trace_closest(rays, prim, tuv, ii))
+    ms_a = timed(lambda: tl.trace_any(rays, occ))
+    hit = float((prim != -1).float().mean())
+    agree = bool(((occ != 0) == (prim != -1)).all())
+    info = tl.info()
+    tl.close()
+    q.close()
+    ctx.close()
+    return {"workload": "2000 instances of a 20k-triangle object (40M instanced triangles), 4,194,304 rays",
+            "closest_mrays_per_s": round(n / ms_c / 1e3, 1), "closest_ms": round(ms_c, 4),
+            "any_mrays_per_s": round(n / ms_a / 1e3, 1), "any_ms": round(ms_a, 4), "hit_rate": round(hit, 4),
+            "any_agrees_with_closest": agree, "tlas_nodes": info["n_nodes"], "tlas_depth": info["max_depth"],
+            "build_s": round(build_s, 3)}
+
+
 def bench_guided(args, la, torch, np):
     """Learned path guiding on the slit-room workload: the terrain workload
     (4,992,800 triangles + 10k spheres, Lambert / GGX gold / dielectric) in a
@@ -495,6 +559,7 @@ def main() -> int:
     ap.add_argument("--no-terrain", action="store_true", help="skip the terrain path-tracer leg")
     ap.add_argument("--no-direct", action="store_true", help="skip the DirectLighting leg")
     ap.add_argument("--no-film", action="store_true", help="skip the film output (tone mapping) leg")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the instancing leg")
     ap.add_argument("--terrain-frames", type=int, default=2)
     ap.add_argument("--profile-workload", action="store_true",
                     help="run only a short slit-room plain + guided sequence for profilers")
@@ -627,6 +692,7 @@ def main() -> int:
     render = None if args.no_render else bench_render(args, la, oracle, torch, np)
     direct = None if args.no_direct else bench_direct(args, la, oracle, torch, np)
     film = None if args.no_film else bench_film(args, la, torch, np)
+    instanced = None if args.no_instanced else bench_instanced(args, la, torch, np)
     terrain = None if args.no_terrain else bench_terrain(args, la, torch, np)
     guided = None if args.no_guided else bench_guided(args, la, torch, np)
     batch = None if args.no_batch else bench_batch(args, la, torch, np)
@@ -684,6 +750,7 @@ def main() -> int:
         "render": render,
         "direct": direct,
         "film": film,
+        "instanced": instanced,
         "terrain": terrain,
         "guided": guided,
         "batch": batch,

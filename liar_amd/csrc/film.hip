@@ -52,10 +52,15 @@ struct ToneArgs {
     float exposure_stops, correction_stops, middle_grey;
 };
 
+// x^g as exp2(g log2 x) on the hardware exp2 / log2 (relative error ~1e-6
+// for the exponents and ranges of display curves; powf's range reduction
+// made the per-pixel kernel compute bound).
+__device__ inline float fast_pow(float x, float g) { return x > 0.0f ? __powf(x, g) : (g > 0.0f ? 0.0f : 1.0f / x); }
+
 __device__ inline float filmic(float x) {  // raster.cpp:62-67
     x = fmaxf(0.0f, x - 0.004f);
     const float y = (x * (6.2f * x + 0.5f)) / (x * (6.2f * x + 1.7f) + 0.06f);
-    return powf(y, 2.2f);
+    return fast_pow(y, 2.2f);
 }
 
 __device__ inline float inv_filmic(float y) {  // raster.cpp:69-80
@@ -70,7 +75,7 @@ __device__ inline float inv_filmic(float y) {  // raster.cpp:69-80
 }
 
 __device__ inline float tonecurve(float x, float g) {  // rgb_space.cpp:156-162
-    return x < 0.0f ? -powf(-x, g) : powf(x, g);
+    return x < 0.0f ? -fast_pow(-x, g) : fast_pow(x, g);
 }
 
 __global__ __launch_bounds__(kStatsBlock) void k_film_stats(const float* __restrict__ xyz,
@@ -107,17 +112,41 @@ __global__ __launch_bounds__(kStatsBlock) void k_film_stats(const float* __restr
     if (threadIdx.x == 0) part[blockIdx.x] = StatsPartial{s_sum[0], s_cnt[0], s_max[0]};
 }
 
-// Raster::exposureStops / sceneGain (raster.cpp:99-150, 613-617).
-__global__ void k_exposure(const StatsPartial* __restrict__ part, uint32_t nb, ToneArgs A, ToneState* out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double sum = 0.0;
-    uint32_t cnt = 0;
-    float mx = 0.0f;
-    for (uint32_t b = 0; b < nb; ++b) {
-        sum += part[b].sum_log;
-        cnt += part[b].count;
-        mx = fmaxf(mx, part[b].max_y);
+// Raster::exposureStops / sceneGain (raster.cpp:99-150, 613-617).  One
+// block: each thread sums the partials b = tid, tid + 256, ... in order, then
+// a fixed-shape tree, so the result does not depend on scheduling.
+__global__ __launch_bounds__(kStatsBlock) void k_exposure(const StatsPartial* __restrict__ part, uint32_t nb,
+                                                         ToneArgs A, ToneState* out) {
+    __shared__ double s_sum[kStatsBlock];
+    __shared__ uint32_t s_cnt[kStatsBlock];
+    __shared__ float s_max[kStatsBlock];
+    {
+        double ps = 0.0;
+        uint32_t pc = 0;
+        float pm = 0.0f;
+        for (uint32_t b = threadIdx.x; b < nb; b += kStatsBlock) {
+            const StatsPartial q = part[b];
+            ps += q.sum_log;
+            pc += q.count;
+            pm = fmaxf(pm, q.max_y);
+        }
+        s_sum[threadIdx.x] = ps;
+        s_cnt[threadIdx.x] = pc;
+        s_max[threadIdx.x] = pm;
     }
+    __syncthreads();
+    for (int h = kStatsBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + h];
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + h];
+            s_max[threadIdx.x] = fmaxf(s_max[threadIdx.x], s_max[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double sum = s_sum[0];
+    const uint32_t cnt = s_cnt[0];
+    const float mx = s_max[0];
     const float avg = cnt ? (float)exp(sum / (double)cnt) : 0.0f;
     float stops = A.exposure_stops;
     if (A.auto_exposure) {
@@ -358,7 +387,7 @@ int lr_tonemap(lr_context* ctx, const float* film_xyz, const float* film_weight,
     StatsPartial* part = static_cast<StatsPartial*>(scratch);
     ToneState* st = reinterpret_cast<ToneState*>(static_cast<char*>(scratch) + part_bytes);
     hipLaunchKernelGGL(k_film_stats, dim3(nb), dim3(kStatsBlock), 0, s, film_xyz, film_weight, n, part);
-    hipLaunchKernelGGL(k_exposure, dim3(1), dim3(64), 0, s, part, nb, A, st);
+    hipLaunchKernelGGL(k_exposure, dim3(1), dim3(kStatsBlock), 0, s, part, nb, A, st);
     hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, film_xyz, film_weight, film_alpha, n, A,
                        st, reinterpret_cast<float4*>(rgba), reinterpret_cast<uchar4*>(rgba8));
     LR_HIP_CHECK(hipGetLastError());

@@ -114,34 +114,62 @@ def compile_scene(scene) -> CompiledScene:
         tri_mat.append(np.full(len(t), mat, np.uint32))
         nverts[0] += len(v)
 
-    def walk(obj, inherited):
+    def point(M, p):
+        p = np.asarray(p, np.float64)
+        return p if M is None else (M[:3, :3] @ p + M[:3, 3])
+
+    def vector(M, v):
+        v = np.asarray(v, np.float64)
+        return v if M is None else M[:3, :3] @ v
+
+    def walk(obj, inherited, M=None):
         shader = obj.shader if obj.shader is not None else inherited
-        if isinstance(obj, scenery.List):
+        if isinstance(obj, scenery.Transformation):
+            L = obj.localToWorld if M is None else M @ obj.localToWorld
+            walk(obj.child, shader, L)
+        elif isinstance(obj, scenery.List):
             for c in obj.children:
-                walk(c, shader)
+                walk(c, shader, M)
         elif isinstance(obj, scenery.TriangleMesh):
-            add_mesh(obj.vertices(), obj.triangles(), material(shader))
+            v = obj.vertices()
+            if M is not None:
+                v = (np.asarray(v, np.float64) @ M[:3, :3].T + M[:3, 3]).astype(np.float32)
+            add_mesh(v, obj.triangles(), material(shader))
         elif isinstance(obj, scenery.Parallelogram):
-            add_mesh(*obj._mesh(), material(shader))
+            v, t = obj._mesh()
+            if M is not None:
+                v = (np.asarray(v, np.float64) @ M[:3, :3].T + M[:3, 3]).astype(np.float32)
+            add_mesh(v, t, material(shader))
         elif isinstance(obj, scenery.Sphere):
-            sph.append([*obj.center, obj.radius])
+            r = obj.radius
+            if M is not None:
+                A = M[:3, :3]
+                k = np.cbrt(abs(np.linalg.det(A)))
+                if not np.allclose(A.T @ A, k * k * np.eye(3), rtol=1e-6, atol=1e-9 * k * k):
+                    raise NotImplementedError("a Sphere under a non-uniform scale or shear is not supported")
+                r = obj.radius * k
+            sph.append([*point(M, obj.center), r])
             sph_mat.append(material(shader))
         elif isinstance(obj, scenery.LightArea):
             surf = obj.surface
             li = len(lights)
+            o, a, b = point(M, surf.support), vector(M, surf.sizeU), vector(M, surf.sizeV)
             lights.append({"type": N.LIGHT_PARALLELOGRAM,
                            "radiance": _spectrum_of(obj.radiance, table, spec_index),
                            "double_sided": bool(obj.isDoubleSided),
-                           "origin": surf.support, "dx": surf.sizeU, "dy": surf.sizeV,
+                           "origin": o, "dx": a, "dy": b,
                            "samples": int(obj.numberOfEmissionSamples)})
             s = obj.shader if obj.shader is not None else surf.shader
-            add_mesh(*surf._mesh(), material(s, li))
+            v = np.stack([o, o + a, o + a + b, o + b]).astype(np.float32)
+            add_mesh(v, np.array([[0, 1, 2], [0, 2, 3]], np.uint32), material(s, li))
         elif isinstance(obj, scenery.LightSky):
             lights.append({"type": N.LIGHT_SKY, "radiance": _spectrum_of(obj.radiance, table, spec_index),
                            "samples": int(obj.numberOfEmissionSamples)})
         elif isinstance(obj, scenery.LightSun):
+            dsun = vector(M, obj.direction)
+            dsun = dsun / np.linalg.norm(dsun)
             lights.append({"type": N.LIGHT_SUN, "radiance": _spectrum_of(obj.radiance, table, spec_index),
-                           "direction": obj.direction, "angle": obj.angle,
+                           "direction": tuple(float(x) for x in dsun), "angle": obj.angle,
                            "samples": int(obj.numberOfEmissionSamples)})
         else:
             raise NotImplementedError(f"scene object {type(obj).__name__} is not supported")
