@@ -97,6 +97,18 @@ def test_branching_scene_same_seed_matches_oracle(gpu_ctx):
     assert rel < 5e-3 and bad < 0.05
 
 
+def test_terrain_sun_sky_same_seed_matches_oracle(gpu_ctx):
+    """Small terrain: Lambert, glossy Cook-Torrance gold and dielectric
+    materials under a sun disc and a sky -- glossy continuations (secondary,
+    picked-light estimates), dielectric branches, sun and sky shadow rays."""
+    say("[gpu] direct small terrain 48x27 4spp vs oracle")
+    cs, cam = W.terrain(n_side=101, n_spheres=300, seed=7, tile=4)
+    gx, gw, st = gpu_direct(gpu_ctx, cs, cam, 48, 27, 4, nl=1, seed=11)
+    ox, ow = oracle.render_direct(cs, cam, 48, 27, 4, wavelengths=1, seed=11)
+    rel, bad = compare(gx, gw, ox, ow, "direct terrain")
+    assert rel < 5e-3 and bad < 0.05
+
+
 def test_chunking_and_queue_growth(gpu_ctx):
     """Chunks of 512 paths (queues grown and chunks repeated on overflow) give
     the film of one big chunk, up to the rounding of atomic sums."""
