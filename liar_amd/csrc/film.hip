@@ -11,7 +11,7 @@
 //
 // Here the whole film is one launch sequence on the caller's stream:
 //   k_film_stats   per-block partial sums of log Y and counts, and max Y
-//   k_exposure     one thread: fixed-order sum of the partials, the gain
+//   k_exposure     one block: fixed-order sum of the partials, the gain
 //   k_tonemap      one thread per pixel, float RGBA and/or 8-bit RGBA
 // No host synchronisation unless the caller asks for the exposure values.
 #include <hip/hip_runtime.h>
@@ -27,7 +27,7 @@
 namespace {
 
 constexpr int kStatsBlock = 256;
-constexpr uint32_t kMaxStatsBlocks = 1024;
+constexpr uint32_t kMaxStatsBlocks = 2048;
 
 struct StatsPartial {
     double sum_log;
@@ -52,10 +52,13 @@ struct ToneArgs {
     float exposure_stops, correction_stops, middle_grey;
 };
 
-// x^g as exp2(g log2 x) on the hardware exp2 / log2 (relative error ~1e-6
-// for the exponents and ranges of display curves; powf's range reduction
-// made the per-pixel kernel compute bound).
-__device__ inline float fast_pow(float x, float g) { return x > 0.0f ? __powf(x, g) : (g > 0.0f ? 0.0f : 1.0f / x); }
+// x^g as exp2(g log2 x) on the hardware v_log_f32 / v_exp_f32 (relative
+// error ~1e-6 for the exponents and ranges of display curves).  HIP's powf
+// and __powf are both the accurate library pow, whose range reduction made
+// the per-pixel kernel compute bound.
+__device__ inline float fast_pow(float x, float g) {
+    return x > 0.0f ? __builtin_amdgcn_exp2f(g * __builtin_amdgcn_logf(x)) : (g > 0.0f ? 0.0f : 1.0f / x);
+}
 
 __device__ inline float filmic(float x) {  // raster.cpp:62-67
     x = fmaxf(0.0f, x - 0.004f);

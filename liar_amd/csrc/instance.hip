@@ -260,20 +260,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace_inst(const lr_node* __restrict_
         } else {
             const uint32_t firstp = (cur >> 2) & 0x1FFFFFFFu;
             const uint32_t cnt = (cur & 3u) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* q = pf + (size_t)(firstp + k) * 3;
-                const float4 a = q[0], b = q[1];
+            const float4* q = pf + (size_t)firstp * 3;
+            auto test = [&](const float4 a, const float4 b, const float4 c) {
                 const uint32_t id = __float_as_uint(a.w);
                 if (__float_as_uint(b.w) == LR_PRIM_TRIANGLE) {
-                    const float4 c = q[2];
                     float th, uh, vh;
                     if (lr::tri_intersect(ox, oy, oz, dx, dy, dz, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, th,
                                           uh, vh)) {
                         if (ANY) {
-                            if (th > nl && th < fl) {
-                                any_hit = true;
-                                break;
-                            }
+                            if (th > nl && th < fl) any_hit = true;
                         } else if (th > nl && th < far_lim && (th < tri_t || (th == tri_t && id < tri_id))) {
                             tri_t = th;
                             tri_u = uh;
@@ -288,12 +283,39 @@ __global__ __launch_bounds__(BLOCK) void k_trace_inst(const lr_node* __restrict_
                         lr::in_range(th, nl, fl)) {
                         if (ANY) {
                             any_hit = true;
-                            break;
                         } else if (th < sph_t || (th == sph_t && id < sph_id)) {
                             sph_t = th;
                             sph_id = id;
                         }
                     }
+                }
+            };
+            // a leaf's records (48 B each) are requested before any is tested:
+            // the whole leaf for any-hit, pairs for closest-hit
+            if constexpr (ANY) {
+                float4 ra[4], rb[4], rc[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if ((uint32_t)k < cnt) {
+                        ra[k] = q[3 * k];
+                        rb[k] = q[3 * k + 1];
+                        rc[k] = q[3 * k + 2];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((uint32_t)k < cnt && !any_hit) test(ra[k], rb[k], rc[k]);
+            } else {
+                for (uint32_t k0 = 0; k0 < cnt; k0 += 2) {
+                    const float4 ra0 = q[3 * k0], rb0 = q[3 * k0 + 1], rc0 = q[3 * k0 + 2];
+                    float4 ra1 = ra0, rb1 = rb0, rc1 = rc0;
+                    if (k0 + 1 < cnt) {
+                        ra1 = q[3 * k0 + 3];
+                        rb1 = q[3 * k0 + 4];
+                        rc1 = q[3 * k0 + 5];
+                    }
+                    test(ra0, rb0, rc0);
+                    if (k0 + 1 < cnt) test(ra1, rb1, rc1);
                 }
             }
             if (ANY && any_hit) {
