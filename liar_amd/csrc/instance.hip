@@ -87,11 +87,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_inst(const lr_node* __restrict_
     const float4* __restrict__ nf = reinterpret_cast<const float4*>(nodes);
     const float4* __restrict__ pf = reinterpret_cast<const float4*>(prims);
     const uint32_t lane = threadIdx.x & 63u;
-    bool first = true;
-    const uint32_t static_total = gridDim.x * BLOCK;
+    RayClaim claim;
+    claim.init(n, blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6), gridDim.x * (BLOCK / 64));
 
     uint32_t ray = kNone;
-    bool no_more = false;
+    bool no_more = claim.exhausted();  // wave-uniform
     // world ray
     float wox = 0, woy = 0, woz = 0, wdx = 0, wdy = 0, wdz = 0, tnear = 0, tfar = 0;
     // current ray (world, or the current instance's local space)
@@ -189,43 +189,25 @@ __global__ __launch_bounds__(BLOCK) void k_trace_inst(const lr_node* __restrict_
             const uint64_t need = __ballot(ray == kNone);
             const uint32_t idle = (uint32_t)__popcll(need);
             if (idle && (idle >= refill_min || idle == 64u)) {
-                const uint32_t cnt = idle;
-                const int leader = __ffsll((unsigned long long)need) - 1;
-                uint32_t base = 0;
-                if (first) {
-                    base = blockIdx.x * BLOCK + (threadIdx.x & ~63u);
-                } else {
-                    if ((int)lane == leader) base = atomicAdd(counter, cnt);
-                    base = static_total + __shfl(base, leader);
+                const uint32_t my = claim.claim(need, idle, lane, ray == kNone, counter);
+                if (my != kNone) {
+                    ray = my;
+                    wox = rays.ox[my];
+                    woy = rays.oy[my];
+                    woz = rays.oz[my];
+                    wdx = rays.dx[my];
+                    wdy = rays.dy[my];
+                    wdz = rays.dz[my];
+                    tnear = rays.tnear[my];
+                    tfar = rays.tfar[my];
+                    to_world();
+                    best_t = INFINITY;
+                    best_u = best_v = 0.0f;
+                    best_prim = best_inst = LR_MISS;
+                    cur = 0u;
+                    st.sp = 0;
                 }
-                if (ray == kNone) {
-                    const uint32_t my = base + __builtin_amdgcn_mbcnt_hi(
-                                                   (uint32_t)(need >> 32),
-                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
-                    if (my < n) {
-                        ray = my;
-                        wox = rays.ox[my];
-                        woy = rays.oy[my];
-                        woz = rays.oz[my];
-                        wdx = rays.dx[my];
-                        wdy = rays.dy[my];
-                        wdz = rays.dz[my];
-                        tnear = rays.tnear[my];
-                        tfar = rays.tfar[my];
-                        to_world();
-                        best_t = INFINITY;
-                        best_u = best_v = 0.0f;
-                        best_prim = best_inst = LR_MISS;
-                        cur = 0u;
-                        st.sp = 0;
-                    }
-                }
-                if (first) {
-                    if (static_total >= n) no_more = true;
-                    first = false;
-                } else if ((uint64_t)base + cnt >= n) {
-                    no_more = true;
-                }
+                no_more = claim.exhausted();
             }
         }
         if (__ballot(ray != kNone) == 0) break;
@@ -410,6 +392,7 @@ bool affine_inverse(const float* m, double* w) {
 template <bool ANY>
 int launch_inst(lr_tlas* T, const lr_rays_soa* rays, uint32_t n, HitsArg ha, uint32_t* inst, uint8_t* occ,
                 hipStream_t s) {
+    if (n > kMaxRays) return lr::fail(LR_E_INVALID, "at most 4,278,190,080 rays per launch");
     lr_context* ctx = T->ctx;
     const auto fn = k_trace_inst<kBlock, kShort, ANY>;
     lr_trace_config cfg;

@@ -66,6 +66,87 @@ __device__ __forceinline__ void node_test(const float4* __restrict__ np, float o
         e[b] = s_ ? ea_ : eb_;             \
     }
 
+// Largest ray count per launch: RayClaim's counter may run past n by up to
+// 1024 per wave (at most 8192 waves), which must not wrap 32 bits.
+constexpr uint32_t kMaxRays = 0xFF000000u;
+
+__device__ __forceinline__ uint32_t min_u64_u32(uint64_t a, uint32_t b) { return a < b ? (uint32_t)a : b; }
+
+// Ray assignment for the persistent traversal kernels.  About half of the
+// rays are split statically, interleaved in groups of 64: wave w of W owns
+// groups w, w + W, w + 2W, ... of the first S*W rays (S a multiple of 64 per
+// wave), so the rays in flight at any time stay in one window of the ray
+// arrays.  The rest are claimed from one global counter in chunks whose size
+// shrinks with what is left (guided self-scheduling: remaining / (2 W), at
+// most 1024, at least what the wave needs now); a wave hands out a claimed
+// chunk over several refills.  One atomic on one address costs ~10 ns,
+// serialised over the whole GPU, so claiming only what one refill needs made
+// the counter the limit for short rays (a few node visits each).  All members
+// are wave-uniform; claim() is called by the whole wave.
+struct RayClaim {
+    uint32_t n, n_waves, wave, static_total;
+    uint32_t lo, hi;    // the wave's reservation not yet handed out: [lo, hi)
+    bool in_static;     // [lo, hi) counts the wave's static rays (else ray indices)
+    uint32_t dyn_left;  // rays the counter had not handed out, as last seen
+    bool dyn_done;      // the counter has nothing left
+
+    __device__ __forceinline__ void init(uint32_t n_rays, uint32_t w, uint32_t waves) {
+        n = n_rays;
+        n_waves = waves;
+        wave = __builtin_amdgcn_readfirstlane(w);  // uniform: keep it (and what follows) in SGPRs
+        const uint32_t per = 64u * max(1u, n / (128u * waves));
+        static_total = min_u64_u32((uint64_t)per * waves, n);
+        in_static = true;
+        lo = 0;
+        // per == 64 when 64 * waves may exceed n: wave w then owns [64 w, 64 w + 64) clipped to n
+        hi = (uint64_t)per * waves <= n ? per : min_u64_u32(n > 64ull * w ? n - 64ull * w : 0ull, 64u);
+        dyn_left = n - static_total;
+        dyn_done = dyn_left == 0;
+    }
+    __device__ __forceinline__ bool exhausted() const { return dyn_done && lo >= hi; }
+    __device__ __forceinline__ uint32_t static_ray(uint32_t j) const {
+        return ((j >> 6) * n_waves + wave) * 64u + (j & 63u);
+    }
+
+    // Hands rays to the `idle` lanes set in `need`, in lane order; returns
+    // this lane's ray index, or 0xFFFFFFFF when it gets none.
+    __device__ __forceinline__ uint32_t claim(uint64_t need, uint32_t idle, uint32_t lane, bool is_idle,
+                                              uint32_t* counter) {
+        const uint32_t have = hi - lo;
+        uint32_t nlo = 0, nhi = 0;
+        if (have < idle && !dyn_done) {
+            const uint32_t want = idle - have;
+            const uint32_t chunk = max(want, min(1024u, (dyn_left / (2u * n_waves)) & ~63u));
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(counter, chunk);
+            base = __builtin_amdgcn_readlane(base, leader);
+            const uint64_t a = (uint64_t)static_total + base, b = a + chunk;
+            nlo = min_u64_u32(a, n);
+            nhi = min_u64_u32(b, n);
+            dyn_left = n - nhi;
+            dyn_done = b >= n;
+        }
+        uint32_t my = 0xFFFFFFFFu;
+        if (is_idle) {
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+            if (rank < have)
+                my = in_static ? static_ray(lo + rank) : lo + rank;
+            else if (rank - have < nhi - nlo)
+                my = nlo + (rank - have);
+        }
+        if (have >= idle) {
+            lo += idle;
+        } else {
+            in_static = false;
+            lo = min(nlo + (idle - have), nhi);
+            hi = nhi;
+        }
+        return my;
+    }
+};
+
 template <int BLOCK, int SHORT>
 struct Stack {
     uint32_t* s_e;

@@ -27,9 +27,10 @@ namespace {
 using namespace lr::trav;
 
 // Lane-refill persistent traversal.  Every lane runs one traversal step per
-// iteration (a node visit or a leaf's primitive tests); a lane whose ray is
-// finished takes the next ray index from a global counter, one atomic per
-// wave per refill (ballot + mbcnt), so no lane idles while rays remain.
+// iteration (a node visit or a leaf's primitive tests); lanes whose rays are
+// finished take the next ray indices from their wave's reservation (a static
+// share, then chunks claimed from a global counter: RayClaim), so no lane
+// idles while rays remain.
 template <int BLOCK, int SHORT, bool ANY>
 __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nodes,
                                                  const lr_prim* __restrict__ prims, RaysArg rays, uint32_t n,
@@ -55,13 +56,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
 
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     uint32_t ray = kNone;
-    bool no_more = false;  // wave-uniform
-    // The first fill of every wave is its own slice of [0, grid * BLOCK)
-    // (no atomic: one atomic per wave on one address costs ~10 ns each,
-    // serialised, which dominated short and empty launches); later refills
-    // take from the counter, which numbers the rays after those slices.
-    bool first = true;
-    const uint32_t static_total = gridDim.x * BLOCK;
+    RayClaim claim;
+    claim.init(n, blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6), gridDim.x * (BLOCK / 64));
+    bool no_more = claim.exhausted();  // wave-uniform
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0;
     float tnear = 0, tfar = 0, far_lim = 0, near_lim = 0;
     float tri_t = 0, tri_u = 0, tri_v = 0, sph_t = 0;
@@ -73,51 +70,33 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
             const uint64_t need = __ballot(ray == kNone);
             const uint32_t idle = (uint32_t)__popcll(need);
             // Refill only when enough lanes are idle (or all are): each refill
-            // costs the wave one atomic and one ray-load round trip.
+            // costs the wave a ray-load round trip.
             if (idle && (idle >= refill_min || idle == 64u)) {
-                const uint32_t cnt = (uint32_t)__popcll(need);
-                const int leader = __ffsll((unsigned long long)need) - 1;
-                uint32_t base = 0;
-                if (first) {
-                    base = blockIdx.x * BLOCK + (threadIdx.x & ~63u);  // all 64 lanes are idle
-                } else {
-                    if ((int)lane == leader) base = atomicAdd(counter, cnt);
-                    base = static_total + __shfl(base, leader);
+                const uint32_t my = claim.claim(need, idle, lane, ray == kNone, counter);
+                if (my != kNone) {
+                    ray = my;
+                    ox = rays.ox[my];
+                    oy = rays.oy[my];
+                    oz = rays.oz[my];
+                    dx = rays.dx[my];
+                    dy = rays.dy[my];
+                    dz = rays.dz[my];
+                    tnear = rays.tnear[my];
+                    tfar = rays.tfar[my];
+                    ix = safe_inv(dx);
+                    iy = safe_inv(dy);
+                    iz = safe_inv(dz);
+                    far_lim = tfar * (1.0f - 1e-6f);
+                    near_lim = tnear * (1.0f + 1e-6f);
+                    tri_t = far_lim;
+                    sph_t = far_lim;
+                    tri_u = tri_v = 0.0f;
+                    tri_id = sph_id = LR_MISS;
+                    doomed = false;
+                    cur = rays.root ? rays.root[my] : 0u;
+                    st.sp = 0;
                 }
-                if (ray == kNone) {
-                    const uint32_t my = base + __builtin_amdgcn_mbcnt_hi(
-                                                   (uint32_t)(need >> 32),
-                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
-                    if (my < n) {
-                        ray = my;
-                        ox = rays.ox[my];
-                        oy = rays.oy[my];
-                        oz = rays.oz[my];
-                        dx = rays.dx[my];
-                        dy = rays.dy[my];
-                        dz = rays.dz[my];
-                        tnear = rays.tnear[my];
-                        tfar = rays.tfar[my];
-                        ix = safe_inv(dx);
-                        iy = safe_inv(dy);
-                        iz = safe_inv(dz);
-                        far_lim = tfar * (1.0f - 1e-6f);
-                        near_lim = tnear * (1.0f + 1e-6f);
-                        tri_t = far_lim;
-                        sph_t = far_lim;
-                        tri_u = tri_v = 0.0f;
-                        tri_id = sph_id = LR_MISS;
-                        doomed = false;
-                        cur = rays.root ? rays.root[my] : 0u;
-                        st.sp = 0;
-                    }
-                }
-                if (first) {
-                    if (static_total >= n) no_more = true;
-                    first = false;
-                } else if ((uint64_t)base + cnt >= n) {
-                    no_more = true;
-                }
+                no_more = claim.exhausted();
             }
         }
         if (__ballot(ray != kNone) == 0) break;
@@ -354,6 +333,8 @@ int prepare(lr_qbvh* bvh, uint32_t n, bool any, hipStream_t stream, Launch& L) {
 
 int launch(lr_qbvh* bvh, const Launch& L, const RaysArg& ra, uint32_t n, const HitsArg& ha, uint8_t* occ,
            hipStream_t s, const uint32_t* n_dev = nullptr) {
+    // the claim counter runs past n by at most 1024 per wave
+    if (n > kMaxRays) return lr::fail(LR_E_INVALID, "at most 4,278,190,080 rays per launch");
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, L.counter);
     hipLaunchKernelGGL(L.fn, dim3(L.grid), dim3(L.block), 0, s, bvh->d_nodes, bvh->d_prims, ra, n, ha, occ,
                        L.spill, L.cap, L.counter, L.refill_min, n_dev);

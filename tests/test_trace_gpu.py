@@ -114,6 +114,36 @@ def test_launch_configs_agree(gpu_ctx, block, lds):
         gpu_ctx.set_trace_config(**old)
 
 
+@pytest.mark.parametrize("refill", [8, 32, 64])
+def test_every_ray_traced_once_across_claim_boundaries(gpu_ctx, refill):
+    """A 512-wave grid (block 64, 2 blocks per CU): ray counts below one ray
+    per lane, at the edge of the static share, and with a dynamic remainder
+    handed out in shrinking chunks.  Hit buffers start as a sentinel, so a
+    ray that no lane traced shows up as a mismatch."""
+    import torch
+
+    v, t, s = scene(3000, 100, 12)
+    b = la.Qbvh(gpu_ctx, v, t, s)
+    old = gpu_ctx.trace_config()
+    try:
+        gpu_ctx.set_trace_config(block=64, lds_stack=8, blocks_per_cu=2, refill_min=refill)
+        for n in (1, 65, 32767, 32769, 3 * 32768 + 17, 200003):
+            say(f"[gpu] claim boundaries: refill {refill}, {n} rays")
+            r = la.random_rays(n, seed=n)
+            prim = torch.full((n,), -2, dtype=torch.int32, device="cuda")
+            tuv = torch.full((3, n), float("nan"), dtype=torch.float32, device="cuda")
+            b.trace_closest(to_dev(r), prim, tuv)
+            occ = torch.full((n,), 7, dtype=torch.uint8, device="cuda")
+            b.trace_any(to_dev(r), occ)
+            torch.cuda.synchronize()
+            tuv = tuv.cpu().numpy()
+            got = (prim.cpu().numpy().view(np.uint32), tuv[0], tuv[1], tuv[2])
+            assert_hits_equal(oracle.brute_closest(v, t, s, r), got, f"n={n}")
+            np.testing.assert_array_equal(oracle.brute_any(v, t, s, r), occ.cpu().numpy())
+    finally:
+        gpu_ctx.set_trace_config(**old)
+
+
 def test_autotuner_installs_fastest_config_and_keeps_hits(gpu_ctx):
     """lr_autotune_trace times all 9 kernels x 4 blocks_per_cu x 4 refill
     thresholds, installs the fastest, and traversal under the tuned
