@@ -88,7 +88,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_inst(const lr_node* __restrict_
     const float4* __restrict__ pf = reinterpret_cast<const float4*>(prims);
     const uint32_t lane = threadIdx.x & 63u;
     RayClaim claim;
-    claim.init(n, blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6), gridDim.x * (BLOCK / 64));
+    claim.init(n, blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6), gridDim.x * (BLOCK / 64), false);
 
     uint32_t ray = kNone;
     bool no_more = claim.exhausted();  // wave-uniform

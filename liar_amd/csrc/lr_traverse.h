@@ -83,18 +83,23 @@ __device__ __forceinline__ uint32_t min_u64_u32(uint64_t a, uint32_t b) { return
 // serialised over the whole GPU, so claiming only what one refill needs made
 // the counter the limit for short rays (a few node visits each).  All members
 // are wave-uniform; claim() is called by the whole wave.
+// With guided = false, the static share is one group of 64 per wave and each
+// refill claims exactly what it needs: for long, uneven rays (the instanced
+// traversal) that balances better and the atomics are rare anyway.
 struct RayClaim {
     uint32_t n, n_waves, wave, static_total;
+    bool guided;
     uint32_t lo, hi;    // the wave's reservation not yet handed out: [lo, hi)
     bool in_static;     // [lo, hi) counts the wave's static rays (else ray indices)
     uint32_t dyn_left;  // rays the counter had not handed out, as last seen
     bool dyn_done;      // the counter has nothing left
 
-    __device__ __forceinline__ void init(uint32_t n_rays, uint32_t w, uint32_t waves) {
+    __device__ __forceinline__ void init(uint32_t n_rays, uint32_t w, uint32_t waves, bool guided_share = true) {
         n = n_rays;
         n_waves = waves;
         wave = __builtin_amdgcn_readfirstlane(w);  // uniform: keep it (and what follows) in SGPRs
-        const uint32_t per = 64u * max(1u, n / (128u * waves));
+        guided = guided_share;
+        const uint32_t per = guided ? 64u * max(1u, n / (128u * waves)) : 64u;
         static_total = min_u64_u32((uint64_t)per * waves, n);
         in_static = true;
         lo = 0;
@@ -116,7 +121,7 @@ struct RayClaim {
         uint32_t nlo = 0, nhi = 0;
         if (have < idle && !dyn_done) {
             const uint32_t want = idle - have;
-            const uint32_t chunk = max(want, min(1024u, (dyn_left / (2u * n_waves)) & ~63u));
+            const uint32_t chunk = guided ? max(want, min(1024u, (dyn_left / (2u * n_waves)) & ~63u)) : want;
             const int leader = __ffsll((unsigned long long)need) - 1;
             uint32_t base = 0;
             if ((int)lane == leader) base = atomicAdd(counter, chunk);
