@@ -7,10 +7,14 @@ Workload: the random-triangle microbenchmark -- 1,000,000 triangles (centre
 tfar +inf, seed 3).  One step = one closest-hit launch over all rays, inputs
 resident in HBM.  Also reports any-hit (tfar ~ U[0.01, 2], seed 4).
 
-Prints one JSON line.  The roofline uses algorithmic bytes per ray
+Prints one JSON line.  The roofline uses the canonical algorithmic bytes
+per ray of the FP32 4-wide QBVH
   B_ray = 32 (ray) + 16 (hit) + 128 * nodes/ray + 48 * prims/ray
 with nodes/ray and prims/ray counted by the CPU oracle's traversal of the
-same QBVH on a sample of the same rays.  The CPU baseline is that oracle
+same QBVH with FP32 boxes on a sample of the same rays.  The GPU fetches
+64 B quantised nodes (lr_node_q8); the bytes of that traversal (the
+oracle's count over the decoded quantised boxes, 64 B per node) are
+reported beside it as device_bytes_per_ray_*.  The CPU baseline is that oracle
 traversal (a multi-threaded C restatement of LiAR's QBVH query) timed on the
 host cores on a bounded sample.
 """
@@ -28,6 +32,8 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s closest-hit at 1 GPU; time-to-equal-MSE guided vs plain GPU tracer"
 HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 128  # canonical: one 4-wide FP32 SoA node
+DEVICE_NODE_BYTES = 64  # lr_node_q8, what the GPU fetches
 N_TRIS = 1_000_000
 N_RAYS = 1 << 22
 
@@ -657,6 +663,10 @@ def main() -> int:
     ref = oracle.qbvh_closest(nodes, prims, rays_h[:, :S])
     t_count = time.perf_counter() - t0
     ref_any = oracle.qbvh_any(nodes, prims, rays_any_h[:, :S])
+    # node / primitive counts of the traversal the GPU runs: the quantised boxes
+    dec = oracle.decode_q8(bvh.export_q8())
+    cnt = oracle.qbvh_closest(dec, prims, rays_h[:, :S])
+    cnt_any = oracle.qbvh_any(dec, prims, rays_any_h[:, :S])
     g_prim = prim[:S].cpu().numpy().view(np.uint32)
     g_tuv = tuv[:, :S].cpu().numpy()
     mism = int((g_prim != ref[0]).sum()) + sum(
@@ -664,10 +674,12 @@ def main() -> int:
     mism_any = int((occ[:S].cpu().numpy() != ref_any[0]).sum())
     nodes_per_ray = ref[4] / S
     prims_per_ray = ref[5] / S
-    b_ray = 32 + 16 + 128 * nodes_per_ray + 48 * prims_per_ray
+    b_ray = 32 + 16 + NODE_BYTES * nodes_per_ray + 48 * prims_per_ray
     any_nodes = ref_any[1] / S
     any_prims = ref_any[2] / S
-    b_ray_any = 32 + 1 + 128 * any_nodes + 48 * any_prims
+    b_ray_any = 32 + 1 + NODE_BYTES * any_nodes + 48 * any_prims
+    dev_b_ray = 32 + 16 + DEVICE_NODE_BYTES * cnt[4] / S + 48 * cnt[5] / S
+    dev_b_ray_any = 32 + 1 + DEVICE_NODE_BYTES * cnt_any[1] / S + 48 * cnt_any[2] / S
     achieved = N_RAYS * b_ray / (kern_c * 1e-3) / 1e9
     achieved_any = N_RAYS * b_ray_any / (kern_a * 1e-3) / 1e9
 
@@ -733,6 +745,12 @@ def main() -> int:
             "bytes_per_ray_closest": round(b_ray, 1),
             "nodes_per_ray_closest": round(nodes_per_ray, 3),
             "prims_per_ray_closest": round(prims_per_ray, 3),
+            "device_node_bytes": DEVICE_NODE_BYTES,
+            "device_nodes_per_ray_closest": round(cnt[4] / S, 3),
+            "device_prims_per_ray_closest": round(cnt[5] / S, 3),
+            "device_bytes_per_ray_closest": round(dev_b_ray, 1),
+            "device_gbs_closest": round(N_RAYS * dev_b_ray / (kern_c * 1e-3) / 1e9, 1),
+            "device_bytes_per_ray_any": round(dev_b_ray_any, 1),
             "any_hit_mrays_s": round(mrays_any, 3),
             "kernel_ms_any": round(kern_a, 4),
             "bytes_per_ray_any": round(b_ray_any, 1),

@@ -73,6 +73,25 @@ typedef struct lr_node {
     uint32_t reserved[4];
 } lr_node;
 
+/* The device traversal's node, 64 B: lr_node with each child plane stored
+ * as 8 bits on a per-node, per-axis grid.  A plane decodes as
+ *   fmaf((float)q, scale[axis], origin[axis])
+ * (scale a power of two, so q * scale is exact and the decode has one
+ * rounding, the same on the host and the GPU).  lo planes are rounded down
+ * and hi planes up, so every decoded box contains the lr_node box: box tests
+ * stay conservative and the hits of a traversal are unchanged, while a node
+ * fetch is half an L2 line.  No counterpart in LiAR, whose TriangleMesh
+ * keeps a Lass QBVH with FP32 boxes (src/liar/scenery/triangle_mesh.cpp:69-85);
+ * the library derives it from lr_node. */
+typedef struct lr_node_q8 {
+    float origin[3];  /* per axis: the smallest lo of the node's children */
+    float scale[3];   /* per axis: grid step, a power of two */
+    uint8_t qlo_x[4], qhi_x[4];
+    uint8_t qlo_y[4], qhi_y[4];
+    uint8_t qlo_z[4], qhi_z[4];
+    uint32_t child[4]; /* as lr_node.child */
+} lr_node_q8;
+
 /* Primitive record, 48 B, stored in leaf order.
  *   triangle: a = (v0, id), b = (v1 - v0, kind=0), c = (v2 - v0, 0)
  *   sphere:   a = (centre, id), b = (radius, radius^2, 0, kind=1), c = 0
@@ -144,6 +163,10 @@ int lr_qbvh_get_info(const lr_qbvh* bvh, lr_qbvh_info* info);
 /* Copies the flattened arrays back to host memory (nodes: n_nodes records,
  * prims: n_prims records); either pointer may be NULL. */
 int lr_qbvh_export(const lr_qbvh* bvh, lr_node* nodes, lr_prim* prims);
+/* Copies the quantised nodes the GPU traverses (n_nodes records, the same
+ * indexing and child encoding as lr_qbvh_export's nodes).  No counterpart in
+ * LiAR: it exposes the device format so that it can be checked. */
+int lr_qbvh_export_q8(const lr_qbvh* bvh, lr_node_q8* nodes);
 
 /* Closest hit for n rays.  Replaces SceneObject::intersect over the scene
  * (src/liar/kernel/scene_object.h:110-113) and its TriangleMesh / Sphere /

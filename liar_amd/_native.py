@@ -14,6 +14,7 @@ LR_OK = 0
 LR_MISS = 0xFFFFFFFF
 LR_CHILD_EMPTY = 0xFFFFFFFF
 NODE_BYTES = 128
+NODE_Q8_BYTES = 64  # lr_node_q8, the device traversal's node
 PRIM_BYTES = 48
 
 
@@ -162,6 +163,7 @@ SIGNATURES = {
     "lr_qbvh_destroy": (_I, [_P]),
     "lr_qbvh_get_info": (_I, [_P, C.POINTER(QbvhInfo)]),
     "lr_qbvh_export": (_I, [_P, _P, _P]),
+    "lr_qbvh_export_q8": (_I, [_P, _P]),
     "lr_trace_closest": (_I, [_P, C.POINTER(RaysSoa), _U32, C.POINTER(HitsSoa), _P]),
     "lr_trace_any": (_I, [_P, C.POINTER(RaysSoa), _U32, _P, _P]),
     "lr_set_trace_config": (_I, [_P, C.POINTER(TraceConfig)]),

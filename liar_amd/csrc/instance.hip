@@ -45,9 +45,9 @@ static_assert(sizeof(InstRec) == 64, "instance record is one 64 B line");
 
 struct lr_tlas {
     lr_context* ctx = nullptr;
-    lr_node* d_tnodes = nullptr;
+    lr_node_q8* d_tnodes = nullptr;
     lr::inst::InstRec* d_inst = nullptr;
-    lr_node* d_nodes = nullptr;
+    lr_node_q8* d_nodes = nullptr;
     lr_prim* d_prims = nullptr;
     lr_tlas_info info{};
     uint32_t stack_cap = 0;
@@ -65,9 +65,9 @@ constexpr int kBlock = 256;
 constexpr int kShort = 16;
 
 template <int BLOCK, int SHORT, bool ANY>
-__global__ __launch_bounds__(BLOCK) void k_trace_inst(const lr_node* __restrict__ tnodes,
+__global__ __launch_bounds__(BLOCK) void k_trace_inst(const lr_node_q8* __restrict__ tnodes,
                                                       const InstRec* __restrict__ insts,
-                                                      const lr_node* __restrict__ nodes,
+                                                      const lr_node_q8* __restrict__ nodes,
                                                       const lr_prim* __restrict__ prims, RaysArg rays, uint32_t n,
                                                       HitsArg hits, uint32_t* __restrict__ hit_inst,
                                                       uint8_t* __restrict__ occluded, uint2* __restrict__ spill,
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_inst(const lr_node* __restrict_
         if (!(cur & 0x80000000u)) {
             float t[4];
             uint32_t e[4];
-            node_test((local ? nf : tnf) + (size_t)cur * 8, ox, oy, oz, ix, iy, iz, nl, cull_now(), t, e);
+            node_test((local ? nf : tnf) + (size_t)cur * kNodeF4, ox, oy, oz, ix, iy, iz, nl, cull_now(), t, e);
             LR_CSWAP(0, 1);
             LR_CSWAP(2, 3);
             LR_CSWAP(0, 2);
@@ -548,15 +548,16 @@ int lr_tlas_create(lr_context* ctx, lr_qbvh* const* geometry, uint32_t n_geometr
     }
     hipError_t e = hipSuccess;
     if (!top.nodes.empty()) {
-        e = hipMalloc(&T->d_tnodes, sizeof(lr_node) * top.nodes.size());
+        const std::vector<lr_node_q8> tq = lr::quantize_nodes(top.nodes), gq = lr::quantize_nodes(hn);
+        e = hipMalloc(&T->d_tnodes, sizeof(lr_node_q8) * tq.size());
         if (e == hipSuccess) e = hipMalloc(&T->d_inst, sizeof(InstRec) * leaf_recs.size());
-        if (e == hipSuccess) e = hipMalloc(&T->d_nodes, sizeof(lr_node) * hn.size());
+        if (e == hipSuccess) e = hipMalloc(&T->d_nodes, sizeof(lr_node_q8) * gq.size());
         if (e == hipSuccess) e = hipMalloc(&T->d_prims, sizeof(lr_prim) * hp.size());
         if (e == hipSuccess)
-            e = hipMemcpy(T->d_tnodes, top.nodes.data(), sizeof(lr_node) * top.nodes.size(), hipMemcpyHostToDevice);
+            e = hipMemcpy(T->d_tnodes, tq.data(), sizeof(lr_node_q8) * tq.size(), hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(T->d_inst, leaf_recs.data(), sizeof(InstRec) * leaf_recs.size(), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(T->d_nodes, hn.data(), sizeof(lr_node) * hn.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(T->d_nodes, gq.data(), sizeof(lr_node_q8) * gq.size(), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(T->d_prims, hp.data(), sizeof(lr_prim) * hp.size(), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {

@@ -26,4 +26,10 @@ int build_qbvh(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t n
 // reference box slots in leaf order) and order[slot] = input box index.
 int build_boxes(const float* boxes, uint32_t n, BuiltQbvh& out, std::vector<uint32_t>& order, std::string& err);
 
+// The device node format: each node's child boxes quantised to 8 bits per
+// plane, rounded outward (lr_node_q8).  Indices and child encodings are
+// copied, so this applies to any node array, concatenated ones included.
+void quantize_nodes(const lr_node* in, size_t n, lr_node_q8* out);
+std::vector<lr_node_q8> quantize_nodes(const std::vector<lr_node>& in);
+
 }  // namespace lr

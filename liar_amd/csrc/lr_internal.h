@@ -38,9 +38,10 @@ struct lr_context {
 
 struct lr_qbvh {
     lr_context* ctx = nullptr;
-    lr_node* d_nodes = nullptr;
+    lr_node_q8* d_nodes = nullptr;  // the device traversal's nodes (quantised boxes)
     lr_prim* d_prims = nullptr;
     lr::BuiltQbvh host;
+    std::vector<lr_node_q8> host_q8;  // host copy of d_nodes
     lr_qbvh_info info{};
 };
 

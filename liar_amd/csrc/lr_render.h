@@ -50,7 +50,7 @@ struct DevObserver {
 };
 
 struct DevScene {
-    const lr_node* nodes;
+    const lr_node_q8* nodes;
     const lr_prim* prims;
     const uint32_t* prim_mat;
     const DevMaterial* mats;

@@ -32,28 +32,37 @@ __device__ __forceinline__ float safe_inv(float d) {
     return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
 }
 
-// Slab test of the four children of a node; returns per-child entry distance
-// (+inf when culled).
+// Slab test of the four children of a quantised node (lr_node_q8, 64 B =
+// four dwordx4 loads); returns per-child entry distance (+inf when culled).
+// Each plane decodes as fmaf(q, scale, origin), bit for bit the builder's
+// value, then the FP32 slab test runs on those planes.
 __device__ __forceinline__ void node_test(const float4* __restrict__ np, float ox, float oy, float oz, float ix,
                                           float iy, float iz, float tnear, float cull, float t[4], uint32_t e[4]) {
-    const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-    const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    const float4 g0 = np[0], g1 = np[1];
+    const uint4 q = reinterpret_cast<const uint4*>(np)[2];
+    const uint4 ch = reinterpret_cast<const uint4*>(np)[3];
+    const float orx = g0.x, ory = g0.y, orz = g0.z, sx = g0.w, sy = g1.x, sz = g1.y;
+    const uint32_t qlx = __float_as_uint(g1.z), qhx = __float_as_uint(g1.w);
     const uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float ax = (lx[i] - ox) * ix, bx = (hx[i] - ox) * ix;
-        const float ay = (ly[i] - oy) * iy, by = (hy[i] - oy) * iy;
-        const float az = (lz[i] - oz) * iz, bz = (hz[i] - oz) * iz;
+        const int sh = 8 * i;
+        const float lx = fmaf((float)((qlx >> sh) & 255u), sx, orx), hx = fmaf((float)((qhx >> sh) & 255u), sx, orx);
+        const float ly = fmaf((float)((q.x >> sh) & 255u), sy, ory), hy = fmaf((float)((q.y >> sh) & 255u), sy, ory);
+        const float lz = fmaf((float)((q.z >> sh) & 255u), sz, orz), hz = fmaf((float)((q.w >> sh) & 255u), sz, orz);
+        const float ax = (lx - ox) * ix, bx = (hx - ox) * ix;
+        const float ay = (ly - oy) * iy, by = (hy - oy) * iy;
+        const float az = (lz - oz) * iz, bz = (hz - oz) * iz;
         const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tnear));
-        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)) , fmaxf(az, bz)) * kIze;
+        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)) * kIze;
         const bool hit = tn <= fminf(tf, cull) && c[i] != LR_CHILD_EMPTY;
         t[i] = hit ? tn : INFINITY;
         e[i] = c[i];
     }
 }
+// Node stride in float4 units.
+constexpr size_t kNodeF4 = sizeof(lr_node_q8) / 16;
+static_assert(sizeof(lr_node_q8) == 64, "lr_node_q8 is 64 B");
 
 #define LR_CSWAP(a, b)                     \
     {                                      \

@@ -408,6 +408,68 @@ int build_boxes(const float* boxes, uint32_t n, BuiltQbvh& out, std::vector<uint
     return 0;
 }
 
+// Per node and axis: origin = the smallest lo of the valid children, step
+// s = the smallest power of two with fmaf(255, s, origin) >= the largest hi.
+// A lo plane takes the largest q whose decode fmaf(q, s, origin) is <= the
+// FP32 plane (q = 0 decodes to origin itself), a hi plane the smallest q whose
+// decode is >= it (q = 255 always is).  q * s is exact, so std::fmaf here and
+// the kernel's fmaf give the same floats.
+void quantize_nodes(const lr_node* in, size_t n, lr_node_q8* out) {
+    for (size_t k = 0; k < n; ++k) {
+        const lr_node& a = in[k];
+        lr_node_q8& o = out[k];
+        std::memset(&o, 0, sizeof(o));
+        const float* lo[3] = {a.lo_x, a.lo_y, a.lo_z};
+        const float* hi[3] = {a.hi_x, a.hi_y, a.hi_z};
+        uint8_t* qlo[3] = {o.qlo_x, o.qlo_y, o.qlo_z};
+        uint8_t* qhi[3] = {o.qhi_x, o.qhi_y, o.qhi_z};
+        for (int i = 0; i < 4; ++i) o.child[i] = a.child[i];
+        for (int ax = 0; ax < 3; ++ax) {
+            float org = INFINITY, top = -INFINITY;
+            for (int i = 0; i < 4; ++i) {
+                if (a.child[i] == LR_CHILD_EMPTY) continue;
+                org = std::min(org, lo[ax][i]);
+                top = std::max(top, hi[ax][i]);
+            }
+            if (!(org <= top)) {  // no child: any grid (the kernel skips empty children)
+                org = 0.0f;
+                top = 0.0f;
+            }
+            const double ext = (double)top - (double)org;
+            int e = !(ext < 1e38) ? 127 : ext > 0.0 ? (int)std::ceil(std::log2(ext / 255.0)) : -126;
+            e = std::max(-126, std::min(127, e));
+            while (e < 127 && std::fmaf(255.0f, std::ldexp(1.0f, e), org) < top) ++e;
+            const float sc = std::ldexp(1.0f, e);
+            o.origin[ax] = org;
+            o.scale[ax] = sc;
+            auto dec = [&](int q) { return std::fmaf((float)q, sc, org); };
+            for (int i = 0; i < 4; ++i) {
+                if (a.child[i] == LR_CHILD_EMPTY) {
+                    qlo[ax][i] = 255;
+                    qhi[ax][i] = 0;
+                    continue;
+                }
+                int q = (int)std::floor(((double)lo[ax][i] - org) / sc);
+                q = std::max(0, std::min(255, q));
+                while (q > 0 && dec(q) > lo[ax][i]) --q;
+                while (q < 255 && dec(q + 1) <= lo[ax][i]) ++q;
+                qlo[ax][i] = (uint8_t)q;
+                q = (int)std::ceil(((double)hi[ax][i] - org) / sc);
+                q = std::max(0, std::min(255, q));
+                while (q < 255 && dec(q) < hi[ax][i]) ++q;
+                while (q > 0 && dec(q - 1) >= hi[ax][i]) --q;
+                qhi[ax][i] = (uint8_t)q;
+            }
+        }
+    }
+}
+
+std::vector<lr_node_q8> quantize_nodes(const std::vector<lr_node>& in) {
+    std::vector<lr_node_q8> out(in.size());
+    if (!in.empty()) quantize_nodes(in.data(), in.size(), out.data());
+    return out;
+}
+
 namespace {
 
 void build_nodes(std::vector<Ref>& refs, const Box& scene, BuiltQbvh& out) {

@@ -1496,10 +1496,11 @@ int lr_batch_create(lr_context* ctx, lr_scene* const* scenes, uint32_t n, lr_bat
     T.info.n_prims = (uint32_t)hp.size();
     T.info.max_depth = max_depth;
     T.info.max_stack = max_stack;
-    hipError_t e = hipMalloc(&T.d_nodes, sizeof(lr_node) * hn.size());
+    const std::vector<lr_node_q8> hq = lr::quantize_nodes(hn);
+    hipError_t e = hipMalloc(&T.d_nodes, sizeof(lr_node_q8) * hq.size());
     if (e == hipSuccess) e = hipMalloc(&T.d_prims, sizeof(lr_prim) * hp.size());
     if (e == hipSuccess) e = hipMalloc(&b->d_roots, sizeof(uint32_t) * n);
-    if (e == hipSuccess) e = hipMemcpy(T.d_nodes, hn.data(), sizeof(lr_node) * hn.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(T.d_nodes, hq.data(), sizeof(lr_node_q8) * hq.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(T.d_prims, hp.data(), sizeof(lr_prim) * hp.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_roots, b->roots.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&b->d_scenes, sizeof(DevScene) * n);

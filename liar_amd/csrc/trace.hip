@@ -32,7 +32,7 @@ using namespace lr::trav;
 // share, then chunks claimed from a global counter: RayClaim), so no lane
 // idles while rays remain.
 template <int BLOCK, int SHORT, bool ANY>
-__global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nodes,
+__global__ __launch_bounds__(BLOCK) void k_trace(const lr_node_q8* __restrict__ nodes,
                                                  const lr_prim* __restrict__ prims, RaysArg rays, uint32_t n,
                                                  HitsArg hits, uint8_t* __restrict__ occluded,
                                                  uint2* __restrict__ spill, int cap,
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node* __restrict__ nod
         if (!(cur & 0x80000000u)) {
             float t[4];
             uint32_t e[4];
-            node_test(nf + (size_t)cur * 8, ox, oy, oz, ix, iy, iz, tnear, cull, t, e);
+            node_test(nf + (size_t)cur * kNodeF4, ox, oy, oz, ix, iy, iz, tnear, cull, t, e);
             LR_CSWAP(0, 1);
             LR_CSWAP(2, 3);
             LR_CSWAP(0, 2);
@@ -248,7 +248,7 @@ __global__ void k_fill_u8(uint32_t n, uint8_t* p, uint8_t v) {
 
 __global__ void k_zero_u32(uint32_t* p) { p[threadIdx.x] = 0u; }
 
-using TraceFn = void (*)(const lr_node*, const lr_prim*, RaysArg, uint32_t, HitsArg, uint8_t*, uint2*, int,
+using TraceFn = void (*)(const lr_node_q8*, const lr_prim*, RaysArg, uint32_t, HitsArg, uint8_t*, uint2*, int,
                          uint32_t*, uint32_t, const uint32_t*);
 
 struct KernelEntry {
@@ -585,12 +585,13 @@ int lr_qbvh_create(lr_context* ctx, const float* verts, uint32_t nv, const uint3
         I.scene_lo[k] = b->host.scene_lo[k];
         I.scene_hi[k] = b->host.scene_hi[k];
     }
+    b->host_q8 = lr::quantize_nodes(b->host.nodes);
     int rc = ctx ? lr::bind(ctx) : LR_OK;
     if (!rc && ctx && I.n_nodes) {
-        hipError_t e = hipMalloc(&b->d_nodes, sizeof(lr_node) * I.n_nodes);
+        hipError_t e = hipMalloc(&b->d_nodes, sizeof(lr_node_q8) * I.n_nodes);
         if (e == hipSuccess) e = hipMalloc(&b->d_prims, sizeof(lr_prim) * I.n_prims);
         if (e == hipSuccess)
-            e = hipMemcpy(b->d_nodes, b->host.nodes.data(), sizeof(lr_node) * I.n_nodes, hipMemcpyHostToDevice);
+            e = hipMemcpy(b->d_nodes, b->host_q8.data(), sizeof(lr_node_q8) * I.n_nodes, hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(b->d_prims, b->host.prims.data(), sizeof(lr_prim) * I.n_prims, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = lr::fail(LR_E_DEVICE, std::string("lr_qbvh_create upload: ") + hipGetErrorString(e));
@@ -624,6 +625,12 @@ int lr_qbvh_export(const lr_qbvh* b, lr_node* nodes, lr_prim* prims) {
         std::memcpy(nodes, b->host.nodes.data(), sizeof(lr_node) * b->host.nodes.size());
     if (prims && !b->host.prims.empty())
         std::memcpy(prims, b->host.prims.data(), sizeof(lr_prim) * b->host.prims.size());
+    return LR_OK;
+}
+
+int lr_qbvh_export_q8(const lr_qbvh* b, lr_node_q8* nodes) {
+    if (!b) return lr::fail(LR_E_INVALID, "NULL qbvh");
+    if (nodes && !b->host_q8.empty()) std::memcpy(nodes, b->host_q8.data(), sizeof(lr_node_q8) * b->host_q8.size());
     return LR_OK;
 }
 

@@ -133,6 +133,12 @@ class Qbvh:
         N.check(N.lib().lr_qbvh_export(self.handle, N.ptr(nodes), N.ptr(prims)), "lr_qbvh_export")
         return nodes, prims
 
+    def export_q8(self):
+        """uint8 (n_nodes, 64): the quantised nodes the GPU traverses (lr_node_q8)."""
+        q = np.zeros((self.info()["n_nodes"], N.NODE_Q8_BYTES), dtype=np.uint8)
+        N.check(N.lib().lr_qbvh_export_q8(self.handle, N.ptr(q)), "lr_qbvh_export_q8")
+        return q
+
     # -- device traces (torch CUDA tensors) ---------------------------------
     @staticmethod
     def _rays_struct(rays) -> N.RaysSoa:

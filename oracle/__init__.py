@@ -179,6 +179,29 @@ def qbvh_any(nodes, prims, rays, threads: int | None = None):
     return occ, nv.value, npr.value
 
 
+def decode_q8(q8):
+    """lr_node_q8 records (uint8 (n, 64)) -> lr_node records (uint8 (n, 128)).
+
+    Every plane is fmaf(q, scale, origin).  scale is a power of two and
+    q < 256, so the FP32 product q * scale is exact and one FP32 addition
+    gives the fused value bit for bit; the child words are copied.  The
+    oracle's QBVH traversal over the decoded nodes therefore makes exactly the
+    box tests of the GPU traversal over the quantised ones.  (The quantised
+    format is the library's device format; LiAR's QBVH keeps FP32 boxes.)"""
+    q8 = np.ascontiguousarray(q8, dtype=np.uint8)
+    n = q8.shape[0]
+    hdr = q8[:, :24].copy().view(np.float32)  # origin x, y, z, scale x, y, z
+    q = q8[:, 24:48].reshape(n, 6, 4).astype(np.float32)  # qlo_x, qhi_x, qlo_y, qhi_y, qlo_z, qhi_z
+    planes = np.empty((n, 6, 4), np.float32)
+    for k in range(6):
+        ax = k // 2
+        planes[:, k, :] = hdr[:, ax:ax + 1] + q[:, k, :] * hdr[:, 3 + ax:4 + ax]
+    out = np.zeros((n, 128), np.uint8)
+    out[:, :96] = planes.reshape(n, 24).view(np.uint8)
+    out[:, 96:112] = q8[:, 48:64]
+    return out
+
+
 def qbvh_check(nodes, prims, verts, tris, spheres=None):
     """Structural invariants of a flattened QBVH; returns (ok, message, max_depth)."""
     verts, tris, spheres = _scene(verts, tris, spheres)
