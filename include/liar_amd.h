@@ -237,11 +237,19 @@ int lr_trace_any_instanced(lr_tlas* tlas, const lr_rays_soa* rays, uint32_t n, u
  *                  entries spill to global memory
  *   refill_min     a wave fetches new rays once at least this many of its 64
  *                  lanes are idle (1..64) */
+/* schedule: LR_TRACE_SCHEDULE_STEP -- each lane visits one node or one leaf
+ * per iteration (the two code paths share the iteration); PHASED -- the
+ * wave alternates node and leaf iterations, a lane postponing one leaf and
+ * visiting nodes on until every lane has one (fuller waves for short rays,
+ * more registers).  In lr_set_trace_config a 0 field keeps its value. */
+#define LR_TRACE_SCHEDULE_STEP 1u
+#define LR_TRACE_SCHEDULE_PHASED 2u
 typedef struct lr_trace_config {
     uint32_t block;
     uint32_t blocks_per_cu;
     uint32_t lds_stack;
     uint32_t refill_min;
+    uint32_t schedule;
 } lr_trace_config;
 int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg);
 int lr_get_trace_config(const lr_context* ctx, lr_trace_config* cfg);
@@ -249,7 +257,7 @@ int lr_get_trace_config(const lr_context* ctx, lr_trace_config* cfg);
 /* Traversal autotuner (no counterpart in LiAR, whose CPU tracer has no launch
  * parameters).  Times the caller's own batch under every instantiated
  * (block, lds_stack) kernel x blocks_per_cu in {2, 4, 8, 16} x refill_min in
- * {8, 16, 32, 48}: one warm-up launch, then `repeats` (0: 3) launches between
+ * {8, 16, 32, 48} x both schedules (288 configurations): one warm-up launch, then `repeats` (0: 3) launches between
  * HIP events on `stream`.  Tunes closest hit when hits is non-NULL, else any
  * hit into occluded.  The fastest configuration becomes the context's
  * (lr_set_trace_config) and is returned in *best; entries (optional, up to

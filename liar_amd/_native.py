@@ -47,7 +47,7 @@ class QbvhInfo(C.Structure):
 
 class TraceConfig(C.Structure):
     _fields_ = [("block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_stack", C.c_uint32),
-                ("refill_min", C.c_uint32)]
+                ("refill_min", C.c_uint32), ("schedule", C.c_uint32)]
 
 
 class TuneEntry(C.Structure):

@@ -19,12 +19,13 @@ BLOCKS = (128, 256)
 LDS_STACKS = (8, 16, 24)
 BLOCKS_PER_CU = (4, 8, 16)
 REFILL = (16, 32, 48)
+SCHEDULES = (1, 2)  # LR_TRACE_SCHEDULE_STEP, LR_TRACE_SCHEDULE_PHASED
 
 
 def tune_render(gs: "E.GpuScene", camera, params, repeats: int = 2, configs=None) -> dict:
     """Times lr_render(gs, camera, params) under every (block, lds_stack,
-    blocks_per_cu, refill_min) in ``configs`` (default: the product of the
-    module's value lists, 54 configurations) and sets the fastest as the
+    blocks_per_cu, refill_min, schedule) in ``configs`` (default: the product
+    of the module's value lists, 108 configurations) and sets the fastest as the
     context's traversal configuration.  Returns {"best", "best_ms",
     "default_ms", "table"}; the film written meanwhile is scratch."""
     import torch
@@ -32,8 +33,8 @@ def tune_render(gs: "E.GpuScene", camera, params, repeats: int = 2, configs=None
     ctx = gs.ctx
     start = ctx.trace_config()
     if configs is None:
-        configs = [dict(block=b, lds_stack=l, blocks_per_cu=c, refill_min=r)
-                   for b, l, c, r in itertools.product(BLOCKS, LDS_STACKS, BLOCKS_PER_CU, REFILL)]
+        configs = [dict(block=b, lds_stack=l, blocks_per_cu=c, refill_min=r, schedule=sc)
+                   for b, l, c, r, sc in itertools.product(BLOCKS, LDS_STACKS, BLOCKS_PER_CU, REFILL, SCHEDULES)]
     w, h = params.width, params.height
     fx = torch.zeros((3, h, w), dtype=torch.float32, device="cuda")
     fw = torch.zeros((h, w), dtype=torch.float32, device="cuda")

@@ -22,7 +22,7 @@ struct lr_scratch {
 struct lr_context {
     int device = 0;
     int n_cu = 0;
-    lr_trace_config cfg{256, 8, 8, 32};
+    lr_trace_config cfg{256, 8, 8, 32, LR_TRACE_SCHEDULE_STEP};
     std::mutex mu;
     std::unordered_map<hipStream_t, lr_scratch> spill;
     // resident workgroups per CU of each traversal kernel at its block size

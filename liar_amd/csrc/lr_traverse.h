@@ -60,6 +60,12 @@ __device__ __forceinline__ void node_test(const float4* __restrict__ np, float o
         e[i] = c[i];
     }
 }
+// Marks a loaded value as used at this point (an empty asm statement, no
+// instruction).  The loads of a leaf's records are all issued above it and
+// awaited once, instead of the compiler sinking each record's load into the
+// branch that reads it, which serialises the fetches.
+__device__ __forceinline__ void keep(const float4& v) { asm volatile("" : : "v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w)); }
+
 // Node stride in float4 units.
 constexpr size_t kNodeF4 = sizeof(lr_node_q8) / 16;
 static_assert(sizeof(lr_node_q8) == 64, "lr_node_q8 is 64 B");

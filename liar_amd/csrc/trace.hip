@@ -31,7 +31,7 @@ using namespace lr::trav;
 // finished take the next ray indices from their wave's reservation (a static
 // share, then chunks claimed from a global counter: RayClaim), so no lane
 // idles while rays remain.
-template <int BLOCK, int SHORT, bool ANY>
+template <int BLOCK, int SHORT, bool ANY, bool PHASED>
 __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node_q8* __restrict__ nodes,
                                                  const lr_prim* __restrict__ prims, RaysArg rays, uint32_t n,
                                                  HitsArg hits, uint8_t* __restrict__ occluded,
@@ -64,6 +64,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node_q8* __restrict__ 
     float tri_t = 0, tri_u = 0, tri_v = 0, sph_t = 0;
     uint32_t tri_id = LR_MISS, sph_id = LR_MISS, cur = 0;
     bool doomed = false;
+    // cur: the next entry to visit (inner node, leaf, or kEmpty); leaf: the
+    // postponed leaf (or kEmpty); cur_t / leaf_t: their entry distances.
+    constexpr uint32_t kEmpty = LR_CHILD_EMPTY;
+    uint32_t leaf = kEmpty;
+    float cur_t = 0.0f, leaf_t = 0.0f;
 
     for (;;) {
         if (!no_more) {
@@ -94,19 +99,34 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node_q8* __restrict__ 
                     tri_id = sph_id = LR_MISS;
                     doomed = false;
                     cur = rays.root ? rays.root[my] : 0u;
+                    cur_t = tnear;
+                    leaf = kEmpty;
                     st.sp = 0;
                 }
                 no_more = claim.exhausted();
             }
         }
         if (__ballot(ray != kNone) == 0) break;
+        // STEP: each lane visits its node or its leaf every iteration (the two
+        // code paths share the iteration).  PHASED: the phase of an iteration
+        // is the same for the whole wave -- node steps while any lane is still
+        // searching for a leaf (no leaf postponed and an inner node to visit),
+        // else leaf steps; a lane that reaches a leaf postpones it and visits
+        // nodes on (speculatively) until every lane has one, so each phase runs
+        // with most lanes active.
+        bool leaf_phase = false;
+        if constexpr (PHASED) {
+            const bool searching = ray != kNone && leaf == kEmpty && !(cur & 0x80000000u);
+            leaf_phase = __ballot(searching) == 0;
+        }
         if (ray == kNone) continue;
 
         bool finished = false;
-        bool need_pop = true;
         bool any_hit = false;
         const float cull = ANY ? tfar * kCullSlack : fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
-        if (!(cur & 0x80000000u)) {
+        // Visits inner node `cur`: pushes the far children, returns whether any
+        // child was hit (then cur = the nearest, its entry distance in t0).
+        auto visit_node = [&](float& t0) -> bool {
             float t[4];
             uint32_t e[4];
             node_test(nf + (size_t)cur * kNodeF4, ox, oy, oz, ix, iy, iz, tnear, cull, t, e);
@@ -115,21 +135,22 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node_q8* __restrict__ 
             LR_CSWAP(0, 2);
             LR_CSWAP(1, 3);
             LR_CSWAP(1, 2);
-            if (t[0] != INFINITY) {
-                if (t[3] != INFINITY) st.push(e[3], t[3]);
-                if (t[2] != INFINITY) st.push(e[2], t[2]);
-                if (t[1] != INFINITY) st.push(e[1], t[1]);
-                cur = e[0];
-                need_pop = false;
-            }
-        } else {
-            const uint32_t first = (cur >> 2) & 0x1FFFFFFFu;
-            const uint32_t cnt = (cur & 3u) + 1u;
+            if (t[0] == INFINITY) return false;
+            if (t[3] != INFINITY) st.push(e[3], t[3]);
+            if (t[2] != INFINITY) st.push(e[2], t[2]);
+            if (t[1] != INFINITY) st.push(e[1], t[1]);
+            cur = e[0];
+            t0 = t[0];
+            return true;
+        };
+        // Tests the primitives of leaf entry `lf`.  The records are requested
+        // before they are tested (one memory round trip per group instead of
+        // two per primitive): any-hit asks for all four at once; closest-hit,
+        // which keeps more state live, two at a time.
+        auto visit_leaf = [&](uint32_t lf) {
+            const uint32_t first = (lf >> 2) & 0x1FFFFFFFu;
+            const uint32_t cnt = (lf & 3u) + 1u;
             const float4* q = pf + (size_t)first * 3;
-            // The leaf's records are requested before they are tested (one
-            // memory round trip per group instead of two per primitive):
-            // any-hit asks for all four at once; closest-hit, which keeps more
-            // state live, two at a time so that it keeps 6 waves per SIMD.
             auto test = [&](const float4 a, const float4 b, const float4 c) {
                 const uint32_t id = __float_as_uint(a.w);
                 if (__float_as_uint(b.w) == LR_PRIM_TRIANGLE) {
@@ -159,51 +180,94 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const lr_node_q8* __restrict__ 
                     }
                 }
             };
+            // Slots past the leaf's count re-read its last record (same
+            // address, no extra memory traffic) so that every load is
+            // unconditional.
             if constexpr (ANY) {
                 float4 ra[4], rb[4], rc[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    if ((uint32_t)k < cnt) {
-                        ra[k] = q[3 * k];
-                        rb[k] = q[3 * k + 1];
-                        rc[k] = q[3 * k + 2];
-                    }
+                    const uint32_t r = 3u * min((uint32_t)k, cnt - 1u);
+                    ra[k] = q[r];
+                    rb[k] = q[r + 1];
+                    rc[k] = q[r + 2];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    keep(ra[k]);
+                    keep(rb[k]);
+                    keep(rc[k]);
                 }
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
                     if ((uint32_t)k < cnt && !any_hit) test(ra[k], rb[k], rc[k]);
             } else {
                 for (uint32_t k0 = 0; k0 < cnt; k0 += 2) {
+                    const uint32_t r1 = 3u * min(k0 + 1u, cnt - 1u);
                     const float4 ra0 = q[3 * k0], rb0 = q[3 * k0 + 1], rc0 = q[3 * k0 + 2];
-                    float4 ra1 = ra0, rb1 = rb0, rc1 = rc0;
-                    if (k0 + 1 < cnt) {
-                        ra1 = q[3 * k0 + 3];
-                        rb1 = q[3 * k0 + 4];
-                        rc1 = q[3 * k0 + 5];
-                    }
+                    const float4 ra1 = q[r1], rb1 = q[r1 + 1], rc1 = q[r1 + 2];
+                    keep(ra0);
+                    keep(rb0);
+                    keep(rc0);
+                    keep(ra1);
+                    keep(rb1);
+                    keep(rc1);
                     test(ra0, rb0, rc0);
                     if (k0 + 1 < cnt) test(ra1, rb1, rc1);
                 }
             }
-            if (ANY && any_hit) {
-                finished = true;
-                need_pop = false;
-            }
-        }
-        if (need_pop) {
-            // pop the next entry that can still hold a better hit
-            const float cull2 =
-                ANY ? cull : fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
-            finished = true;
+        };
+        // Pops the next entry that can still hold a better hit (false: none).
+        auto pop_next = [&](uint32_t& e_out, float& t_out) -> bool {
+            const float cull2 = ANY ? cull : fminf(doomed ? INFINITY : tri_t, sph_t) * kCullSlack;
             while (st.sp > 0) {
                 uint32_t e;
                 float te;
                 st.pop(e, te);
                 if (te <= cull2) {
-                    cur = e;
-                    finished = false;
-                    break;
+                    e_out = e;
+                    t_out = te;
+                    return true;
                 }
+            }
+            return false;
+        };
+
+        if constexpr (!PHASED) {
+            bool need_pop = true;
+            if (!(cur & 0x80000000u)) {
+                float t0;
+                need_pop = !visit_node(t0);
+            } else {
+                visit_leaf(cur);
+                if (ANY && any_hit) {
+                    finished = true;
+                    need_pop = false;
+                }
+            }
+            if (need_pop) {
+                float te;
+                finished = !pop_next(cur, te);
+            }
+        } else {
+            if (!leaf_phase) {
+                if (!(cur & 0x80000000u) && !visit_node(cur_t)) cur = kEmpty;  // inner node (kEmpty has bit 31 set)
+            } else if (leaf != kEmpty) {
+                // a leaf postponed behind a hit found since is skipped
+                if (leaf_t <= cull) visit_leaf(leaf);
+                leaf = kEmpty;
+                if (ANY && any_hit) finished = true;
+            }
+            if (!finished) {
+                // when there is no entry to visit, pop one; a leaf moves to the
+                // free leaf slot and the next entry is popped at once
+                if (cur == kEmpty && !pop_next(cur, cur_t)) cur = kEmpty;
+                if (cur != kEmpty && (cur & 0x80000000u) && leaf == kEmpty) {
+                    leaf = cur;
+                    leaf_t = cur_t;
+                    if (!pop_next(cur, cur_t)) cur = kEmpty;
+                }
+                finished = cur == kEmpty && leaf == kEmpty;
             }
         }
         if (finished) {
@@ -253,11 +317,12 @@ using TraceFn = void (*)(const lr_node_q8*, const lr_prim*, RaysArg, uint32_t, H
 
 struct KernelEntry {
     uint32_t block, lds_stack;
-    TraceFn closest;
-    TraceFn any;
+    TraceFn closest[2];  // [schedule - 1]: LR_TRACE_SCHEDULE_STEP, LR_TRACE_SCHEDULE_PHASED
+    TraceFn any[2];
 };
 
-#define LR_K(B, S) {B, S, k_trace<B, S, false>, k_trace<B, S, true>}
+#define LR_K(B, S) \
+    {B, S, {k_trace<B, S, false, false>, k_trace<B, S, false, true>}, {k_trace<B, S, true, false>, k_trace<B, S, true, true>}}
 const KernelEntry kKernels[] = {
     LR_K(64, 8),  LR_K(64, 16),  LR_K(64, 24),  LR_K(128, 8), LR_K(128, 16),
     LR_K(128, 24), LR_K(256, 8), LR_K(256, 16), LR_K(256, 24),
@@ -300,7 +365,7 @@ int prepare(lr_qbvh* bvh, uint32_t n, bool any, hipStream_t stream, Launch& L) {
     }
     const KernelEntry* k = find_kernel(cfg.block, cfg.lds_stack);
     if (!k) return lr::fail(LR_E_INVALID, "no traversal kernel for this block / lds_stack");
-    L.fn = any ? k->any : k->closest;
+    L.fn = any ? k->any[cfg.schedule - 1] : k->closest[cfg.schedule - 1];
     L.block = cfg.block;
     L.refill_min = cfg.refill_min;
     int occ = 0;
@@ -468,7 +533,10 @@ int lr_set_trace_config(lr_context* ctx, const lr_trace_config* cfg) {
     if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
     if (cfg->lds_stack) c.lds_stack = cfg->lds_stack;
     if (cfg->refill_min) c.refill_min = cfg->refill_min;
+    if (cfg->schedule) c.schedule = cfg->schedule;
     if (c.refill_min > 64) return lr::fail(LR_E_INVALID, "refill_min must be in 1..64");
+    if (c.schedule != LR_TRACE_SCHEDULE_STEP && c.schedule != LR_TRACE_SCHEDULE_PHASED)
+        return lr::fail(LR_E_INVALID, "schedule must be LR_TRACE_SCHEDULE_STEP or LR_TRACE_SCHEDULE_PHASED");
     if (!find_kernel(c.block, c.lds_stack))
         return lr::fail(LR_E_INVALID, "unsupported (block, lds_stack) = (" + std::to_string(c.block) + ", " +
                                           std::to_string(c.lds_stack) +
@@ -512,14 +580,16 @@ int lr_autotune_trace(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits
     LR_HIP_CHECK(hipEventCreate(&e1));
     static const uint32_t kBpc[] = {2, 4, 8, 16};
     static const uint32_t kRefill[] = {8, 16, 32, 48};
+    static const uint32_t kSchedule[] = {LR_TRACE_SCHEDULE_STEP, LR_TRACE_SCHEDULE_PHASED};
     lr_trace_config win = saved;
     float win_ms = INFINITY;
     uint32_t count = 0;
     int rc = LR_OK;
     for (const auto& k : kKernels) {
         for (uint32_t bpc : kBpc) {
-            for (uint32_t rf : kRefill) {
-                const lr_trace_config c{k.block, bpc, k.lds_stack, rf};
+            for (uint32_t rfs = 0; rfs < 8 && !rc; ++rfs) {
+                const uint32_t rf = kRefill[rfs & 3], sched = kSchedule[rfs >> 2];
+                const lr_trace_config c{k.block, bpc, k.lds_stack, rf, sched};
                 {
                     std::lock_guard<std::mutex> g(ctx->mu);
                     ctx->cfg = c;

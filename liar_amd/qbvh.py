@@ -57,11 +57,13 @@ class Context:
         cfg = N.TraceConfig()
         N.check(N.lib().lr_get_trace_config(self.handle, C.byref(cfg)), "lr_get_trace_config")
         return {"block": cfg.block, "blocks_per_cu": cfg.blocks_per_cu, "lds_stack": cfg.lds_stack,
-                "refill_min": cfg.refill_min}
+                "refill_min": cfg.refill_min, "schedule": cfg.schedule}
 
     def set_trace_config(self, block: int = 0, blocks_per_cu: int = 0, lds_stack: int = 0,
-                         refill_min: int = 0) -> None:
-        cfg = N.TraceConfig(block, blocks_per_cu, lds_stack, refill_min)
+                         refill_min: int = 0, schedule: int = 0) -> None:
+        """0 keeps a field; schedule 1 = one node or leaf per lane and
+        iteration, 2 = phased node / leaf iterations (LR_TRACE_SCHEDULE_*)."""
+        cfg = N.TraceConfig(block, blocks_per_cu, lds_stack, refill_min, schedule)
         N.check(N.lib().lr_set_trace_config(self.handle, C.byref(cfg)), "lr_set_trace_config")
 
 
@@ -207,7 +209,7 @@ class Qbvh:
 
         def cfg(c):
             return {"block": c.block, "blocks_per_cu": c.blocks_per_cu, "lds_stack": c.lds_stack,
-                    "refill_min": c.refill_min}
+                    "refill_min": c.refill_min, "schedule": c.schedule}
 
         table = [(cfg(entries[k].cfg), float(entries[k].ms)) for k in range(count.value)]
         return {"best": cfg(best), "best_ms": min(ms for _, ms in table), "table": table}

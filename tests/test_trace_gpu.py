@@ -93,11 +93,13 @@ def test_interval_quirk_and_ties_on_gpu(gpu_ctx):
     assert ref[0][2] == 2  # duplicated quads (ids 2 and 4) tie at t = 0.5: smaller id
 
 
+@pytest.mark.parametrize("schedule", [1, 2])
 @pytest.mark.parametrize("block,lds", [(64, 8), (128, 16), (256, 24), (256, 8)])
-def test_launch_configs_agree(gpu_ctx, block, lds):
-    """Every (block, LDS stack) instance, including ones whose LDS stack is
-    shorter than the tree's bound (global spill path), gives the same hits."""
-    say(f"[gpu] launch config block={block} lds={lds}")
+def test_launch_configs_agree(gpu_ctx, block, lds, schedule):
+    """Every (block, LDS stack, schedule) instance, including ones whose LDS
+    stack is shorter than the tree's bound (global spill path), gives the
+    same hits."""
+    say(f"[gpu] launch config block={block} lds={lds} schedule={schedule}")
     v, t, s = scene(200000, 2000, 3)
     b = la.Qbvh(gpu_ctx, v, t, s)
     assert b.info()["max_stack"] > 8
@@ -106,16 +108,16 @@ def test_launch_configs_agree(gpu_ctx, block, lds):
     ref = oracle.qbvh_closest(nodes, prims, r)[:4]
     old = gpu_ctx.trace_config()
     try:
-        gpu_ctx.set_trace_config(block=block, lds_stack=lds, blocks_per_cu=2)
-        assert_hits_equal(ref, gpu_closest(b, r), f"block={block} lds={lds}")
+        gpu_ctx.set_trace_config(block=block, lds_stack=lds, blocks_per_cu=2, schedule=schedule)
+        assert_hits_equal(ref, gpu_closest(b, r), f"block={block} lds={lds} schedule={schedule}")
         r2 = la.random_rays(1 << 16, seed=6, tfar_lo=0.01, tfar_hi=2.0)
         np.testing.assert_array_equal(oracle.qbvh_any(nodes, prims, r2)[0], gpu_any(b, r2))
     finally:
         gpu_ctx.set_trace_config(**old)
 
 
-@pytest.mark.parametrize("refill", [8, 32, 64])
-def test_every_ray_traced_once_across_claim_boundaries(gpu_ctx, refill):
+@pytest.mark.parametrize("refill,schedule", [(8, 1), (32, 1), (64, 1), (32, 2)])
+def test_every_ray_traced_once_across_claim_boundaries(gpu_ctx, refill, schedule):
     """A 512-wave grid (block 64, 2 blocks per CU): ray counts below one ray
     per lane, at the edge of the static share, and with a dynamic remainder
     handed out in shrinking chunks.  Hit buffers start as a sentinel, so a
@@ -126,7 +128,7 @@ def test_every_ray_traced_once_across_claim_boundaries(gpu_ctx, refill):
     b = la.Qbvh(gpu_ctx, v, t, s)
     old = gpu_ctx.trace_config()
     try:
-        gpu_ctx.set_trace_config(block=64, lds_stack=8, blocks_per_cu=2, refill_min=refill)
+        gpu_ctx.set_trace_config(block=64, lds_stack=8, blocks_per_cu=2, refill_min=refill, schedule=schedule)
         for n in (1, 65, 32767, 32769, 3 * 32768 + 17, 200003):
             say(f"[gpu] claim boundaries: refill {refill}, {n} rays")
             r = la.random_rays(n, seed=n)
@@ -146,7 +148,7 @@ def test_every_ray_traced_once_across_claim_boundaries(gpu_ctx, refill):
 
 def test_autotuner_installs_fastest_config_and_keeps_hits(gpu_ctx):
     """lr_autotune_trace times all 9 kernels x 4 blocks_per_cu x 4 refill
-    thresholds, installs the fastest, and traversal under the tuned
+    thresholds x 2 schedules, installs the fastest, and traversal under the tuned
     configuration still matches the oracle bit-exactly (closest and any)."""
     say("[gpu] autotuner")
     v, t, s = scene(100000, 1000, 4)
@@ -156,7 +158,8 @@ def test_autotuner_installs_fastest_config_and_keeps_hits(gpu_ctx):
     old = gpu_ctx.trace_config()
     try:
         res = b.autotune(to_dev(r), repeats=2)
-        assert len(res["table"]) == 9 * 4 * 4
+        assert len(res["table"]) == 9 * 4 * 4 * 2
+        assert {c["schedule"] for c, _ in res["table"]} == {1, 2}
         assert all(ms > 0 for _, ms in res["table"])
         assert res["best_ms"] == min(ms for _, ms in res["table"])
         assert gpu_ctx.trace_config() == res["best"]
