@@ -123,6 +123,12 @@ def bench_direct(args, la, oracle, torch, np):
     camera = E.camera_struct(cam)
     film = torch.zeros((3, H_, W_), dtype=torch.float32, device="cuda")
     wgt = torch.zeros((H_, W_), dtype=torch.float32, device="cuda")
+    tuned = None
+    if not args.no_tune:
+        # traversal configuration for this integrand, timed on a 4-spp frame
+        from liar_amd.autotune import tune_render
+
+        tuned = tune_render(gs, camera, E.render_params(W_, H_, 4, NL, 0, 8, 0.5, (380.0, 780.0), 99), direct=True)
     stats = []
     for k in range(args.warmup + args.render_steps):
         film.zero_()
@@ -142,6 +148,7 @@ def bench_direct(args, la, oracle, torch, np):
         "paths_per_s": round(paths / secs, 1),
         "mrays_per_s": round(rays / secs / 1e6, 2),
         "rays_per_path": round(rays / paths, 3),
+        "trace_config": None if tuned is None else tuned["best"],
     }
     if not args.no_cpu:
         cw, ch = 128, 128

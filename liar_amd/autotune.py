@@ -22,12 +22,13 @@ REFILL = (16, 32, 48)
 SCHEDULES = (1, 2)  # LR_TRACE_SCHEDULE_STEP, LR_TRACE_SCHEDULE_PHASED
 
 
-def tune_render(gs: "E.GpuScene", camera, params, repeats: int = 2, configs=None) -> dict:
-    """Times lr_render(gs, camera, params) under every (block, lds_stack,
-    blocks_per_cu, refill_min, schedule) in ``configs`` (default: the product
-    of the module's value lists, 108 configurations) and sets the fastest as the
-    context's traversal configuration.  Returns {"best", "best_ms",
-    "default_ms", "table"}; the film written meanwhile is scratch."""
+def tune_render(gs: "E.GpuScene", camera, params, repeats: int = 2, configs=None, direct: bool = False) -> dict:
+    """Times lr_render(gs, camera, params) -- lr_render_direct with
+    ``direct`` -- under every (block, lds_stack, blocks_per_cu, refill_min,
+    schedule) in ``configs`` (default: the product of the module's value
+    lists, 108 configurations) and sets the fastest as the context's traversal
+    configuration.  Returns {"best", "best_ms", "default_ms", "table"}; the
+    film written meanwhile is scratch."""
     import torch
 
     ctx = gs.ctx
@@ -39,12 +40,14 @@ def tune_render(gs: "E.GpuScene", camera, params, repeats: int = 2, configs=None
     fx = torch.zeros((3, h, w), dtype=torch.float32, device="cuda")
     fw = torch.zeros((h, w), dtype=torch.float32, device="cuda")
 
+    render = gs.render_direct if direct else gs.render
+
     def timed() -> float:
-        gs.render(camera, params, fx, fw)  # warm-up (scratch sized for this configuration)
+        render(camera, params, fx, fw)  # warm-up (scratch sized for this configuration)
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
         for _ in range(repeats):
-            gs.render(camera, params, fx, fw)
+            render(camera, params, fx, fw)
         b.record()
         b.synchronize()
         return a.elapsed_time(b) / repeats
