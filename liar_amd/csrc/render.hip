@@ -1887,12 +1887,50 @@ __global__ void k_dl_fill_miss(const uint32_t* count, uint32_t cap, Hits h) {
 // One generation's hits: emission, shadow-ray candidates, continuations.
 // Grid-stride over the queue in whole workgroups (every thread of a block
 // runs the same number of appends).
+// Shadow rays of a block are staged in LDS and flushed to the shadow queue
+// with one atomic and coalesced copies per flush (every 512-768 rays),
+// instead of one atomic per block and candidate slot: the queue's single
+// counter serialises its atomics (~10 ns each), which set the kernel's time.
+constexpr uint32_t kStage = 768;
+struct ShadowStage {
+    float r[8][kStage];
+    uint32_t pid[kStage];
+    float c[kStage];
+};
+
+// Copies the block's `fill` staged rays to the shadow queue (all threads).
+__device__ void stage_flush(ShadowStage& st, uint32_t fill, ShadowQueue sq, uint32_t* scount, uint32_t scap,
+                            uint32_t* sover) {
+    __shared__ uint32_t s_gbase;
+    __syncthreads();  // staged writes done
+    if (threadIdx.x == 0) {
+        s_gbase = atomicAdd(scount, fill);
+        if (s_gbase + (uint64_t)fill > scap) atomicAdd(sover, (uint32_t)(s_gbase + (uint64_t)fill - max(s_gbase, scap)));
+    }
+    __syncthreads();
+    const uint32_t gb = s_gbase;
+    for (uint32_t k = threadIdx.x; k < fill; k += blockDim.x) {
+        const uint32_t g = gb + k;
+        if (g < scap) {
+#pragma unroll
+            for (int w = 0; w < 8; ++w) sq.r[w][g] = st.r[w][k];
+            sq.pid[g] = st.pid[k];
+            sq.c[g] = st.c[k];
+        }
+    }
+    __syncthreads();  // the stage is reused
+}
+
 __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t gen, Hits h,
                                                   const float4* __restrict__ prim_geom, DLQueue q,
                                                   const uint32_t* count, PathState ps, DLQueue qn,
                                                   uint32_t* count_next, ShadowQueue sq, uint32_t* scount,
                                                   uint64_t path0, DLArgs A) {
     const uint32_t n = min(*count, A.qcap);
+    __shared__ ShadowStage stage;
+    __shared__ uint32_t s_wcnt[2][16];  // per-wave counts, double-buffered by slot parity
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t fill = 0, parity = 0;  // block-uniform
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         DLHit H;
@@ -1953,22 +1991,35 @@ __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t
             float c = 0.0f, far = 0.0f;
             V3 dir = v3(0, 0, 1);
             const bool alive = surf && dl_candidate(S, P, A, H, j, c, dir, far);
-            const uint32_t pos = block_append(alive, scount);
+            // block-local compaction into the stage: one barrier per slot
+            const uint64_t mask = __ballot(alive);
+            if (lane == 0) s_wcnt[parity][wave] = (uint32_t)__popcll(mask);
+            __syncthreads();
+            uint32_t off = fill, tot = 0;
+            for (uint32_t w = 0; w < nw; ++w) {
+                const uint32_t cw = s_wcnt[parity][w];
+                if (w < wave) off += cw;
+                tot += cw;
+            }
+            parity ^= 1u;
             if (alive) {
-                if (pos < A.scap) {
-                    sq.r[0][pos] = H.start.x;
-                    sq.r[1][pos] = H.start.y;
-                    sq.r[2][pos] = H.start.z;
-                    sq.r[3][pos] = dir.x;
-                    sq.r[4][pos] = dir.y;
-                    sq.r[5][pos] = dir.z;
-                    sq.r[6][pos] = kTolerance;
-                    sq.r[7][pos] = far;
-                    sq.pid[pos] = pid;
-                    sq.c[pos] = c;
-                } else {
-                    atomicAdd(A.sover, 1u);
-                }
+                const uint32_t k = off + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                stage.r[0][k] = H.start.x;
+                stage.r[1][k] = H.start.y;
+                stage.r[2][k] = H.start.z;
+                stage.r[3][k] = dir.x;
+                stage.r[4][k] = dir.y;
+                stage.r[5][k] = dir.z;
+                stage.r[6][k] = kTolerance;
+                stage.r[7][k] = far;
+                stage.pid[k] = pid;
+                stage.c[k] = c;
+            }
+            fill += tot;
+            if (fill + blockDim.x > kStage) {
+                stage_flush(stage, fill, sq, scount, A.scap, A.sover);
+                fill = 0;
             }
         }
         // specular / glossy continuations (direct_lighting.cpp:314-377):
@@ -2014,6 +2065,7 @@ __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t
             }
         }
     }
+    if (fill) stage_flush(stage, fill, sq, scount, A.scap, A.sover);
 }
 
 // Unoccluded shadow rays add their radiance to their camera path.
