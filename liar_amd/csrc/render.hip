@@ -107,6 +107,10 @@ __device__ inline lr::u4 rng(const Params& P, uint64_t path, uint32_t gen, uint3
     return lr::philox4x32(P.seed_lo, P.seed_hi, lr::u4{(uint32_t)path, (uint32_t)(path >> 32), gen, purpose});
 }
 
+// Block size of the kernels that append to a ray queue with block_append:
+// one atomic per block, so 1024 threads make 4x fewer than 256.
+constexpr uint32_t kAppendBlock = 1024;
+
 // Slot of this lane's entry in a queue that the whole workgroup appends to:
 // wave counts by ballot, one atomic per workgroup (a single counter address
 // serialises its atomics, so fewer of them matter).  Every thread of the
@@ -1299,9 +1303,10 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         const uint32_t n = (uint32_t)(npix * ppp);
         const uint64_t path0 = pix0 * ppp;
         const dim3 blk(256), grd((n + 255) / 256);
+        const dim3 ablk(kAppendBlock), agrd((n + kAppendBlock - 1) / kAppendBlock);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 1u);
         if (guided) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts + 8, 1u);
-        hipLaunchKernelGGL(k_camera<false>, grd, blk, 0, s, S, C, P, path0, n, ps, q[0], counts, NB);
+        hipLaunchKernelGGL(k_camera<false>, agrd, ablk, 0, s, S, C, P, path0, n, ps, q[0], counts, NB);
         for (uint32_t gen = 0; gen < P.max_gen; ++gen) {
             const int cur = gen & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(k_next_gen, dim3(1), dim3(64), 0, s, counts, (uint32_t)cur, (uint32_t)nxt, rays_total);
@@ -1316,10 +1321,10 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             if (guided) {
                 hipLaunchKernelGGL(k_guide_inputs, grd, blk, 0, s, S, q[cur], counts + cur, H, ps, GA, g_in);
                 if (int rc = lr_guide_infer(guide, g_in, n, counts + cur, g_mix, s)) return rc;
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), agrd, ablk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
                                    sc->d_prim_geom, ps, q[nxt], counts + nxt, GA, NB);
             } else {
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, false>), grd, blk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, false>), agrd, ablk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
                                    sc->d_prim_geom, ps, q[nxt], counts + nxt, GA, NB);
             }
         }
@@ -1650,9 +1655,10 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
             const ChunkPlan& ch = chunks[c];
             B.seg = b->d_seg + ch.seg_begin;
             B.n_seg = ch.n_seg;
-            const dim3 blk(256), grd((ch.n_paths + 255) / 256);
+            const dim3 blk(256);
+            const dim3 ablk(kAppendBlock), agrd((ch.n_paths + kAppendBlock - 1) / kAppendBlock);
             hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 1u);
-            hipLaunchKernelGGL(k_camera<true>, grd, blk, 0, s, S0, C0, P0, (uint64_t)0, ch.n_paths, ps, q[0], counts,
+            hipLaunchKernelGGL(k_camera<true>, agrd, ablk, 0, s, S0, C0, P0, (uint64_t)0, ch.n_paths, ps, q[0], counts,
                                B);
             for (uint32_t gen = 0; gen < ch.max_gen; ++gen) {
                 const int cu = gen & 1, nx = cu ^ 1;
@@ -1662,7 +1668,7 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
                 lr_hits_soa hits{H.prim, H.t, H.u, H.v};
                 if (int rc = lr::trace_closest_queue(&b->tree, &rays, ch.n_paths, counts + cu, &hits, s, q[cu].root))
                     return rc;
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, true>), grd, blk, 0, s, S0, P0, gen, (uint64_t)0,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, true>), agrd, ablk, 0, s, S0, P0, gen, (uint64_t)0,
                                    q[cu], counts + cu, H, hgeo[0], ps, q[nx], counts + nx, NG, B);
             }
             hipLaunchKernelGGL(k_splat<true>, dim3((ch.n_pix + 255) / 256), blk, 0, s, S0, P0, (uint64_t)0, ch.n_pix,
@@ -2182,7 +2188,8 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
             cq.pid = q[0].pid;
             cq.root = nullptr;
             hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counts, 8u);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_camera<false>), dim3((n + 255) / 256), blk, 0, s, S, C, P, path0, n,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_camera<false>), dim3((n + kAppendBlock - 1) / kAppendBlock),
+                               dim3(kAppendBlock), 0, s, S, C, P, path0, n,
                                ps, cq, counts, NB);
             hipLaunchKernelGGL(k_dl_init, dim3((n + 255) / 256), blk, 0, s, counts, ps, q[0]);
             // no specular (or traced glossy) material: no hit continues, so
