@@ -673,6 +673,75 @@ __global__ void k_splat(DevScene S0, Params P0, uint64_t pix0, uint32_t npix, Pa
     film_w[px] += (float)ppp;
 }
 
+// Film splat of one scene's chunk: a block takes pb = max(1, 256 / ppp) pixels.
+// Each thread turns one path's (L, lambda, pdf) into its XYZ contribution in
+// LDS, then one thread per pixel adds its paths' contributions up in path
+// order -- the same terms, in the same order, as k_splat's thread walking the
+// pixel's paths, so the film is bit for bit the same -- while the table
+// lookups and loads run one path per lane instead of one pixel per lane.
+__global__ __launch_bounds__(256) void k_splat_tiles(DevScene S, Params P, uint64_t pix0, uint32_t npix,
+                                                     PathState ps, float* film_xyz, float* film_w) {
+    __shared__ float c[3][256];
+    __shared__ uint8_t has[256];
+    const uint32_t ppp = P.spp * P.nl;
+    const uint32_t pb = ppp >= 256u ? 1u : 256u / ppp;      // pixels per block
+    const uint32_t tile = ppp >= 256u ? 256u : pb * ppp;    // paths per tile
+    const uint32_t pixb = blockIdx.x * pb;
+    if (pixb >= npix) return;  // the whole block
+    const uint32_t npb = min(pb, npix - pixb);
+    const uint32_t total = npb * ppp;
+    const uint32_t path_begin = pixb * ppp;
+    const uint32_t t = threadIdx.x;
+    float X = 0.0f, Y = 0.0f, Z = 0.0f;
+    for (uint32_t t0 = 0; t0 < total; t0 += tile) {
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        bool h = false;
+        const uint32_t k = t0 + t;
+        if (t < tile && k < total) {
+            const uint32_t i = path_begin + k;
+            const float L = ps.L[i];
+            if (L != 0.0f) {
+                float xyz[3];
+                observer_xyz(S.obs, ps.lam[i], xyz);
+                float sc = L / ps.lpdf[i];  // Spectral::xyz, spectral.cpp:212-221
+                if (P.max_lum > 0.0f && sc * xyz[1] > P.max_lum) sc *= P.max_lum / (sc * xyz[1]);  // raster.cpp:567-570
+                cx = sc * xyz[0];
+                cy = sc * xyz[1];
+                cz = sc * xyz[2];
+                h = true;
+            }
+        }
+        c[0][t] = cx;
+        c[1][t] = cy;
+        c[2][t] = cz;
+        has[t] = h ? 1 : 0;
+        __syncthreads();
+        if (t < npb) {
+            const uint32_t lo = max(t * ppp, t0), hi = min((t + 1) * ppp, t0 + tile);
+            for (uint32_t kk = lo; kk < hi; ++kk) {
+                if (!has[kk - t0]) continue;
+                X += c[0][kk - t0];
+                Y += c[1][kk - t0];
+                Z += c[2][kk - t0];
+            }
+        }
+        __syncthreads();
+    }
+    if (t < npb) {
+        const uint64_t px = pix0 + pixb + t;
+        const uint64_t np = (uint64_t)P.width * P.height;
+        film_xyz[px] += X;
+        film_xyz[np + px] += Y;
+        film_xyz[2 * np + px] += Z;
+        film_w[px] += (float)ppp;
+    }
+}
+
+inline uint32_t splat_blocks(uint32_t npix, uint32_t ppp) {
+    const uint32_t pb = ppp >= 256u ? 1u : 256u / ppp;
+    return (npix + pb - 1) / pb;
+}
+
 // -- host helpers --------------------------------------------------------------
 
 template <class T>
@@ -1345,8 +1414,8 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
                 if (int rc = lr_guide_train_step(guide, b_in, b_dir, b_w, batch, nullptr, nullptr, s)) return rc;
             }
         }
-        hipLaunchKernelGGL(k_splat<false>, dim3((npix + 255) / 256), blk, 0, s, S, P, pix0, npix, ps, film_xyz, film_w,
-                           NB);
+        hipLaunchKernelGGL(k_splat_tiles, dim3(splat_blocks(npix, (uint32_t)ppp)), blk, 0, s, S, P, pix0, npix, ps,
+                           film_xyz, film_w);
         LR_HIP_CHECK(hipGetLastError());
         ++chunks;
     }
@@ -2237,8 +2306,8 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
             if (qcap > (1ull << 30) || scap > (1ull << 30))
                 return lr::fail(LR_E_INVALID, "lr_render_direct: ray tree too large; use a smaller chunk_paths");
         }
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_splat<false>), dim3((npix + 255) / 256), blk, 0, s, S, P, pix0, npix, ps,
-                           film_xyz, film_w, NB);
+        hipLaunchKernelGGL(k_splat_tiles, dim3(splat_blocks(npix, P.spp * P.nl)), blk, 0, s, S, P, pix0, npix, ps,
+                           film_xyz, film_w);
         LR_HIP_CHECK(hipGetLastError());
         ++chunks;
     }
