@@ -135,6 +135,14 @@ def compile_scene(scene) -> CompiledScene:
             if M is not None:
                 v = (np.asarray(v, np.float64) @ M[:3, :3].T + M[:3, 3]).astype(np.float32)
             add_mesh(v, obj.triangles(), material(shader))
+        elif isinstance(obj, scenery.TriangleMeshComposite):
+            # one triangle set is this object's rule; each child keeps its shader
+            for c in obj.children():
+                cs = c.shader if c.shader is not None else shader
+                v = c.vertices()
+                if M is not None:
+                    v = (np.asarray(v, np.float64) @ M[:3, :3].T + M[:3, 3]).astype(np.float32)
+                add_mesh(v, c.triangles(), material(cs))
         elif isinstance(obj, scenery.Parallelogram):
             v, t = obj._mesh()
             if M is not None:

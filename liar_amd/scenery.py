@@ -79,6 +79,60 @@ class TriangleMesh(SceneObject):
         return self._uvs
 
 
+class TriangleMeshComposite(SceneObject):
+    """``TriangleMeshComposite(children)``
+    (src/liar/scenery/triangle_mesh_composite.cpp:94-183): the triangles of
+    several TriangleMesh children in one mesh, each child keeping its own
+    shader.  An intersection applies the mesh rule to all of them at once --
+    the nearest triangle with t > near, then LiAR's interval check on it --
+    and reports the child and the triangle's index in that child (:186-216).
+
+    The engine merges every mesh of a scene into one triangle set, which is
+    this rule; ``mesh()`` gives the merged arrays for ray queries
+    (``liar_amd.Qbvh``) and ``child_of(k)`` maps a hit's triangle index back."""
+
+    def __init__(self, children=()):
+        super().__init__()
+        self._children = []
+        self.add(children)
+
+    def add(self, child):
+        """Adds a TriangleMesh or a sequence of them (``add(child)`` /
+        ``add(children)``, :54-64)."""
+        items = child if isinstance(child, (list, tuple)) else [child]
+        for c in items:
+            if not isinstance(c, TriangleMesh):
+                raise TypeError("TriangleMeshComposite children are TriangleMesh objects")
+            self._children.append(c)
+
+    def children(self):
+        return list(self._children)
+
+    def offsets(self):
+        """First merged triangle index of each child (the back links, :172)."""
+        return np.cumsum([0] + [len(c.triangles()) for c in self._children[:-1]]).astype(np.int64)
+
+    def mesh(self):
+        """(vertices float32 (nv, 3), triangles uint32 (nt, 3)): children's
+        arrays concatenated in order, vertex indices rebased (:127-173)."""
+        verts, tris, base = [], [], 0
+        for c in self._children:
+            verts.append(np.asarray(c.vertices(), np.float32).reshape(-1, 3))
+            tris.append(np.asarray(c.triangles(), np.uint32).reshape(-1, 3) + np.uint32(base))
+            base += len(verts[-1])
+        if not verts:
+            return np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)
+        return np.concatenate(verts), np.concatenate(tris)
+
+    def child_of(self, k):
+        """Merged triangle index -> (child index, triangle index in the child)."""
+        off = self.offsets()
+        c = int(np.searchsorted(off, k, side="right") - 1)
+        if k < 0 or c < 0 or k - off[c] >= len(self._children[c].triangles()):
+            raise IndexError("triangle index out of range")
+        return c, int(k - off[c])
+
+
 class Sphere(SceneObject):
     def __init__(self, center=(0.0, 0.0, 0.0), radius=1.0):
         super().__init__()
