@@ -80,3 +80,32 @@ def test_compile_scene_matches_listed_meshes_and_shaders():
     assert np.array_equal(c1.verts, c2.verts) and np.array_equal(c1.tris, c2.tris)
     assert np.array_equal(c1.prim_material, c2.prim_material)
     assert len(c1.materials) == 2 and c1.prim_material[0] != c1.prim_material[2]
+
+
+@pytest.mark.gpu
+def test_composite_ray_queries_on_gpu(gpu_ctx):
+    """A composite of three random meshes traced through lr_qbvh over its
+    merged arrays: ids bit-exact against the oracle, and every hit maps back
+    to a child whose own triangle, transformed by nothing, is the one hit."""
+    import torch
+
+    print("[gpu] composite queries", flush=True)
+    meshes = []
+    for k in range(3):
+        v, t = la.random_triangles(2000 + 500 * k, seed=40 + k)
+        meshes.append(scenery.TriangleMesh(v, (), (), t))
+    comp = scenery.TriangleMeshComposite(meshes)
+    v, t = comp.mesh()
+    b = la.Qbvh(gpu_ctx, v, t)
+    r = la.random_rays(30000, seed=44)
+    prim, tuv = b.trace_closest(torch.from_numpy(r).cuda())
+    torch.cuda.synchronize()
+    prim = prim.cpu().numpy().view(np.uint32)
+    ref = oracle.brute_closest(v, t, None, r)
+    assert np.array_equal(prim, ref[0]) and np.array_equal(tuv.cpu().numpy()[0].view(np.uint32), ref[1].view(np.uint32))
+    hit = np.nonzero(prim != oracle.MISS)[0]
+    assert hit.size > 1000
+    for k in hit[:200]:
+        c, j = comp.child_of(int(prim[k]))
+        assert np.array_equal(meshes[c].vertices()[meshes[c].triangles()[j]], v[t[prim[k]]])
+    b.close()
