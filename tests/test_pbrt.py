@@ -227,7 +227,7 @@ def test_pbrt_render_furnace_and_orientation(gpu_ctx, tmp_path):
         Shape "sphere" "float radius" 1.5
         WorldEnd""")
     s.render()
-    y = s.image.xyz()[1]
+    y = s.image.xyz()[..., 1]
     yy, xx = np.mgrid[0:48, 0:64]
     disc = (xx - 31.5) ** 2 + (yy - 23.5) ** 2 < 9 ** 2
     Y = y[disc]
@@ -246,7 +246,7 @@ def test_pbrt_render_furnace_and_orientation(gpu_ctx, tmp_path):
         Shape "sphere" "float radius" 0.8
         WorldEnd""")
     s.render()
-    y = s.image.xyz()[1]
+    y = s.image.xyz()[..., 1]
     left, right = y[:, :32].mean(), y[:, 32:].mean()
     print(f"[gpu] white sphere at -x, black at +x: left half {left:.3f}, right half {right:.3f}", flush=True)
     assert left > 0.05 and right < 1e-6
