@@ -584,8 +584,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        log("bench.py measures one GPU only: this build has no multi-GPU path "
-            "(the ray query shards by rays with a replicated QBVH; see DESIGN.md).")
+        log("bench.py measures one GPU only: there is no multi-GPU path "
+            "(the ray query would shard by rays over a replicated QBVH).")
         return 2
 
     import numpy as np
