@@ -632,65 +632,20 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
 }
 
 // -- film: Raster box filter, fixed-order per-pixel sums (raster.cpp:547-606)
-template <bool BATCH>
-__global__ void k_splat(DevScene S0, Params P0, uint64_t pix0, uint32_t npix, PathState ps, float* film_xyz0,
-                        float* film_w0, BatchArgs B) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= npix) return;
-    DevScene S = S0;
-    Params P = P0;
-    float* film_xyz = film_xyz0;
-    float* film_w = film_w0;
-    uint64_t px = pix0 + j;
-    uint32_t first = j * (P.spp * P.nl);  // chunk-local index of the pixel's first path
-    if (BATCH) {
-        const Segment sg = B.seg[find_segment(B, j, true)];
-        S = B.scenes[sg.scene];
-        P = B.params[sg.scene];
-        film_xyz = B.films[sg.scene].xyz;
-        film_w = B.films[sg.scene].w;
-        px = sg.pixel0 + (j - sg.pix_begin);
-        first = sg.path_begin + (j - sg.pix_begin) * (P.spp * P.nl);
-    }
-    const uint32_t ppp = P.spp * P.nl;
-    float X = 0.0f, Y = 0.0f, Z = 0.0f;
-    for (uint32_t k = 0; k < ppp; ++k) {
-        const uint32_t i = first + k;
-        const float L = ps.L[i];
-        if (L == 0.0f) continue;
-        float xyz[3];
-        observer_xyz(S.obs, ps.lam[i], xyz);
-        float s = L / ps.lpdf[i];  // Spectral::xyz, spectral.cpp:212-221
-        if (P.max_lum > 0.0f && s * xyz[1] > P.max_lum) s *= P.max_lum / (s * xyz[1]);  // raster.cpp:567-570
-        X += s * xyz[0];
-        Y += s * xyz[1];
-        Z += s * xyz[2];
-    }
-    const uint64_t np = (uint64_t)P.width * P.height;
-    film_xyz[px] += X;
-    film_xyz[np + px] += Y;
-    film_xyz[2 * np + px] += Z;
-    film_w[px] += (float)ppp;
-}
-
-// Film splat of one scene's chunk: a block takes pb = max(1, 256 / ppp) pixels.
+// Film splat in tiles: a block takes up to pb = max(1, 256 / ppp) pixels of
+// one scene (ppp = spp x wavelengths paths each, stored pixel after pixel).
 // Each thread turns one path's (L, lambda, pdf) into its XYZ contribution in
 // LDS, then one thread per pixel adds its paths' contributions up in path
-// order -- the same terms, in the same order, as k_splat's thread walking the
-// pixel's paths, so the film is bit for bit the same -- while the table
-// lookups and loads run one path per lane instead of one pixel per lane.
-__global__ __launch_bounds__(256) void k_splat_tiles(DevScene S, Params P, uint64_t pix0, uint32_t npix,
-                                                     PathState ps, float* film_xyz, float* film_w) {
+// order -- the same terms, in the same order, as a thread walking the
+// pixel's paths, so films are the same bit for bit -- while the table lookups
+// and loads run one path per lane instead of one pixel per lane.
+__device__ void splat_tile(const DevScene& S, const Params& P, uint32_t npb, uint32_t path_begin, uint64_t px0,
+                           const PathState& ps, float* film_xyz, float* film_w) {
     __shared__ float c[3][256];
     __shared__ uint8_t has[256];
     const uint32_t ppp = P.spp * P.nl;
-    const uint32_t pb = ppp >= 256u ? 1u : 256u / ppp;      // pixels per block
-    const uint32_t tile = ppp >= 256u ? 256u : pb * ppp;    // paths per tile
-    const uint32_t pixb = blockIdx.x * pb;
-    if (pixb >= npix) return;  // the whole block
-    const uint32_t npb = min(pb, npix - pixb);
+    const uint32_t tile = ppp >= 256u ? 256u : (256u / ppp) * ppp;  // paths per pass
     const uint32_t total = npb * ppp;
-    const uint32_t path_begin = pixb * ppp;
     const uint32_t t = threadIdx.x;
     float X = 0.0f, Y = 0.0f, Z = 0.0f;
     for (uint32_t t0 = 0; t0 < total; t0 += tile) {
@@ -728,13 +683,41 @@ __global__ __launch_bounds__(256) void k_splat_tiles(DevScene S, Params P, uint6
         __syncthreads();
     }
     if (t < npb) {
-        const uint64_t px = pix0 + pixb + t;
+        const uint64_t px = px0 + t;
         const uint64_t np = (uint64_t)P.width * P.height;
         film_xyz[px] += X;
         film_xyz[np + px] += Y;
         film_xyz[2 * np + px] += Z;
         film_w[px] += (float)ppp;
     }
+}
+
+// One scene's chunk: block b takes pixels [b pb, b pb + pb) of the chunk.
+__global__ __launch_bounds__(256) void k_splat_tiles(DevScene S, Params P, uint64_t pix0, uint32_t npix,
+                                                     PathState ps, float* film_xyz, float* film_w) {
+    const uint32_t ppp = P.spp * P.nl;
+    const uint32_t pb = ppp >= 256u ? 1u : 256u / ppp;
+    const uint32_t pixb = blockIdx.x * pb;
+    if (pixb >= npix) return;  // the whole block
+    splat_tile(S, P, min(pb, npix - pixb), pixb * ppp, pix0 + pixb, ps, film_xyz, film_w);
+}
+
+// A batch chunk: block b takes the pixels of tile b, all of one segment.
+struct SplatTile {
+    uint32_t seg;   // chunk-local segment
+    uint32_t pix;   // first chunk-local pixel
+    uint32_t npix;  // pixels (<= max(1, 256 / ppp))
+};
+
+__global__ __launch_bounds__(256) void k_splat_tiles_batch(BatchArgs B, const SplatTile* __restrict__ tiles,
+                                                           PathState ps) {
+    const SplatTile T = tiles[blockIdx.x];
+    const Segment sg = B.seg[T.seg];
+    const Params P = B.params[sg.scene];
+    const uint32_t ppp = P.spp * P.nl;
+    const uint32_t off = T.pix - sg.pix_begin;
+    splat_tile(B.scenes[sg.scene], P, T.npix, sg.path_begin + off * ppp, (uint64_t)sg.pixel0 + off, ps,
+               B.films[sg.scene].xyz, B.films[sg.scene].w);
 }
 
 inline uint32_t splat_blocks(uint32_t npix, uint32_t ppp) {
@@ -1467,6 +1450,8 @@ struct lr_batch {
     SceneFilm* d_films = nullptr;
     Segment* d_seg = nullptr;
     size_t seg_cap = 0;
+    SplatTile* d_tiles = nullptr;
+    size_t tile_cap = 0;
     uint64_t* d_stamps = nullptr;
     size_t stamp_cap = 0;
     // wavefront workspace (grow-only): path state 4, two queues 2 x 10, hits 4
@@ -1492,7 +1477,7 @@ void free_batch(lr_batch* b) {
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
     if (b->cstream) (void)hipStreamDestroy(b->cstream);
     void* ptrs[] = {b->tree.d_nodes, b->tree.d_prims, b->d_roots, b->d_scenes, b->d_params, b->d_cams,
-                    (void*)b->d_geoms, b->d_films, b->d_seg, b->d_stamps, b->d_work, b->d_counts};
+                    (void*)b->d_geoms, b->d_films, b->d_seg, b->d_tiles, b->d_stamps, b->d_work, b->d_counts};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1658,10 +1643,27 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
         }
     }
     if (cur.n_seg) chunks.push_back(cur);
+    // splat tiles: per chunk, each segment's pixels in groups of max(1, 256 / ppp)
+    std::vector<SplatTile> tiles;
+    std::vector<uint32_t> tile0(chunks.size() + 1, 0);
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        const ChunkPlan& ch = chunks[c];
+        tile0[c] = (uint32_t)tiles.size();
+        for (uint32_t j = 0; j < ch.n_seg; ++j) {
+            const Segment& sg = segs[ch.seg_begin + j];
+            const uint32_t end = j + 1 < ch.n_seg ? segs[ch.seg_begin + j + 1].pix_begin : ch.n_pix;
+            const uint32_t ppp = hprm[sg.scene].spp * hprm[sg.scene].nl;
+            const uint32_t pb = ppp >= 256u ? 1u : 256u / ppp;
+            for (uint32_t p = sg.pix_begin; p < end; p += pb) tiles.push_back(SplatTile{j, p, std::min(pb, end - p)});
+        }
+    }
+    tile0[chunks.size()] = (uint32_t)tiles.size();
     // the previous call may still read the tables: wait for it before rewriting them
     if (b->cstream) LR_HIP_CHECK(hipStreamSynchronize(b->cstream));
     LR_HIP_CHECK(hipStreamSynchronize(s));
     if (int rc = grow(&b->d_seg, b->seg_cap, segs.size())) return rc;
+    if (int rc = grow(&b->d_tiles, b->tile_cap, tiles.size())) return rc;
+    LR_HIP_CHECK(hipMemcpy(b->d_tiles, tiles.data(), sizeof(SplatTile) * tiles.size(), hipMemcpyHostToDevice));
     if (int rc = grow(&b->d_stamps, b->stamp_cap, chunks.size() + 1)) return rc;
     LR_HIP_CHECK(hipMemcpy(b->d_seg, segs.data(), sizeof(Segment) * segs.size(), hipMemcpyHostToDevice));
     LR_HIP_CHECK(hipMemcpy(b->d_scenes, hs.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice));
@@ -1740,8 +1742,8 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, true>), agrd, ablk, 0, s, S0, P0, gen, (uint64_t)0,
                                    q[cu], counts + cu, H, hgeo[0], ps, q[nx], counts + nx, NG, B);
             }
-            hipLaunchKernelGGL(k_splat<true>, dim3((ch.n_pix + 255) / 256), blk, 0, s, S0, P0, (uint64_t)0, ch.n_pix,
-                               ps, (float*)nullptr, (float*)nullptr, B);
+            hipLaunchKernelGGL(k_splat_tiles_batch, dim3(tile0[c + 1] - tile0[c]), blk, 0, s, B, b->d_tiles + tile0[c],
+                               ps);
             hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, b->d_stamps + c + 1);
             LR_HIP_CHECK(hipGetLastError());
         }
