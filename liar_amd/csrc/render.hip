@@ -109,10 +109,10 @@ __device__ inline lr::u4 rng(const Params& P, uint64_t path, uint32_t gen, uint3
 
 // Block size of the kernels that append to a ray queue with block_append:
 // one atomic per block, so 1024 threads make 4x fewer than 256.  The guided
-// k_shade (72 VGPRs) fits one 16-wave block per CU, 4 waves per SIMD, so it
-// runs 512-thread blocks instead (6 waves per SIMD).
+// and batched k_shade (72 VGPRs) fit one 16-wave block per CU, 4 waves per
+// SIMD, so they run 512-thread blocks instead (6 waves per SIMD).
 constexpr uint32_t kAppendBlock = 1024;
-constexpr uint32_t kAppendBlockGuided = 512;
+constexpr uint32_t kAppendBlockWide = 512;
 
 // Slot of this lane's entry in a queue that the whole workgroup appends to:
 // wave counts by ballot, one atomic per workgroup (a single counter address
@@ -1376,8 +1376,8 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             if (guided) {
                 hipLaunchKernelGGL(k_guide_inputs, grd, blk, 0, s, S, q[cur], counts + cur, H, ps, GA, g_in);
                 if (int rc = lr_guide_infer(guide, g_in, n, counts + cur, g_mix, s)) return rc;
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), dim3((n + kAppendBlockGuided - 1) / kAppendBlockGuided),
-                                   dim3(kAppendBlockGuided), 0, s, S, P, gen, path0, q[cur], counts + cur, H,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), dim3((n + kAppendBlockWide - 1) / kAppendBlockWide),
+                                   dim3(kAppendBlockWide), 0, s, S, P, gen, path0, q[cur], counts + cur, H,
                                    sc->d_prim_geom, ps, q[nxt], counts + nxt, GA, NB);
             } else {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, false>), agrd, ablk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
@@ -1743,7 +1743,9 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
                 lr_hits_soa hits{H.prim, H.t, H.u, H.v};
                 if (int rc = lr::trace_closest_queue(&b->tree, &rays, ch.n_paths, counts + cu, &hits, s, q[cu].root))
                     return rc;
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, true>), agrd, ablk, 0, s, S0, P0, gen, (uint64_t)0,
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, true>),
+                                   dim3((ch.n_paths + kAppendBlockWide - 1) / kAppendBlockWide), dim3(kAppendBlockWide), 0,
+                                   s, S0, P0, gen, (uint64_t)0,
                                    q[cu], counts + cu, H, hgeo[0], ps, q[nx], counts + nx, NG, B);
             }
             hipLaunchKernelGGL(k_splat_tiles_batch, dim3(tile0[c + 1] - tile0[c]), blk, 0, s, B, b->d_tiles + tile0[c],
