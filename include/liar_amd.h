@@ -427,10 +427,11 @@ typedef struct lr_direct_params {
 } lr_direct_params;
 
 /* params->max_depth, rr_depth, rr_survival and graph are ignored;
- * chunk_paths 0 means 1M camera paths per chunk (each chunk keeps a branch
- * queue of 4x its paths and a shadow queue of as many rays as its hits can
- * make: a scene whose branches overflow returns LR_E_INVALID and asks for a
- * smaller chunk).  dparams NULL: LiAR's defaults (8, 1, true). */
+ * chunk_paths 0 means 4M camera paths per chunk (each chunk keeps two branch
+ * queues of 2x its paths and a shadow queue of as many rays as its hits can
+ * make; a queue that overflows is doubled and the chunk repeated, and one
+ * that would pass 2^30 entries returns LR_E_INVALID and asks for a smaller
+ * chunk).  dparams NULL: LiAR's defaults (8, 1, true). */
 int lr_render_direct(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
                      const lr_direct_params* dparams, float* film_xyz, float* film_weight,
                      lr_render_stats* stats, void* stream);

@@ -2212,7 +2212,7 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
 
     const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
     const uint64_t n_pix = (uint64_t)R.width * R.height;
-    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 20);
+    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 22);
     const uint64_t pix_per_chunk = std::max<uint64_t>(1, chunk_req / ppp);
     const uint64_t cap64 = pix_per_chunk * ppp;
     if (cap64 >= (1ull << 26)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");
