@@ -1878,8 +1878,11 @@ struct DLHit {
 // picked by power and whose estimate is divided by (samples x pick pdf)
 // (direct_lighting.cpp:259-310).  Returns false when the candidate makes no
 // ray; c is its radiance if unoccluded, [kTolerance, far) its extent.
+// le_k / le_v cache the radiance of the last light a candidate of this hit
+// used (candidates of one light are consecutive slots): a sampled spectrum
+// costs a binary search through global memory.
 __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DLArgs& A, const DLHit& H, uint32_t j,
-                                    float& c, V3& dir, float& far) {
+                                    float& c, V3& dir, float& far, uint32_t& le_k, float& le_v) {
     uint32_t k = 0, local = 0;
     float nl = 1.0f, scale = 1.0f;
     if (H.hq) {
@@ -1906,7 +1909,11 @@ __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DL
         scale = (float)A.secondary * psel;
     }
     const DevLight& Lk = S.lights[k];
-    const float Le = spectrum_eval(S, Lk.radiance, H.lam, false);
+    if (k != le_k) {
+        le_v = spectrum_eval(S, Lk.radiance, H.lam, false);
+        le_k = k;
+    }
+    const float Le = le_v;
     if (Le == 0.0f) return false;
     const lr::u4 r = dl_rng(P, H.gp, H.gen, H.br, 100u + j);
     float f = 0.0f, pb = 0.0f, pl = 0.0f;
@@ -2070,10 +2077,12 @@ __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t
                 H.start = x + (10.0f * kTolerance) * H.nf;
             }
         }
+        uint32_t le_k = ~0u;
+        float le_v = 0.0f;
         for (uint32_t j = 0; j < A.slots; ++j) {
             float c = 0.0f, far = 0.0f;
             V3 dir = v3(0, 0, 1);
-            const bool alive = surf && dl_candidate(S, P, A, H, j, c, dir, far);
+            const bool alive = surf && dl_candidate(S, P, A, H, j, c, dir, far, le_k, le_v);
             // block-local compaction into the stage: one barrier per slot
             const uint64_t mask = __ballot(alive);
             if (lane == 0) s_wcnt[parity][wave] = (uint32_t)__popcll(mask);
