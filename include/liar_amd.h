@@ -416,8 +416,11 @@ int lr_render(lr_scene* scene, const lr_camera* camera, const lr_render_params* 
  * rays and specular continuations of high quality): every light with its
  * own number of samples (lr_light.samples), each a light sample and a BSDF
  * sample combined by the power heuristic; otherwise secondary_light_samples
- * lights picked by power.  Rays that escape add nothing.  Deterministic for
- * a seed; the film is accumulated as by lr_render. */
+ * lights picked by power.  Rays that escape add nothing.  The random
+ * numbers are fixed by the seed, but a path's unoccluded shadow-ray radiance
+ * is added with float atomics in scheduling order, so films are reproducible
+ * up to FP32 summation order (not bit for bit); the film is accumulated as by
+ * lr_render. */
 typedef struct lr_direct_params {
     uint32_t max_generation;           /* RayTracer::maxRayGeneration: rays of
                                           generation 0 (camera) .. this are cast

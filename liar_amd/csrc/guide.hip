@@ -799,7 +799,8 @@ void wgrad_chunks(uint32_t n_pad, uint32_t& chunk_rows, uint32_t& chunks) {
 }
 
 int grow_training(lr_guide* g, uint32_t n_pad, uint32_t chunks, hipStream_t s) {
-    if (g->cap_rows >= n_pad && g->cap_chunks >= chunks) return LR_OK;
+    if (g->A && g->G && g->partial && g->row_loss && g->cap_rows >= n_pad && g->cap_chunks >= chunks)
+        return LR_OK;
     n_pad = std::max<uint32_t>(n_pad, (uint32_t)g->cap_rows);
     chunks = std::max(chunks, g->cap_chunks);
     {
@@ -810,6 +811,10 @@ int grow_training(lr_guide* g, uint32_t n_pad, uint32_t chunks, hipStream_t s) {
         if (g->row_loss) LR_HIP_CHECK(hipFree(g->row_loss));
         g->A = g->G = nullptr;
         g->partial = g->row_loss = nullptr;
+        // capacities drop with the buffers: a failed allocation below leaves
+        // a state that the next call grows again
+        g->cap_rows = 0;
+        g->cap_chunks = 0;
         LR_HIP_CHECK(hipMalloc(&g->A, sizeof(uint16_t) * kLayers * kW * (size_t)n_pad));
         LR_HIP_CHECK(hipMalloc(&g->G, sizeof(uint16_t) * kLayers * kW * (size_t)n_pad));
         LR_HIP_CHECK(hipMalloc(&g->partial, sizeof(float) * kParams * (size_t)chunks));

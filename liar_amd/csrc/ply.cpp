@@ -13,7 +13,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -120,13 +123,17 @@ int bad(const std::string& path, const std::string& what) {
     return lr::fail(LR_E_INVALID, "lr_load_ply(" + path + "): " + what);
 }
 
-}  // namespace
+// Fewest body bytes one record of `e` can take: its fixed-size values and
+// list counts in binary, one byte per number in ascii (each number is at
+// least one character).  Counts in the header are checked against it before
+// anything is sized by them, so a short file cannot ask for large buffers.
+uint64_t min_record_bytes(const Element& e, bool ascii) {
+    uint64_t b = 0;
+    for (const Property& pr : e.props) b += ascii ? 1 : type_size(pr.list ? pr.count : pr.item);
+    return b;
+}
 
-extern "C" {
-
-int lr_load_ply(const char* path, uint32_t flags, lr_mesh_data* out) {
-    if (!path || !out) return lr::fail(LR_E_INVALID, "lr_load_ply: NULL argument");
-    memset(out, 0, sizeof(*out));
+int load_ply(const char* path, uint32_t flags, lr_mesh_data* out) {
     const std::string p(path);
     Reader R;
     {
@@ -216,6 +223,9 @@ int lr_load_ply(const char* path, uint32_t flags, lr_mesh_data* out) {
         }
         if (is_v) {
             if (ix < 0 || iy < 0 || iz < 0) return bad(p, "vertex element without x, y, z");
+            const uint64_t left = R.buf.size() - std::min(R.pos, R.buf.size());
+            const uint64_t rec = min_record_bytes(e, fmt == 0);
+            if (rec && e.n > left / rec) return bad(p, "vertex element has more records than the file holds");
             if (e.n >= (1ull << 32)) return bad(p, "too many vertices");
             have_vertex = true;
             nv = e.n;
@@ -224,6 +234,11 @@ int lr_load_ply(const char* path, uint32_t flags, lr_mesh_data* out) {
             if (iu >= 0 && iv >= 0) uvs.resize(2 * e.n);
         }
         if (is_f && ifc < 0) return bad(p, "face element without vertex_indices");
+        {
+            const uint64_t left = R.buf.size() - std::min(R.pos, R.buf.size());
+            const uint64_t rec = min_record_bytes(e, fmt == 0);
+            if (rec && e.n > left / rec) return bad(p, "element " + e.name + " has more records than the file holds");
+        }
         std::vector<double> row(e.props.size());
         std::vector<uint64_t> idx;
         for (uint64_t r = 0; r < e.n; ++r) {
@@ -236,6 +251,11 @@ int lr_load_ply(const char* path, uint32_t flags, lr_mesh_data* out) {
                 double c;
                 if (!num(pr.count, c) || c < 0 || c != std::floor(c)) return bad(p, "bad list count in " + e.name);
                 const uint64_t cnt = (uint64_t)c;
+                {
+                    const uint64_t left = R.buf.size() - std::min(R.pos, R.buf.size());
+                    const uint64_t item = fmt == 0 ? 1 : type_size(pr.item);
+                    if (cnt > left / item) return bad(p, "list in " + e.name + " is longer than the file");
+                }
                 if ((int)k == ifc) idx.resize(cnt);
                 for (uint64_t q = 0; q < cnt; ++q) {
                     double x;
@@ -290,6 +310,24 @@ int lr_load_ply(const char* path, uint32_t flags, lr_mesh_data* out) {
         return lr::fail(LR_E_NOMEM, "lr_load_ply: out of host memory");
     }
     return LR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lr_load_ply(const char* path, uint32_t flags, lr_mesh_data* out) {
+    if (!path || !out) return lr::fail(LR_E_INVALID, "lr_load_ply: NULL argument");
+    memset(out, 0, sizeof(*out));
+    try {
+        return load_ply(path, flags, out);
+    } catch (const std::bad_alloc&) {
+        lr_mesh_data_free(out);
+        return lr::fail(LR_E_NOMEM, std::string("lr_load_ply(") + path + "): out of host memory");
+    } catch (const std::exception& ex) {
+        lr_mesh_data_free(out);
+        return lr::fail(LR_E_INVALID, std::string("lr_load_ply(") + path + "): " + ex.what());
+    }
 }
 
 void lr_mesh_data_free(lr_mesh_data* m) {

@@ -907,13 +907,16 @@ int build_observer(lr_scene* sc, float lo, float hi) {
 // inputs [8], direction [3], aux [4], weight [1]; per batch row inputs [8],
 // direction [3], weight [1].
 int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, size_t batch, hipStream_t s) {
-    if (sc->gcap >= paths && sc->grec >= records && sc->gbatch >= batch) return LR_OK;
+    if (sc->d_gwork && sc->gcap >= paths && sc->grec >= records && sc->gbatch >= batch) return LR_OK;
     LR_HIP_CHECK(hipStreamSynchronize(s));
-    if (sc->d_gwork) LR_HIP_CHECK(hipFree(sc->d_gwork));
-    sc->d_gwork = nullptr;
     paths = std::max(paths, sc->gcap);
     records = std::max(records, sc->grec);
     batch = std::max(batch, sc->gbatch);
+    // capacities drop to zero with the buffer, so a failed allocation below
+    // leaves a state that the next call grows again instead of trusting
+    if (sc->d_gwork) LR_HIP_CHECK(hipFree(sc->d_gwork));
+    sc->d_gwork = nullptr;
+    sc->gcap = sc->grec = sc->gbatch = 0;
     const size_t words = 56 * paths + 16 * records + 12 * batch;
     LR_HIP_CHECK(hipMalloc(&sc->d_gwork, sizeof(float) * words));
     sc->gcap = paths;
@@ -923,10 +926,11 @@ int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, size_t batch, 
 }
 
 int ensure_work(lr_scene* sc, size_t paths, hipStream_t s) {
-    if (sc->cap >= paths) return LR_OK;
+    if (sc->d_work && sc->d_counts && sc->cap >= paths) return LR_OK;
     LR_HIP_CHECK(hipStreamSynchronize(s));
     if (sc->d_work) LR_HIP_CHECK(hipFree(sc->d_work));
     sc->d_work = nullptr;
+    sc->cap = 0;  // a failed allocation below leaves no stale capacity behind
     // path state 4, two queues 2 * 9, hits 4 (32-bit words per path)
     LR_HIP_CHECK(hipMalloc(&sc->d_work, sizeof(float) * 26 * paths));
     if (!sc->d_counts) LR_HIP_CHECK(hipMalloc(&sc->d_counts, 64));
@@ -2173,15 +2177,15 @@ __global__ void k_dl_resolve(const uint32_t* scount, uint32_t scap, ShadowQueue 
 // per entry), their hits (4 words) and the shadow queue (10 words per ray +
 // an occlusion byte).
 int ensure_dl_work(lr_scene* sc, size_t qcap, size_t scap, hipStream_t s) {
-    if (sc->dl_qcap >= qcap && sc->dl_scap >= scap) return LR_OK;
+    if (sc->d_dlwork && sc->d_dlocc && sc->dl_qcap >= qcap && sc->dl_scap >= scap) return LR_OK;
     LR_HIP_CHECK(hipStreamSynchronize(s));
+    qcap = std::max(qcap, sc->dl_qcap);
+    scap = std::max(scap, sc->dl_scap);
     if (sc->d_dlwork) LR_HIP_CHECK(hipFree(sc->d_dlwork));
     if (sc->d_dlocc) LR_HIP_CHECK(hipFree(sc->d_dlocc));
     sc->d_dlwork = nullptr;
     sc->d_dlocc = nullptr;
     sc->dl_qcap = sc->dl_scap = 0;
-    qcap = std::max(qcap, sc->dl_qcap);
-    scap = std::max(scap, sc->dl_scap);
     LR_HIP_CHECK(hipMalloc(&sc->d_dlwork, sizeof(float) * (26 * qcap + 10 * scap)));
     LR_HIP_CHECK(hipMalloc(&sc->d_dlocc, scap));
     sc->dl_qcap = qcap;
@@ -2222,7 +2226,14 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
     const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
     const uint64_t n_pix = (uint64_t)R.width * R.height;
     const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 22);
-    const uint64_t pix_per_chunk = std::max<uint64_t>(1, chunk_req / ppp);
+    // The shadow queue holds slots rays per camera path and its 32-bit
+    // counters must not wrap: a chunk's slots * paths stays within 2^30.
+    const uint64_t kQueueMax = 1ull << 30;
+    const uint64_t slots_eff = std::max(A.slots, 1u);
+    if (slots_eff * ppp > kQueueMax)
+        return lr::fail(LR_E_INVALID, "lr_render_direct: light samples * spp * wavelengths exceed 2^30 rays per pixel");
+    const uint64_t pix_per_chunk =
+        std::min<uint64_t>(std::max<uint64_t>(1, chunk_req / ppp), kQueueMax / (slots_eff * ppp));
     const uint64_t cap64 = pix_per_chunk * ppp;
     if (cap64 >= (1ull << 26)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");
     const uint32_t cap = (uint32_t)cap64;

@@ -117,6 +117,29 @@ def test_ply_malformed_raise(tmp_path, text, msg):
         scenery.TriangleMesh.loadPly(p)
 
 
+@pytest.mark.parametrize("fmt", ["ascii", "binary_little_endian"])
+def test_ply_overstated_counts_raise_without_allocating(tmp_path, fmt):
+    """A header whose counts exceed what the body can hold is rejected before
+    any buffer is sized by it (a few-byte file must not ask for 50 GB)."""
+    head = (f"ply\nformat {fmt} 1.0\nelement vertex 4294967295\nproperty float x\nproperty float y\n"
+            "property float z\nend_header\n").encode()
+    p = tmp_path / "huge_v.ply"
+    p.write_bytes(head + (b"0 0 0\n" if fmt == "ascii" else bytes(12)))
+    with pytest.raises(la.LiarError, match="more records than the file"):
+        scenery.TriangleMesh.loadPly(p)
+    # one face whose list count claims 2^32 - 1 indices
+    head = (f"ply\nformat {fmt} 1.0\nelement vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+            "element face 1\nproperty list uint int vertex_indices\nend_header\n").encode()
+    if fmt == "ascii":
+        body = b"0 0 0\n1 0 0\n0 1 0\n4294967295 0 1 2\n"
+    else:
+        body = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], "<f4").tobytes() + np.array([2**32 - 1, 0, 1, 2], "<u4").tobytes()
+    p = tmp_path / "huge_f.ply"
+    p.write_bytes(head + body)
+    with pytest.raises(la.LiarError, match="longer than the file"):
+        scenery.TriangleMesh.loadPly(p)
+
+
 def test_triangulation_matches_liar_vectors():
     cases = json.loads((GOLDEN / "triangulation_vectors.json").read_text())
     assert len(cases) == 21
