@@ -175,6 +175,20 @@ int lr_qbvh_export_q8(const lr_qbvh* bvh, lr_node_q8* nodes);
 int lr_trace_closest(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n,
                      lr_hits_soa* hits, void* stream);
 
+/* lr_trace_closest without allocation: no LiAR counterpart.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * lr_trace_closest (hits are bit-identical); the traversal's work counter
+ * and stack spill live in a fixed 32 MiB device array per GPU, held until the
+ * process ends, instead of the context's grow-only per-stream scratch.  The
+ * persistent grid shrinks so that its spill fits; a tree whose stack needs
+ * more than the array holds for one workgroup returns LR_E_INVALID.  Every
+ * call on one GPU shares that array: calls that overlap on a GPU (other
+ * streams, other contexts, other threads) race and must be serialised by
+ * the caller. */
+int lr_trace_closest_noalloc(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n,
+                             lr_hits_soa* hits, void* stream);
+
 /* Any hit (occlusion) for n rays; occluded[i] = 1 or 0.  Replaces
  * SceneObject::isIntersecting (scene_object.h:110-113, triangle_mesh.cpp:321-330,
  * sphere.cpp:122-127, object_trees.h:137-143, list.cpp:137-150). */

@@ -165,6 +165,7 @@ SIGNATURES = {
     "lr_qbvh_export": (_I, [_P, _P, _P]),
     "lr_qbvh_export_q8": (_I, [_P, _P]),
     "lr_trace_closest": (_I, [_P, C.POINTER(RaysSoa), _U32, C.POINTER(HitsSoa), _P]),
+    "lr_trace_closest_noalloc": (_I, [_P, C.POINTER(RaysSoa), _U32, C.POINTER(HitsSoa), _P]),
     "lr_trace_any": (_I, [_P, C.POINTER(RaysSoa), _U32, _P, _P]),
     "lr_set_trace_config": (_I, [_P, C.POINTER(TraceConfig)]),
     "lr_get_trace_config": (_I, [_P, C.POINTER(TraceConfig)]),

@@ -28,6 +28,9 @@ struct lr_context {
     // resident workgroups per CU of each traversal kernel at its block size
     // (queried once: launches under graph capture make no runtime queries)
     std::unordered_map<const void*, int> occupancy;
+    // device address of this GPU's fixed traversal scratch (a file-scope
+    // __device__ array of trace.hip), used by lr_trace_closest_noalloc
+    void* fixed_scratch = nullptr;
 
     // Returns a device buffer of at least `bytes` for `stream` (locked).
     int scratch(hipStream_t stream, size_t bytes, void** out);

@@ -356,7 +356,11 @@ struct Launch {
 // Scratch layout per stream: [work counter, 256 B][spill stack: (cap - lds) x threads x 8 B].
 constexpr size_t kCounterBytes = 256;
 
-int prepare(lr_qbvh* bvh, uint32_t n, bool any, hipStream_t stream, Launch& L) {
+// Chooses the kernel and the persistent grid of a launch over n rays; the
+// scratch pointers are bound by the caller (prepare, prepare_fixed).
+// cache_occupancy = false: a missing occupancy entry is queried but not
+// stored (the map insertion would allocate host memory).
+int plan(lr_qbvh* bvh, uint32_t n, bool any, Launch& L, uint32_t& lds_stack, bool cache_occupancy = true) {
     lr_context* ctx = bvh->ctx;
     lr_trace_config cfg;
     {
@@ -377,22 +381,69 @@ int prepare(lr_qbvh* bvh, uint32_t n, bool any, hipStream_t stream, Launch& L) {
     if (occ == 0) {
         LR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(L.fn),
                                                                   (int)cfg.block, 0));
-        std::lock_guard<std::mutex> g(ctx->mu);
-        ctx->occupancy[reinterpret_cast<const void*>(L.fn)] = occ;
+        if (cache_occupancy) {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            ctx->occupancy[reinterpret_cast<const void*>(L.fn)] = occ;
+        }
     }
     occ = std::max(1, std::min(occ, (int)cfg.blocks_per_cu));
     const uint64_t want = ((uint64_t)n + cfg.block - 1) / cfg.block;
     const uint64_t maxg = (uint64_t)ctx->n_cu * (uint64_t)occ;
     L.grid = (uint32_t)std::max<uint64_t>(1, std::min(want, maxg));
     L.cap = (int)bvh->info.max_stack;
+    lds_stack = cfg.lds_stack;
+    return LR_OK;
+}
+
+// Bytes of scratch a launch needs: the work counter, then the spilled stack
+// entries below the LDS part, (cap - lds_stack) x grid x block x 8 B.
+size_t scratch_bytes(const Launch& L, uint32_t lds_stack) {
     size_t bytes = kCounterBytes;
-    if ((uint32_t)L.cap > cfg.lds_stack) bytes += (size_t)(L.cap - cfg.lds_stack) * L.grid * cfg.block * sizeof(uint2);
-    void* p = nullptr;
-    if (int rc = ctx->scratch(stream, bytes, &p)) return rc;
+    if ((uint32_t)L.cap > lds_stack) bytes += (size_t)(L.cap - lds_stack) * L.grid * L.block * sizeof(uint2);
+    return bytes;
+}
+
+void bind_scratch(Launch& L, uint32_t lds_stack, void* p) {
     L.counter = static_cast<uint32_t*>(p);
-    L.spill = (uint32_t)L.cap > cfg.lds_stack
-                  ? reinterpret_cast<uint2*>(static_cast<char*>(p) + kCounterBytes)
-                  : nullptr;
+    L.spill = (uint32_t)L.cap > lds_stack ? reinterpret_cast<uint2*>(static_cast<char*>(p) + kCounterBytes) : nullptr;
+}
+
+int prepare(lr_qbvh* bvh, uint32_t n, bool any, hipStream_t stream, Launch& L) {
+    uint32_t lds = 0;
+    if (int rc = plan(bvh, n, any, L, lds)) return rc;
+    void* p = nullptr;
+    if (int rc = bvh->ctx->scratch(stream, scratch_bytes(L, lds), &p)) return rc;
+    bind_scratch(L, lds, p);
+    return LR_OK;
+}
+
+// The fixed scratch of lr_trace_closest_noalloc: one array per GPU, alive
+// until the process ends.  Host code never uses this name as an address:
+// lr_context_create looks up the device address of the current GPU's copy
+// (lr_context::fixed_scratch) and launches receive that pointer.
+constexpr size_t kFixedScratchBytes = 32u << 20;
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+
+// As prepare, on the context's fixed scratch: the persistent grid shrinks
+// until its spill slices fit (the kernel's spill stride is its own grid), and
+// a tree too deep for even one workgroup's slice is rejected.
+int prepare_fixed(lr_qbvh* bvh, uint32_t n, Launch& L) {
+    lr_context* ctx = bvh->ctx;
+    if (!ctx->fixed_scratch) return lr::fail(LR_E_DEVICE, "fixed traversal scratch was not set up for this context");
+    uint32_t lds = 0;
+    if (int rc = plan(bvh, n, false, L, lds, false)) return rc;
+    if ((uint32_t)L.cap > lds) {
+        const size_t per_block = (size_t)(L.cap - lds) * L.block * sizeof(uint2);
+        const size_t fit = (kFixedScratchBytes - kCounterBytes) / per_block;
+        if (fit == 0)
+            return lr::fail(LR_E_INVALID, "lr_trace_closest_noalloc: a traversal stack of " + std::to_string(L.cap) +
+                                              " entries needs more than the fixed " +
+                                              std::to_string(kFixedScratchBytes >> 20) + " MiB scratch holds");
+        L.grid = (uint32_t)std::min<size_t>(L.grid, fit);
+    }
+    if (scratch_bytes(L, lds) > kFixedScratchBytes)
+        return lr::fail(LR_E_DEVICE, "lr_trace_closest_noalloc: launch does not fit the fixed scratch");
+    bind_scratch(L, lds, ctx->fixed_scratch);
     return LR_OK;
 }
 
@@ -502,10 +553,16 @@ int lr_context_create(int device, lr_context** out) {
     LR_HIP_CHECK(hipGetDeviceProperties(&prop, device));
     if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
         return lr::fail(LR_E_UNSUPPORTED, std::string("device is ") + prop.gcnArchName + ", built for gfx950 only");
+    // device address of this GPU's copy of the fixed scratch (the device is
+    // current); lr_trace_closest_noalloc uses it and never the array's name
+    void* fixed = nullptr;
+    LR_HIP_CHECK(hipGetSymbolAddress(&fixed, HIP_SYMBOL(scratch)));
+    if (!fixed) return lr::fail(LR_E_DEVICE, "no device address for the fixed traversal scratch");
     lr_context* ctx = new (std::nothrow) lr_context();
     if (!ctx) return lr::fail(LR_E_NOMEM, "out of host memory");
     ctx->device = device;
     ctx->n_cu = prop.multiProcessorCount;
+    ctx->fixed_scratch = fixed;
     *out = ctx;
     return LR_OK;
 }
@@ -722,6 +779,28 @@ int lr_trace_closest(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits_
     }
     Launch L;
     if (int rc = prepare(bvh, n, false, s, L)) return rc;
+    return launch(bvh, L, ra, n, ha, nullptr, s);
+}
+
+int lr_trace_closest_noalloc(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits_soa* hits, void* stream) {
+    if (!bvh) return lr::fail(LR_E_INVALID, "NULL qbvh");
+    if (!bvh->ctx) return lr::fail(LR_E_INVALID, "QBVH was built host-only (NULL context); it cannot be traced");
+    if (int rc = check_rays(rays, n)) return rc;
+    if (!hits || (n && (!hits->prim || !hits->t || !hits->u || !hits->v)))
+        return lr::fail(LR_E_INVALID, "a hit array pointer is NULL");
+    if (n == 0) return LR_OK;
+    if (int rc = lr::bind(bvh->ctx)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const RaysArg ra{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tnear, rays->tfar, nullptr};
+    const HitsArg ha{hits->prim, hits->t, hits->u, hits->v};
+    if (bvh->info.n_nodes == 0) {
+        hipLaunchKernelGGL(k_fill_miss, dim3((n + 255) / 256), dim3(256), 0, s, n, ha);
+        LR_HIP_CHECK(hipGetLastError());
+        return LR_OK;
+    }
+    Launch L;
+    if (int rc = prepare_fixed(bvh, n, L)) return rc;
+    // launch() clears the work counter with a kernel (k_zero_u32) first
     return launch(bvh, L, ra, n, ha, nullptr, s);
 }
 
