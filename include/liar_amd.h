@@ -318,7 +318,7 @@ typedef struct lr_spectrum {
  * maximum exceeds 1 is divided by that maximum). */
 #define LR_MAT_NONE 0u          /* no BSDF (absorbs)                         */
 #define LR_MAT_LAMBERT 1u       /* lambert.cpp:76-124                        */
-#define LR_MAT_MIRROR 2u        /* mirror.cpp:99-161 (no fuzz)              */
+#define LR_MAT_MIRROR 2u        /* mirror.cpp:99-161, with fuzz               */
 #define LR_MAT_DIELECTRIC 3u    /* dielectric.cpp:148-247                    */
 #define LR_MAT_CONDUCTOR 4u     /* conductor.cpp:125-198                     */
 #define LR_MAT_COOK_TORRANCE 5u /* cook_torrance.cpp:189-294 with the
@@ -336,8 +336,46 @@ typedef struct lr_material {
     uint32_t kappa;         /* Conductor/CookTorrance k; Dielectric outer IOR */
     uint32_t emission;      /* Unshaded emission seen at generation 0        */
     uint32_t light;         /* index of the area light this surface is       */
-    float roughness;        /* CookTorrance: alpha = max(roughness, 1e-3)^2  */
+    float roughness;        /* CookTorrance roughnessU: alpha_u =
+                               max(roughness, 1e-3)^2 along the BSDF frame's
+                               u tangent (cook_torrance.cpp:189-200)         */
+    float roughness_v;      /* CookTorrance roughnessV (alpha_v likewise)    */
+    float fuzz;             /* Mirror fuzz (mirror.cpp:142-161), >= 0; 0 is a
+                               perfect mirror                               */
 } lr_material;
+
+/* ---- batched shading queries ---------------------------------------------
+ * One BSDF, its inputs already looked up at one wavelength, in its local
+ * frame (z = shading normal towards the viewer, wi.z > 0): what
+ * Shader::bsdf(sample, context) returns for a hit (shader.h:60-87). */
+typedef struct lr_bsdf {
+    uint32_t type;           /* LR_MAT_LAMBERT .. LR_MAT_COOK_TORRANCE_BECKMANN */
+    float reflectance;       /* Lambert diffuse / reflectance (Reflectant)   */
+    float transmittance;     /* Dielectric                                   */
+    float eta, kappa;        /* Conductor / CookTorrance n, k (>= 1e-9)      */
+    float ior;               /* Dielectric n_incident / n_transmitted        */
+    float roughness_u, roughness_v; /* CookTorrance                          */
+    float fuzz;              /* Mirror                                       */
+} lr_bsdf;
+
+/* Bsdf::evaluate(wi, wo, all caps) for n direction pairs (device arrays,
+ * 3 floats each) -> value, pdf (0, 0 where the BSDF has no value), with the
+ * geometric normal equal to the shading normal (bsdf.h:163, bsdf.cpp:52-76). */
+int lr_bsdf_eval(lr_context* ctx, const lr_bsdf* bsdf, const float* wi, const float* wo, uint32_t n,
+                 float* value, float* pdf, void* stream);
+/* Bsdf::sample(wi, (u0, u1), u2, caps) for n inputs (device; u: 3 floats
+ * each) -> wo (3 floats), value, pdf (pdf 0: no sample).  allow: bit 0
+ * reflection, bit 1 transmission caps (bsdf.h:164, bsdf.cpp:80-95). */
+int lr_bsdf_sample(lr_context* ctx, const lr_bsdf* bsdf, const float* wi, const float* u, uint32_t n,
+                   uint32_t allow, float* wo, float* value, float* pdf, void* stream);
+/* A spectrum's value at n wavelengths [nm] (device arrays), as the renderer
+ * evaluates it for one sampled wavelength: Sampled::doEvaluate /
+ * Spectral::fromSampled in Single mode (sampled.cpp:102-131,
+ * spectral.cpp:185-210: 0 outside [w0, wn) and at the last wavelength,
+ * reflectant != 0 divides by the table maximum when it exceeds 1), BlackBody
+ * (black_body.cpp:96-118).  Synchronous. */
+int lr_spectrum_eval(lr_context* ctx, const lr_spectrum* spectrum, const float* wavelengths, uint32_t n,
+                     uint32_t reflectant, float* out, void* stream);
 
 /* Lights (src/liar/scenery/light_area.cpp, light_sky.cpp). */
 #define LR_LIGHT_PARALLELOGRAM 0u /* LightArea over a Parallelogram         */
@@ -411,6 +449,16 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* geometry, const uint32_t* prim_mat
                     const lr_spectrum* spectra, uint32_t n_spectra,
                     const lr_light* lights, uint32_t n_lights, lr_scene** out);
 int lr_scene_destroy(lr_scene* scene);
+
+/* Shading normals of the scene's triangles: TriangleMesh vertex normals,
+ * interpolated at the hit's barycentric (u, v) (triangle_mesh.cpp:334-358,
+ * flipped towards the viewer and used for the BSDF frame,
+ * intersection_context.cpp:197-223,278-291).  corner_normals holds 9 floats
+ * per triangle, (n0, n1, n2) of its corners in primitive-id order, for
+ * n_triangles = the scene's triangle count; a triangle whose 9 values are all
+ * zero keeps its geometric normal.  NULL removes them.  Call before
+ * rendering and before lr_batch_create takes the scene. */
+int lr_scene_set_normals(lr_scene* scene, const float* corner_normals, uint32_t n_triangles);
 
 /* Renders with LiAR's AdjointPhotonTracer integrand
  * (src/liar/tracers/adjoint_photon_tracer.cpp:77-292) as a wavefront path

@@ -7,7 +7,7 @@ interfaces on top of that C-ABI (``scenery``, ``shaders``, ``textures``,
 ``spectra``, ``cameras``, ``samplers``, ``tracers``, ``output``,
 ``RenderEngine``).
 """
-from . import cameras, output, samplers, scenery, shaders, spectra, textures, tracers  # noqa: F401
+from . import cameras, output, samplers, scenery, shaders, shading, spectra, textures, tracers  # noqa: F401
 from ._native import LIB_PATH, LiarError  # noqa: F401
 from .engine import RenderEngine, compile_scene  # noqa: F401
 from .qbvh import Context, Qbvh, philox4x32, random_rays, random_triangles  # noqa: F401

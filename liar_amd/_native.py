@@ -63,7 +63,22 @@ class Spectrum(C.Structure):
 class Material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("reflectance", C.c_uint32), ("transmittance", C.c_uint32),
                 ("eta", C.c_uint32), ("kappa", C.c_uint32), ("emission", C.c_uint32), ("light", C.c_uint32),
-                ("roughness", C.c_float)]
+                ("roughness", C.c_float), ("roughness_v", C.c_float), ("fuzz", C.c_float)]
+
+
+def material_struct(m: dict) -> "Material":
+    """lr_material of a compiled material dict (roughness_v defaults to
+    roughness: isotropic; fuzz to 0: a perfect mirror)."""
+    return Material(m["type"], m["reflectance"], m["transmittance"], m["eta"], m["kappa"], m["emission"],
+                    m["light"], float(m["roughness"]), float(m.get("roughness_v", m["roughness"])),
+                    float(m.get("fuzz", 0.0)))
+
+
+class BsdfDesc(C.Structure):
+    """lr_bsdf."""
+    _fields_ = [("type", C.c_uint32), ("reflectance", C.c_float), ("transmittance", C.c_float),
+                ("eta", C.c_float), ("kappa", C.c_float), ("ior", C.c_float), ("roughness_u", C.c_float),
+                ("roughness_v", C.c_float), ("fuzz", C.c_float)]
 
 
 class Light(C.Structure):
@@ -192,6 +207,10 @@ SIGNATURES = {
     "lr_set_standard_observer": (_I, [_P, _P, _U32]),
     "lr_scene_create": (_I, [_P, _P, _P, _P, _U32, _P, _U32, _P, _U32, C.POINTER(_P)]),
     "lr_scene_destroy": (_I, [_P]),
+    "lr_scene_set_normals": (_I, [_P, _P, _U32]),
+    "lr_bsdf_eval": (_I, [_P, _P, _P, _P, _U32, _P, _P, _P]),
+    "lr_bsdf_sample": (_I, [_P, _P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
+    "lr_spectrum_eval": (_I, [_P, _P, _P, _U32, _U32, _P, _P]),
     "lr_render": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, _P, C.POINTER(RenderStats), _P]),
     "lr_render_direct": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(DirectParams), _P, _P,
                               C.POINTER(RenderStats), _P]),

@@ -37,7 +37,7 @@ struct DevLight {
 
 struct DevMaterial {
     uint32_t type, refl, trans, eta, kappa, emission, light;
-    float roughness;
+    float roughness, roughness_v, fuzz;  // CookTorrance roughness along u / v; Mirror fuzz
 };
 
 // Observer tables restricted to the sampling domain (n entries).
@@ -212,17 +212,18 @@ __device__ inline V3 uniform_sphere(float u1, float u2) {
 }
 
 // ---------------------------------------------------------------------------
-// BSDFs in the local frame (z = normal flipped towards the incoming ray,
-// omega_in.z > 0).  Shading normal == geometric normal, so LiAR's
-// geometric-hemisphere caps masking (src/liar/kernel/bsdf.cpp:52-76) reduces
-// to the sign of omega_out.z.
+// BSDFs in the local frame (z = shading normal flipped towards the incoming
+// ray, omega_in.z > 0).  LiAR masks the caps by the geometric hemispheres
+// (src/liar/kernel/bsdf.cpp:52-76): bsdf_evaluate takes the geometric
+// normal in the local frame for that test.
 // ---------------------------------------------------------------------------
 struct BsdfParams {
     uint32_t type;
-    float refl, trans;  // spectral values at the path's wavelength
-    float eta, kappa;   // conductor n, k
-    float ior;          // dielectric n_incident / n_transmitted
-    float alpha;        // Cook-Torrance roughness^2
+    float refl, trans;       // spectral values at the path's wavelength
+    float eta, kappa;        // conductor n, k
+    float ior;               // dielectric n_incident / n_transmitted
+    float alpha_u, alpha_v;  // Cook-Torrance max(roughness, 1e-3)^2 along the frame's u / v
+    float fuzz;              // Mirror fuzz (0: a perfect mirror)
 };
 
 __device__ inline bool bsdf_is_diffuse_or_glossy(uint32_t type) {
@@ -242,27 +243,64 @@ __device__ inline float fresnel_conductor(float eta, float kappa, float cos_i) {
     return 0.5f * (r_orth2 + r_par2);
 }
 
-// Trowbridge-Reitz D and pdf of h (microfacet_trowbridge_reitz.cpp:46-58),
-// isotropic alpha.
-__device__ inline float ggx_d(V3 h, float alpha, float& pdf_h) {
-    const float hx = h.x / alpha, hy = h.y / alpha;
+// MicrofacetDistribution::samplePhi (src/liar/kernel/microfacet.cpp:102-123).
+__device__ inline float sample_phi(float u, float au, float av) {
+    if (au == av) return 2.0f * kPi * u;
+    const float phi = atanf(av / au * tanf(2.0f * kPi * u));
+    if (u <= 0.25f) return phi;
+    if (u < 0.75f) return phi + kPi;
+    return phi + 2.0f * kPi;
+}
+
+// Trowbridge-Reitz D and pdf of h (microfacet_trowbridge_reitz.cpp:46-58).
+__device__ inline float ggx_d(V3 h, float au, float av, float& pdf_h) {
+    const float hx = h.x / au, hy = h.y / av;
     const float q = hx * hx + hy * hy + h.z * h.z;
-    const float d = 1.0f / (kPi * alpha * alpha * q * q);
+    const float d = 1.0f / (kPi * au * av * q * q);
     pdf_h = d * h.z;
     return d;
 }
 
-// Beckmann D and pdf of h (microfacet_beckmann.cpp:50-72), isotropic alpha.
-__device__ inline float beckmann_d(V3 h, float alpha, float& pdf_h) {
+// Beckmann D and pdf of h (microfacet_beckmann.cpp:50-72).
+__device__ inline float beckmann_d(V3 h, float au, float av, float& pdf_h) {
     const float c2 = h.z * h.z;
     if (c2 == 0.0f) {
         pdf_h = 0.0f;
         return 0.0f;
     }
-    const float a2 = alpha * alpha;
-    const float d = expf(-(h.x * h.x + h.y * h.y) / (a2 * c2)) / (kPi * a2 * c2 * c2);
+    const float hx = h.x / au, hy = h.y / av;
+    const float d = expf(-(hx * hx + hy * hy) / c2) / (kPi * au * av * c2 * c2);
     pdf_h = d * h.z;
     return d;
+}
+
+// sampleH of the two distributions: Trowbridge-Reitz
+// (microfacet_trowbridge_reitz.cpp:62-72), Beckmann (microfacet_beckmann.cpp:
+// 76-106, whose anisotropic branch draws phi from u1, the number that also
+// sets the radial part, as the reference does).
+__device__ inline V3 sample_h(uint32_t type, float au, float av, float u1, float u2) {
+    if (type == LR_MAT_COOK_TORRANCE) {
+        const float phi = sample_phi(u1, au, av);
+        const float s = sqrtf(u2 / (1.0f - u2));
+        return normalize(v3(s * au * cosf(phi), s * av * sinf(phi), 1.0f));
+    }
+    const float s = logf(1.0f - u1);
+    float cp, sp, tan2;
+    if (au == av) {
+        const float phi = 2.0f * kPi * u2;
+        cp = cosf(phi);
+        sp = sinf(phi);
+        tan2 = -s * au * au;
+    } else {
+        const float phi = sample_phi(u1, au, av);
+        cp = cosf(phi);
+        sp = sinf(phi);
+        const float a = cp / au, b = sp / av;
+        tan2 = -s / (a * a + b * b);
+    }
+    const float cos2 = 1.0f / (1.0f + tan2);
+    const float ct = sqrtf(cos2), st = sqrtf(fmaxf(1.0f - cos2, 0.0f));
+    return v3(cp * st, sp * st, ct);
 }
 
 // Cook-Torrance V-cavity G (cook_torrance.cpp:234-238).
@@ -271,19 +309,20 @@ __device__ inline float ct_g(V3 wi, V3 wo, V3 h) {
     return fminf(1.0f, fminf(2.0f * h.z * wi.z / doh, 2.0f * h.z * wo.z / doh));
 }
 
-// Bsdf::evaluate for the non-delta BSDFs; returns false for "no value".
+// doEvaluate of the non-delta BSDFs; returns false for "no value".
 __device__ inline bool bsdf_eval(const BsdfParams& P, V3 wi, V3 wo, float& value, float& pdf) {
     if (wo.z == 0.0f) return false;
-    if (!(wo.z > 0.0f)) return false;  // transmission masked for reflective BSDFs
     if (P.type == LR_MAT_LAMBERT) {  // lambert.cpp:105-114
         value = P.refl / kPi;
         pdf = fabsf(wo.z) / kPi;
         return pdf > 0.0f;
     }
     if (P.type == LR_MAT_COOK_TORRANCE || P.type == LR_MAT_COOK_TORRANCE_BECKMANN) {  // cook_torrance.cpp:262-280
+        if (!(wo.z > 0.0f)) return false;
         const V3 h = normalize(wi + wo);
         float pdf_h;
-        const float d = P.type == LR_MAT_COOK_TORRANCE ? ggx_d(h, P.alpha, pdf_h) : beckmann_d(h, P.alpha, pdf_h);
+        const float d = P.type == LR_MAT_COOK_TORRANCE ? ggx_d(h, P.alpha_u, P.alpha_v, pdf_h)
+                                                       : beckmann_d(h, P.alpha_u, P.alpha_v, pdf_h);
         const float g = ct_g(wi, wo, h);
         const float woh = dot(wi, h);
         pdf = pdf_h / (4.0f * woh);
@@ -292,6 +331,15 @@ __device__ inline bool bsdf_eval(const BsdfParams& P, V3 wi, V3 wo, float& value
         return pdf > 0.0f;
     }
     return false;  // delta BSDFs evaluate to zero (mirror.cpp:135-138 etc.)
+}
+
+// Bsdf::evaluate (bsdf.cpp:52-76): a wo on the other side of the geometric
+// normal (ngl, local frame, towards the viewer) than wi masks the reflection
+// caps, and every BSDF here evaluates reflection only.
+__device__ inline bool bsdf_evaluate(const BsdfParams& P, V3 wi, V3 wo, V3 ngl, float& value, float& pdf) {
+    if (wo.z == 0.0f) return false;
+    if ((dot(wi, ngl) > 0.0f) != (dot(wo, ngl) > 0.0f)) return false;
+    return bsdf_eval(P, wi, wo, value, pdf);
 }
 
 // Bsdf::sample.  u1, u2: 2D sample; uc: component sample.
@@ -307,7 +355,14 @@ __device__ inline bool bsdf_sample(const BsdfParams& P, V3 wi, float u1, float u
     }
     case LR_MAT_MIRROR: {  // mirror.cpp:142-161: value = reflectance, pdf = cos
         wo = v3(-wi.x, -wi.y, wi.z);
-        value = P.refl;
+        if (P.fuzz > 0.0f) {
+            // fuzz: the reflection plus fuzz x a uniform-sphere point, left
+            // unnormalised (its z is the pdf, so value |cos| / pdf = refl)
+            wo = wo + P.fuzz * uniform_sphere(u1, u2);
+            value = wo.z > 0.0f ? P.refl : 0.0f;
+        } else {
+            value = P.refl;
+        }
         pdf = wo.z;
         return pdf > 0.0f;
     }
@@ -342,21 +397,9 @@ __device__ inline bool bsdf_sample(const BsdfParams& P, V3 wi, float u1, float u
         }
         return pdf > 0.0f;
     }
-    case LR_MAT_COOK_TORRANCE: {  // cook_torrance.cpp:284-294, GGX sampleH
-        const float phi = 2.0f * kPi * u1;
-        const float s = sqrtf(u2 / (1.0f - u2));
-        const V3 h = normalize(v3(s * P.alpha * cosf(phi), s * P.alpha * sinf(phi), 1.0f));
-        wo = (2.0f * dot(wi, h)) * h - wi;
-        if (!(wo.z > 0.0f)) return false;
-        return bsdf_eval(P, wi, wo, value, pdf);
-    }
-    case LR_MAT_COOK_TORRANCE_BECKMANN: {  // microfacet_beckmann.cpp:76-106 (isotropic)
-        const float s = logf(1.0f - u1);
-        const float phi = 2.0f * kPi * u2;
-        const float tan2 = -s * P.alpha * P.alpha;
-        const float cos2 = 1.0f / (1.0f + tan2);
-        const float ct = sqrtf(cos2), st = sqrtf(fmaxf(1.0f - cos2, 0.0f));
-        const V3 h = v3(cosf(phi) * st, sinf(phi) * st, ct);
+    case LR_MAT_COOK_TORRANCE:
+    case LR_MAT_COOK_TORRANCE_BECKMANN: {  // cook_torrance.cpp:284-294
+        const V3 h = sample_h(P.type, P.alpha_u, P.alpha_v, u1, u2);
         wo = (2.0f * dot(wi, h)) * h - wi;
         if (!(wo.z > 0.0f)) return false;
         return bsdf_eval(P, wi, wo, value, pdf);
@@ -364,6 +407,113 @@ __device__ inline bool bsdf_sample(const BsdfParams& P, V3 wi, float u1, float u
     default:
         return false;
     }
+}
+
+// ---------------------------------------------------------------------------
+// hit -> surface frame (IntersectionContext, src/liar/kernel/
+// intersection_context.cpp:41-72,197-223,278-291)
+// ---------------------------------------------------------------------------
+
+// Per-primitive shading geometry of a scene (by primitive id):
+//   geom[p]: triangle (unit geometric normal, corner-normal record index or
+//            LR_NONE as bits); sphere (centre, radius)
+//   tan[p]:  triangle frame for the geometric normal: (unit vector, code),
+//            code 0 = any orthonormal pair (both edges shorter than
+//            sqrt(1e-3)), 1 = the vector is u (v = n x u), 2 = it is v
+//            (u = v x n)
+//   cn[4 r]: corner-normal record r: (n0, e1.x) (n1, e1.y) (n2, e1.z) (e2, 0)
+struct GeomArgs {
+    const float4* geom;
+    const float4* tan;
+    const float4* cn;
+};
+
+struct HitFrame {
+    V3 x;       // hit point
+    V3 ns;      // shading normal, towards the viewer
+    V3 ng;      // geometric normal, towards the viewer
+    V3 tt, bb;  // BSDF frame tangents (u, v)
+    bool leaving;
+    bool has_ns;  // ns comes from corner normals (else ns == ng)
+};
+
+// bsdfToLocal's choice: u = ns.reject(dP/du), v = ns.reject(dP/dv); both
+// shorter than sqrt(1e-3): any orthonormal pair; then the shorter one
+// is replaced by the cross product with ns of the longer one.
+__device__ inline void frame_from_derivatives(V3 ns, V3 dpdu, V3 dpdv, V3& tt, V3& bb) {
+    V3 u = dpdu - dot(dpdu, ns) * ns;
+    V3 v = dpdv - dot(dpdv, ns) * ns;
+    const float su = dot(u, u), sv = dot(v, v);
+    if (fmaxf(su, sv) < 1e-3f) onb(ns, u, v);
+    if (su >= sv) v = cross(ns, u);
+    else u = cross(v, ns);
+    tt = normalize(u);
+    bb = normalize(v);
+}
+
+__device__ inline HitFrame hit_frame(const GeomArgs& G, uint32_t n_triangles, uint32_t prim, V3 o, V3 d, float tnear,
+                                     float t, float bu, float bv) {
+    HitFrame F;
+    F.x = o + t * d;
+    const float4 g = G.geom[prim];
+    F.has_ns = false;
+    if (prim < n_triangles) {
+        const V3 ng = v3(g.x, g.y, g.z);
+        F.leaving = dot(d, ng) > 0.0f;  // triangle_mesh.cpp:304-309
+        const uint32_t rec = __float_as_uint(g.w);
+        F.ng = dot(ng, d) > 0.0f ? -ng : ng;
+        if (rec == LR_NONE) {
+            F.ns = F.ng;
+            const float4 tv = G.tan[prim];
+            const uint32_t code = __float_as_uint(tv.w);
+            const V3 a = v3(tv.x, tv.y, tv.z);
+            if (code == 1u) {
+                F.tt = a;
+                F.bb = normalize(cross(F.ns, a));
+            } else if (code == 2u) {
+                F.bb = a;
+                F.tt = normalize(cross(a, F.ns));
+            } else {
+                onb(F.ns, F.tt, F.bb);
+            }
+        } else {
+            // interpolated corner normals (triangle_mesh.cpp:334-358)
+            const float4* r = G.cn + 4 * (size_t)rec;
+            const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            const float b0 = 1.0f - bu - bv;
+            V3 ns = normalize(b0 * v3(r0.x, r0.y, r0.z) + bu * v3(r1.x, r1.y, r1.z) + bv * v3(r2.x, r2.y, r2.z));
+            if (dot(ns, d) > 0.0f) ns = -ns;
+            F.ns = ns;
+            F.has_ns = true;
+            frame_from_derivatives(ns, v3(r0.w, r1.w, r2.w), v3(r3.x, r3.y, r3.z), F.tt, F.bb);
+        }
+    } else {
+        const V3 c = v3(g.x, g.y, g.z);
+        const float r = g.w;
+        const V3 n = normalize(F.x - c);
+        const V3 s0 = (o + tnear * d) - c;
+        F.leaving = dot(s0, s0) < r * r;  // sphere.cpp:110-111
+        // (phi / 2pi, theta / pi) parameter derivatives (sphere.cpp:130-171)
+        const float st = sqrtf(fmaxf(0.0f, 1.0f - n.z * n.z));
+        const V3 dpdu = (2.0f * kPi * r) * v3(-n.y, n.x, 0.0f);
+        V3 dpdv;
+        if (st != 0.0f) {
+            const float cs = n.z / st;
+            dpdv = (kPi * r) * v3(cs * n.x, cs * n.y, -st);
+        } else {
+            const float phi = atan2f(n.y, n.x);
+            dpdv = (kPi * r) * v3(cosf(phi), sinf(phi), 0.0f);
+        }
+        F.ng = dot(n, d) > 0.0f ? -n : n;
+        F.ns = F.ng;
+        frame_from_derivatives(F.ns, dpdu, dpdv, F.tt, F.bb);
+    }
+    return F;
+}
+
+// The geometric normal in the BSDF frame (bsdf_evaluate's hemisphere test).
+__device__ inline V3 local_geometric_normal(const HitFrame& F) {
+    return F.has_ns ? v3(dot(F.ng, F.tt), dot(F.ng, F.bb), dot(F.ng, F.ns)) : v3(0.0f, 0.0f, 1.0f);
 }
 
 // ---------------------------------------------------------------------------

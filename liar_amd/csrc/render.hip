@@ -74,7 +74,7 @@ struct BatchArgs {
     const DevScene* scenes;
     const Params* params;
     const DevCamera* cams;
-    const float4* const* geoms;  // per scene: prim_geom
+    const GeomArgs* geoms;       // per scene: shading geometry
     const uint32_t* roots;
     const SceneFilm* films;
 };
@@ -395,7 +395,7 @@ __global__ void k_gather_batch(uint32_t cap, const uint32_t* __restrict__ count,
 // -- one generation: emission + scatter (adjoint_photon_tracer.cpp:140-292) --
 template <bool GUIDED, bool BATCH>
 __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Queue q, const uint32_t* count,
-                        Hits h, const float4* __restrict__ prim_geom0, PathState ps, Queue qn, uint32_t* count_next,
+                        Hits h, GeomArgs GA0, PathState ps, Queue qn, uint32_t* count_next,
                         GuideArgs G, BatchArgs B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < *count;
@@ -406,13 +406,13 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
         pid = q.pid[i];
         DevScene S = S0;
         Params P = P0;
-        const float4* __restrict__ prim_geom = prim_geom0;
+        GeomArgs Gm = GA0;
         uint64_t gp = path0 + pid;  // the path's index in its scene (random streams)
         if (BATCH) {
             const Segment sg = B.seg[find_segment(B, pid, false)];
             S = B.scenes[sg.scene];
             P = B.params[sg.scene];
-            prim_geom = B.geoms[sg.scene];
+            Gm = B.geoms[sg.scene];
             gp = (uint64_t)sg.pixel0 * (P.spp * P.nl) + (pid - sg.path_begin);
             root = q.root[i];
         }
@@ -434,21 +434,9 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                 }
             }
         } else {
-            const float t = h.t[i];
-            const V3 x = o + t * d;
-            const float4 g = prim_geom[prim];
-            V3 ng;
-            bool leaving;
-            if (prim < S.n_triangles) {
-                ng = v3(g.x, g.y, g.z);
-                leaving = dot(d, ng) > 0.0f;  // triangle_mesh.cpp:304-309
-            } else {
-                const V3 c = v3(g.x, g.y, g.z);
-                ng = (1.0f / g.w) * (x - c);
-                const V3 s0 = (o + tnear * d) - c;
-                leaving = dot(s0, s0) < g.w * g.w;  // sphere.cpp:110-111
-            }
-            const V3 nf = dot(ng, d) < 0.0f ? ng : -ng;  // flipped towards the viewer
+            const HitFrame F = hit_frame(Gm, S.n_triangles, prim, o, d, tnear, h.t[i], h.u[i], h.v[i]);
+            const V3 x = F.x, nf = F.ns, tt = F.tt, bb = F.bb;
+            const bool leaving = F.leaving;
             const DevMaterial m = S.mats[S.prim_mat[prim]];
             if (gen > 0 && m.light != LR_NONE) {
                 const DevLight& Lk = S.lights[m.light];
@@ -465,11 +453,11 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                 B.eta = fmaxf(spectrum_eval(S, m.eta, lam, false), 1e-9f);
                 B.kappa = fmaxf(spectrum_eval(S, m.kappa, lam, false), 1e-9f);
                 B.ior = leaving ? B.eta / B.kappa : B.kappa / B.eta;  // dielectric.cpp:150-155
-                const float r = fmaxf(m.roughness, 1e-3f);
-                B.alpha = r * r;
-
-                V3 tt, bb;
-                onb(nf, tt, bb);
+                const float ru = fmaxf(m.roughness, 1e-3f), rv = fmaxf(m.roughness_v, 1e-3f);
+                B.alpha_u = ru * ru;
+                B.alpha_v = rv * rv;
+                B.fuzz = m.fuzz;
+                const V3 ngl = local_geometric_normal(F);
                 const V3 wiw = -d;
                 const V3 wi = v3(dot(wiw, tt), dot(wiw, bb), fmaxf(dot(wiw, nf), 1.1920929e-7f));
                 const lr::u4 r0 = rng(P, gp, gen, 1u);
@@ -520,7 +508,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                                 for (uint32_t j2 = 0; j2 < S.n_lights; ++j2)
                                     if (j2 != k) pl += S.lights[j2].sel_pdf * light_pdf(S.lights[j2], x, dw);
                                 wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
-                                ok = bsdf_eval(B, wi, wo, f, pb) && f != 0.0f;
+                                ok = bsdf_evaluate(B, wi, wo, ngl, f, pb) && f != 0.0f;
                                 pg = vmf_mix_pdf(mx, dw);
                                 pc = cl * pl;
                             }
@@ -529,7 +517,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         const lr::u4 r2 = rng(P, gp, gen, 3u);
                         dw = vmf_mix_sample(mx, lr::u01(r2.x), lr::u01(r2.y), lr::u01(r2.z));
                         wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
-                        ok = bsdf_eval(B, wi, wo, f, pb) && f != 0.0f;
+                        ok = bsdf_evaluate(B, wi, wo, ngl, f, pb) && f != 0.0f;
                         if (ok) {
                             for (uint32_t k = 0; k < S.n_lights; ++k)
                                 pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], x, dw);
@@ -574,7 +562,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         pl *= (1.0f - strategy);
                         wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
                         float vb, pb;
-                        if (bsdf_eval(B, wi, wo, vb, pb) && vb != 0.0f) {
+                        if (bsdf_evaluate(B, wi, wo, ngl, vb, pb) && vb != 0.0f) {
                             const float pbs = pb * strategy;
                             value = vb * (pl * pl) / (pl * pl + pbs * pbs);
                             pdf = pl;
@@ -585,7 +573,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                 if (ok && pdf > 0.0f) {
                     const float cos_t = wo.z;
                     thr *= value * fabsf(cos_t) / pdf;
-                    start = x + ((cos_t > 0.0f ? 10.0f : -10.0f) * kTolerance) * nf;
+                    start = x + ((cos_t > 0.0f ? 10.0f : -10.0f) * kTolerance) * F.ng;  // adjoint_photon_tracer.cpp:177
                     dir = normalize(wo.x * tt + wo.y * bb + wo.z * nf);
                     if (GUIDED && G.record && gen < G.record_depth && rec_fcp > 0.0f && isfinite(thr) && thr > 0.0f) {
                         const uint32_t slot = atomicAdd(G.count, 1u);
@@ -790,7 +778,9 @@ struct lr_scene {
     lr_qbvh* bvh = nullptr;
     uint32_t n_triangles = 0, n_lights = 0;
     uint32_t* d_prim_mat = nullptr;
-    float4* d_prim_geom = nullptr;
+    float4* d_prim_geom = nullptr;  // GeomArgs::geom
+    float4* d_prim_tan = nullptr;   // GeomArgs::tan
+    float4* d_corner = nullptr;     // GeomArgs::cn (lr_scene_set_normals)
     DevMaterial* d_mats = nullptr;
     DevSpectrum* d_spectra = nullptr;
     float* d_table = nullptr;
@@ -831,12 +821,15 @@ void free_scene(lr_scene* s) {
     if (s->ev_in) (void)hipEventDestroy(s->ev_in);
     if (s->ev_out) (void)hipEventDestroy(s->ev_out);
     if (s->cstream) (void)hipStreamDestroy(s->cstream);
-    void* ptrs[] = {s->d_prim_mat, s->d_prim_geom, s->d_mats, s->d_spectra, s->d_table, s->d_lights,
-                    s->d_obs,      s->d_work,      s->d_counts, s->d_gwork, s->d_dlwork, s->d_dlocc};
+    void* ptrs[] = {s->d_prim_mat, s->d_prim_geom, s->d_prim_tan, s->d_corner, s->d_mats,   s->d_spectra,
+                    s->d_table,    s->d_lights,    s->d_obs,      s->d_work,   s->d_counts, s->d_gwork,
+                    s->d_dlwork,   s->d_dlocc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
 }
+
+GeomArgs geom_args(const lr_scene* sc) { return GeomArgs{sc->d_prim_geom, sc->d_prim_tan, sc->d_corner}; }
 
 int build_observer(lr_scene* sc, float lo, float hi) {
     if (sc->d_obs && sc->obs_lo == lo && sc->obs_hi == hi) return LR_OK;
@@ -940,36 +933,15 @@ int ensure_work(lr_scene* sc, size_t paths, hipStream_t s) {
 
 }  // namespace
 
-extern "C" {
+namespace lr {
 
-int lr_set_standard_observer(const float* w, const float* xyz, uint32_t n) {
-    if (!w || !xyz || n < 2) return lr::fail(LR_E_INVALID, "observer needs at least two wavelengths");
-    for (uint32_t k = 0; k + 1 < n; ++k)
-        if (!(w[k + 1] > w[k])) return lr::fail(LR_E_INVALID, "observer wavelengths must be strictly increasing");
-    std::lock_guard<std::mutex> g(g_obs_mu);
-    g_obs_w.assign(w, w + n);
-    g_obs_xyz.assign(xyz, xyz + 3 * (size_t)n);
-    return LR_OK;
-}
-
-int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material, const lr_material* materials,
-                    uint32_t n_materials, const lr_spectrum* spectra, uint32_t n_spectra, const lr_light* lights,
-                    uint32_t n_lights, lr_scene** out) {
-    if (!ctx || !bvh || !out) return lr::fail(LR_E_INVALID, "NULL context, geometry or out");
-    *out = nullptr;
-    if (bvh->ctx != ctx) return lr::fail(LR_E_INVALID, "geometry belongs to another context");
-    const uint32_t n_prims = bvh->info.n_prims;
-    if (n_prims && !prim_material) return lr::fail(LR_E_INVALID, "prim_material is NULL");
-    if ((n_materials && !materials) || (n_spectra && !spectra) || (n_lights && !lights))
-        return lr::fail(LR_E_INVALID, "NULL materials / spectra / lights array");
-    {
-        std::lock_guard<std::mutex> g(g_obs_mu);
-        if (g_obs_w.size() < 2) return lr::fail(LR_E_INVALID, "no standard observer: call lr_set_standard_observer first");
-    }
-    auto spec_ok = [&](uint32_t i) { return i == LR_NONE || i < n_spectra; };
-    // spectra -> device table
-    std::vector<DevSpectrum> dspec(n_spectra);
-    std::vector<float> table;
+// lr_spectrum descriptions -> device records and their value table (sampled
+// wavelengths then values, binned values), with LiAR's Reflectant scale
+// (spectral.cpp:200-207) and the BlackBody peak normalisation.
+int build_spectra(const lr_spectrum* spectra, uint32_t n_spectra, std::vector<DevSpectrum>& dspec,
+                  std::vector<float>& table) {
+    dspec.assign(n_spectra, DevSpectrum{});
+    table.clear();
     for (uint32_t i = 0; i < n_spectra; ++i) {
         const lr_spectrum& s = spectra[i];
         DevSpectrum& d = dspec[i];
@@ -1013,6 +985,42 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
             return lr::fail(LR_E_INVALID, "unknown spectrum kind " + std::to_string(s.kind));
         }
     }
+    return LR_OK;
+}
+
+}  // namespace lr
+
+extern "C" {
+
+int lr_set_standard_observer(const float* w, const float* xyz, uint32_t n) {
+    if (!w || !xyz || n < 2) return lr::fail(LR_E_INVALID, "observer needs at least two wavelengths");
+    for (uint32_t k = 0; k + 1 < n; ++k)
+        if (!(w[k + 1] > w[k])) return lr::fail(LR_E_INVALID, "observer wavelengths must be strictly increasing");
+    std::lock_guard<std::mutex> g(g_obs_mu);
+    g_obs_w.assign(w, w + n);
+    g_obs_xyz.assign(xyz, xyz + 3 * (size_t)n);
+    return LR_OK;
+}
+
+int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material, const lr_material* materials,
+                    uint32_t n_materials, const lr_spectrum* spectra, uint32_t n_spectra, const lr_light* lights,
+                    uint32_t n_lights, lr_scene** out) {
+    if (!ctx || !bvh || !out) return lr::fail(LR_E_INVALID, "NULL context, geometry or out");
+    *out = nullptr;
+    if (bvh->ctx != ctx) return lr::fail(LR_E_INVALID, "geometry belongs to another context");
+    const uint32_t n_prims = bvh->info.n_prims;
+    if (n_prims && !prim_material) return lr::fail(LR_E_INVALID, "prim_material is NULL");
+    if ((n_materials && !materials) || (n_spectra && !spectra) || (n_lights && !lights))
+        return lr::fail(LR_E_INVALID, "NULL materials / spectra / lights array");
+    {
+        std::lock_guard<std::mutex> g(g_obs_mu);
+        if (g_obs_w.size() < 2) return lr::fail(LR_E_INVALID, "no standard observer: call lr_set_standard_observer first");
+    }
+    auto spec_ok = [&](uint32_t i) { return i == LR_NONE || i < n_spectra; };
+    // spectra -> device table
+    std::vector<DevSpectrum> dspec;
+    std::vector<float> table;
+    if (int rc = lr::build_spectra(spectra, n_spectra, dspec, table)) return rc;
     std::vector<DevMaterial> dmat(n_materials);
     for (uint32_t i = 0; i < n_materials; ++i) {
         const lr_material& m = materials[i];
@@ -1022,28 +1030,48 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
             return lr::fail(LR_E_INVALID, "material " + std::to_string(i) + " references a missing spectrum");
         if (m.light != LR_NONE && m.light >= n_lights)
             return lr::fail(LR_E_INVALID, "material " + std::to_string(i) + " references a missing light");
-        dmat[i] = DevMaterial{m.type, m.reflectance, m.transmittance, m.eta, m.kappa, m.emission, m.light, m.roughness};
+        if (!(m.fuzz >= 0.0f) || !std::isfinite(m.fuzz))
+            return lr::fail(LR_E_INVALID, "material " + std::to_string(i) + ": fuzz must be finite and >= 0");
+        dmat[i] = DevMaterial{m.type,     m.reflectance, m.transmittance, m.eta,        m.kappa,
+                              m.emission, m.light,       m.roughness,     m.roughness_v, m.fuzz};
     }
     std::vector<uint32_t> pm(prim_material, prim_material + n_prims);
     for (uint32_t i = 0; i < n_prims; ++i)
         if (pm[i] >= n_materials) return lr::fail(LR_E_INVALID, "prim " + std::to_string(i) + " has no material");
-    // per-prim geometry by primitive id: triangle unit normal / sphere (c, r)
-    std::vector<float4> geom(n_prims);
+    // per-prim shading geometry by primitive id (GeomArgs): triangle unit
+    // normal + corner-normal record (none yet) and its BSDF frame's fixed
+    // tangent (intersection_context.cpp:197-223 with dP/du, dP/dv = the
+    // edges); sphere (c, r)
+    std::vector<float4> geom(n_prims), tan(n_prims, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    const float kNoRecord = lr::as_f32(LR_NONE);
     for (const lr_prim& q : bvh->host.prims) {
         const uint32_t id = lr::as_u32(q.a[3]);
         if (lr::as_u32(q.b[3]) == LR_PRIM_TRIANGLE) {
-            double nx = (double)q.b[1] * q.c[2] - (double)q.b[2] * q.c[1];
-            double ny = (double)q.b[2] * q.c[0] - (double)q.b[0] * q.c[2];
-            double nz = (double)q.b[0] * q.c[1] - (double)q.b[1] * q.c[0];
-            const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+            const double e1[3] = {q.b[0], q.b[1], q.b[2]}, e2[3] = {q.c[0], q.c[1], q.c[2]};
+            double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
             if (len > 0) {
-                nx /= len;
-                ny /= len;
-                nz /= len;
+                for (double& c : n) c /= len;
             } else {
-                nz = 1.0;
+                n[0] = n[1] = 0.0;
+                n[2] = 1.0;
             }
-            geom[id] = make_float4((float)nx, (float)ny, (float)nz, 0.0f);
+            geom[id] = make_float4((float)n[0], (float)n[1], (float)n[2], kNoRecord);
+            double u[3], v[3];
+            const double du = e1[0] * n[0] + e1[1] * n[1] + e1[2] * n[2];
+            const double dv = e2[0] * n[0] + e2[1] * n[1] + e2[2] * n[2];
+            for (int k = 0; k < 3; ++k) {
+                u[k] = e1[k] - du * n[k];
+                v[k] = e2[k] - dv * n[k];
+            }
+            const double su = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            const double sv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (std::max(su, sv) >= 1e-3) {
+                const double* a = su >= sv ? u : v;
+                const double la = std::sqrt(su >= sv ? su : sv);
+                tan[id] = make_float4((float)(a[0] / la), (float)(a[1] / la), (float)(a[2] / la),
+                                      lr::as_f32(su >= sv ? 1u : 2u));
+            }
         } else {
             geom[id] = make_float4(q.a[0], q.a[1], q.a[2], q.b[0]);
         }
@@ -1135,6 +1163,7 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
     int rc = lr::bind(ctx);
     if (!rc) rc = upload(pm, &sc->d_prim_mat);
     if (!rc) rc = upload(geom, &sc->d_prim_geom);
+    if (!rc) rc = upload(tan, &sc->d_prim_tan);
     if (!rc) rc = upload(dmat, &sc->d_mats);
     if (!rc) rc = upload(dspec, &sc->d_spectra);
     if (!rc) rc = upload(table, &sc->d_table);
@@ -1144,6 +1173,55 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
         return rc;
     }
     *out = sc;
+    return LR_OK;
+}
+
+int lr_scene_set_normals(lr_scene* sc, const float* cn, uint32_t n_triangles) {
+    if (!sc) return lr::fail(LR_E_INVALID, "NULL scene");
+    if (n_triangles != sc->n_triangles)
+        return lr::fail(LR_E_INVALID, "lr_scene_set_normals: " + std::to_string(n_triangles) +
+                                          " triangles given, the scene has " + std::to_string(sc->n_triangles));
+    if (int rc = lr::bind(sc->ctx)) return rc;
+    std::vector<float4> geom(sc->bvh->info.n_prims);
+    if (!geom.empty())
+        LR_HIP_CHECK(hipMemcpy(geom.data(), sc->d_prim_geom, sizeof(float4) * geom.size(), hipMemcpyDeviceToHost));
+    // records (n0, e1.x) (n1, e1.y) (n2, e1.z) (e2, 0) for triangles with
+    // normals; edges as the primitive records hold them
+    std::vector<float4> rec;
+    std::vector<uint32_t> slot(n_triangles, LR_NONE);
+    if (cn) {
+        for (uint32_t t = 0; t < n_triangles; ++t) {
+            const float* n = cn + 9 * (size_t)t;
+            bool any = false;
+            for (int k = 0; k < 9; ++k) {
+                if (!std::isfinite(n[k]))
+                    return lr::fail(LR_E_INVALID, "lr_scene_set_normals: non-finite normal of triangle " +
+                                                      std::to_string(t));
+                any = any || n[k] != 0.0f;
+            }
+            if (any) {
+                slot[t] = (uint32_t)(rec.size() / 4);
+                rec.resize(rec.size() + 4);
+            }
+        }
+        for (const lr_prim& q : sc->bvh->host.prims) {
+            const uint32_t id = lr::as_u32(q.a[3]);
+            if (lr::as_u32(q.b[3]) != LR_PRIM_TRIANGLE || slot[id] == LR_NONE) continue;
+            const float* n = cn + 9 * (size_t)id;
+            float4* r = rec.data() + 4 * (size_t)slot[id];
+            r[0] = make_float4(n[0], n[1], n[2], q.b[0]);
+            r[1] = make_float4(n[3], n[4], n[5], q.b[1]);
+            r[2] = make_float4(n[6], n[7], n[8], q.b[2]);
+            r[3] = make_float4(q.c[0], q.c[1], q.c[2], 0.0f);
+        }
+    }
+    for (uint32_t t = 0; t < n_triangles; ++t) geom[t].w = lr::as_f32(slot[t]);
+    LR_HIP_CHECK(hipDeviceSynchronize());
+    if (sc->d_corner) LR_HIP_CHECK(hipFree(sc->d_corner));
+    sc->d_corner = nullptr;
+    if (int rc = upload(rec, &sc->d_corner)) return rc;
+    if (!geom.empty())
+        LR_HIP_CHECK(hipMemcpy(sc->d_prim_geom, geom.data(), sizeof(float4) * geom.size(), hipMemcpyHostToDevice));
     return LR_OK;
 }
 
@@ -1382,10 +1460,10 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
                 if (int rc = lr_guide_infer(guide, g_in, n, counts + cur, g_mix, s)) return rc;
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), dim3((n + kAppendBlockWide - 1) / kAppendBlockWide),
                                    dim3(kAppendBlockWide), 0, s, S, P, gen, path0, q[cur], counts + cur, H,
-                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, GA, NB);
+                                   geom_args(sc), ps, q[nxt], counts + nxt, GA, NB);
             } else {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false, false>), agrd, ablk, 0, s, S, P, gen, path0, q[cur], counts + cur, H,
-                                   sc->d_prim_geom, ps, q[nxt], counts + nxt, GA, NB);
+                                   geom_args(sc), ps, q[nxt], counts + nxt, GA, NB);
             }
         }
         if (training) {
@@ -1454,7 +1532,7 @@ struct lr_batch {
     DevScene* d_scenes = nullptr;
     Params* d_params = nullptr;
     DevCamera* d_cams = nullptr;
-    const float4** d_geoms = nullptr;
+    GeomArgs* d_geoms = nullptr;
     SceneFilm* d_films = nullptr;
     Segment* d_seg = nullptr;
     size_t seg_cap = 0;
@@ -1573,7 +1651,7 @@ int lr_batch_create(lr_context* ctx, lr_scene* const* scenes, uint32_t n, lr_bat
     if (e == hipSuccess) e = hipMalloc(&b->d_scenes, sizeof(DevScene) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_params, sizeof(Params) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_cams, sizeof(DevCamera) * n);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_geoms, sizeof(float4*) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_geoms, sizeof(GeomArgs) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_films, sizeof(SceneFilm) * n);
     if (e == hipSuccess) e = hipMalloc(&b->d_counts, 64 * sizeof(uint32_t));
     if (e != hipSuccess) {
@@ -1599,7 +1677,7 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
     std::vector<DevScene> hs(n);
     std::vector<Params> hprm(n);
     std::vector<DevCamera> hcam(n);
-    std::vector<const float4*> hgeo(n);
+    std::vector<GeomArgs> hgeo(n);
     std::vector<SceneFilm> hfilm(n);
     std::vector<uint64_t> paths(n);
     for (uint32_t k = 0; k < n; ++k) {
@@ -1610,7 +1688,7 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
         lr_scene* sc = b->scenes[k];
         if (int rc = build_observer(sc, it.params.lambda_min, it.params.lambda_max)) return rc;
         hs[k] = dev_scene(sc);
-        hgeo[k] = sc->d_prim_geom;
+        hgeo[k] = geom_args(sc);
         hfilm[k] = SceneFilm{it.film_xyz, it.film_weight};
         paths[k] = (uint64_t)it.params.width * it.params.height * it.params.spp * it.params.wavelengths;
     }
@@ -1677,7 +1755,7 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
     LR_HIP_CHECK(hipMemcpy(b->d_scenes, hs.data(), sizeof(DevScene) * n, hipMemcpyHostToDevice));
     LR_HIP_CHECK(hipMemcpy(b->d_params, hprm.data(), sizeof(Params) * n, hipMemcpyHostToDevice));
     LR_HIP_CHECK(hipMemcpy(b->d_cams, hcam.data(), sizeof(DevCamera) * n, hipMemcpyHostToDevice));
-    LR_HIP_CHECK(hipMemcpy((void*)b->d_geoms, hgeo.data(), sizeof(float4*) * n, hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMemcpy((void*)b->d_geoms, hgeo.data(), sizeof(GeomArgs) * n, hipMemcpyHostToDevice));
     LR_HIP_CHECK(hipMemcpy(b->d_films, hfilm.data(), sizeof(SceneFilm) * n, hipMemcpyHostToDevice));
     if (b->cap < cap) {
         if (b->d_work) LR_HIP_CHECK(hipFree(b->d_work));
@@ -1868,7 +1946,7 @@ __device__ inline lr::u4 dl_rng(const Params& P, uint64_t gp, uint32_t gen, uint
 
 struct DLHit {
     BsdfParams B;
-    V3 wi, tt, bb, nf, start;
+    V3 wi, tt, bb, nf, ngl, start;
     float w, lam;
     uint64_t gp;
     uint32_t gen, br;
@@ -1926,7 +2004,7 @@ __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DL
         dir = light_sample(Lk, H.start, lr::u01(r.x), lr::u01(r.y), pl);
         if (!(pl > 0.0f)) return false;
         const V3 wo = v3(dot(dir, H.tt), dot(dir, H.bb), dot(dir, H.nf));
-        if (!bsdf_eval(H.B, H.wi, wo, f, pb) || f == 0.0f) return false;
+        if (!bsdf_evaluate(H.B, H.wi, wo, H.ngl, f, pb) || f == 0.0f) return false;
         c = H.w * f * Le * power_heuristic(nl * pl, nl * pb) * fabsf(wo.z) / (nl * pl * scale);
     } else {
         // BSDF sample that reaches the light (ray_tracer.cpp:234-264)
@@ -2016,7 +2094,7 @@ __device__ void stage_flush(ShadowStage& st, uint32_t fill, ShadowQueue sq, uint
 }
 
 __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t gen, Hits h,
-                                                  const float4* __restrict__ prim_geom, DLQueue q,
+                                                  GeomArgs Gm, DLQueue q,
                                                   const uint32_t* count, PathState ps, DLQueue qn,
                                                   uint32_t* count_next, ShadowQueue sq, uint32_t* scount,
                                                   uint64_t path0, DLArgs A) {
@@ -2044,20 +2122,13 @@ __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t
             H.gen = gen;
             H.gp = path0 + pid;
             H.lam = ps.lam[pid];
-            x = o + h.t[i] * d;
-            const float4 g = prim_geom[prim];
-            V3 ng;
-            bool leaving;
-            if (prim < S.n_triangles) {
-                ng = v3(g.x, g.y, g.z);
-                leaving = dot(d, ng) > 0.0f;
-            } else {
-                const V3 c = v3(g.x, g.y, g.z);
-                ng = (1.0f / g.w) * (x - c);
-                const V3 s0 = (o + tnear * d) - c;
-                leaving = dot(s0, s0) < g.w * g.w;
-            }
-            H.nf = dot(ng, d) < 0.0f ? ng : -ng;
+            const HitFrame F = hit_frame(Gm, S.n_triangles, prim, o, d, tnear, h.t[i], h.u[i], h.v[i]);
+            x = F.x;
+            const bool leaving = F.leaving;
+            H.nf = F.ns;
+            H.tt = F.tt;
+            H.bb = F.bb;
+            H.ngl = local_geometric_normal(F);
             const DevMaterial m = S.mats[S.prim_mat[prim]];
             // shader emission: camera rays and objects that are not lights
             // (direct_lighting.cpp:248-255)
@@ -2073,9 +2144,10 @@ __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t
                 H.B.eta = fmaxf(spectrum_eval(S, m.eta, H.lam, false), 1e-9f);
                 H.B.kappa = fmaxf(spectrum_eval(S, m.kappa, H.lam, false), 1e-9f);
                 H.B.ior = leaving ? H.B.eta / H.B.kappa : H.B.kappa / H.B.eta;
-                const float rgh = fmaxf(m.roughness, 1e-3f);
-                H.B.alpha = rgh * rgh;
-                onb(H.nf, H.tt, H.bb);
+                const float ru = fmaxf(m.roughness, 1e-3f), rv = fmaxf(m.roughness_v, 1e-3f);
+                H.B.alpha_u = ru * ru;
+                H.B.alpha_v = rv * rv;
+                H.B.fuzz = m.fuzz;
                 const V3 wiw = -d;
                 H.wi = v3(dot(wiw, H.tt), dot(wiw, H.bb), fmaxf(dot(wiw, H.nf), 1.1920929e-7f));
                 H.start = x + (10.0f * kTolerance) * H.nf;
@@ -2306,7 +2378,7 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
                 } else if (int rc = lr::trace_closest_queue(sc->bvh, &rs, glimit, counts + cur, &hs, s)) {
                     return rc;
                 }
-                hipLaunchKernelGGL(k_dl_shade, dim3(grid), blk, 0, s, S, P, gen, H, sc->d_prim_geom, q[cur],
+                hipLaunchKernelGGL(k_dl_shade, dim3(grid), blk, 0, s, S, P, gen, H, geom_args(sc), q[cur],
                                    counts + cur, ps, q[nxt], counts + nxt, sq, counts + 2, path0, A);
                 const uint8_t* occ = nullptr;
                 if (A.slots && sc->bvh->info.n_nodes) {

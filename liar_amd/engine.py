@@ -27,6 +27,7 @@ class CompiledScene:
     materials: list = field(default_factory=list)   # dicts (type, slots, roughness)
     spectra: list = field(default_factory=list)     # spectra.Spectrum objects
     lights: list = field(default_factory=list)      # dicts
+    tri_normals: np.ndarray = None  # (nt, 9) float32 corner normals, or None (geometric normals only)
 
 
 def _spectrum_of(tex, table, index):
@@ -49,7 +50,8 @@ def _scalar_of(tex, default):
 
 def _material_of(shader, table, index):
     m = {"type": N.MAT_NONE, "reflectance": N.LR_NONE, "transmittance": N.LR_NONE, "eta": N.LR_NONE,
-         "kappa": N.LR_NONE, "emission": N.LR_NONE, "light": N.LR_NONE, "roughness": 0.0}
+         "kappa": N.LR_NONE, "emission": N.LR_NONE, "light": N.LR_NONE, "roughness": 0.0, "roughness_v": 0.0,
+         "fuzz": 0.0}
     parts = shader.children if isinstance(shader, shaders.Sum) else ([shader] if shader is not None else [])
     bsdfs = [p for p in parts if not isinstance(p, shaders.Unshaded)]
     emit = [p for p in parts if isinstance(p, shaders.Unshaded)]
@@ -64,9 +66,10 @@ def _material_of(shader, table, index):
         if isinstance(s, shaders.Lambert):
             m.update(type=N.MAT_LAMBERT, reflectance=_spectrum_of(s.diffuse, table, index))
         elif isinstance(s, shaders.Mirror):
-            if s.fuzz is not None and _scalar_of(s.fuzz, 0.0) > 0:
-                raise NotImplementedError("Mirror.fuzz is not supported")
-            m.update(type=N.MAT_MIRROR, reflectance=_spectrum_of(s.reflectance, table, index))
+            fuzz = _scalar_of(s.fuzz, 0.0) if s.fuzz is not None else 0.0
+            if not (fuzz >= 0.0 and math.isfinite(fuzz)):
+                raise ValueError("Mirror.fuzz must be finite and >= 0")
+            m.update(type=N.MAT_MIRROR, reflectance=_spectrum_of(s.reflectance, table, index), fuzz=float(fuzz))
         elif isinstance(s, shaders.Dielectric):
             m.update(type=N.MAT_DIELECTRIC, reflectance=_spectrum_of(s.reflectance, table, index),
                      transmittance=_spectrum_of(s.transmittance, table, index),
@@ -78,13 +81,12 @@ def _material_of(shader, table, index):
                      kappa=_spectrum_of(s.absorptionCoefficient, table, index))
         elif isinstance(s, shaders.CookTorrance):
             ru, rv = _scalar_of(s.roughnessU, 0.1), _scalar_of(s.roughnessV, 0.1)
-            if ru != rv:
-                raise NotImplementedError("anisotropic CookTorrance roughness is not supported")
             t = N.MAT_COOK_TORRANCE if isinstance(s.mdf, shaders.MicrofacetTrowbridgeReitz) \
                 else N.MAT_COOK_TORRANCE_BECKMANN
             m.update(type=t, reflectance=_spectrum_of(s.reflectance, table, index),
                      eta=_spectrum_of(s.refractionIndex, table, index),
-                     kappa=_spectrum_of(s.absorptionCoefficient, table, index), roughness=float(ru))
+                     kappa=_spectrum_of(s.absorptionCoefficient, table, index), roughness=float(ru),
+                     roughness_v=float(rv))
         else:
             raise NotImplementedError(f"shader {type(s).__name__} is not supported")
     return m
@@ -92,7 +94,7 @@ def _material_of(shader, table, index):
 
 def compile_scene(scene) -> CompiledScene:
     """Flattens a scene graph into arrays (LiAR's object graph -> one QBVH)."""
-    verts, tris, tri_mat = [], [], []
+    verts, tris, tri_mat, tri_nrm = [], [], [], []
     sph, sph_mat = [], []
     materials, mat_index = [], {}
     table, spec_index = [], {}
@@ -108,11 +110,15 @@ def compile_scene(scene) -> CompiledScene:
             materials.append(m)
         return mat_index[key]
 
-    def add_mesh(v, t, mat):
+    def add_mesh(v, t, mat, cn=None):
         verts.append(np.asarray(v, np.float32))
         tris.append(np.asarray(t, np.uint32) + nverts[0])
         tri_mat.append(np.full(len(t), mat, np.uint32))
+        tri_nrm.append(cn if cn is not None else np.zeros((len(t), 9), np.float32))
+        tri_nrm_any[0] = tri_nrm_any[0] or cn is not None
         nverts[0] += len(v)
+
+    tri_nrm_any = [False]
 
     def point(M, p):
         p = np.asarray(p, np.float64)
@@ -134,7 +140,7 @@ def compile_scene(scene) -> CompiledScene:
             v = obj.vertices()
             if M is not None:
                 v = (np.asarray(v, np.float64) @ M[:3, :3].T + M[:3, 3]).astype(np.float32)
-            add_mesh(v, obj.triangles(), material(shader))
+            add_mesh(v, obj.triangles(), material(shader), obj.corner_normals(M))
         elif isinstance(obj, scenery.TriangleMeshComposite):
             # one triangle set is this object's rule; each child keeps its shader
             for c in obj.children():
@@ -142,7 +148,7 @@ def compile_scene(scene) -> CompiledScene:
                 v = c.vertices()
                 if M is not None:
                     v = (np.asarray(v, np.float64) @ M[:3, :3].T + M[:3, 3]).astype(np.float32)
-                add_mesh(v, c.triangles(), material(cs))
+                add_mesh(v, c.triangles(), material(cs), c.corner_normals(M))
         elif isinstance(obj, scenery.Parallelogram):
             v, t = obj._mesh()
             if M is not None:
@@ -188,7 +194,8 @@ def compile_scene(scene) -> CompiledScene:
     TM = np.concatenate(tri_mat) if tri_mat else np.zeros(0, np.uint32)
     S = np.asarray(sph, np.float32).reshape(-1, 4)
     PM = np.concatenate([TM, np.asarray(sph_mat, np.uint32)]).astype(np.uint32)
-    return CompiledScene(V, T, S, PM, materials, table, lights)
+    CN = np.concatenate(tri_nrm).astype(np.float32) if tri_nrm_any[0] else None
+    return CompiledScene(V, T, S, PM, materials, table, lights, CN)
 
 
 def camera_struct(cam: cameras.PerspectiveCamera) -> N.Camera:
@@ -214,9 +221,7 @@ class GpuScene:
         self.qbvh = Qbvh(ctx, cs.verts, cs.tris, cs.spheres)
         keep = []
         specs = (N.Spectrum * max(1, len(cs.spectra)))(*[s._describe(keep) for s in cs.spectra])
-        mats = (N.Material * max(1, len(cs.materials)))(*[
-            N.Material(m["type"], m["reflectance"], m["transmittance"], m["eta"], m["kappa"], m["emission"],
-                       m["light"], m["roughness"]) for m in cs.materials])
+        mats = (N.Material * max(1, len(cs.materials)))(*[N.material_struct(m) for m in cs.materials])
         ls = (N.Light * max(1, len(cs.lights)))()
         for i, L in enumerate(cs.lights):
             ls[i].type = L["type"]
@@ -232,6 +237,9 @@ class GpuScene:
         N.check(N.lib().lr_scene_create(ctx.handle, self.qbvh.handle, N.ptr(pm), mats, len(cs.materials), specs,
                                         len(cs.spectra), ls, len(cs.lights), C.byref(h)), "lr_scene_create")
         self._h = h
+        if cs.tri_normals is not None:
+            cn = np.ascontiguousarray(cs.tri_normals, np.float32)
+            N.check(N.lib().lr_scene_set_normals(h, N.ptr(cn), len(cs.tris)), "lr_scene_set_normals")
 
     def close(self):
         if getattr(self, "_h", None) is not None:

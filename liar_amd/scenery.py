@@ -18,25 +18,51 @@ class SceneObject:
 class TriangleMesh(SceneObject):
     """``TriangleMesh(vertices, normals, uvs, triangles)``; each triangle is a
     3-tuple of vertex indices or of ``(vertex[, normal[, uv]])`` index tuples
-    (src/liar/scenery/triangle_mesh.cpp:39-85).  Shading uses the geometric
-    normal."""
+    (src/liar/scenery/triangle_mesh.cpp:39-85), a None index meaning none.
+    A triangle whose three corners name normals shades with those normals
+    interpolated at the hit (triangle_mesh.cpp:334-358); otherwise, and for a
+    plain vertex index, with its geometric normal.  Texture coordinates are
+    kept but not used (the engine has constant textures only)."""
 
     def __init__(self, vertices, normals=(), uvs=(), triangles=()):
         super().__init__()
         self._vertices = np.asarray(vertices, dtype=np.float32).reshape(-1, 3)
         self._normals = normals if isinstance(normals, np.ndarray) else list(normals)
         self._uvs = uvs if isinstance(uvs, np.ndarray) else list(uvs)
+        self._normal_idx = None  # (nt, 3) int64, -1 = no normal
         if isinstance(triangles, np.ndarray) and triangles.ndim == 2 and triangles.shape[1] == 3:
             self._triangles = np.ascontiguousarray(triangles, dtype=np.uint32)
         else:
-            idx = []
+            idx, nidx = [], []
             for tri in triangles:
                 if len(tri) != 3:
                     raise ValueError("a triangle needs three corners")
                 idx.append([c[0] if isinstance(c, (tuple, list)) else c for c in tri])
+                nidx.append([c[1] if isinstance(c, (tuple, list)) and len(c) > 1 and c[1] is not None else -1
+                             for c in tri])
             self._triangles = np.asarray(idx, dtype=np.uint32).reshape(-1, 3)
+            nidx = np.asarray(nidx, dtype=np.int64).reshape(-1, 3)
+            if (nidx >= 0).any():
+                if nidx.max() >= len(self._normals):
+                    raise ValueError("triangle references a missing normal")
+                self._normal_idx = nidx
         if len(self._triangles) and self._triangles.max() >= len(self._vertices):
             raise ValueError("triangle references a missing vertex")
+
+    def corner_normals(self, M=None):
+        """(nt, 9) float32: each triangle's three corner normals, zero for a
+        triangle that does not name all three; None when no triangle has
+        normals.  M: a local-to-world 4x4 matrix; normals transform by its
+        inverse transpose (LiAR's normalTransform)."""
+        if self._normal_idx is None:
+            return None
+        ns = np.asarray(self._normals, np.float64).reshape(-1, 3)
+        if M is not None:
+            ns = ns @ np.linalg.inv(np.asarray(M, np.float64)[:3, :3])  # rows: (M^-T n)^T
+        ok = (self._normal_idx >= 0).all(axis=1)
+        out = np.zeros((len(self._triangles), 9), np.float64)
+        out[ok] = ns[self._normal_idx[ok]].reshape(-1, 9)
+        return out.astype(np.float32)
 
     @staticmethod
     def loadPly(path, fan_all=False):

@@ -22,6 +22,11 @@ class Spectrum:
     def _describe(self, keep: list) -> N.Spectrum:
         raise NotImplementedError
 
+    def evaluate(self, ctx, wavelengths_nm, reflectant=False):
+        """The values the GPU renderer uses at these wavelengths [nm]
+        (lr_spectrum_eval on ``ctx``'s device)."""
+        return _evaluate(self, ctx, wavelengths_nm, reflectant)
+
 
 class Sampled(Spectrum):
     """Piecewise-linear tabulated spectrum, zero outside the table
@@ -38,15 +43,58 @@ class Sampled(Spectrum):
         self.values = v
 
     def __call__(self, wavelength):
+        """``Sampled.__call__`` (sampled.cpp:102-123): linear interpolation,
+        the last value at the last wavelength, 0 outside the table.  (The
+        renderer's per-sample evaluation, Spectral::fromSampled, is 0 at the
+        last wavelength too: ``lr_spectrum_eval``.)"""
         w = float(_to_nm([wavelength])[0])
-        if w < self.wavelengths[0] or w >= self.wavelengths[-1]:
+        ws, vs = self.wavelengths.astype(np.float64), self.values.astype(np.float64)
+        if w < ws[0] or w > ws[-1]:
             return 0.0
-        return float(np.interp(w, self.wavelengths, self.values))
+        k = int(np.searchsorted(ws, w, side="right"))  # upper_bound
+        if k >= len(ws):
+            return float(vs[-1])
+        t = (w - ws[k - 1]) / (ws[k] - ws[k - 1])
+        return float(vs[k - 1] + t * (vs[k] - vs[k - 1]))
+
+    def resample(self, wavelengths):
+        """``Sampled.resample(wavelengths)`` (sampled.cpp:63-93): a Sampled
+        on the given ascending wavelengths, each value interpolated in the
+        table interval [w_k-1, w_k) that holds it, 0 below the first and from
+        the last wavelength on.  Wavelengths in metres or nanometres (kept
+        in nanometres, like every Sampled)."""
+        ws = _to_nm(np.asarray(wavelengths, np.float64))
+        tw, tv = self.wavelengths.astype(np.float64), self.values.astype(np.float64)
+        out = np.zeros(len(ws), np.float64)
+        i, n = 0, len(ws)
+        while i < n and ws[i] < tw[0]:
+            i += 1
+        for k in range(1, len(tw)):
+            while i < n and ws[i] < tw[k]:
+                out[i] = tv[k - 1] + (ws[i] - tw[k - 1]) / (tw[k] - tw[k - 1]) * (tv[k] - tv[k - 1])
+                i += 1
+        return Sampled(ws, out)
 
     def _describe(self, keep):
         keep += [self.wavelengths, self.values]
         return N.Spectrum(N.SPEC_SAMPLED, len(self.values), self.wavelengths.ctypes.data,
                           self.values.ctypes.data, 0.0, 0.0, 0.0, 0.0, 0.0)
+
+
+def _evaluate(sp, ctx, wavelengths_nm, reflectant):
+    import ctypes as C
+
+    import torch
+
+    w = torch.as_tensor(np.asarray(wavelengths_nm, np.float32), device=f"cuda:{ctx.device}")
+    out = torch.empty_like(w)
+    keep = []
+    desc = sp._describe(keep)
+    s = torch.cuda.current_stream(w.device).cuda_stream
+    N.check(N.lib().lr_spectrum_eval(ctx.handle, C.byref(desc), C.c_void_p(w.data_ptr()), len(w),
+                                     int(bool(reflectant)), C.c_void_p(out.data_ptr()), C.c_void_p(s)),
+            "lr_spectrum_eval")
+    return out.cpu().numpy()
 
 
 class Binned(Spectrum):

@@ -241,7 +241,7 @@ class _Spec(C.Structure):
 
 class _Mat(C.Structure):
     _fields_ = [(k, C.c_uint32) for k in ("type", "refl", "trans", "eta", "kappa", "emission", "light")] + \
-               [("roughness", C.c_float)]
+               [("roughness", C.c_float), ("roughness_v", C.c_float), ("fuzz", C.c_float)]
 
 
 class _Light(C.Structure):
@@ -303,7 +303,7 @@ def _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_dep
             observer, threads, max_sample_luminance, dl_cfg):
     L = lib()
     if not hasattr(L, "_render_sig"):
-        head = [C.c_void_p] * 2 + [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+        head = [C.c_void_p] * 2 + [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.c_void_p, C.c_void_p]
         L.or_render.argtypes = head + [C.c_void_p, C.c_void_p, C.c_int]
@@ -320,7 +320,9 @@ def _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_dep
     specs = (_Spec * max(1, len(cs.spectra)))(*[_spec_struct(s, keep) for s in cs.spectra])
     mats = (_Mat * max(1, len(cs.materials)))(*[
         _Mat(m["type"], m["reflectance"], m["transmittance"], m["eta"], m["kappa"], m["emission"], m["light"],
-             m["roughness"]) for m in cs.materials])
+             m["roughness"], m.get("roughness_v", m["roughness"]), m.get("fuzz", 0.0)) for m in cs.materials])
+    CN = getattr(cs, "tri_normals", None)
+    CN = None if CN is None else np.ascontiguousarray(CN, np.float32)
     lights = (_Light * max(1, len(cs.lights)))()
     for i, Ld in enumerate(cs.lights):
         lights[i].type = Ld["type"]
@@ -345,7 +347,8 @@ def _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_dep
     oxyz = np.ascontiguousarray(observer[1], np.float32)
     xyz = np.zeros((3, height, width), np.float64)
     wgt = np.zeros((height, width), np.float64)
-    args = [V.ctypes.data, T.ctypes.data, len(T), S.ctypes.data if len(S) else None, len(S), PM.ctypes.data, mats,
+    args = [V.ctypes.data, T.ctypes.data, len(T), S.ctypes.data if len(S) else None, len(S),
+            CN.ctypes.data if CN is not None else None, PM.ctypes.data, mats,
             specs, len(cs.spectra), lights, len(cs.lights), ow.ctypes.data, oxyz.ctypes.data, len(ow),
             C.byref(cam), C.byref(P)]
     tail = [xyz.ctypes.data, wgt.ctypes.data, threads or default_threads()]
@@ -361,3 +364,106 @@ def set_planted_mistake(k: int) -> None:
     """Test hook of the oracle renderer: 1 drops the path tracer's MIS
     weights, 2 those of the DirectLighting light estimate."""
     lib().or_set_planted_mistake(int(k))
+
+
+# ---------------------------------------------------------------------------
+# single-function probes of the renderer oracle (BSDFs, distributions,
+# spectra), for the tests that pin them
+# ---------------------------------------------------------------------------
+
+MAT_LAMBERT, MAT_MIRROR, MAT_DIELECTRIC, MAT_CONDUCTOR, MAT_GGX, MAT_BECKMANN = 1, 2, 3, 4, 5, 6
+
+
+def bsdf_params(kind, refl=1.0, trans=1.0, eta=1.0, kappa=0.0, ior=1.0, roughness_u=0.1, roughness_v=None,
+                fuzz=0.0):
+    """The oracle's Bsdf record: alpha = max(roughness, 1e-3)^2 per axis and
+    eta, kappa floored at 1e-9, as the shaders' doBsdf set them."""
+    rv = roughness_u if roughness_v is None else roughness_v
+    au, av = max(roughness_u, 1e-3) ** 2, max(rv, 1e-3) ** 2
+    return np.array([kind, refl, trans, max(eta, 1e-9), max(kappa, 1e-9), ior, au, av, fuzz], np.float64)
+
+
+def _probe_sigs(L):
+    if getattr(L, "_probe_sig", False):
+        return
+    P, U32, D = C.c_void_p, C.c_uint32, C.c_double
+    L.or_bsdf_eval_n.argtypes = [P, P, P, U32, P, P]
+    L.or_bsdf_sample_n.argtypes = [P, P, P, U32, U32, P, P]
+    L.or_fresnel_conductor.argtypes = [D, D, D]
+    L.or_fresnel_conductor.restype = D
+    L.or_mdf_d.argtypes = [U32, D, D, P, P]
+    L.or_mdf_d.restype = D
+    L.or_mdf_sample_h.argtypes = [U32, D, D, D, D, P]
+    L.or_spectrum_eval.argtypes = [P, P, U32, C.c_int, P]
+    L._probe_sig = True
+
+
+def bsdf_eval(prm, wi, wo):
+    """Bsdf doEvaluate of the oracle for wi, wo (n, 3) -> (ok bool (n,),
+    value (n,), pdf (n,)); no geometric-hemisphere masking (callers pass wo
+    on the shading side, as the renderer's bsdf_evaluate lets through)."""
+    L = lib()
+    _probe_sigs(L)
+    wi = np.ascontiguousarray(np.broadcast_to(wi, np.shape(wo)), np.float64)
+    wo = np.ascontiguousarray(wo, np.float64)
+    n = len(wo)
+    out = np.zeros((n, 2), np.float64)
+    ok = np.zeros(n, np.uint8)
+    L.or_bsdf_eval_n(_p(np.ascontiguousarray(prm, np.float64)), _p(wi), _p(wo), n, _p(out), _p(ok))
+    return ok.astype(bool), out[:, 0], out[:, 1]
+
+
+def bsdf_sample(prm, wi, u, allow=3):
+    """Bsdf::sample of the oracle: wi (3,) or (n, 3), u (n, 3) -> (ok, wo (n, 3),
+    value, pdf)."""
+    L = lib()
+    _probe_sigs(L)
+    u = np.ascontiguousarray(u, np.float64)
+    n = len(u)
+    wi = np.ascontiguousarray(np.broadcast_to(wi, (n, 3)), np.float64)
+    out = np.zeros((n, 5), np.float64)
+    ok = np.zeros(n, np.uint8)
+    L.or_bsdf_sample_n(_p(np.ascontiguousarray(prm, np.float64)), _p(wi), _p(u), n, int(allow), _p(out), _p(ok))
+    return ok.astype(bool), out[:, :3], out[:, 3], out[:, 4]
+
+
+def fresnel_conductor(eta, kappa, cos_i):
+    L = lib()
+    _probe_sigs(L)
+    return float(L.or_fresnel_conductor(float(eta), float(kappa), float(cos_i)))
+
+
+def mdf_d(kind, alpha_u, alpha_v, h):
+    """D(h) and pdf_h = D cos(theta_h) of the oracle's distribution (kind
+    MAT_GGX or MAT_BECKMANN) at unit vectors h (n, 3)."""
+    L = lib()
+    _probe_sigs(L)
+    h = np.ascontiguousarray(np.atleast_2d(h), np.float64)
+    d = np.empty(len(h))
+    p = np.empty(len(h))
+    pk = np.zeros(1, np.float64)
+    for i in range(len(h)):
+        d[i] = L.or_mdf_d(int(kind), float(alpha_u), float(alpha_v), _p(h[i]), _p(pk))
+        p[i] = pk[0]
+    return d, p
+
+
+def mdf_sample_h(kind, alpha_u, alpha_v, u1, u2):
+    L = lib()
+    _probe_sigs(L)
+    out = np.zeros(3, np.float64)
+    L.or_mdf_sample_h(int(kind), float(alpha_u), float(alpha_v), float(u1), float(u2), _p(out))
+    return out
+
+
+def spectrum_eval(spectrum, wavelengths_nm, reflectant=False):
+    """A liar_amd.spectra object evaluated by the oracle at wavelengths [nm],
+    as its renderer does for one sampled wavelength."""
+    L = lib()
+    _probe_sigs(L)
+    keep = []
+    sp = _spec_struct(spectrum, keep)
+    w = np.ascontiguousarray(wavelengths_nm, np.float64)
+    out = np.zeros(len(w), np.float64)
+    L.or_spectrum_eval(C.byref(sp), _p(w), len(w), int(bool(reflectant)), _p(out))
+    return out

@@ -35,8 +35,11 @@
 static const double PI = 3.14159265358979323846;
 static const double TOL = 1e-6;
 
-/* Test hook: 1 = drop both one-sample MIS weights (a planted mistake that
- * the furnace tests must detect).  Never set outside tests. */
+/* Test hook, a planted mistake that a pinning test must detect; never set
+ * outside tests.  1: drop both one-sample MIS weights (furnace tests);
+ * 2: drop the DirectLighting light-estimate weights; 3: CookTorrance pdf
+ * without the half-vector Jacobian 1 / (4 wo.h) (chi-square test);
+ * 5: microfacet D without its 1/pi (normalisation test). */
 static int g_plant = 0;
 void or_set_planted_mistake(int k) { g_plant = k; }
 
@@ -58,7 +61,7 @@ static double u01(uint32_t x) { return (double)(x >> 8) * (1.0 / 16777216.0); }
 
 /* ---- scene description (layouts of include/liar_amd.h) ---------------- */
 typedef struct { uint32_t kind, count; const float *w, *v; float value, lo, hi, temperature, scale; } or_spectrum;
-typedef struct { uint32_t type, refl, trans, eta, kappa, emission, light; float roughness; } or_material;
+typedef struct { uint32_t type, refl, trans, eta, kappa, emission, light; float roughness, roughness_v, fuzz; } or_material;
 typedef struct {
     uint32_t type, radiance, double_sided;
     float origin[3], dx[3], dy[3], direction[3], angle;
@@ -85,6 +88,7 @@ static V f3(const float* p) { return v(p[0], p[1], p[2]); }
 
 typedef struct {
     const float* verts; const uint32_t* tris; uint32_t nt;
+    const float* tri_normals;  /* 9 per triangle (corner normals) or NULL; all-zero = none */
     const float* spheres; uint32_t ns;
     const uint32_t* prim_mat;
     const or_material* mats;
@@ -258,6 +262,87 @@ static uint32_t intersect(const Scene* S, V o, V d, double tn, double* th) {
     return rid;
 }
 
+/* Barycentric (u, v) of a triangle hit (the Moller-Trumbore values). */
+static void tri_uv(const Scene* S, uint32_t k, V o, V d, double* bu, double* bv) {
+    const float* a = S->verts + 3 * (size_t)S->tris[3 * k];
+    const float* b = S->verts + 3 * (size_t)S->tris[3 * k + 1];
+    const float* c = S->verts + 3 * (size_t)S->tris[3 * k + 2];
+    V v0 = f3(a), e1 = sub(f3(b), v0), e2 = sub(f3(c), v0);
+    V p = cross(d, e2);
+    double det = dot(e1, p);
+    V s = sub(o, v0), q = cross(s, e1);
+    *bu = det != 0 ? dot(s, p) / det : 0;
+    *bv = det != 0 ? dot(d, q) / det : 0;
+}
+static void onb(V n, V* t, V* b);
+
+/* Surface frame of a hit (IntersectionContext, src/liar/kernel/
+ * intersection_context.cpp:41-72,197-223,278-291):
+ *   geometric normal ng; shading normal ns = the corner normals interpolated
+ *   at (u, v) when the mesh has them (triangle_mesh.cpp:334-358), else ng;
+ *   dP/du, dP/dv: triangle edges v1 - v0, v2 - v0 (barycentric parameters,
+ *   the mesh having no uvs), sphere (phi / 2pi, theta / pi) derivatives
+ *   (sphere.cpp:130-171); both normals flipped towards -d (flipTo); the BSDF
+ *   frame u = ns.reject(dP/du), v = ns.reject(dP/dv), or any orthonormal pair
+ *   when both are shorter than sqrt(1e-3), then completed by a cross product
+ *   with ns from the longer one (bsdfToLocal). */
+typedef struct { V x, ns, ng, tt, bb; int leaving, has_ns; } Frame;
+static void hit_frame(const Scene* S, uint32_t prim, V o, V d, double tn, double t, Frame* F) {
+    V x = add(o, mul(t, d)), ng, ns, dpdu, dpdv;
+    if (prim < S->nt) {
+        const float* a = S->verts + 3 * (size_t)S->tris[3 * prim];
+        const float* b = S->verts + 3 * (size_t)S->tris[3 * prim + 1];
+        const float* cc = S->verts + 3 * (size_t)S->tris[3 * prim + 2];
+        /* the edges as the library's primitive records hold them (FP32) */
+        dpdu = v((float)(b[0] - a[0]), (float)(b[1] - a[1]), (float)(b[2] - a[2]));
+        dpdv = v((float)(cc[0] - a[0]), (float)(cc[1] - a[1]), (float)(cc[2] - a[2]));
+        ng = nrm(cross(sub(f3(b), f3(a)), sub(f3(cc), f3(a))));
+        F->leaving = dot(d, ng) > 0;  /* triangle_mesh.cpp:304-309 */
+        ns = ng;
+        F->has_ns = 0;
+        if (S->tri_normals) {
+            const float* n = S->tri_normals + 9 * (size_t)prim;
+            V n0 = f3(n), n1 = f3(n + 3), n2 = f3(n + 6);
+            if (dot(n0, n0) + dot(n1, n1) + dot(n2, n2) > 0) {
+                double bu, bv;
+                tri_uv(S, prim, o, d, &bu, &bv);
+                ns = nrm(add(add(mul(1 - bu - bv, n0), mul(bu, n1)), mul(bv, n2)));
+                F->has_ns = 1;
+            }
+        }
+    } else {
+        const float* sp = S->spheres + 4 * (size_t)(prim - S->nt);
+        double r = sp[3];
+        ng = nrm(sub(x, f3(sp)));
+        V s0 = sub(add(o, mul(tn, d)), f3(sp));
+        F->leaving = dot(s0, s0) < r * r;  /* sphere.cpp:110-111 */
+        ns = ng;
+        F->has_ns = 0;
+        V N = ng;
+        double st = sqrt(fmax(0, 1 - N.z * N.z));
+        dpdu = mul(2 * PI * r, v(-N.y, N.x, 0));
+        if (st != 0) {
+            double cs = N.z / st;
+            dpdv = mul(PI * r, v(cs * N.x, cs * N.y, -st));
+        } else {
+            double phi = atan2(N.y, N.x);
+            dpdv = mul(PI * r, v(cos(phi), sin(phi), 0));
+        }
+    }
+    if (dot(ns, d) > 0) ns = mul(-1, ns);
+    if (dot(ng, d) > 0) ng = mul(-1, ng);
+    V u = sub(dpdu, mul(dot(dpdu, ns), ns)), w = sub(dpdv, mul(dot(dpdv, ns), ns));
+    double su = dot(u, u), sv = dot(w, w);
+    if (fmax(su, sv) < 1e-3) onb(ns, &u, &w);
+    if (su >= sv) w = cross(ns, u);
+    else u = cross(w, ns);
+    F->x = x;
+    F->ns = ns;
+    F->ng = ng;
+    F->tt = nrm(u);
+    F->bb = nrm(w);
+}
+
 /* ---- lights ------------------------------------------------------------------ */
 static double para_hit(const Scene* S, uint32_t k, V p, V d, double tn) {
     V n = S->light_n[k];
@@ -323,36 +408,88 @@ static uint32_t light_pick(const Scene* S, double u) {
     return k < S->n_lights ? k : S->n_lights - 1;
 }
 
-/* ---- BSDFs (local frame, z = normal towards the viewer) -------------------- */
-typedef struct { uint32_t type; double refl, trans, eta, kappa, ior, alpha; } Bsdf;
+/* ---- BSDFs (local frame, z = shading normal towards the viewer) ------------ */
+/* alpha_u, alpha_v: CookTorrance max(roughness, 1e-3)^2 along the frame's
+ * tangents (cook_torrance.cpp:189-200); fuzz: Mirror (mirror.cpp:142-161). */
+typedef struct { uint32_t type; double refl, trans, eta, kappa, ior, alpha_u, alpha_v, fuzz; } Bsdf;
 
 static double fresnel_cond(double eta, double kappa, double ci) {  /* conductor.cpp:184-194 */
     double c2 = ci * ci, a = eta * eta + kappa * kappa, b = 2 * ci * eta;
     double ao = a + c2, ap = a * c2 + 1;
     return ((ao - b) / (ao + b) + (ap - b) / (ap + b)) / 2;
 }
+/* MicrofacetDistribution::samplePhi (src/liar/kernel/microfacet.cpp:102-123) */
+static double sample_phi(double u, double au, double av) {
+    if (au == av) return 2 * PI * u;
+    double phi = atan(av / au * tan(2 * PI * u));
+    if (u <= 0.25) return phi;
+    if (u < 0.75) return phi + PI;
+    return phi + 2 * PI;
+}
+/* D(h) and its pdf D cos(theta_h): Trowbridge-Reitz
+ * (microfacet_trowbridge_reitz.cpp:46-58), Beckmann (microfacet_beckmann.cpp:50-72). */
+static double mdf_d(uint32_t type, double au, double av, V h, double* pdf_h) {
+    double d;
+    if (type == 5) {
+        double q = (h.x / au) * (h.x / au) + (h.y / av) * (h.y / av) + h.z * h.z;
+        d = 1 / (PI * au * av * q * q);
+    } else {
+        double c2 = h.z * h.z;
+        if (c2 == 0) { *pdf_h = 0; return 0; }
+        d = exp(-((h.x / au) * (h.x / au) + (h.y / av) * (h.y / av)) / c2) / (PI * au * av * c2 * c2);
+    }
+    if (g_plant == 5) d *= PI;
+    *pdf_h = d * h.z;
+    return d;
+}
+/* sampleH: Trowbridge-Reitz (microfacet_trowbridge_reitz.cpp:62-72), Beckmann
+ * (microfacet_beckmann.cpp:76-106; its anisotropic branch takes phi from
+ * sample.x, the same number as the radial part, as the reference does). */
+static V mdf_sample_h(uint32_t type, double au, double av, double u1, double u2) {
+    if (type == 5) {
+        double phi = sample_phi(u1, au, av), s = sqrt(u2 / (1 - u2));
+        return nrm(v(s * au * cos(phi), s * av * sin(phi), 1));
+    }
+    double s = log(1 - u1), cp, sp, t2;
+    if (au == av) {
+        double phi = 2 * PI * u2;
+        cp = cos(phi); sp = sin(phi);
+        t2 = -s * au * au;
+    } else {
+        double phi = sample_phi(u1, au, av);
+        cp = cos(phi); sp = sin(phi);
+        t2 = -s / ((cp / au) * (cp / au) + (sp / av) * (sp / av));
+    }
+    double c2 = 1 / (1 + t2), ct = sqrt(c2), st = sqrt(fmax(1 - c2, 0));
+    return v(cp * st, sp * st, ct);
+}
+/* BSDF value and pdf of wo, caps already filtered by the caller's geometric
+ * hemisphere test (Bsdf::evaluate, bsdf.cpp:52-76): Lambert (lambert.cpp:105-114)
+ * and CookTorrance (cook_torrance.cpp:262-280); delta BSDFs evaluate to 0. */
 static int bsdf_eval(const Bsdf* B, V wi, V wo, double* val, double* pdf) {
-    if (wo.z == 0 || !(wo.z > 0)) return 0;
+    if (wo.z == 0) return 0;
     if (B->type == 1) { *val = B->refl / PI; *pdf = fabs(wo.z) / PI; return *pdf > 0; }
-    if (B->type == 5 || B->type == 6) {  /* cook_torrance.cpp:262-280 */
+    if (B->type == 5 || B->type == 6) {
+        if (!(wo.z > 0)) return 0;
         V h = nrm(add(wi, wo));
-        double a = B->alpha, d;
-        if (B->type == 5) {
-            double q = (h.x / a) * (h.x / a) + (h.y / a) * (h.y / a) + h.z * h.z;
-            d = 1 / (PI * a * a * q * q);
-        } else {
-            double c2 = h.z * h.z;
-            if (c2 == 0) return 0;
-            d = exp(-(h.x * h.x + h.y * h.y) / (a * a * c2)) / (PI * a * a * c2 * c2);
-        }
-        double pdfh = d * h.z, doh = fabs(dot(wi, h));
-        double g = fmin(1, fmin(2 * h.z * wi.z / doh, 2 * h.z * wo.z / doh));
+        double pdfh, d = mdf_d(B->type, B->alpha_u, B->alpha_v, h, &pdfh);
+        double doh = fabs(dot(wi, h));
+        double g = fmin(1, fmin(2 * h.z * wi.z / doh, 2 * h.z * wo.z / doh));  /* cook_torrance.cpp:234-238 */
         double woh = dot(wi, h);
-        *pdf = pdfh / (4 * woh);
+        *pdf = g_plant == 3 ? pdfh : pdfh / (4 * woh);
         *val = B->refl * d * g / (4 * wi.z * wo.z) * fresnel_cond(B->eta, B->kappa, fabs(woh));
         return *pdf > 0;
     }
     return 0;
+}
+/* Bsdf::evaluate (bsdf.cpp:52-76): wo on the other side of the geometric
+ * normal (ngl, in the local frame, towards the viewer) than wi masks the
+ * reflection caps; every BSDF here evaluates only reflection (the delta
+ * ones evaluate to 0 anyway), so such a wo has no value. */
+static int bsdf_evaluate(const Bsdf* B, V wi, V wo, V ngl, double* val, double* pdf) {
+    if (wo.z == 0) return 0;
+    if ((dot(wi, ngl) > 0) != (dot(wo, ngl) > 0)) return 0;
+    return bsdf_eval(B, wi, wo, val, pdf);
 }
 /* allow: bit 0 reflection, bit 1 transmission (the caps the caller permits;
  * only the dielectric has both and chooses among the permitted ones). */
@@ -370,9 +507,16 @@ static int bsdf_sample(const Bsdf* B, V wi, double u1, double u2, double uc, V* 
         *val = B->refl / PI;
         return *pdf > 0;
     }
-    case 2:
+    case 2:  /* mirror.cpp:142-161: value = reflectance, pdf = cos; fuzz adds
+                fuzz x a uniform-sphere point to the reflection, unnormalised */
         *wo = v(-wi.x, -wi.y, wi.z);
-        *val = B->refl;
+        if (B->fuzz > 0) {
+            double z = 1 - 2 * u1, r = sqrt(fmax(0, 1 - z * z)), phi = 2 * PI * u2;
+            *wo = add(*wo, mul(B->fuzz, v(r * cos(phi), r * sin(phi), z)));
+            *val = wo->z > 0 ? B->refl : 0;
+        } else {
+            *val = B->refl;
+        }
         *pdf = wo->z;
         return *pdf > 0;
     case 4:
@@ -393,23 +537,80 @@ static int bsdf_sample(const Bsdf* B, V wi, double u1, double u2, double uc, V* 
         else { *wo = v(-ior * wi.x, -ior * wi.y, -ct); *val = B->trans * (1 - fr) / ct; *pdf = 1 - prob; }
         return *pdf > 0;
     }
-    case 5: {  /* GGX sampleH, microfacet_trowbridge_reitz.cpp:62-72 */
-        double phi = 2 * PI * u1, s = sqrt(u2 / (1 - u2));
-        V h = nrm(v(s * B->alpha * cos(phi), s * B->alpha * sin(phi), 1));
-        *wo = sub(mul(2 * dot(wi, h), h), wi);
-        if (!(wo->z > 0)) return 0;
-        return bsdf_eval(B, wi, *wo, val, pdf);
-    }
-    case 6: {  /* Beckmann sampleH, microfacet_beckmann.cpp:76-106 */
-        double s = log(1 - u1), phi = 2 * PI * u2, t2 = -s * B->alpha * B->alpha;
-        double c2 = 1 / (1 + t2), ct = sqrt(c2), st = sqrt(fmax(1 - c2, 0));
-        V h = v(cos(phi) * st, sin(phi) * st, ct);
+    case 5:
+    case 6: {  /* cook_torrance.cpp:284-294: sample h, reflect, evaluate */
+        V h = mdf_sample_h(B->type, B->alpha_u, B->alpha_v, u1, u2);
         *wo = sub(mul(2 * dot(wi, h), h), wi);
         if (!(wo->z > 0)) return 0;
         return bsdf_eval(B, wi, *wo, val, pdf);
     }
     }
     return 0;
+}
+
+/* ---- probes (tests only): one BSDF / distribution / spectrum evaluation ---- */
+/* prm: type, refl, trans, eta, kappa, ior, alpha_u, alpha_v, fuzz */
+static Bsdf probe_bsdf(const double* prm) {
+    Bsdf B = {(uint32_t)prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8]};
+    return B;
+}
+/* out: value, pdf; returns 1 when the BSDF has a value for (wi, wo) */
+int or_bsdf_eval(const double* prm, const double* wi, const double* wo, double* out) {
+    Bsdf B = probe_bsdf(prm);
+    out[0] = out[1] = 0;
+    return bsdf_eval(&B, v(wi[0], wi[1], wi[2]), v(wo[0], wo[1], wo[2]), &out[0], &out[1]);
+}
+/* u: u1, u2, component sample; out: wo (3), value, pdf */
+int or_bsdf_sample(const double* prm, const double* wi, const double* u, uint32_t allow, double* out) {
+    Bsdf B = probe_bsdf(prm);
+    V wo = v(0, 0, 0);
+    double val = 0, pdf = 0;
+    int ok = bsdf_sample(&B, v(wi[0], wi[1], wi[2]), u[0], u[1], u[2], &wo, &val, &pdf, allow);
+    out[0] = wo.x; out[1] = wo.y; out[2] = wo.z; out[3] = val; out[4] = pdf;
+    return ok;
+}
+double or_fresnel_conductor(double eta, double kappa, double cos_i) { return fresnel_cond(eta, kappa, cos_i); }
+/* Batched probes: n inputs of 3 doubles (wi, wo or u); out 2 (value, pdf)
+ * or 5 (wo, value, pdf) doubles per input; ok[i] = the call's result. */
+void or_bsdf_eval_n(const double* prm, const double* wi, const double* wo, uint32_t n, double* out, uint8_t* ok) {
+    Bsdf B = probe_bsdf(prm);
+    for (uint32_t i = 0; i < n; ++i) {
+        out[2 * i] = out[2 * i + 1] = 0;
+        ok[i] = (uint8_t)bsdf_eval(&B, v(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]),
+                                   v(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), &out[2 * i], &out[2 * i + 1]);
+    }
+}
+void or_bsdf_sample_n(const double* prm, const double* wi, const double* u, uint32_t n, uint32_t allow, double* out,
+                      uint8_t* ok) {
+    Bsdf B = probe_bsdf(prm);
+    for (uint32_t i = 0; i < n; ++i) {
+        V wo = v(0, 0, 0);
+        double val = 0, pdf = 0;
+        ok[i] = (uint8_t)bsdf_sample(&B, v(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), u[3 * i], u[3 * i + 1],
+                                     u[3 * i + 2], &wo, &val, &pdf, allow);
+        double* o = out + 5 * (size_t)i;
+        o[0] = wo.x; o[1] = wo.y; o[2] = wo.z; o[3] = val; o[4] = pdf;
+    }
+}
+/* One spectrum at n wavelengths [nm], as the renderer evaluates it. */
+void or_spectrum_eval(const or_spectrum* sp, const double* w, uint32_t n, int reflectant, double* out) {
+    Scene S;
+    memset(&S, 0, sizeof S);
+    S.spectra = sp;
+    S.n_spectra = 1;
+    double mx = -INFINITY, scale = 1;
+    if (sp->kind == 1 || sp->kind == 2)
+        for (uint32_t k = 0; k < sp->count; ++k) mx = fmax(mx, sp->v[k]);
+    if (mx > 1) scale = 1 / mx;
+    S.spec_reflscale = &scale;
+    for (uint32_t i = 0; i < n; ++i) out[i] = spec_eval(&S, 0, w[i], reflectant);
+}
+double or_mdf_d(uint32_t type, double au, double av, const double* h, double* pdf_h) {
+    return mdf_d(type, au, av, v(h[0], h[1], h[2]), pdf_h);
+}
+void or_mdf_sample_h(uint32_t type, double au, double av, double u1, double u2, double* h) {
+    V r = mdf_sample_h(type, au, av, u1, u2);
+    h[0] = r.x; h[1] = r.y; h[2] = r.z;
 }
 
 /* ---- one path -------------------------------------------------------------- */
@@ -438,22 +639,11 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
                         L += thr * spec_eval(S, S->lights[k].radiance, lam, 0);
             return L;
         }
-        V x = add(o, mul(t, d));
-        V ng;
-        int leaving;
-        if (prim < S->nt) {
-            const float* a = S->verts + 3 * (size_t)S->tris[3 * prim];
-            const float* b = S->verts + 3 * (size_t)S->tris[3 * prim + 1];
-            const float* cc = S->verts + 3 * (size_t)S->tris[3 * prim + 2];
-            ng = nrm(cross(sub(f3(b), f3(a)), sub(f3(cc), f3(a))));
-            leaving = dot(d, ng) > 0;
-        } else {
-            const float* s = S->spheres + 4 * (size_t)(prim - S->nt);
-            ng = mul(1.0 / s[3], sub(x, f3(s)));
-            V s0 = sub(add(o, mul(tn, d)), f3(s));
-            leaving = dot(s0, s0) < (double)s[3] * s[3];
-        }
-        V nf = dot(ng, d) < 0 ? ng : mul(-1, ng);
+        Frame F;
+        hit_frame(S, prim, o, d, tn, t, &F);
+        const V x = F.x, nf = F.ns, tt = F.tt, bb = F.bb;
+        const int leaving = F.leaving;
+        const V ngl = F.has_ns ? v(dot(F.ng, tt), dot(F.ng, bb), dot(F.ng, nf)) : v(0, 0, 1);
         const or_material* m = &S->mats[S->prim_mat[prim]];
         if (gen > 0 && m->light != RNONE && light_sees(S, m->light, d))
             L += thr * spec_eval(S, S->lights[m->light].radiance, lam, 0);
@@ -466,10 +656,10 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
         B.eta = fmax(spec_eval(S, m->eta, lam, 0), 1e-9);
         B.kappa = fmax(spec_eval(S, m->kappa, lam, 0), 1e-9);
         B.ior = leaving ? B.eta / B.kappa : B.kappa / B.eta;
-        double r = fmax(m->roughness, 1e-3);
-        B.alpha = r * r;
-        V tt, bb;
-        onb(nf, &tt, &bb);
+        double ru = fmax(m->roughness, 1e-3), rv = fmax(m->roughness_v, 1e-3);
+        B.alpha_u = ru * ru;
+        B.alpha_v = rv * rv;
+        B.fuzz = m->fuzz;
         V wiw = mul(-1, d);
         V wi = v(dot(wiw, tt), dot(wiw, bb), fmax(dot(wiw, nf), 1.1920929e-7));
         uint32_t r0[4], r1[4];
@@ -504,7 +694,7 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
                 pl *= 1 - strategy;
                 wo = v(dot(dw, tt), dot(dw, bb), dot(dw, nf));
                 double vb, pb;
-                if (bsdf_eval(&B, wi, wo, &vb, &pb) && vb != 0) {
+                if (bsdf_evaluate(&B, wi, wo, ngl, &vb, &pb) && vb != 0) {
                     double pbs = pb * strategy;
                     val = g_plant == 1 ? vb : vb * pl * pl / (pl * pl + pbs * pbs);
                     pdf = pl;
@@ -515,7 +705,7 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
         if (!(ok && pdf > 0)) return L;
         double ct = wo.z;
         thr *= val * fabs(ct) / pdf;
-        V start = add(x, mul((ct > 0 ? 10.0 : -10.0) * TOL, nf));
+        V start = add(x, mul((ct > 0 ? 10.0 : -10.0) * TOL, F.ng));  /* adjoint_photon_tracer.cpp:177 */
         d = nrm(add(add(mul(wo.x, tt), mul(wo.y, bb)), mul(wo.z, nf)));
         o = start;
         tn = TOL;
@@ -553,7 +743,7 @@ static int is_specular(uint32_t type) { return type == 2 || type == 3 || type ==
 
 typedef struct {
     Bsdf B;
-    V wi, tt, bb, nf, start;
+    V wi, tt, bb, nf, ngl, start;
     double lam;
     uint64_t p;
     uint32_t key;  /* generation | branch << 8: the node's random-number key */
@@ -603,7 +793,7 @@ static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
         dir = light_sample(S, k, H->start, u01(r[0]), u01(r[1]), &pl);
         if (!(pl > 0)) return 0;
         V wo = v(dot(dir, H->tt), dot(dir, H->bb), dot(dir, H->nf));
-        if (!bsdf_eval(&H->B, H->wi, wo, &f, &pb) || f == 0) return 0;
+        if (!bsdf_evaluate(&H->B, H->wi, wo, H->ngl, &f, &pb) || f == 0) return 0;
         double wt = g_plant == 2 ? 1 : sq_heuristic(nl * pl, nl * pb);
         contrib = f * Le * wt * fabs(wo.z) / (nl * pl * scale);
     } else {
@@ -636,22 +826,10 @@ static double dl_cast(const Ctx* c, uint64_t p, V o, V d, double tn, double lam,
     double t;
     uint32_t prim = intersect(S, o, d, tn, &t);
     if (prim == RMISS) return 0;  /* alpha = 0: escaped rays add nothing */
-    V x = add(o, mul(t, d));
-    V ng;
-    int leaving;
-    if (prim < S->nt) {
-        const float* a = S->verts + 3 * (size_t)S->tris[3 * prim];
-        const float* b = S->verts + 3 * (size_t)S->tris[3 * prim + 1];
-        const float* cc = S->verts + 3 * (size_t)S->tris[3 * prim + 2];
-        ng = nrm(cross(sub(f3(b), f3(a)), sub(f3(cc), f3(a))));
-        leaving = dot(d, ng) > 0;
-    } else {
-        const float* s = S->spheres + 4 * (size_t)(prim - S->nt);
-        ng = mul(1.0 / s[3], sub(x, f3(s)));
-        V s0 = sub(add(o, mul(tn, d)), f3(s));
-        leaving = dot(s0, s0) < (double)s[3] * s[3];
-    }
-    V nf = dot(ng, d) < 0 ? ng : mul(-1, ng);
+    Frame F;
+    hit_frame(S, prim, o, d, tn, t, &F);
+    const V x = F.x, nf = F.ns;
+    const int leaving = F.leaving;
     const or_material* m = &S->mats[S->prim_mat[prim]];
     double L = 0;
     if ((gen == 0 || m->light == RNONE) && m->emission != RNONE) L += spec_eval(S, m->emission, lam, 0);
@@ -663,10 +841,14 @@ static double dl_cast(const Ctx* c, uint64_t p, V o, V d, double tn, double lam,
     H.B.eta = fmax(spec_eval(S, m->eta, lam, 0), 1e-9);
     H.B.kappa = fmax(spec_eval(S, m->kappa, lam, 0), 1e-9);
     H.B.ior = leaving ? H.B.eta / H.B.kappa : H.B.kappa / H.B.eta;
-    double rg = fmax(m->roughness, 1e-3);
-    H.B.alpha = rg * rg;
+    double rgu = fmax(m->roughness, 1e-3), rgv = fmax(m->roughness_v, 1e-3);
+    H.B.alpha_u = rgu * rgu;
+    H.B.alpha_v = rgv * rgv;
+    H.B.fuzz = m->fuzz;
     H.nf = nf;
-    onb(nf, &H.tt, &H.bb);
+    H.tt = F.tt;
+    H.bb = F.bb;
+    H.ngl = F.has_ns ? v(dot(F.ng, F.tt), dot(F.ng, F.bb), dot(F.ng, nf)) : v(0, 0, 1);
     V wiw = mul(-1, d);
     H.wi = v(dot(wiw, H.tt), dot(wiw, H.bb), fmax(dot(wiw, nf), 1.1920929e-7));
     H.start = add(x, mul(10 * TOL, nf));
@@ -778,13 +960,13 @@ static double lum(const Scene* S, uint32_t i, const float* w, const float* xyz, 
 }
 
 static int render_impl(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres, uint32_t ns,
-                       const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
+                       const float* tri_normals, const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
                        uint32_t n_spectra, const or_light* lights, uint32_t n_lights, const float* obs_w,
                        const float* obs_xyz, uint32_t obs_n, const or_camera* cam, const or_params* P, double* xyz,
                        double* weight, int nthreads, const DLCfg* dl) {
     Scene S;
     memset(&S, 0, sizeof S);
-    S.verts = verts; S.tris = tris; S.nt = nt; S.spheres = spheres; S.ns = ns; S.prim_mat = prim_mat;
+    S.verts = verts; S.tris = tris; S.nt = nt; S.tri_normals = tri_normals; S.spheres = spheres; S.ns = ns; S.prim_mat = prim_mat;
     S.mats = mats; S.spectra = spectra; S.n_spectra = n_spectra; S.lights = lights; S.n_lights = n_lights;
     if (build_observer(&S, obs_w, obs_xyz, (int)obs_n, P->lambda_min, P->lambda_max)) return -1;
     S.spec_reflscale = (double*)malloc(sizeof(double) * (n_spectra + 1));
@@ -900,22 +1082,22 @@ static int render_impl(const float* verts, const uint32_t* tris, uint32_t nt, co
 /* AdjointPhotonTracer render into xyz (3*W*H doubles, += sums) and weight
  * (W*H, += counts). */
 int or_render(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres, uint32_t ns,
-              const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
+              const float* tri_normals, const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
               uint32_t n_spectra, const or_light* lights, uint32_t n_lights, const float* obs_w,
               const float* obs_xyz, uint32_t obs_n, const or_camera* cam, const or_params* P, double* xyz,
               double* weight, int nthreads) {
-    return render_impl(verts, tris, nt, spheres, ns, prim_mat, mats, spectra, n_spectra, lights, n_lights, obs_w,
+    return render_impl(verts, tris, nt, spheres, ns, tri_normals, prim_mat, mats, spectra, n_spectra, lights, n_lights, obs_w,
                        obs_xyz, obs_n, cam, P, xyz, weight, nthreads, NULL);
 }
 
 /* DirectLighting render (cfg: max generation, secondary light samples,
  * trace glossy); P's depth and roulette fields are ignored. */
 int or_render_direct(const float* verts, const uint32_t* tris, uint32_t nt, const float* spheres, uint32_t ns,
-                     const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
+                     const float* tri_normals, const uint32_t* prim_mat, const or_material* mats, const or_spectrum* spectra,
                      uint32_t n_spectra, const or_light* lights, uint32_t n_lights, const float* obs_w,
                      const float* obs_xyz, uint32_t obs_n, const or_camera* cam, const or_params* P,
                      const uint32_t* cfg, double* xyz, double* weight, int nthreads) {
     DLCfg dl = {cfg[0], cfg[1] ? cfg[1] : 1u, cfg[2]};
-    return render_impl(verts, tris, nt, spheres, ns, prim_mat, mats, spectra, n_spectra, lights, n_lights, obs_w,
+    return render_impl(verts, tris, nt, spheres, ns, tri_normals, prim_mat, mats, spectra, n_spectra, lights, n_lights, obs_w,
                        obs_xyz, obs_n, cam, P, xyz, weight, nthreads, &dl);
 }

@@ -39,14 +39,16 @@ def test_guided_furnace_unbiased_with_untrained_guide(gpu_ctx):
     sc, cam = W.furnace_sphere(shaders.Lambert(textures.Constant(1.0)))
     gs = E.GpuScene(gpu_ctx, la.compile_scene(sc))
     g = Guide(gpu_ctx, seed=7)
-    fx, fw = _film(64, 64)
-    E.guided_render(gs, g, E.camera_struct(cam), E.render_params(64, 64, 1024, 1, 0, 8, 0.5, (380, 780), 3),
+    fx, fw = _film(128, 128)
+    E.guided_render(gs, g, E.camera_struct(cam), E.render_params(128, 128, 32768, 1, 0, 8, 0.5, (380, 780), 3),
                     fx, fw, selection=0.5)
     y = _y(fx, fw)
-    yy, xx = np.mgrid[0:64, 0:64]
-    Y = y[(xx - 31.5) ** 2 + (yy - 31.5) ** 2 < 18 ** 2]
-    print(f"[gpu] guided furnace: mean {Y.mean():.5f} std {Y.std():.4f}")
-    assert abs(Y.mean() - 1.0) < 3e-3
+    yy, xx = np.mgrid[0:128, 0:128]
+    Y = y[(xx - 63.5) ** 2 + (yy - 63.5) ** 2 < 36 ** 2]
+    three_sigma = 3 * Y.std() / math.sqrt(Y.size)
+    print(f"[gpu] guided furnace: mean {Y.mean():.5f}, 3 sigma {three_sigma:.1e}", flush=True)
+    assert three_sigma < 5e-4
+    assert abs(Y.mean() - 1.0) < 1e-3
     g.close()
     gs.close()
 

@@ -124,17 +124,20 @@ def test_white_furnace(gpu_ctx, name, shader, expect):
     say(f"[gpu] white furnace {name}")
     sc, cam = W.furnace_sphere(shader)
     cs = la.compile_scene(sc)
-    gx, gw, _ = gpu_render(gpu_ctx, cs, cam, 64, 64, 1024, nl=1, max_depth=0, rr_depth=8, seed=3)
+    gx, gw, _ = gpu_render(gpu_ctx, cs, cam, 128, 128, 32768, nl=1, max_depth=0, rr_depth=8, seed=3)
     img = gx / gw
-    # the sphere covers a disc of radius ~30 px at the image centre (fov 11
+    # the sphere covers a disc of radius ~60 px at the image centre (fov 11
     # degrees at distance sqrt(125)); test pixels well inside it
-    yy, xx = np.mgrid[0:64, 0:64]
-    centre = (xx - 31.5) ** 2 + (yy - 31.5) ** 2 < 18 ** 2
+    yy, xx = np.mgrid[0:128, 0:128]
+    centre = (xx - 63.5) ** 2 + (yy - 63.5) ** 2 < 36 ** 2
     Y = img[1][centre]
-    say(f"[gpu] furnace {name}: mean Y {Y.mean():.6f} (expect {expect}), std {Y.std():.4f}, n {Y.size}")
-    assert abs(Y.mean() - expect) < 1e-3 * max(expect, 1.0) + 3 * Y.std() / np.sqrt(Y.size)
-    assert abs(Y.mean() - expect) < 2e-3
-    corner = img[1][:4, :4]
+    three_sigma = 3 * Y.std() / np.sqrt(Y.size)
+    say(f"[gpu] furnace {name}: mean Y {Y.mean():.6f} (expect {expect}), 3 sigma {three_sigma:.1e}, n {Y.size}")
+    # enough samples that the noise is well inside the 1e-3 energy bound
+    # (test/white_furnace/ashikhmin_shirley.py:20-79), which then holds alone
+    assert three_sigma < 5e-4
+    assert abs(Y.mean() - expect) < 1e-3
+    corner = img[1][:8, :8]
     assert (corner == 0).all()  # camera rays that miss see no sky (generation 0)
 
 
