@@ -32,6 +32,7 @@ using namespace lr;
 namespace {
 
 std::mutex g_obs_mu;
+uint64_t g_obs_version = 1;  // bumped by lr_set_standard_observer (scenes rebuild their tables)
 std::vector<float> g_obs_w, g_obs_xyz;  // standard observer (nm, xyz interleaved)
 
 struct DevCamera {
@@ -788,6 +789,7 @@ struct lr_scene {
     // observer restricted to the last render's sampling domain
     float obs_lo = -1.0f, obs_hi = -1.0f;
     uint32_t obs_n = 0;
+    uint64_t obs_version = 0;
     float* d_obs = nullptr;
     // wavefront workspace (grow-only)
     size_t cap = 0;
@@ -832,10 +834,15 @@ void free_scene(lr_scene* s) {
 GeomArgs geom_args(const lr_scene* sc) { return GeomArgs{sc->d_prim_geom, sc->d_prim_tan, sc->d_corner}; }
 
 int build_observer(lr_scene* sc, float lo, float hi) {
-    if (sc->d_obs && sc->obs_lo == lo && sc->obs_hi == hi) return LR_OK;
-    std::vector<double> w, x, y, z;
     {
         std::lock_guard<std::mutex> g(g_obs_mu);
+        if (sc->d_obs && sc->obs_lo == lo && sc->obs_hi == hi && sc->obs_version == g_obs_version) return LR_OK;
+    }
+    std::vector<double> w, x, y, z;
+    uint64_t version = 0;
+    {
+        std::lock_guard<std::mutex> g(g_obs_mu);
+        version = g_obs_version;
         const size_t n = g_obs_w.size();
         if (n < 2) return lr::fail(LR_E_INVALID, "no standard observer: call lr_set_standard_observer first");
         auto sens = [&](double wl, int a) -> double {
@@ -893,6 +900,7 @@ int build_observer(lr_scene* sc, float lo, float hi) {
     sc->obs_lo = lo;
     sc->obs_hi = hi;
     sc->obs_n = (uint32_t)n;
+    sc->obs_version = version;
     return LR_OK;
 }
 
@@ -998,6 +1006,7 @@ int lr_set_standard_observer(const float* w, const float* xyz, uint32_t n) {
         if (!(w[k + 1] > w[k])) return lr::fail(LR_E_INVALID, "observer wavelengths must be strictly increasing");
     std::lock_guard<std::mutex> g(g_obs_mu);
     g_obs_w.assign(w, w + n);
+    ++g_obs_version;
     g_obs_xyz.assign(xyz, xyz + 3 * (size_t)n);
     return LR_OK;
 }

@@ -25,7 +25,18 @@ TONE_MAPPINGS = ("Linear", "CompressY", "CompressRGB", "Reinhard2002Y", "Reinhar
 LINEAR_CODECS = (".exr", ".hdr", ".rgbe", ".pic", ".pfm")
 
 
+class _ToneMappingNames:
+    """``Raster.ToneMapping.<name>``: the operator names, usable wherever a
+    toneMapping string is (raster.cpp:30-40)."""
+
+
+for _name in TONE_MAPPINGS:
+    setattr(_ToneMappingNames, _name, _name)
+
+
 class Raster:
+    ToneMapping = _ToneMappingNames
+
     def __init__(self, resolution=(320, 240)):
         self.resolution = (int(resolution[0]), int(resolution[1]))
         w, h = self.resolution
