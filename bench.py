@@ -52,6 +52,28 @@ def _traffic(kind: str):
         return None
 
 
+def pin_config(args, ctx) -> bool:
+    """--trace-config: install the given traversal configuration in ctx and
+    skip the autotuner (profiling runs time exactly one configuration)."""
+    if not args.trace_config:
+        return False
+    ctx.set_trace_config(**args.trace_config)
+    return True
+
+
+def host_info() -> dict:
+    """CPU of the host that runs the CPU baseline."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
 def bench_render(args, la, oracle, torch, np):
     """Path-tracer leg: the Cornell-box workload (256x256, 16 spp, 4 stratified
     wavelengths, depth <= 8, Russian roulette from generation 3) rendered
@@ -68,7 +90,7 @@ def bench_render(args, la, oracle, torch, np):
     film = torch.zeros((3, H_, W_), dtype=torch.float32, device="cuda")
     wgt = torch.zeros((H_, W_), dtype=torch.float32, device="cuda")
     tuned = None
-    if not args.no_tune:
+    if not pin_config(args, ctx) and not args.no_tune:
         from liar_amd.autotune import tune_render
 
         tuned = tune_render(gs, camera, E.render_params(W_, H_, SPP, NL, 8, 3, 0.5, (380.0, 780.0), 99))
@@ -99,7 +121,7 @@ def bench_render(args, la, oracle, torch, np):
         oracle.render(cs, cam, cw, ch, SPP, wavelengths=NL, max_depth=8, rr_depth=3, seed=1, threads=threads)
         t_cpu = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(cw * ch * SPP * NL / t_cpu, 1), "unit": "paths/s",
-                               "cores": threads, "kind": "port",
+                               "cores": threads, "kind": "port", "host": host_info(),
                                "sample": f"{cw}x{ch} pixels of the same frame geometry (FP64 oracle), "
                                          f"{t_cpu:.2f} s"}
     gs.close()
@@ -124,7 +146,7 @@ def bench_direct(args, la, oracle, torch, np):
     film = torch.zeros((3, H_, W_), dtype=torch.float32, device="cuda")
     wgt = torch.zeros((H_, W_), dtype=torch.float32, device="cuda")
     tuned = None
-    if not args.no_tune:
+    if not pin_config(args, ctx) and not args.no_tune:
         # traversal configuration for this integrand, timed on a 4-spp frame
         from liar_amd.autotune import tune_render
 
@@ -157,7 +179,7 @@ def bench_direct(args, la, oracle, torch, np):
         oracle.render_direct(cs, cam, cw, ch, SPP, wavelengths=NL, seed=1, threads=threads)
         t_cpu = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(cw * ch * SPP * NL / t_cpu, 1), "unit": "paths/s",
-                               "cores": threads, "kind": "port",
+                               "cores": threads, "kind": "port", "host": host_info(),
                                "sample": f"{cw}x{ch} pixels of the same frame geometry (FP64 oracle, brute-force "
                                          f"shadow rays), {t_cpu:.2f} s"}
     gs.close()
@@ -303,7 +325,7 @@ def bench_guided(args, la, torch, np):
         return E.render_params(Wd, Hd, spp, 1, 16, 16, 0.5, (380.0, 780.0), seed, 0, ML)
 
     tuned = None
-    if not args.no_tune:
+    if not pin_config(args, ctx) and not args.no_tune:
         from liar_amd.autotune import tune_render
 
         log("[bench] guided: autotuning traversal on the plain render")
@@ -387,7 +409,7 @@ def bench_terrain(args, la, torch, np):
     gs = E.GpuScene(ctx, cs)
     camera = E.camera_struct(cam)
     tuned = None
-    if not args.no_tune:
+    if not pin_config(args, ctx) and not args.no_tune:
         from liar_amd.autotune import tune_render
 
         log("[bench] terrain: autotuning traversal")
@@ -456,8 +478,11 @@ def bench_batch(args, la, torch, np):
     cam = specs[big]["camera"]
     n_tune = 1 << 20
     tune_rays = _camera_rays(np, torch, cam, n_tune)
-    res = gss[big].qbvh.autotune(tune_rays, repeats=2)
-    ctx.set_trace_config(**res["best"])
+    if pin_config(args, ctx):
+        res = {"best": ctx.trace_config()}
+    else:
+        res = gss[big].qbvh.autotune(tune_rays, repeats=2)
+        ctx.set_trace_config(**res["best"])
     batch.render(items(), graph=True)  # warm-up: code load, workspace, graph instantiation
     torch.cuda.synchronize()
     it = items()
@@ -581,7 +606,16 @@ def main() -> int:
     ap.add_argument("--render-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (PMC byte-counter calibration)")
+    ap.add_argument("--trace-config", default=None,
+                    help="block,lds_stack,blocks_per_cu,refill_min,schedule: use this traversal "
+                         "configuration everywhere instead of autotuning (for profiler runs)")
+    ap.add_argument("--only", default=None,
+                    choices=["render", "direct", "film", "instanced", "terrain", "guided", "batch"],
+                    help="run only this leg (no headline) and print its JSON object")
     args = ap.parse_args()
+    if args.trace_config:
+        b, l, bpc, r, sch = (int(x) for x in args.trace_config.split(","))
+        args.trace_config = {"block": b, "lds_stack": l, "blocks_per_cu": bpc, "refill_min": r, "schedule": sch}
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         log("bench.py measures one GPU only: there is no multi-GPU path "
@@ -597,6 +631,16 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.profile_workload:
         return profile_workload(args, la, torch)
+    if args.only:
+        legs = {"render": lambda: bench_render(args, la, oracle, torch, np),
+                "direct": lambda: bench_direct(args, la, oracle, torch, np),
+                "film": lambda: bench_film(args, la, torch, np),
+                "instanced": lambda: bench_instanced(args, la, torch, np),
+                "terrain": lambda: bench_terrain(args, la, torch, np),
+                "guided": lambda: bench_guided(args, la, torch, np),
+                "batch": lambda: bench_batch(args, la, torch, np)}
+        print(json.dumps({args.only: legs[args.only]()}), flush=True)
+        return 0
     dev = torch.device("cuda:0")
     ctx = la.Context(0)
 
@@ -642,7 +686,7 @@ def main() -> int:
     def tune(r, any_hit):
         # traversal block size, LDS stack depth, residency and refill threshold
         # from the autotuner, on the rays about to be timed (outside the timed region)
-        if args.no_tune:
+        if pin_config(args, ctx) or args.no_tune:
             return None
         res = bvh.autotune(r, any_hit=any_hit, repeats=2)
         d = next((ms for c, ms in res["table"] if c == default_cfg), None)
@@ -667,7 +711,8 @@ def main() -> int:
     S = min(args.count_sample, N_RAYS)
     log(f"[bench] oracle counts / parity on {S} rays")
     t0 = time.perf_counter()
-    ref = oracle.qbvh_closest(nodes, prims, rays_h[:, :S])
+    ref_p, ref_t, ref_u, ref_v, tied, ref_nodes, ref_prims = oracle.qbvh_closest_ties(nodes, prims, rays_h[:, :S])
+    ref = (ref_p, ref_t, ref_u, ref_v, ref_nodes, ref_prims)
     t_count = time.perf_counter() - t0
     ref_any = oracle.qbvh_any(nodes, prims, rays_any_h[:, :S])
     # node / primitive counts of the traversal the GPU runs: the quantised boxes
@@ -679,6 +724,10 @@ def main() -> int:
     mism = int((g_prim != ref[0]).sum()) + sum(
         int((g_tuv[k].view(np.uint32) != ref[1 + k].view(np.uint32)).sum()) for k in range(3))
     mism_any = int((occ[:S].cpu().numpy() != ref_any[0]).sum())
+    # numerically tied hits (two or more primitives at the hit distance): the
+    # GPU must break them as the oracle does, by the smaller primitive id
+    n_tied = int(tied.sum())
+    mism_tied = int((g_prim[tied != 0] != ref[0][tied != 0]).sum())
     nodes_per_ray = ref[4] / S
     prims_per_ray = ref[5] / S
     b_ray = 32 + 16 + NODE_BYTES * nodes_per_ray + 48 * prims_per_ray
@@ -692,6 +741,8 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
+        # every core of this GPU's CPU share (16 per GPU on the boxes this
+        # runs on; fewer when the process is pinned to fewer)
         threads = min(oracle.default_threads(), 16)
         rate0 = S / max(t_count, 1e-6)  # rays/s with default threads
         n_cpu = int(min(N_RAYS, max(1 << 16, rate0 * args.cpu_seconds * threads / oracle.default_threads())))
@@ -706,6 +757,7 @@ def main() -> int:
             "kind": "port",
             "sample": f"first {n_cpu} of the {N_RAYS} closest-hit rays, same QBVH, "
                       f"{t_cpu:.1f} s on {threads} host threads",
+            "host": host_info(),
         }
 
     render = None if args.no_render else bench_render(args, la, oracle, torch, np)
@@ -745,6 +797,12 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": _traffic("closest"),
+            # achieved counts the canonical bytes per ray (128 B FP32 nodes,
+            # oracle visit counts): an effective rate, part of it served by
+            # L2 / MALL; the device's own node bytes (64 B quantised nodes)
+            # give the lower device_frac
+            "basis": "effective: canonical 32 + 16 + 128 N_node + 48 N_prim bytes per ray",
+            "device_frac": round(N_RAYS * dev_b_ray / (kern_c * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
         "cpu_baseline": cpu,
         "detail": {
@@ -767,6 +825,9 @@ def main() -> int:
             "parity_sample": S,
             "parity_mismatches_closest": mism,
             "parity_mismatches_any": mism_any,
+            "tied_hits_closest": n_tied,
+            "tied_hit_fraction_closest": round(n_tied / S, 8),
+            "tied_hit_mismatches_closest": mism_tied,
             "hit_rate_closest": round(float((ref[0] != oracle.MISS).mean()), 4),
             "qbvh_build_s": round(info["build_seconds"], 3),
             "autotune_closest": tuned_c,

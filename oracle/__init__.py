@@ -166,6 +166,30 @@ def qbvh_closest(nodes, prims, rays, threads: int | None = None):
     return prim, t, u, v, nv.value, npr.value
 
 
+def qbvh_closest_ties(nodes, prims, rays, threads: int | None = None):
+    """qbvh_closest plus a per-ray flag: 1 where the hit distance is reached
+    by two or more primitives (a numerical tie, broken by the smaller id).
+    Returns (prim, t, u, v, tied uint8, nodes_visited, prims_tested)."""
+    L = lib()
+    if not getattr(L, "_ties_sig", False):
+        P, U32 = C.c_void_p, C.c_uint32
+        L.or_qbvh_closest_ties.argtypes = [P, U32, P, P, U32, P, P, P, P, P, C.c_int, P, P]
+        L.or_qbvh_closest_ties.restype = C.c_int
+        L._ties_sig = True
+    nodes = np.ascontiguousarray(nodes)
+    prims = np.ascontiguousarray(prims)
+    rays, ra = _rays_arg(rays)
+    n = rays.shape[1]
+    prim = np.empty(n, np.uint32)
+    t, u, v = (np.empty(n, np.float32) for _ in range(3))
+    tied = np.zeros(n, np.uint8)
+    nv, npr = C.c_uint64(), C.c_uint64()
+    rc = L.or_qbvh_closest_ties(_p(nodes), len(nodes), _p(prims), ra, n, _p(prim), _p(t), _p(u), _p(v), _p(tied),
+                                threads or default_threads(), C.byref(nv), C.byref(npr))
+    assert rc == 0, "oracle traversal stack overflow"
+    return prim, t, u, v, tied, nv.value, npr.value
+
+
 def qbvh_any(nodes, prims, rays, threads: int | None = None):
     nodes = np.ascontiguousarray(nodes)
     prims = np.ascontiguousarray(prims)

@@ -296,14 +296,19 @@ static void node_children(const or_node* nd, const float o[3], const float inv[3
     }
 }
 
+/* tied (optional): 1 when the reported hit distance is reached by two or
+ * more primitives (LiAR keeps the first found of them, object_trees.h:243-244,
+ * this library the smallest id).  Culling keeps every box whose entry is <=
+ * the best distance (times the slack), so every primitive at that distance
+ * is tested and the count is complete. */
 static int qbvh_closest_one(const or_node* nodes, const or_prim* prims, uint32_t n_nodes,
                             const float o[3], const float d[3], float tnear, float tfar,
-                            uint32_t* prim, float* th, float* uh, float* vh, or_counts* cnt) {
+                            uint32_t* prim, float* th, float* uh, float* vh, or_counts* cnt, uint8_t* tied) {
     float inv[3] = {safe_inv(d[0]), safe_inv(d[1]), safe_inv(d[2])};
     float far_lim = tfar * (1.0f - OR_EPS), near_lim = tnear * (1.0f + OR_EPS);
     float tri_t = far_lim, tri_u = 0.0f, tri_v = 0.0f, sph_t = far_lim;
     uint32_t tri_id = OR_MISS, sph_id = OR_MISS;
-    int doomed = 0;
+    int doomed = 0, tri_tie = 0, sph_tie = 0;
     uint32_t st_e[OR_STACK];
     float st_t[OR_STACK];
     int sp = 0;
@@ -341,29 +346,39 @@ static int qbvh_closest_one(const or_node* nodes, const or_prim* prims, uint32_t
                 cnt->prims++;
                 if (fbits(q->b[3]) == 0u) {
                     float t, u, v;
-                    if (or_tri_f32(o, d, q->a, q->b, q->c, &t, &u, &v) && t > tnear && t < far_lim &&
-                        (t < tri_t || (t == tri_t && id < tri_id))) {
-                        tri_t = t; tri_u = u; tri_v = v; tri_id = id;
-                        doomed = !(t > near_lim);
+                    if (or_tri_f32(o, d, q->a, q->b, q->c, &t, &u, &v) && t > tnear && t < far_lim) {
+                        if (t == tri_t && tri_id != OR_MISS && id != tri_id) tri_tie = 1;
+                        if (t < tri_t || (t == tri_t && id < tri_id)) {
+                            if (t < tri_t) tri_tie = 0;
+                            tri_t = t; tri_u = u; tri_v = v; tri_id = id;
+                            doomed = !(t > near_lim);
+                        }
                     }
                 } else {
                     float t;
-                    if (or_sphere_f32(o, d, q->a, q->b[1], tnear, &t) && in_range(t, tnear, tfar) &&
-                        (t < sph_t || (t == sph_t && id < sph_id))) {
-                        sph_t = t; sph_id = id;
+                    if (or_sphere_f32(o, d, q->a, q->b[1], tnear, &t) && in_range(t, tnear, tfar)) {
+                        if (t == sph_t && sph_id != OR_MISS && id != sph_id) sph_tie = 1;
+                        if (t < sph_t || (t == sph_t && id < sph_id)) {
+                            if (t < sph_t) sph_tie = 0;
+                            sph_t = t; sph_id = id;
+                        }
                     }
                 }
             }
         }
     }
     int tri_ok = tri_id != OR_MISS && !doomed;
+    int tie = 0;
     if (tri_ok && (sph_id == OR_MISS || tri_t < sph_t || (tri_t == sph_t && tri_id < sph_id))) {
         *prim = tri_id; *th = tri_t; *uh = tri_u; *vh = tri_v;
+        tie = tri_tie || (sph_id != OR_MISS && sph_t == tri_t);
     } else if (sph_id != OR_MISS) {
         *prim = sph_id; *th = sph_t; *uh = 0.0f; *vh = 0.0f;
+        tie = sph_tie || (tri_ok && tri_t == sph_t);
     } else {
         *prim = OR_MISS; *th = INFINITY; *uh = 0.0f; *vh = 0.0f;
     }
+    if (tied) *tied = (uint8_t)tie;
     return 0;
 }
 
@@ -459,6 +474,7 @@ typedef struct {
     float *t, *u, *v;
     double *t64, *u64, *v64;
     uint8_t* occ;
+    uint8_t* tied;
     or_counts counts;
     int status;
     /* instances */
@@ -527,7 +543,7 @@ static void* run_job(void* arg) {
             break;
         case JOB_QBVH_CLOSEST:
             if (qbvh_closest_one(j->nodes, j->prims, j->n_nodes, o, d, tn, tf, j->prim + i, j->t + i,
-                                 j->u + i, j->v + i, &j->counts))
+                                 j->u + i, j->v + i, &j->counts, j->tied ? j->tied + i : NULL))
                 j->status = -1;
             break;
         case JOB_INST_CLOSEST:
@@ -661,6 +677,23 @@ int or_qbvh_closest(const void* nodes, uint32_t n_nodes, const void* prims,
     j.n_nodes = n_nodes;
     j.rays = mk_rays(rays);
     j.prim = prim; j.t = t; j.u = u; j.v = v;
+    return run_threads(j, n, nthreads, nodes_visited, prims_tested);
+}
+
+/* or_qbvh_closest plus tied[i] = 1 where ray i's hit distance is shared by
+ * two or more primitives. */
+int or_qbvh_closest_ties(const void* nodes, uint32_t n_nodes, const void* prims,
+                         const float* const rays[8], uint32_t n, uint32_t* prim, float* t, float* u,
+                         float* v, uint8_t* tied, int nthreads, uint64_t* nodes_visited, uint64_t* prims_tested) {
+    or_job j;
+    memset(&j, 0, sizeof(j));
+    j.kind = JOB_QBVH_CLOSEST;
+    j.nodes = (const or_node*)nodes;
+    j.prims = (const or_prim*)prims;
+    j.n_nodes = n_nodes;
+    j.rays = mk_rays(rays);
+    j.prim = prim; j.t = t; j.u = u; j.v = v;
+    j.tied = tied;
     return run_threads(j, n, nthreads, nodes_visited, prims_tested);
 }
 
