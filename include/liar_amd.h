@@ -381,6 +381,10 @@ int lr_spectrum_eval(lr_context* ctx, const lr_spectrum* spectrum, const float* 
 #define LR_LIGHT_PARALLELOGRAM 0u /* LightArea over a Parallelogram         */
 #define LR_LIGHT_SKY 1u           /* LightSky with a constant radiance      */
 #define LR_LIGHT_SUN 2u           /* distant disc of `angle` diameter      */
+#define LR_LIGHT_MESH 3u          /* LightArea over a TriangleMesh: the
+                                     scene triangles first_triangle ..
+                                     first_triangle + n_triangles - 1
+                                     (triangle_mesh.cpp:414-444)          */
 typedef struct lr_light {
     uint32_t type;
     uint32_t radiance;      /* spectrum index                               */
@@ -392,7 +396,10 @@ typedef struct lr_light {
     uint32_t samples;       /* DirectLighting emission samples per camera
                                sample (LightArea / LightSky
                                numberOfEmissionSamples); 0: LiAR's default,
-                               9 for a parallelogram, 1 for sky and sun     */
+                               9 for a parallelogram or mesh, 1 for sky and
+                               sun                                          */
+    uint32_t first_triangle, n_triangles; /* mesh: its triangles' primitive
+                               ids (at most 4096 triangles)                 */
 } lr_light;
 
 /* PerspectiveCamera (src/liar/cameras/perspective_camera.cpp:406-434),

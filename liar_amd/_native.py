@@ -84,7 +84,8 @@ class BsdfDesc(C.Structure):
 class Light(C.Structure):
     _fields_ = [("type", C.c_uint32), ("radiance", C.c_uint32), ("double_sided", C.c_uint32),
                 ("origin", C.c_float * 3), ("dx", C.c_float * 3), ("dy", C.c_float * 3),
-                ("direction", C.c_float * 3), ("angle", C.c_float), ("samples", C.c_uint32)]
+                ("direction", C.c_float * 3), ("angle", C.c_float), ("samples", C.c_uint32),
+                ("first_triangle", C.c_uint32), ("n_triangles", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -160,7 +161,7 @@ LR_NONE = 0xFFFFFFFF
 SPEC_CONSTANT, SPEC_SAMPLED, SPEC_BINNED, SPEC_BLACKBODY = 0, 1, 2, 3
 MAT_NONE, MAT_LAMBERT, MAT_MIRROR, MAT_DIELECTRIC, MAT_CONDUCTOR, MAT_COOK_TORRANCE = 0, 1, 2, 3, 4, 5
 MAT_COOK_TORRANCE_BECKMANN = 6
-LIGHT_PARALLELOGRAM, LIGHT_SKY, LIGHT_SUN = 0, 1, 2
+LIGHT_PARALLELOGRAM, LIGHT_SKY, LIGHT_SUN, LIGHT_MESH = 0, 1, 2, 3
 
 _P = C.c_void_p
 _I = C.c_int

@@ -33,6 +33,8 @@ struct DevLight {
     float n[3], sel_pdf;     // n: unit front normal (parallelogram) / sun direction
     float du[3], cdf;        // du, dv: dual basis, (p - o).du = u
     float dv[3], pad2;
+    uint32_t tri0, ntri;     // mesh: records tri0 .. tri0 + ntri - 1 of DevScene::light_tris
+    uint32_t pad3, pad4;
 };
 
 struct DevMaterial {
@@ -57,6 +59,7 @@ struct DevScene {
     const DevSpectrum* spectra;
     const float* table;
     const DevLight* lights;
+    const float4* light_tris;  // mesh lights: 3 per triangle, (v0, area cdf) (e1, 0) (e2, 0)
     uint32_t n_lights;
     uint32_t n_triangles;
     DevObserver obs;
@@ -534,38 +537,102 @@ __device__ inline float parallelogram_hit(const DevLight& L, V3 p, V3 d, float t
     return t;
 }
 
-// LightContext::emission pdf (solid angle) of direction d from point p.
-__device__ inline float light_pdf(const DevLight& L, V3 p, V3 d) {
+// A mesh light's own surface along (p, d): TriangleMesh::doIntersect with
+// near = tolerance (triangle_mesh.cpp:291-317): the nearest triangle with
+// t > near, kept only if inRange; returns t and its unit normal, or -1.
+__device__ inline float mesh_light_hit(const DevLight& L, const float4* lt, V3 p, V3 d, V3& n) {
+    float best = INFINITY;
+    uint32_t bk = 0xFFFFFFFFu;
+    for (uint32_t k = 0; k < L.ntri; ++k) {
+        const float4* r = lt + 3 * (size_t)(L.tri0 + k);
+        const float4 a = r[0], b = r[1], c = r[2];
+        float t, u, v;
+        if (tri_intersect(p.x, p.y, p.z, d.x, d.y, d.z, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t, u, v) &&
+            t > kTolerance && t < best) {
+            best = t;
+            bk = k;
+        }
+    }
+    if (bk == 0xFFFFFFFFu || !in_range(best, kTolerance, INFINITY)) return -1.0f;
+    const float4* r = lt + 3 * (size_t)(L.tri0 + bk);
+    n = normalize(cross(v3(r[1].x, r[1].y, r[1].z), v3(r[2].x, r[2].y, r[2].z)));
+    return best;
+}
+
+// LightContext::emission pdf (solid angle) of direction d from point p:
+// LightArea::doEmission through SceneObject::doAngularPdf
+// (light_area.cpp:189-203, scene_object.cpp:252-266) re-intersects the
+// surface; the normal it returns is flipped towards p by the
+// IntersectionContext, so the single-sided test there never rejects.
+__device__ inline float light_pdf(const DevLight& L, const float4* lt, V3 p, V3 d) {
     if (L.type == LR_LIGHT_PARALLELOGRAM) {
         const float t = parallelogram_hit(L, p, d, kTolerance);
         if (t < 0.0f) return 0.0f;
-        const float c = dot(d, ld3(L.n));
-        if (!L.double_sided && c > 0.0f) return 0.0f;
-        return t * t / (L.area * fabsf(c));
+        return t * t / (L.area * fabsf(dot(d, ld3(L.n))));
+    }
+    if (L.type == LR_LIGHT_MESH) {
+        V3 n;
+        const float t = mesh_light_hit(L, lt, p, d, n);
+        if (t < 0.0f) return 0.0f;
+        return t * t / (L.area * fabsf(dot(d, n)));
     }
     if (L.type == LR_LIGHT_SKY) return 1.0f / (4.0f * kPi);
     // sun
     return dot(d, ld3(L.n)) >= L.cos_max ? L.inv_solid : 0.0f;
 }
 
-// Radiance of light L seen along d (for the sky / sun: any ray that escapes;
-// for a parallelogram: the caller already hit it).
+// Does light L emit along d?  For the sky and sun: any ray that escapes in
+// their directions; for an area light: the caller already hit its surface,
+// and (as light_pdf explains) LiAR's emission along a ray is not
+// single-sided.
 __device__ inline bool light_sees(const DevLight& L, V3 d) {
-    if (L.type == LR_LIGHT_PARALLELOGRAM) return L.double_sided || dot(d, ld3(L.n)) < 0.0f;
-    if (L.type == LR_LIGHT_SKY) return true;
+    if (L.type == LR_LIGHT_PARALLELOGRAM || L.type == LR_LIGHT_MESH || L.type == LR_LIGHT_SKY) return true;
     return dot(d, ld3(L.n)) >= L.cos_max;
 }
 
 // LightContext::sampleEmission: direction from p towards the light; pdf in
-// solid angle (0 = failed).
-__device__ inline V3 light_sample(const DevLight& L, V3 p, float u1, float u2, float& pdf) {
-    if (L.type == LR_LIGHT_PARALLELOGRAM) {
-        const V3 q = ld3(L.o) + u1 * ld3(L.dx) + u2 * ld3(L.dy);
+// solid angle (0 = failed).  dist: distance to the sampled point (area
+// lights).  Single-sided area lights reject points seen from behind
+// (light_area.cpp:207-252).
+__device__ inline V3 light_sample(const DevLight& L, const float4* lt, V3 p, float u1, float u2, float& pdf,
+                                  float& dist) {
+    dist = INFINITY;
+    if (L.type == LR_LIGHT_PARALLELOGRAM || L.type == LR_LIGHT_MESH) {
+        V3 q, n;
+        float pdf_area;
+        if (L.type == LR_LIGHT_PARALLELOGRAM) {  // parallelogram.cpp:112-118
+            q = ld3(L.o) + u1 * ld3(L.dx) + u2 * ld3(L.dy);
+            n = ld3(L.n);
+            pdf_area = 1.0f / L.area;
+        } else {  // TriangleMesh::doSampleSurface (triangle_mesh.cpp:414-444)
+            const float4* r = lt + 3 * (size_t)L.tri0;
+            uint32_t lo = 0, hi = L.ntri;  // lower_bound of u1 on the area cdf
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (r[3 * mid].w < u1) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint32_t k = lo < L.ntri ? lo : L.ntri - 1;
+            const float c0 = k > 0 ? r[3 * (k - 1)].w : 0.0f;
+            const float tp = r[3 * k].w - c0;
+            const float di = (u1 - c0) / tp;
+            const float sq = sqrtf(di);
+            const float a = 1.0f - sq, b = sq * u2;
+            const V3 v0 = v3(r[3 * k].x, r[3 * k].y, r[3 * k].z);
+            const V3 e1 = v3(r[3 * k + 1].x, r[3 * k + 1].y, r[3 * k + 1].z);
+            const V3 e2 = v3(r[3 * k + 2].x, r[3 * k + 2].y, r[3 * k + 2].z);
+            q = v0 + a * e1 + b * e2;
+            const V3 nv = cross(e1, e2);
+            const float nn = sqrtf(dot(nv, nv));
+            n = (1.0f / nn) * nv;
+            pdf_area = tp / (0.5f * nn);
+        }
         V3 to = q - p;
         const float d2 = dot(to, to);
-        to = (1.0f / sqrtf(d2)) * to;
-        const float c = dot(ld3(L.n), to);
-        pdf = (1.0f / L.area) * d2 / fabsf(c);
+        dist = sqrtf(d2);
+        to = (1.0f / dist) * to;
+        const float c = dot(n, to);
+        pdf = pdf_area * d2 / fabsf(c);
         if (!L.double_sided && c > 0.0f) pdf = 0.0f;
         if (!(pdf < INFINITY)) pdf = 0.0f;
         return to;

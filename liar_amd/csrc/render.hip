@@ -430,7 +430,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
             if (gen > 0) {
                 for (uint32_t k = 0; k < S.n_lights; ++k) {
                     const DevLight& Lk = S.lights[k];
-                    if (Lk.type != LR_LIGHT_PARALLELOGRAM && light_sees(Lk, d))
+                    if ((Lk.type == LR_LIGHT_SKY || Lk.type == LR_LIGHT_SUN) && light_sees(Lk, d))
                         L += thr * spectrum_eval(S, Lk.radiance, lam, false);
                 }
             }
@@ -494,7 +494,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         if (ok) {
                             dw = wo.x * tt + wo.y * bb + wo.z * nf;
                             for (uint32_t k = 0; k < S.n_lights; ++k)
-                                pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], x, dw);
+                                pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], S.light_tris, x, dw);
                             pg = vmf_mix_pdf(mx, dw);
                             pc = cb * pb;
                         }
@@ -503,11 +503,12 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                             const uint32_t k = light_pick(S, u_light);
                             const DevLight& Lk = S.lights[k];
                             float pdf_l = 0.0f;
-                            dw = light_sample(Lk, x, ul1, ul2, pdf_l);
+                            float dist_l;
+                            dw = light_sample(Lk, S.light_tris, x, ul1, ul2, pdf_l, dist_l);
                             if (Lk.sel_pdf > 0.0f && pdf_l > 0.0f) {
                                 pl = Lk.sel_pdf * pdf_l;
                                 for (uint32_t j2 = 0; j2 < S.n_lights; ++j2)
-                                    if (j2 != k) pl += S.lights[j2].sel_pdf * light_pdf(S.lights[j2], x, dw);
+                                    if (j2 != k) pl += S.lights[j2].sel_pdf * light_pdf(S.lights[j2], S.light_tris, x, dw);
                                 wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
                                 ok = bsdf_evaluate(B, wi, wo, ngl, f, pb) && f != 0.0f;
                                 pg = vmf_mix_pdf(mx, dw);
@@ -521,7 +522,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         ok = bsdf_evaluate(B, wi, wo, ngl, f, pb) && f != 0.0f;
                         if (ok) {
                             for (uint32_t k = 0; k < S.n_lights; ++k)
-                                pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], x, dw);
+                                pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], S.light_tris, x, dw);
                             pg = vmf_mix_pdf(mx, dw);
                             pc = cg * pg;
                         }
@@ -545,7 +546,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         if (strategy < 1.0f) {
                             const V3 dw = wo.x * tt + wo.y * bb + wo.z * nf;
                             for (uint32_t k = 0; k < S.n_lights; ++k)
-                                pdf_lights += S.lights[k].sel_pdf * light_pdf(S.lights[k], x, dw);
+                                pdf_lights += S.lights[k].sel_pdf * light_pdf(S.lights[k], S.light_tris, x, dw);
                             pdf_lights *= (1.0f - strategy);
                         }
                         value *= (pdf * pdf) / (pdf * pdf + pdf_lights * pdf_lights);
@@ -555,11 +556,12 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                     const uint32_t k = light_pick(S, u_light);
                     const DevLight& Lk = S.lights[k];
                     float pdf_l = 0.0f;
-                    const V3 dw = light_sample(Lk, x, ul1, ul2, pdf_l);
+                    float dist_l;
+                    const V3 dw = light_sample(Lk, S.light_tris, x, ul1, ul2, pdf_l, dist_l);
                     if (Lk.sel_pdf > 0.0f && pdf_l > 0.0f) {
                         float pl = Lk.sel_pdf * pdf_l;
                         for (uint32_t j = 0; j < S.n_lights; ++j)
-                            if (j != k) pl += S.lights[j].sel_pdf * light_pdf(S.lights[j], x, dw);
+                            if (j != k) pl += S.lights[j].sel_pdf * light_pdf(S.lights[j], S.light_tris, x, dw);
                         pl *= (1.0f - strategy);
                         wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
                         float vb, pb;
@@ -786,6 +788,7 @@ struct lr_scene {
     DevSpectrum* d_spectra = nullptr;
     float* d_table = nullptr;
     DevLight* d_lights = nullptr;
+    float4* d_light_tris = nullptr;  // mesh lights (DevScene::light_tris)
     // observer restricted to the last render's sampling domain
     float obs_lo = -1.0f, obs_hi = -1.0f;
     uint32_t obs_n = 0;
@@ -825,7 +828,7 @@ void free_scene(lr_scene* s) {
     if (s->cstream) (void)hipStreamDestroy(s->cstream);
     void* ptrs[] = {s->d_prim_mat, s->d_prim_geom, s->d_prim_tan, s->d_corner, s->d_mats,   s->d_spectra,
                     s->d_table,    s->d_lights,    s->d_obs,      s->d_work,   s->d_counts, s->d_gwork,
-                    s->d_dlwork,   s->d_dlocc};
+                    s->d_dlwork,   s->d_dlocc,     s->d_light_tris};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -1093,6 +1096,8 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
     const double far_away = R > 0 ? 1e3 * R : 1e3;
     std::vector<DevLight> dl(n_lights);
     std::vector<double> power(n_lights, 0.0);
+    std::vector<float4> light_tris;   // mesh lights' triangles
+    std::vector<uint32_t> prim_rec;   // primitive id -> record index (mesh lights only)
     for (uint32_t i = 0; i < n_lights; ++i) {
         const lr_light& L = lights[i];
         DevLight& d = dl[i];
@@ -1100,7 +1105,7 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
         d.type = L.type;
         d.radiance = L.radiance;
         d.double_sided = L.double_sided ? 1u : 0u;
-        d.samples = L.samples ? L.samples : (L.type == LR_LIGHT_PARALLELOGRAM ? 9u : 1u);
+        d.samples = L.samples ? L.samples : ((L.type == LR_LIGHT_PARALLELOGRAM || L.type == LR_LIGHT_MESH) ? 9u : 1u);
         if (d.samples > 64) return lr::fail(LR_E_INVALID, "light " + std::to_string(i) + ": at most 64 samples");
         if (L.radiance >= n_spectra) return lr::fail(LR_E_INVALID, "light " + std::to_string(i) + " has no radiance");
         const double lum = host_luminance(spectra[L.radiance]);
@@ -1131,6 +1136,40 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
             d.dv[1] = (float)(vy / vd);
             d.dv[2] = (float)(vz / vd);
             power[i] = 3.14159265358979 * area * lum * (L.double_sided ? 2.0 : 1.0);
+        } else if (L.type == LR_LIGHT_MESH) {
+            // TriangleMesh surface sampling (triangle_mesh.cpp:69-85,414-444):
+            // triangles picked by a cumulative area table, uniform inside
+            if (L.n_triangles == 0 || L.n_triangles > 4096 || L.first_triangle > bvh->info.n_triangles ||
+                L.n_triangles > bvh->info.n_triangles - L.first_triangle)
+                return lr::fail(LR_E_INVALID, "light " + std::to_string(i) +
+                                                  ": mesh triangles out of range (1 .. 4096 of the scene's triangles)");
+            if (prim_rec.empty()) {
+                prim_rec.assign(n_prims, LR_NONE);
+                for (size_t r = 0; r < bvh->host.prims.size(); ++r)
+                    prim_rec[lr::as_u32(bvh->host.prims[r].a[3])] = (uint32_t)r;
+            }
+            d.tri0 = (uint32_t)(light_tris.size() / 3);
+            d.ntri = L.n_triangles;
+            std::vector<double> acc(L.n_triangles);
+            double total = 0.0;
+            for (uint32_t k = 0; k < L.n_triangles; ++k) {
+                const lr_prim& q = bvh->host.prims[prim_rec[L.first_triangle + k]];
+                const double cx = (double)q.b[1] * q.c[2] - (double)q.b[2] * q.c[1];
+                const double cy = (double)q.b[2] * q.c[0] - (double)q.b[0] * q.c[2];
+                const double cz = (double)q.b[0] * q.c[1] - (double)q.b[1] * q.c[0];
+                total += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+                acc[k] = total;
+            }
+            if (!(total > 0)) return lr::fail(LR_E_INVALID, "light " + std::to_string(i) + ": mesh has no area");
+            for (uint32_t k = 0; k < L.n_triangles; ++k) {
+                const lr_prim& q = bvh->host.prims[prim_rec[L.first_triangle + k]];
+                const float cdf = k + 1 == L.n_triangles ? 1.0f : (float)(acc[k] / total);
+                light_tris.push_back(make_float4(q.a[0], q.a[1], q.a[2], cdf));
+                light_tris.push_back(make_float4(q.b[0], q.b[1], q.b[2], 0.0f));
+                light_tris.push_back(make_float4(q.c[0], q.c[1], q.c[2], 0.0f));
+            }
+            d.area = (float)total;
+            power[i] = 3.14159265358979 * total * lum * (L.double_sided ? 2.0 : 1.0);
         } else if (L.type == LR_LIGHT_SKY) {
             power[i] = lum * 3.14159265358979 * far_away * far_away * 4.0 * 3.14159265358979;
         } else if (L.type == LR_LIGHT_SUN) {
@@ -1177,6 +1216,7 @@ int lr_scene_create(lr_context* ctx, lr_qbvh* bvh, const uint32_t* prim_material
     if (!rc) rc = upload(dspec, &sc->d_spectra);
     if (!rc) rc = upload(table, &sc->d_table);
     if (!rc) rc = upload(dl, &sc->d_lights);
+    if (!rc) rc = upload(light_tris, &sc->d_light_tris);
     if (rc) {
         free_scene(sc);
         return rc;
@@ -1303,6 +1343,7 @@ static DevScene dev_scene(const lr_scene* sc) {
     S.spectra = sc->d_spectra;
     S.table = sc->d_table;
     S.lights = sc->d_lights;
+    S.light_tris = sc->d_light_tris;
     S.n_lights = sc->n_lights;
     S.n_triangles = sc->n_triangles;
     S.obs.n = sc->obs_n;
@@ -2008,10 +2049,14 @@ __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DL
     if (Le == 0.0f) return false;
     const lr::u4 r = dl_rng(P, H.gp, H.gen, H.br, 100u + j);
     float f = 0.0f, pb = 0.0f, pl = 0.0f;
+    float dist = INFINITY;  // to the mesh light's surface along dir
     if ((local & 1u) == 0u) {
         // light sample, weighted against BSDF sampling (ray_tracer.cpp:211-231)
-        dir = light_sample(Lk, H.start, lr::u01(r.x), lr::u01(r.y), pl);
+        dir = light_sample(Lk, S.light_tris, H.start, lr::u01(r.x), lr::u01(r.y), pl, dist);
         if (!(pl > 0.0f)) return false;
+        // sampleEmission with the target's normal rejects lights behind the
+        // shading normal (light_area.cpp:242-252)
+        if (!(dot(dir, H.nf) > 0.0f)) return false;
         const V3 wo = v3(dot(dir, H.tt), dot(dir, H.bb), dot(dir, H.nf));
         if (!bsdf_evaluate(H.B, H.wi, wo, H.ngl, f, pb) || f == 0.0f) return false;
         c = H.w * f * Le * power_heuristic(nl * pl, nl * pb) * fabsf(wo.z) / (nl * pl * scale);
@@ -2022,8 +2067,12 @@ __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DL
             f == 0.0f)
             return false;
         dir = normalize(wo.x * H.tt + wo.y * H.bb + wo.z * H.nf);
-        pl = light_pdf(Lk, H.start, dir);
+        pl = light_pdf(Lk, S.light_tris, H.start, dir);
         if (!(pl > 0.0f)) return false;
+        if (Lk.type == LR_LIGHT_MESH) {
+            V3 nl_;
+            dist = mesh_light_hit(Lk, S.light_tris, H.start, dir, nl_);
+        }
         const float wt = bsdf_is_specular(H.B.type) ? 1.0f : power_heuristic(nl * pb, nl * pl);
         c = H.w * f * Le * wt * fabsf(wo.z) / (nl * pb * scale);
     }
@@ -2035,6 +2084,10 @@ __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DL
         const float t = dot(ld3(Lk.o) - H.start, n) / dot(dir, n);
         if (!(t > 0.0f)) return false;
         far = t * (1.0f - 1e-4f);
+    } else if (Lk.type == LR_LIGHT_MESH) {
+        // likewise short of the sampled point / the surface the ray reaches
+        if (!(dist > 0.0f) || !isfinite(dist)) return false;
+        far = dist * (1.0f - 1e-4f);
     }
     return c != 0.0f && isfinite(c);
 }

@@ -164,6 +164,18 @@ def compile_scene(scene) -> CompiledScene:
                 r = obj.radius * k
             sph.append([*point(M, obj.center), r])
             sph_mat.append(material(shader))
+        elif isinstance(obj, scenery.LightArea) and isinstance(obj.surface, scenery.TriangleMesh):
+            surf = obj.surface
+            li = len(lights)
+            first = sum(len(t) for t in tris)
+            v = surf.vertices()
+            if M is not None:
+                v = (np.asarray(v, np.float64) @ M[:3, :3].T + M[:3, 3]).astype(np.float32)
+            lights.append({"type": N.LIGHT_MESH, "radiance": _spectrum_of(obj.radiance, table, spec_index),
+                           "double_sided": bool(obj.isDoubleSided), "samples": int(obj.numberOfEmissionSamples),
+                           "first_triangle": first, "n_triangles": len(surf.triangles())})
+            s = obj.shader if obj.shader is not None else surf.shader
+            add_mesh(v, surf.triangles(), material(s, li), surf.corner_normals(M))
         elif isinstance(obj, scenery.LightArea):
             surf = obj.surface
             li = len(lights)
@@ -232,6 +244,8 @@ class GpuScene:
                     getattr(ls[i], name)[:] = [float(x) for x in L[name]]
             ls[i].angle = float(L.get("angle", 0.0))
             ls[i].samples = int(L.get("samples", 0))
+            ls[i].first_triangle = int(L.get("first_triangle", 0))
+            ls[i].n_triangles = int(L.get("n_triangles", 0))
         pm = np.ascontiguousarray(cs.prim_material, np.uint32)
         h = C.c_void_p()
         N.check(N.lib().lr_scene_create(ctx.handle, self.qbvh.handle, N.ptr(pm), mats, len(cs.materials), specs,

@@ -226,13 +226,16 @@ class AabbTree(List):
 
 
 class LightArea(SceneObject):
-    """Area light over a surface (src/liar/scenery/light_area.cpp); the
-    surface must be a Parallelogram."""
+    """Area light over a surface (src/liar/scenery/light_area.cpp): a
+    Parallelogram, or a TriangleMesh of at most 4096 triangles (sampled by
+    area, triangle_mesh.cpp:414-444).  As in LiAR, a single-sided light is
+    sampled on its front face only, but emits from both faces when a ray hits
+    it."""
 
     def __init__(self, surface):
         super().__init__()
-        if not isinstance(surface, Parallelogram):
-            raise TypeError("LightArea surfaces are limited to Parallelogram")
+        if not isinstance(surface, (Parallelogram, TriangleMesh)):
+            raise TypeError("LightArea surfaces are limited to Parallelogram and TriangleMesh")
         self.surface = surface
         self.radiance = as_texture(1.0)
         self.isDoubleSided = False

@@ -271,7 +271,8 @@ class _Mat(C.Structure):
 class _Light(C.Structure):
     _fields_ = [("type", C.c_uint32), ("radiance", C.c_uint32), ("double_sided", C.c_uint32),
                 ("origin", C.c_float * 3), ("dx", C.c_float * 3), ("dy", C.c_float * 3),
-                ("direction", C.c_float * 3), ("angle", C.c_float), ("samples", C.c_uint32)]
+                ("direction", C.c_float * 3), ("angle", C.c_float), ("samples", C.c_uint32),
+                ("first_triangle", C.c_uint32), ("n_triangles", C.c_uint32)]
 
 
 class _Cam(C.Structure):
@@ -357,6 +358,8 @@ def _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_dep
                 getattr(lights[i], name)[:] = [float(x) for x in Ld[name]]
         lights[i].angle = float(Ld.get("angle", 0.0))
         lights[i].samples = int(Ld.get("samples", 0))
+        lights[i].first_triangle = int(Ld.get("first_triangle", 0))
+        lights[i].n_triangles = int(Ld.get("n_triangles", 0))
     cam = _Cam()
     cam.position[:] = [float(x) for x in camera.position]
     cam.direction[:] = [float(x) for x in camera.direction]

@@ -65,7 +65,8 @@ typedef struct { uint32_t type, refl, trans, eta, kappa, emission, light; float 
 typedef struct {
     uint32_t type, radiance, double_sided;
     float origin[3], dx[3], dy[3], direction[3], angle;
-    uint32_t samples; /* DirectLighting emission samples (0: 9 parallelogram, 1 otherwise) */
+    uint32_t samples; /* DirectLighting emission samples (0: 9 parallelogram or mesh, 1 otherwise) */
+    uint32_t first_triangle, n_triangles; /* mesh light: its triangles' primitive ids */
 } or_light;
 typedef struct { float position[3], direction[3], sky[3], fov, aspect, near_limit; } or_camera;
 typedef struct {
@@ -98,6 +99,7 @@ typedef struct {
     double* light_sel; double* light_cdf; double* light_area; V* light_n; V* light_du; V* light_dv;
     double* light_cos; double* light_invsolid;
     uint32_t* light_ns; /* DirectLighting samples per light */
+    double** light_tcdf; /* mesh lights: cumulative area fractions of their triangles */
     double* spec_reflscale;
     /* observer restricted to the domain */
     int on; double *ow, *ocdf, *oxyz;
@@ -355,22 +357,50 @@ static double para_hit(const Scene* S, uint32_t k, V p, V d, double tn) {
     if (!(u >= 0 && u <= 1 && vv >= 0 && vv <= 1)) return -1;
     return t;
 }
+/* The mesh light's own surface along (p, d): TriangleMesh::doIntersect with
+ * near = tolerance (triangle_mesh.cpp:291-317), the nearest triangle with
+ * t > near kept only if inRange; returns t (or -1) and the unit normal. */
+static double mesh_light_hit(const Scene* S, uint32_t k, V p, V d, V* n) {
+    const or_light* L = &S->lights[k];
+    double best = INFINITY;
+    uint32_t bk = RMISS;
+    for (uint32_t j = 0; j < L->n_triangles; ++j) {
+        double t;
+        if (tri_hit(S, L->first_triangle + j, p, d, &t) && t > TOL && t < best) { best = t; bk = j; }
+    }
+    if (bk == RMISS || !in_range(best, TOL, INFINITY)) return -1;
+    uint32_t tri = L->first_triangle + bk;
+    const float* a = S->verts + 3 * (size_t)S->tris[3 * tri];
+    const float* b = S->verts + 3 * (size_t)S->tris[3 * tri + 1];
+    const float* c = S->verts + 3 * (size_t)S->tris[3 * tri + 2];
+    *n = nrm(cross(sub(f3(b), f3(a)), sub(f3(c), f3(a))));
+    return best;
+}
+/* LightContext::emission pdf: LightArea::doEmission through
+ * SceneObject::doAngularPdf (light_area.cpp:189-203, scene_object.cpp:252-266)
+ * re-intersects the surface, whose normal the IntersectionContext has
+ * flipped towards p, so the single-sided test never rejects. */
 static double light_pdf(const Scene* S, uint32_t k, V p, V d) {
     const or_light* L = &S->lights[k];
     if (L->type == 0) {
         double t = para_hit(S, k, p, d, TOL);
         if (t < 0) return 0;
-        double c = dot(d, S->light_n[k]);
-        if (!L->double_sided && c > 0) return 0;
-        return t * t / (S->light_area[k] * fabs(c));
+        return t * t / (S->light_area[k] * fabs(dot(d, S->light_n[k])));
+    }
+    if (L->type == 3) {
+        V n;
+        double t = mesh_light_hit(S, k, p, d, &n);
+        if (t < 0) return 0;
+        return t * t / (S->light_area[k] * fabs(dot(d, n)));
     }
     if (L->type == 1) return 1.0 / (4 * PI);
     return dot(d, S->light_n[k]) >= S->light_cos[k] ? S->light_invsolid[k] : 0.0;
 }
+/* emission along d: area lights (already hit) emit from both faces (see
+ * light_pdf), the sky always, the sun within its disc */
 static int light_sees(const Scene* S, uint32_t k, V d) {
     const or_light* L = &S->lights[k];
-    if (L->type == 0) return L->double_sided || dot(d, S->light_n[k]) < 0;
-    if (L->type == 1) return 1;
+    if (L->type == 0 || L->type == 1 || L->type == 3) return 1;
     return dot(d, S->light_n[k]) >= S->light_cos[k];
 }
 static void onb(V n, V* t, V* b) {  /* Duff et al., JCGT 2017 */
@@ -382,15 +412,41 @@ static V uniform_sphere(double u1, double u2) {
     double z = 1 - 2 * u1, r = sqrt(fmax(0, 1 - z * z)), phi = 2 * PI * u2;
     return v(r * cos(phi), r * sin(phi), z);
 }
-static V light_sample(const Scene* S, uint32_t k, V p, double u1, double u2, double* pdf) {
+/* sampleEmission towards point p (light_area.cpp:207-252); *dist: distance
+ * to the sampled point of an area light */
+static V light_sample(const Scene* S, uint32_t k, V p, double u1, double u2, double* pdf, double* dist) {
     const or_light* L = &S->lights[k];
-    if (L->type == 0) {
-        V q = add(f3(L->origin), add(mul(u1, f3(L->dx)), mul(u2, f3(L->dy))));
+    *dist = INFINITY;
+    if (L->type == 0 || L->type == 3) {
+        V q, n;
+        double pdf_area;
+        if (L->type == 0) {
+            q = add(f3(L->origin), add(mul(u1, f3(L->dx)), mul(u2, f3(L->dy))));
+            n = S->light_n[k];
+            pdf_area = 1 / S->light_area[k];
+        } else {  /* TriangleMesh::doSampleSurface (triangle_mesh.cpp:414-444) */
+            const double* cdf = S->light_tcdf[k];
+            uint32_t ii = 0;
+            while (ii < L->n_triangles && cdf[ii] < u1) ++ii;  /* lower_bound */
+            if (ii >= L->n_triangles) ii = L->n_triangles - 1;
+            double c0 = ii > 0 ? cdf[ii - 1] : 0, tp = cdf[ii] - c0;
+            double di = (u1 - c0) / tp, sq = sqrt(di), a = 1 - sq, b = sq * u2;
+            uint32_t tri = L->first_triangle + ii;
+            const float* va = S->verts + 3 * (size_t)S->tris[3 * tri];
+            const float* vb = S->verts + 3 * (size_t)S->tris[3 * tri + 1];
+            const float* vc = S->verts + 3 * (size_t)S->tris[3 * tri + 2];
+            V v0 = f3(va), e1 = sub(f3(vb), v0), e2 = sub(f3(vc), v0), nv = cross(e1, e2);
+            double nn = sqrt(dot(nv, nv));
+            q = add(v0, add(mul(a, e1), mul(b, e2)));
+            n = mul(1 / nn, nv);
+            pdf_area = tp / (0.5 * nn);
+        }
         V to = sub(q, p);
         double d2 = dot(to, to);
-        to = mul(1 / sqrt(d2), to);
-        double c = dot(S->light_n[k], to);
-        *pdf = d2 / (S->light_area[k] * fabs(c));
+        *dist = sqrt(d2);
+        to = mul(1 / *dist, to);
+        double c = dot(n, to);
+        *pdf = pdf_area * d2 / fabs(c);
         if (!L->double_sided && c > 0) *pdf = 0;
         if (!(*pdf < INFINITY)) *pdf = 0;
         return to;
@@ -635,7 +691,7 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
         if (prim == RMISS) {
             if (gen > 0)
                 for (uint32_t k = 0; k < S->n_lights; ++k)
-                    if (S->lights[k].type != 0 && light_sees(S, k, d))
+                    if ((S->lights[k].type == 1 || S->lights[k].type == 2) && light_sees(S, k, d))
                         L += thr * spec_eval(S, S->lights[k].radiance, lam, 0);
             return L;
         }
@@ -686,7 +742,8 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
         } else if (S->n_lights > 0) {
             uint32_t k = light_pick(S, ul);
             double pdfl;
-            V dw = light_sample(S, k, x, ul1, ul2, &pdfl);
+            double distl;
+            V dw = light_sample(S, k, x, ul1, ul2, &pdfl, &distl);
             if (S->light_sel[k] > 0 && pdfl > 0) {
                 double pl = S->light_sel[k] * pdfl;
                 for (uint32_t j = 0; j < S->n_lights; ++j)
@@ -787,11 +844,13 @@ static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
     if (Le == 0) return 0;
     uint32_t r[4];
     rng(c, H->p, H->key, 100u + j, r);
-    double f = 0, pb = 0, pl = 0, contrib;
+    double f = 0, pb = 0, pl = 0, contrib, dist = INFINITY;
     V dir;
     if ((local & 1u) == 0) {
-        dir = light_sample(S, k, H->start, u01(r[0]), u01(r[1]), &pl);
+        dir = light_sample(S, k, H->start, u01(r[0]), u01(r[1]), &pl, &dist);
         if (!(pl > 0)) return 0;
+        /* sampleEmission with the target normal (light_area.cpp:242-252) */
+        if (!(dot(dir, H->nf) > 0)) return 0;
         V wo = v(dot(dir, H->tt), dot(dir, H->bb), dot(dir, H->nf));
         if (!bsdf_evaluate(&H->B, H->wi, wo, H->ngl, &f, &pb) || f == 0) return 0;
         double wt = g_plant == 2 ? 1 : sq_heuristic(nl * pl, nl * pb);
@@ -803,6 +862,10 @@ static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
         dir = nrm(add(add(mul(wo.x, H->tt), mul(wo.y, H->bb)), mul(wo.z, H->nf)));
         pl = light_pdf(S, k, H->start, dir);
         if (!(pl > 0)) return 0;
+        if (L->type == 3) {
+            V nn;
+            dist = mesh_light_hit(S, k, H->start, dir, &nn);
+        }
         double wt = (is_specular(H->B.type) || g_plant == 2) ? 1 : sq_heuristic(nl * pb, nl * pl);
         contrib = f * Le * wt * fabs(wo.z) / (nl * pb * scale);
     }
@@ -812,6 +875,9 @@ static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
         double t = dot(sub(f3(L->origin), H->start), n) / dot(dir, n);
         if (!(t > 0)) return 0;
         far = t * (1 - 1e-4);
+    } else if (L->type == 3) {
+        if (!(dist > 0) || !isfinite(dist)) return 0;
+        far = dist * (1 - 1e-4);
     }
     if (!(contrib != 0) || !isfinite(contrib)) return 0;
     return occluded(S, H->start, dir, TOL, far) ? 0 : contrib;
@@ -998,8 +1064,10 @@ static int render_impl(const float* verts, const uint32_t* tris, uint32_t nt, co
     S.light_du = calloc(nl, sizeof(V)); S.light_dv = calloc(nl, sizeof(V));
     S.light_cos = calloc(nl, sizeof(double)); S.light_invsolid = calloc(nl, sizeof(double));
     S.light_ns = calloc(nl, sizeof(uint32_t));
+    S.light_tcdf = calloc(nl, sizeof(double*));
     for (uint32_t k = 0; k < n_lights; ++k)
-        S.light_ns[k] = lights[k].samples ? lights[k].samples : (lights[k].type == 0 ? 9u : 1u);
+        S.light_ns[k] = lights[k].samples ? lights[k].samples
+                                          : ((lights[k].type == 0 || lights[k].type == 3) ? 9u : 1u);
     double total = 0;
     double* power = calloc(nl, sizeof(double));
     for (uint32_t k = 0; k < n_lights; ++k) {
@@ -1015,6 +1083,25 @@ static int render_impl(const float* verts, const uint32_t* tris, uint32_t nt, co
             S.light_du[k] = mul(1 / dot(a, u), u);
             S.light_dv[k] = mul(1 / dot(b, w2), w2);
             power[k] = PI * area * Y * (L->double_sided ? 2 : 1);
+        } else if (L->type == 3) {
+            /* TriangleMesh ctor (triangle_mesh.cpp:69-85): cumulative areas,
+             * normalised, the last exactly 1 */
+            double* cdf = malloc(sizeof(double) * (L->n_triangles ? L->n_triangles : 1));
+            double acc = 0;
+            for (uint32_t j = 0; j < L->n_triangles; ++j) {
+                uint32_t tri = L->first_triangle + j;
+                const float* a = verts + 3 * (size_t)tris[3 * tri];
+                const float* b = verts + 3 * (size_t)tris[3 * tri + 1];
+                const float* c = verts + 3 * (size_t)tris[3 * tri + 2];
+                V cr = cross(sub(f3(b), f3(a)), sub(f3(c), f3(a)));
+                acc += 0.5 * sqrt(dot(cr, cr));
+                cdf[j] = acc;
+            }
+            for (uint32_t j = 0; j < L->n_triangles; ++j) cdf[j] /= acc;
+            if (L->n_triangles) cdf[L->n_triangles - 1] = 1;
+            S.light_tcdf[k] = cdf;
+            S.light_area[k] = acc;
+            power[k] = PI * acc * Y * (L->double_sided ? 2 : 1);
         } else if (L->type == 1) {
             power[k] = Y * PI * far_away * far_away * 4 * PI;
         } else {
@@ -1076,6 +1163,8 @@ static int render_impl(const float* verts, const uint32_t* tris, uint32_t nt, co
     free(S.ow); free(S.ocdf); free(S.oxyz); free(S.spec_reflscale);
     free(S.light_sel); free(S.light_cdf); free(S.light_area); free(S.light_n); free(S.light_du);
     free(S.light_dv); free(S.light_cos); free(S.light_invsolid); free(S.light_ns);
+    for (uint32_t k = 0; k < n_lights; ++k) free(S.light_tcdf[k]);
+    free(S.light_tcdf);
     return 0;
 }
 
