@@ -432,7 +432,35 @@ typedef struct lr_render_params {
                               updated in place on later calls, and launched
                               as one; the caller's stream is ordered before
                               and after it by events                         */
+    uint32_t sampler;      /* LR_SAMPLER_STRATIFIER (0) or LR_SAMPLER_HALTON */
+    uint32_t sampler_generations; /* Halton: ray generations that draw from
+                              the sequence (the tracer's maxRayGeneration;
+                              later ones from the counter-based stream)     */
+    uint64_t sample_offset; /* Halton: index of the call's first sample, so
+                              that successive calls continue the sequence  */
 } lr_render_params;
+
+/* Samplers.  LR_SAMPLER_STRATIFIER: spp stratified samples in every pixel,
+ * one per path and wavelength stratum (counter-based numbers).
+ * LR_SAMPLER_HALTON: LiAR's Halton sampler (src/liar/samplers/halton.cpp:
+ * 180-230): width * height * spp * wavelengths samples of one global
+ * sequence over the whole film, sample j at index j + 1 + sample_offset:
+ * screen = radical inverses in bases 2 and 3, wavelength base 13, and for
+ * generation g < sampler_generations the AdjointPhotonTracer's sub-
+ * sequences (per generation 1D strategy, component, light choice, scatter
+ * and 2D BSDF, light, phase) on the primes from 17 on, 1D ones first, each
+ * digit-scrambled by the permutation LiAR's initialize() draws
+ * (std::mt19937 default-seeded, std::shuffle, in prime order).  A sample's
+ * pixel is where its screen point falls, so the film weights vary per
+ * pixel and are added with float atomics (reproducible up to FP32
+ * summation order).  Path tracing (lr_render) only. */
+#define LR_SAMPLER_STRATIFIER 0u
+#define LR_SAMPLER_HALTON 1u
+
+/* The Halton sampler's digit permutation of sub-sequence prime `index` (0:
+ * prime 17, 1: 19, ...; at most 284): *prime and, when out has room for
+ * prime entries, the permutation. */
+int lr_halton_permutation(uint32_t index, uint32_t* prime, uint32_t* out, uint32_t capacity);
 
 typedef struct lr_render_stats {
     uint64_t paths;

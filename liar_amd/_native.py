@@ -98,7 +98,12 @@ class RenderParams(C.Structure):
                 ("wavelengths", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("rr_survival", C.c_float), ("lambda_min", C.c_float), ("lambda_max", C.c_float),
                 ("seed", C.c_uint64), ("chunk_paths", C.c_uint32), ("max_sample_luminance", C.c_float),
-                ("graph", C.c_uint32)]
+                ("graph", C.c_uint32), ("sampler", C.c_uint32), ("sampler_generations", C.c_uint32),
+                ("sample_offset", C.c_uint64)]
+
+
+LR_SAMPLER_STRATIFIER = 0
+LR_SAMPLER_HALTON = 1
 
 
 class MeshData(C.Structure):
@@ -212,6 +217,7 @@ SIGNATURES = {
     "lr_bsdf_eval": (_I, [_P, _P, _P, _P, _U32, _P, _P, _P]),
     "lr_bsdf_sample": (_I, [_P, _P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
     "lr_spectrum_eval": (_I, [_P, _P, _P, _U32, _U32, _P, _P]),
+    "lr_halton_permutation": (_I, [_U32, _P, _P, _U32]),
     "lr_render": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), _P, _P, C.POINTER(RenderStats), _P]),
     "lr_render_direct": (_I, [_P, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(DirectParams), _P, _P,
                               C.POINTER(RenderStats), _P]),

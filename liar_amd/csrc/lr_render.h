@@ -158,6 +158,35 @@ __device__ inline void observer_xyz(const DevObserver& O, float w, float xyz[3])
 }
 
 // ---------------------------------------------------------------------------
+// Halton sampler (src/liar/samplers/halton.cpp:180-230, halton.h:41-66)
+// ---------------------------------------------------------------------------
+
+// ScrambledRadicalInverse of index i in base b: digit d of i (least
+// significant first) contributes perm[d] b^-(k+1), summed in FP32 in that
+// order; perm == nullptr is the plain radical inverse.
+__device__ __host__ inline float radical_inverse(uint64_t i, uint32_t b, const uint16_t* perm) {
+    const float inv = 1.0f / (float)b;
+    float w = inv, r = 0.0f;
+    if ((i >> 32) == 0) {
+        uint32_t j = (uint32_t)i;
+        while (j) {
+            const uint32_t d = j % b;
+            j /= b;
+            r = r + (float)(perm ? perm[d] : d) * w;
+            w = w * inv;
+        }
+        return r;
+    }
+    while (i) {
+        const uint32_t d = (uint32_t)(i % b);
+        i /= b;
+        r = r + (float)(perm ? perm[d] : d) * w;
+        w = w * inv;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
 // vectors and warps
 // ---------------------------------------------------------------------------
 struct V3 {

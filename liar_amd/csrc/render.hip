@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -46,7 +47,26 @@ struct Params {
     float max_lum;  // 0: no clamp
     uint32_t seed_lo, seed_hi;
     uint32_t strata;  // sqrt(spp) if spp is a square, else 0
+    // Halton sampler (LR_SAMPLER_HALTON): path p of the call is sequence
+    // position hoff + p (index hoff + p + 1, and the key of its random
+    // streams); generations < hgens draw from the sub-sequences whose digit
+    // permutations start at perm + perm_off[prime index]
+    uint32_t halton, hgens;
+    uint64_t hoff;
+    const uint16_t* perm;
+    const uint32_t* perm_off;
 };
+
+// The Halton sub-sequence numbers of generation g (halton.cpp:180-230 with
+// the AdjointPhotonTracer's requests, adjoint_photon_tracer.cpp:47-67): per
+// generation the 1D strategy, component, light choice and scatter samples,
+// then (after all 1D ones) the 2D BSDF, light and phase samples, each
+// coordinate on the next prime.
+__device__ inline float halton_sub(const Params& P, uint64_t index, uint32_t prime_index) {
+    const uint32_t off = P.perm_off[prime_index];
+    const uint32_t prime = P.perm_off[prime_index + 1] - off;
+    return radical_inverse(index, prime, P.perm + off);
+}
 
 struct Queue {
     float* r[8];  // ox oy oz dx dy dz tnear tfar
@@ -162,7 +182,7 @@ __global__ void k_camera(DevScene S0, DevCamera C0, Params P0, uint64_t path0, u
     if (i < n) {
     DevScene S = S0;
     Params P = P0;
-    uint64_t p = path0 + i;
+    uint64_t p = P0.hoff + path0 + i;  // Halton: the path's index in the sequence
     if (BATCH) {
         const Segment sg = B.seg[find_segment(B, i, false)];
         S = B.scenes[sg.scene];
@@ -171,25 +191,35 @@ __global__ void k_camera(DevScene S0, DevCamera C0, Params P0, uint64_t path0, u
         p = (uint64_t)sg.pixel0 * (P.spp * P.nl) + (i - sg.path_begin);
         root = B.roots[sg.scene];
     }
-    const uint32_t l = (uint32_t)(p % P.nl);
-    const uint64_t sidx = p / P.nl;
-    const uint32_t s = (uint32_t)(sidx % P.spp);
-    const uint64_t pixel = sidx / P.spp;
-    const uint32_t px = (uint32_t)(pixel % P.width), py = (uint32_t)(pixel / P.width);
-    const lr::u4 rs = rng(P, sidx, 0xFFFFu, 0u);  // per camera sample (shared by its wavelengths)
     const lr::u4 rp = rng(P, p, 0xFFFEu, 0u);     // per path
-    float sx = lr::u01(rs.x), sy = lr::u01(rs.y);
-    if (P.strata) {
-        sx = ((float)(s % P.strata) + sx) / (float)P.strata;
-        sy = ((float)(s / P.strata) + sy) / (float)P.strata;
+    float X, Y, uw;
+    if (P.halton) {
+        // one global sequence over the film: screen in bases 2 and 3,
+        // wavelength in base 13 (halton.cpp:191-212)
+        const uint64_t j = p + 1;
+        X = radical_inverse(j, 2u, nullptr);
+        Y = radical_inverse(j, 3u, nullptr);
+        uw = radical_inverse(j, 13u, nullptr);
+    } else {
+        const uint32_t l = (uint32_t)(p % P.nl);
+        const uint64_t sidx = p / P.nl;
+        const uint32_t s = (uint32_t)(sidx % P.spp);
+        const uint64_t pixel = sidx / P.spp;
+        const uint32_t px = (uint32_t)(pixel % P.width), py = (uint32_t)(pixel / P.width);
+        const lr::u4 rs = rng(P, sidx, 0xFFFFu, 0u);  // per camera sample (shared by its wavelengths)
+        float sx = lr::u01(rs.x), sy = lr::u01(rs.y);
+        if (P.strata) {
+            sx = ((float)(s % P.strata) + sx) / (float)P.strata;
+            sy = ((float)(s / P.strata) + sy) / (float)P.strata;
+        }
+        X = ((float)px + sx) / (float)P.width;
+        Y = ((float)py + sy) / (float)P.height;
+        // stratified wavelength (stratifier.cpp:191-196 with nl strata per sample)
+        uw = ((float)l + lr::u01(rp.x)) / (float)P.nl;
     }
-    const float X = ((float)px + sx) / (float)P.width;
-    const float Y = ((float)py + sy) / (float)P.height;
     d = ld3(C.dir) + (X - 0.5f) * ld3(C.right) + (Y - 0.5f) * ld3(C.down);
     d = normalize(d);
     ray_factor = 1.0f / dot(d, ld3(C.dir_n));
-    // stratified wavelength (stratifier.cpp:191-196 with nl strata per sample)
-    const float uw = ((float)l + lr::u01(rp.x)) / (float)P.nl;
     float lpdf;
     const float lam = observer_sample(S.obs, uw, lpdf);
     float thr = lpdf > 0.0f ? 1.0f : 0.0f;
@@ -408,7 +438,8 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
         DevScene S = S0;
         Params P = P0;
         GeomArgs Gm = GA0;
-        uint64_t gp = path0 + pid;  // the path's index in its scene (random streams)
+        // the path's index in its scene (random streams; Halton: in the sequence)
+        uint64_t gp = P0.hoff + path0 + pid;
         if (BATCH) {
             const Segment sg = B.seg[find_segment(B, pid, false)];
             S = B.scenes[sg.scene];
@@ -463,9 +494,20 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                 const V3 wi = v3(dot(wiw, tt), dot(wiw, bb), fmaxf(dot(wiw, nf), 1.1920929e-7f));
                 const lr::u4 r0 = rng(P, gp, gen, 1u);
                 const lr::u4 r1 = rng(P, gp, gen, 2u);
-                const float u_strat = lr::u01(r0.x), u1 = lr::u01(r0.y), u2 = lr::u01(r0.z);
-                const float u_comp = lr::u01(r0.w), u_light = lr::u01(r1.x), ul1 = lr::u01(r1.y),
-                            ul2 = lr::u01(r1.z), u_rr = lr::u01(r1.w);
+                float u_strat = lr::u01(r0.x), u1 = lr::u01(r0.y), u2 = lr::u01(r0.z);
+                float u_comp = lr::u01(r0.w), u_light = lr::u01(r1.x), ul1 = lr::u01(r1.y), ul2 = lr::u01(r1.z);
+                const float u_rr = lr::u01(r1.w);
+                if (!BATCH && !GUIDED && P.halton && gen < P.hgens) {
+                    const uint64_t j = gp + 1;
+                    const uint32_t n1 = 4u * P.hgens, b2 = n1 + 6u * gen;
+                    u_strat = halton_sub(P, j, 4u * gen);
+                    u_comp = halton_sub(P, j, 4u * gen + 1u);
+                    u_light = halton_sub(P, j, 4u * gen + 2u);
+                    u1 = halton_sub(P, j, b2);
+                    u2 = halton_sub(P, j, b2 + 1u);
+                    ul1 = halton_sub(P, j, b2 + 2u);
+                    ul2 = halton_sub(P, j, b2 + 3u);
+                }
                 const float strategy = bsdf_is_diffuse_or_glossy(m.type) ? 0.5f : 1.0f;
                 V3 wo;
                 float value = 0.0f, pdf = 0.0f;
@@ -696,6 +738,30 @@ __global__ __launch_bounds__(256) void k_splat_tiles(DevScene S, Params P, uint6
     splat_tile(S, P, min(pb, npix - pixb), pixb * ppp, pix0 + pixb, ps, film_xyz, film_w);
 }
 
+// Halton chunk: each path's contribution goes to the pixel its screen point
+// falls in (Raster::doWriteRender, raster.cpp:547-606), with float atomics.
+__global__ __launch_bounds__(256) void k_splat_scatter(DevScene S, Params P, uint64_t path0, uint32_t n, PathState ps,
+                                                       float* film_xyz, float* film_w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t j = P.hoff + path0 + i + 1;
+    const float X = radical_inverse(j, 2u, nullptr), Y = radical_inverse(j, 3u, nullptr);
+    const uint32_t px = min((uint32_t)(X * (float)P.width), P.width - 1u);
+    const uint32_t py = min((uint32_t)(Y * (float)P.height), P.height - 1u);
+    const uint64_t pix = (uint64_t)py * P.width + px;
+    const uint64_t np = (uint64_t)P.width * P.height;
+    atomicAdd(&film_w[pix], 1.0f);
+    const float L = ps.L[i];
+    if (L == 0.0f) return;
+    float xyz[3];
+    observer_xyz(S.obs, ps.lam[i], xyz);
+    float sc = L / ps.lpdf[i];  // Spectral::xyz, spectral.cpp:212-221
+    if (P.max_lum > 0.0f && sc * xyz[1] > P.max_lum) sc *= P.max_lum / (sc * xyz[1]);  // raster.cpp:567-570
+    atomicAdd(&film_xyz[pix], sc * xyz[0]);
+    atomicAdd(&film_xyz[np + pix], sc * xyz[1]);
+    atomicAdd(&film_xyz[2 * np + pix], sc * xyz[2]);
+}
+
 // A batch chunk: block b takes the pixels of tile b, all of one segment.
 struct SplatTile {
     uint32_t seg;   // chunk-local segment
@@ -789,6 +855,8 @@ struct lr_scene {
     float* d_table = nullptr;
     DevLight* d_lights = nullptr;
     float4* d_light_tris = nullptr;  // mesh lights (DevScene::light_tris)
+    uint16_t* d_halton_perm = nullptr;  // Halton sampler tables (ensure_halton)
+    uint32_t* d_halton_off = nullptr;
     // observer restricted to the last render's sampling domain
     float obs_lo = -1.0f, obs_hi = -1.0f;
     uint32_t obs_n = 0;
@@ -828,7 +896,7 @@ void free_scene(lr_scene* s) {
     if (s->cstream) (void)hipStreamDestroy(s->cstream);
     void* ptrs[] = {s->d_prim_mat, s->d_prim_geom, s->d_prim_tan, s->d_corner, s->d_mats,   s->d_spectra,
                     s->d_table,    s->d_lights,    s->d_obs,      s->d_work,   s->d_counts, s->d_gwork,
-                    s->d_dlwork,   s->d_dlocc,     s->d_light_tris};
+                    s->d_dlwork,   s->d_dlocc,     s->d_light_tris, s->d_halton_perm, s->d_halton_off};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -1312,6 +1380,56 @@ static int make_camera(const lr_camera* cam, DevCamera& C) {
 }
 
 // Validated kernel parameters of an lr_render_params.
+// The Halton sampler's sub-sequence primes (17, 19, ..., 1889: the 284
+// primes after 13) and their digit permutations as LiAR's
+// Halton::initialize draws them (halton.cpp:102-118): one default-seeded
+// std::mt19937 shuffling 0 .. p-1 for each prime in order.
+constexpr uint32_t kHaltonPrimes = 284;
+
+struct HaltonTables {
+    std::vector<uint16_t> perm;  // concatenated permutations
+    std::vector<uint32_t> off;   // kHaltonPrimes + 1 offsets (off[k+1] - off[k] = prime k)
+};
+
+const HaltonTables& halton_tables() {
+    static HaltonTables T;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::vector<uint32_t> primes;
+        for (uint32_t n = 17; primes.size() < kHaltonPrimes; ++n) {
+            bool prime = true;
+            for (uint32_t d = 2; d * d <= n; ++d)
+                if (n % d == 0) {
+                    prime = false;
+                    break;
+                }
+            if (prime) primes.push_back(n);
+        }
+        std::mt19937 rng;
+        T.off.push_back(0);
+        std::vector<size_t> v;
+        for (uint32_t p : primes) {
+            v.resize(p);
+            for (uint32_t k = 0; k < p; ++k) v[k] = k;
+            std::shuffle(v.begin(), v.end(), rng);
+            for (size_t x : v) T.perm.push_back((uint16_t)x);
+            T.off.push_back((uint32_t)T.perm.size());
+        }
+    });
+    return T;
+}
+
+// Device copies of the Halton tables for the scene's device (made once).
+int ensure_halton(lr_scene* sc) {
+    if (sc->d_halton_perm) return LR_OK;
+    const HaltonTables& T = halton_tables();
+    LR_HIP_CHECK(hipMalloc(&sc->d_halton_perm, sizeof(uint16_t) * T.perm.size()));
+    LR_HIP_CHECK(hipMemcpy(sc->d_halton_perm, T.perm.data(), sizeof(uint16_t) * T.perm.size(), hipMemcpyHostToDevice));
+    LR_HIP_CHECK(hipMalloc(&sc->d_halton_off, sizeof(uint32_t) * T.off.size()));
+    LR_HIP_CHECK(hipMemcpy(sc->d_halton_off, T.off.data(), sizeof(uint32_t) * T.off.size(), hipMemcpyHostToDevice));
+    return LR_OK;
+}
+
 static int make_params(const lr_render_params& R, Params& P) {
     if (!R.width || !R.height || !R.spp || !R.wavelengths)
         return lr::fail(LR_E_INVALID, "width, height, spp and wavelengths must be > 0");
@@ -1330,6 +1448,13 @@ static int make_params(const lr_render_params& R, Params& P) {
     P.seed_hi = (uint32_t)(R.seed >> 32);
     const uint32_t q = (uint32_t)std::lround(std::sqrt((double)R.spp));
     P.strata = q * q == R.spp ? q : 0u;
+    if (R.sampler > LR_SAMPLER_HALTON) return lr::fail(LR_E_INVALID, "unknown sampler");
+    P.halton = R.sampler == LR_SAMPLER_HALTON ? 1u : 0u;
+    P.hgens = P.halton ? R.sampler_generations : 0u;
+    if (P.hgens * 10u > kHaltonPrimes) return lr::fail(LR_E_INVALID, "Halton: at most 28 sampler generations");
+    P.hoff = P.halton ? R.sample_offset : 0u;
+    P.perm = nullptr;
+    P.perm_off = nullptr;
     return LR_OK;
 }
 
@@ -1388,6 +1513,12 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     if (int rc = make_camera(cam, C)) return rc;
     Params P;
     if (int rc = make_params(R, P)) return rc;
+    if (P.halton) {
+        if (guide) return lr::fail(LR_E_UNSUPPORTED, "the Halton sampler is for plain path tracing (lr_render)");
+        if (int rc = ensure_halton(sc)) return rc;
+        P.perm = sc->d_halton_perm;
+        P.perm_off = sc->d_halton_off;
+    }
     const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
     const uint64_t n_pix = (uint64_t)R.width * R.height;
     const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 22);
@@ -1533,8 +1664,11 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
                 if (int rc = lr_guide_train_step(guide, b_in, b_dir, b_w, batch, nullptr, nullptr, s)) return rc;
             }
         }
-        hipLaunchKernelGGL(k_splat_tiles, dim3(splat_blocks(npix, (uint32_t)ppp)), blk, 0, s, S, P, pix0, npix, ps,
-                           film_xyz, film_w);
+        if (P.halton)
+            hipLaunchKernelGGL(k_splat_scatter, grd, blk, 0, s, S, P, path0, n, ps, film_xyz, film_w);
+        else
+            hipLaunchKernelGGL(k_splat_tiles, dim3(splat_blocks(npix, (uint32_t)ppp)), blk, 0, s, S, P, pix0, npix,
+                               ps, film_xyz, film_w);
         LR_HIP_CHECK(hipGetLastError());
         ++chunks;
     }
@@ -1735,6 +1869,7 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
         if (!it.film_xyz || !it.film_weight) return lr::fail(LR_E_INVALID, "item " + std::to_string(k) + ": NULL film");
         if (int rc = make_camera(&it.camera, hcam[k])) return rc;
         if (int rc = make_params(it.params, hprm[k])) return rc;
+        if (hprm[k].halton) return lr::fail(LR_E_UNSUPPORTED, "batches render with the Stratifier only");
         lr_scene* sc = b->scenes[k];
         if (int rc = build_observer(sc, it.params.lambda_min, it.params.lambda_max)) return rc;
         hs[k] = dev_scene(sc);
@@ -2343,6 +2478,7 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
     if (int rc = make_camera(cam, C)) return rc;
     Params P;
     if (int rc = make_params(R, P)) return rc;
+    if (P.halton) return lr::fail(LR_E_UNSUPPORTED, "DirectLighting renders with the Stratifier only");
     P.max_gen = 1;              // the camera kernel makes generation 0 only
     P.rr_depth = 0xFFFFFFFFu;   // no roulette
     DLArgs A;
@@ -2480,6 +2616,21 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
         stats->chunks = chunks;
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+    return LR_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int lr_halton_permutation(uint32_t index, uint32_t* prime, uint32_t* out, uint32_t capacity) {
+    if (!prime) return lr::fail(LR_E_INVALID, "NULL prime");
+    if (index >= kHaltonPrimes) return lr::fail(LR_E_INVALID, "Halton prime index must be < 284");
+    const HaltonTables& T = halton_tables();
+    const uint32_t p = T.off[index + 1] - T.off[index];
+    *prime = p;
+    if (out && capacity >= p)
+        for (uint32_t k = 0; k < p; ++k) out[k] = T.perm[T.off[index] + k];
     return LR_OK;
 }
 

@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native as N
-from . import cameras, scenery, shaders, spectra, textures, tracers
+from . import cameras, samplers, scenery, shaders, spectra, textures, tracers
 from .observer import ensure_standard_observer
 
 
@@ -361,10 +361,12 @@ def guided_render(gs: "GpuScene", guide, camera: N.Camera, params: N.RenderParam
 
 def render_params(width, height, spp, wavelengths=1, max_depth=0, rr_depth=8, rr_survival=0.5,
                   lambda_range=(380.0, 780.0), seed=1, chunk_paths=0, max_sample_luminance=0.0,
-                  graph=False) -> N.RenderParams:
+                  graph=False, sampler=N.LR_SAMPLER_STRATIFIER, sampler_generations=0,
+                  sample_offset=0) -> N.RenderParams:
     return N.RenderParams(int(width), int(height), int(spp), int(wavelengths), int(max_depth), int(rr_depth),
                           float(rr_survival), float(lambda_range[0]), float(lambda_range[1]), int(seed),
-                          int(chunk_paths), float(max_sample_luminance), int(bool(graph)))
+                          int(chunk_paths), float(max_sample_luminance), int(bool(graph)), int(sampler),
+                          int(sampler_generations), int(sample_offset))
 
 
 class RenderEngine:
@@ -401,9 +403,17 @@ class RenderEngine:
         rr_start = tr.maxRayGeneration if rr_start is None else rr_start
         lr = self.wavelengthRange or (float(standard_observer()[0][0]), float(standard_observer()[0][-1]))
         msl = float(getattr(self.target, "maxSampleLuminance", math.inf))
+        halton = isinstance(self.sampler, samplers.Halton)
+        if halton:
+            if isinstance(tr, tracers.DirectLighting):
+                raise NotImplementedError("DirectLighting renders with the Stratifier only")
+            self.sampler._check_generations(tr.maxRayGeneration)
         params = render_params(w, h, self.sampler.samplesPerPixel, self.wavelengthsPerSample,
                                getattr(tr, "maxDepth", None) or 0, rr_start, getattr(tr, "survivalProbability", 0.5),
-                               lr, self.seed, max_sample_luminance=msl if math.isfinite(msl) else 0.0)
+                               lr, self.seed, max_sample_luminance=msl if math.isfinite(msl) else 0.0,
+                               sampler=N.LR_SAMPLER_HALTON if halton else N.LR_SAMPLER_STRATIFIER,
+                               sampler_generations=tr.maxRayGeneration if halton else 0,
+                               sample_offset=self.sampler.nextSample if halton else 0)
         ctx = Context(self.device)
         gs = GpuScene(ctx, compile_scene(self.scene))
         dev = torch.device("cuda", self.device)
@@ -415,6 +425,8 @@ class RenderEngine:
         else:
             self.lastStats = gs.render(camera_struct(self.camera), params, film, wgt)
         torch.cuda.synchronize(dev)
+        if halton:
+            self.sampler.nextSample += w * h * params.spp * params.wavelengths
         self.target.accumulate(film.cpu().numpy().astype(np.float64), wgt.cpu().numpy().astype(np.float64))
         if hasattr(self.target, "save"):
             self.target.save()

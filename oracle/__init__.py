@@ -284,7 +284,8 @@ class _Params(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("wavelengths", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("rr_survival", C.c_float), ("lambda_min", C.c_float), ("lambda_max", C.c_float),
-                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32), ("max_sample_luminance", C.c_float)]
+                ("seed", C.c_uint64), ("chunk_paths", C.c_uint32), ("max_sample_luminance", C.c_float),
+                ("sampler", C.c_uint32), ("sampler_generations", C.c_uint32), ("sample_offset", C.c_uint64)]
 
 
 def _spec_struct(sp, keep):
@@ -307,11 +308,17 @@ def _spec_struct(sp, keep):
 
 
 def render(compiled, camera, width, height, spp, wavelengths=1, max_depth=0, rr_depth=8, rr_survival=0.5,
-           lambda_range=(380.0, 780.0), seed=1, observer=None, threads=None, max_sample_luminance=0.0):
+           lambda_range=(380.0, 780.0), seed=1, observer=None, threads=None, max_sample_luminance=0.0,
+           halton=False, sampler_generations=0, sample_offset=0):
     """FP64 CPU render (AdjointPhotonTracer) of a compiled scene -> (xyz sums
-    (3, h, w), weight (h, w))."""
+    (3, h, w), weight (h, w)).  ``halton``: the Halton sampler over the whole
+    film (width * height * spp * wavelengths samples starting at sequence
+    index sample_offset + 1), generations below sampler_generations drawing
+    from its scrambled sub-sequences; the weight then counts the samples that
+    fell in each pixel."""
     return _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_depth, rr_survival,
-                   lambda_range, seed, observer, threads, max_sample_luminance, None)
+                   lambda_range, seed, observer, threads, max_sample_luminance, None,
+                   (1 if halton else 0, sampler_generations, sample_offset))
 
 
 def render_direct(compiled, camera, width, height, spp, wavelengths=1, max_generation=8,
@@ -325,7 +332,7 @@ def render_direct(compiled, camera, width, height, spp, wavelengths=1, max_gener
 
 
 def _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_depth, rr_survival, lambda_range, seed,
-            observer, threads, max_sample_luminance, dl_cfg):
+            observer, threads, max_sample_luminance, dl_cfg, sampler=(0, 0, 0)):
     L = lib()
     if not hasattr(L, "_render_sig"):
         head = [C.c_void_p] * 2 + [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -366,7 +373,7 @@ def _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_dep
     cam.sky[:] = [float(x) for x in camera.sky]
     cam.fov, cam.aspect, cam.near_limit = float(camera.fovAngle), float(camera.aspectRatio), float(camera.nearLimit)
     P = _Params(width, height, spp, wavelengths, max_depth, rr_depth, rr_survival, lambda_range[0],
-                lambda_range[1], seed, 0, max_sample_luminance)
+                lambda_range[1], seed, 0, max_sample_luminance, *sampler)
     if observer is None:
         from liar_amd.observer import standard_observer
         observer = standard_observer()
@@ -385,6 +392,29 @@ def _render(compiled, camera, width, height, spp, wavelengths, max_depth, rr_dep
         rc = L.or_render_direct(*args, dl_cfg.ctypes.data, *tail)
     assert rc == 0, "oracle render failed"
     return xyz, wgt
+
+
+def halton_tables():
+    """The Halton sampler's sub-sequence digit permutations: (perm uint16,
+    off uint32 (285,)); prime k's permutation is perm[off[k]:off[k+1]], of
+    length prime k (17, 19, 23, ...)."""
+    L = lib()
+    L.or_halton_tables.argtypes = [C.c_void_p, C.c_void_p]
+    L.or_halton_tables.restype = C.c_uint32
+    off = np.zeros(285, np.uint32)
+    n = L.or_halton_tables(None, off.ctypes.data)
+    perm = np.zeros(n, np.uint16)
+    L.or_halton_tables(perm.ctypes.data, off.ctypes.data)
+    return perm, off
+
+
+def halton_sample(j: int, base: int = 2, prime_index: int = -1) -> float:
+    """The oracle's FP32 radical inverse of sequence index j: plain in
+    ``base``, or (prime_index >= 0) scrambled on that sub-sequence prime."""
+    L = lib()
+    L.or_halton_sample.argtypes = [C.c_uint64, C.c_uint32, C.c_int]
+    L.or_halton_sample.restype = C.c_float
+    return float(L.or_halton_sample(int(j), int(base), int(prime_index)))
 
 
 def set_planted_mistake(k: int) -> None:

@@ -39,7 +39,8 @@ static const double TOL = 1e-6;
  * outside tests.  1: drop both one-sample MIS weights (furnace tests);
  * 2: drop the DirectLighting light-estimate weights; 3: CookTorrance pdf
  * without the half-vector Jacobian 1 / (4 wo.h) (chi-square test);
- * 5: microfacet D without its 1/pi (normalisation test). */
+ * 5: microfacet D without its 1/pi (normalisation test); 6: Halton BSDF
+ * samples with one prime for both coordinates (convergence test). */
 static int g_plant = 0;
 void or_set_planted_mistake(int k) { g_plant = k; }
 
@@ -75,7 +76,43 @@ typedef struct {
     uint64_t seed;
     uint32_t chunk_paths;
     float max_sample_luminance;
+    uint32_t sampler, sampler_generations; /* 1: Halton (lr_render_params) */
+    uint64_t sample_offset;
 } or_params;
+
+/* ---- Halton sampler (src/liar/samplers/halton.cpp:180-230, halton.h:41-66) -- */
+uint32_t or_halton_tables(uint16_t* perm, uint32_t* off);  /* halton_tables.cpp */
+static uint16_t* g_hperm;
+static uint32_t g_hoff[285];
+static pthread_once_t g_honce = PTHREAD_ONCE_INIT;
+static void halton_init(void) {
+    uint32_t n = or_halton_tables(NULL, g_hoff);
+    g_hperm = (uint16_t*)malloc(sizeof(uint16_t) * n);
+    or_halton_tables(g_hperm, g_hoff);
+}
+/* ScrambledRadicalInverse: digit d of i (least significant first) adds
+ * perm[d] b^-(k+1), in FP32 in that order (perm NULL: plain). */
+static float radical_inverse(uint64_t i, uint32_t b, const uint16_t* perm) {
+    const float inv = 1.0f / (float)b;
+    float w = inv, r = 0.0f;
+    while (i) {
+        uint32_t d = (uint32_t)(i % b);
+        i /= b;
+        r = r + (float)(perm ? perm[d] : d) * w;
+        w = w * inv;
+    }
+    return r;
+}
+static float halton_sub(uint64_t j, uint32_t prime_index) {
+    return radical_inverse(j, g_hoff[prime_index + 1] - g_hoff[prime_index], g_hperm + g_hoff[prime_index]);
+}
+/* probe: sequence index j, plain in base `base` (prime_index < 0) or on the
+ * scrambled sub-sequence of prime index prime_index */
+float or_halton_sample(uint64_t j, uint32_t base, int prime_index) {
+    pthread_once(&g_honce, halton_init);
+    if (prime_index < 0) return radical_inverse(j, base, NULL);
+    return prime_index < 284 ? halton_sub(j, (uint32_t)prime_index) : -1.0f;
+}
 
 typedef struct { double x, y, z; } V;
 static V v(double x, double y, double z) { V r = {x, y, z}; return r; }
@@ -723,6 +760,20 @@ static double trace_path(const Ctx* c, uint64_t p, V o, V d, double tn, double l
         rng(c, p, gen, 2u, r1);
         double us = u01(r0[0]), u1 = u01(r0[1]), u2 = u01(r0[2]), uc = u01(r0[3]);
         double ul = u01(r1[0]), ul1 = u01(r1[1]), ul2 = u01(r1[2]), urr = u01(r1[3]);
+        if (P->sampler == 1 && gen < P->sampler_generations) {
+            /* the AdjointPhotonTracer's sub-sequences of this generation
+             * (adjoint_photon_tracer.cpp:47-67): 1D ids 4g .. 4g+3 on the
+             * first primes, 2D ids 3g .. 3g+2 after all 1D ones */
+            uint64_t j = p + 1;  /* p: the path's position in the sequence */
+            uint32_t b2 = 4 * P->sampler_generations + 6 * gen;
+            us = halton_sub(j, 4 * gen);
+            uc = halton_sub(j, 4 * gen + 1);
+            ul = halton_sub(j, 4 * gen + 2);
+            u1 = halton_sub(j, b2);
+            u2 = halton_sub(j, g_plant == 6 ? b2 : b2 + 1);
+            ul1 = halton_sub(j, b2 + 2);
+            ul2 = halton_sub(j, b2 + 3);
+        }
         double strategy = (m->type == 1 || m->type == 5 || m->type == 6) ? 0.5 : 1.0;
         V wo = v(0, 0, 1);
         double val = 0, pdf = 0;
@@ -957,7 +1008,53 @@ typedef struct {
     double* xyz;     /* 3 planes, W*H each, sums */
     double* weight;
     uint32_t row0, row1;
+    uint64_t s0, s1; /* Halton: this thread's samples (into its own xyz / weight) */
 } Job;
+
+/* Halton samples [s0, s1) of the call: screen point in bases 2 and 3 of the
+ * whole film, wavelength in base 13 (halton.cpp:191-212), accumulated into
+ * the pixel the point falls in (raster.cpp:547-606). */
+static void* run_halton(void* arg) {
+    Job* J = (Job*)arg;
+    const Ctx* c = J->c;
+    const or_params* P = c->P;
+    uint32_t max_gen = P->max_depth ? P->max_depth : 64u;
+    uint64_t np = (uint64_t)P->width * P->height;
+    for (uint64_t p = P->sample_offset + J->s0; p < P->sample_offset + J->s1; ++p) {
+        uint64_t j = p + 1;  /* p keys the path's random streams too */
+        float Xs = radical_inverse(j, 2, NULL), Ys = radical_inverse(j, 3, NULL);
+        float uw = radical_inverse(j, 13, NULL);
+        uint32_t px = (uint32_t)(Xs * (float)P->width), py = (uint32_t)(Ys * (float)P->height);
+        if (px >= P->width) px = P->width - 1;
+        if (py >= P->height) py = P->height - 1;
+        uint64_t pixel = (uint64_t)py * P->width + px;
+        J->weight[pixel] += 1;
+        V d = nrm(add(J->dir, add(mul((double)Xs - 0.5, J->right), mul((double)Ys - 0.5, J->down))));
+        double rf = 1.0 / dot(d, J->dirn);
+        double tn = fmax(TOL, rf * J->near_limit);
+        uint32_t rp[4];
+        rng(c, p, 0xFFFEu, 0u, rp);
+        double lpdf;
+        double lam = obs_sample(c->S, uw, &lpdf);
+        if (!(lpdf > 0) || max_gen == 0) continue;
+        double thr = 1;
+        if (P->rr_depth == 0) {
+            if (u01(rp[1]) < P->rr_survival) thr /= P->rr_survival;
+            else continue;
+        }
+        double L = trace_path(c, p, J->pos, d, tn, lam, max_gen, thr);
+        if (L == 0) continue;
+        double cmf[3];
+        obs_xyz(c->S, lam, cmf);
+        double sc = L / lpdf;
+        if (P->max_sample_luminance > 0 && sc * cmf[1] > P->max_sample_luminance)
+            sc *= P->max_sample_luminance / (sc * cmf[1]);
+        J->xyz[pixel] += sc * cmf[0];
+        J->xyz[np + pixel] += sc * cmf[1];
+        J->xyz[2 * np + pixel] += sc * cmf[2];
+    }
+    return NULL;
+}
 
 static void* run_rows(void* arg) {
     Job* J = (Job*)arg;
@@ -1147,8 +1244,34 @@ static int render_impl(const float* verts, const uint32_t* tris, uint32_t nt, co
     if (nthreads > 256) nthreads = 256;
     pthread_t th[256];
     Job jobs[256];
+    if (P->sampler == 1) {
+        if (dl) return -1;  /* the Halton sampler serves the path tracer */
+        pthread_once(&g_honce, halton_init);
+        if (P->sampler_generations * 10 > 284) return -1;
+        uint64_t np = (uint64_t)P->width * P->height, total = np * P->spp * P->wavelengths;
+        uint64_t per = (total + (uint64_t)nthreads - 1) / (uint64_t)nthreads;
+        int used = 0;
+        for (int k = 0; k < nthreads; ++k) {
+            jobs[k] = proto;
+            jobs[k].s0 = (uint64_t)k * per < total ? (uint64_t)k * per : total;
+            jobs[k].s1 = jobs[k].s0 + per < total ? jobs[k].s0 + per : total;
+            if (jobs[k].s0 >= jobs[k].s1) break;
+            jobs[k].xyz = calloc(3 * np, sizeof(double));
+            jobs[k].weight = calloc(np, sizeof(double));
+            pthread_create(&th[k], NULL, run_halton, &jobs[k]);
+            ++used;
+        }
+        for (int k = 0; k < used; ++k) {
+            pthread_join(th[k], NULL);
+            for (uint64_t i = 0; i < 3 * np; ++i) xyz[i] += jobs[k].xyz[i];
+            for (uint64_t i = 0; i < np; ++i) weight[i] += jobs[k].weight[i];
+            free(jobs[k].xyz);
+            free(jobs[k].weight);
+        }
+        nthreads = 0;  /* done */
+    }
     /* interleave rows across threads in small blocks for balance */
-    uint32_t H = P->height;
+    uint32_t H = nthreads ? P->height : 0;
     uint32_t per = (H + (uint32_t)nthreads - 1) / (uint32_t)nthreads;
     int used = 0;
     for (int k = 0; k < nthreads; ++k) {
