@@ -200,21 +200,40 @@ def _ref_y():
     return d["y"].astype(np.float64)
 
 
+def _mrae(a, b, block=1):
+    """mean |a - b| / mean(b) over block x block pixel means."""
+    n = a.shape[0] // block
+    a = a.reshape(n, block, -1, block).mean((1, 3))
+    b = b.reshape(n, block, -1, block).mean((1, 3))
+    return float(np.abs(a - b).mean() / b.mean())
+
+
 def test_cornell_vs_4096spp_cpu_reference(gpu_ctx):
     """Cornell workload (256x256, 4 wavelengths, depth <= 8, RR from 3) against
     the committed 4096-spp FP64 oracle render (seed 1000).  Measures:
     relative mean absolute error  MRAE = mean|Y - Yref| / mean(Yref)
     bias                          = (mean Y - mean Yref) / mean Yref.
-    Bars: at 1024 spp MRAE < 0.04 and |bias| < 2e-3; at the workload's 16 spp
-    MRAE < 0.35 (noise) and |bias| < 1e-2."""
+    At the workload's 16 spp: MRAE < 0.35 (noise) and |bias| < 1e-2.
+    At 1024 spp: |bias| < 2e-3, and MRAE (per pixel and over 16x16-pixel
+    blocks, where noise falls ~16x and a structured error stays) below the
+    MRAE between two 1024-spp GPU renders of other seeds.  For noise alone
+    the first is ~0.8 of the second (a 1024-spp render's noise plus the
+    4096-spp reference's, against two 1024-spp renders' noise); a per-pixel
+    error of the size of one render's noise pushes it above."""
     ref = _ref_y()
     sc, cam = W.cornell_box()
     cs = la.compile_scene(sc)
-    for spp, mrae_bar, bias_bar in ((16, 0.35, 1e-2), (1024, 0.04, 2e-3)):
-        gx, gw, st = gpu_render(gpu_ctx, cs, cam, 256, 256, spp, nl=4, max_depth=8, rr_depth=3, seed=7)
-        y = (gx / gw)[1]
-        mrae = np.abs(y - ref).mean() / ref.mean()
+    ys = {}
+    for spp, seed in ((16, 7), (1024, 7), (1024, 8)):
+        gx, gw, st = gpu_render(gpu_ctx, cs, cam, 256, 256, spp, nl=4, max_depth=8, rr_depth=3, seed=seed)
+        ys[spp, seed] = y = (gx / gw)[1]
         bias = (y.mean() - ref.mean()) / ref.mean()
-        say(f"[gpu] cornell {spp} spp vs 4096-spp reference: MRAE {mrae:.4f}, bias {bias:+.2e}, "
-            f"{st['seconds'] * 1e3:.1f} ms")
-        assert mrae < mrae_bar and abs(bias) < bias_bar
+        say(f"[gpu] cornell {spp} spp seed {seed} vs 4096-spp reference: MRAE {_mrae(y, ref):.4f}, "
+            f"16x16 blocks {_mrae(y, ref, 16):.4f}, bias {bias:+.2e}, {st['seconds'] * 1e3:.1f} ms")
+        assert abs(bias) < (1e-2 if spp == 16 else 2e-3)
+    assert _mrae(ys[16, 7], ref) < 0.35
+    a, b = ys[1024, 7], ys[1024, 8]
+    for block in (1, 16):
+        noise = _mrae(a, b, block)
+        say(f"[gpu] block {block}: vs reference {_mrae(a, ref, block):.4f}, between GPU seeds {noise:.4f}")
+        assert _mrae(a, ref, block) < noise
