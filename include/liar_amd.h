@@ -706,6 +706,10 @@ typedef struct lr_guide_params {
                            the image dominate the fit rather than the brightest */
     uint32_t record_depth; /* record only the vertices of the first record_depth
                            bounces (0: all) */
+    uint32_t guide_depth;  /* guided MIS at the vertices of the first
+                           guide_depth bounces only; later ones sample as the
+                           plain tracer (LiAR's 0.5 BSDF / 0.5 light) and run
+                           no inference (0: every bounce) */
 } lr_guide_params;
 int lr_render_guided(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
                      lr_guide* guide, const lr_guide_params* guide_params, float* film_xyz,

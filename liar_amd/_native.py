@@ -159,7 +159,7 @@ class GuideParams(C.Structure):
     _fields_ = [("selection", C.c_float), ("train", C.c_uint32), ("train_steps", C.c_uint32),
                 ("train_batch", C.c_uint32), ("max_records", C.c_uint32), ("weight_clamp", C.c_float),
                 ("light_share", C.c_float), ("learned_selection", C.c_uint32), ("balance_heuristic", C.c_uint32),
-                ("image_weighting", C.c_uint32), ("record_depth", C.c_uint32)]
+                ("image_weighting", C.c_uint32), ("record_depth", C.c_uint32), ("guide_depth", C.c_uint32)]
 
 
 LR_NONE = 0xFFFFFFFF

@@ -529,6 +529,12 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         cg = (1.0f - cl) * G.sel;
                         cb = (1.0f - cl) * (1.0f - G.sel);
                     }
+                    if (S.n_lights == 0) {  // no light to sample: its share goes to the others
+                        const float rest = cg + cb;
+                        cg /= rest;
+                        cb /= rest;
+                        cl = 0.0f;
+                    }
                     V3 dw = v3(0, 0, 1);
                     float f = 0.0f, pb = 0.0f, pl = 0.0f, pg = 0.0f, pc = 0.0f;
                     if (u_strat < cb) {
@@ -1534,7 +1540,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     float* g_mix = nullptr;
     float* r_w = nullptr;
     float *b_in = nullptr, *b_dir = nullptr, *b_w = nullptr;
-    uint32_t rec_cap = 0, batch = 0;
+    uint32_t rec_cap = 0, batch = 0, guide_depth = 0;
     if (guided) {
         if (!gparams) return lr::fail(LR_E_INVALID, "guided render needs lr_guide_params");
         if (!(gparams->selection >= 0.0f && gparams->selection < 1.0f))
@@ -1554,6 +1560,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         GA.balance = gparams->balance_heuristic ? 1u : 0u;
         GA.image_weight = gparams->image_weighting ? 1u : 0u;
         GA.record_depth = gparams->record_depth ? gparams->record_depth : 0xFFFFFFFFu;
+        guide_depth = gparams->guide_depth ? gparams->guide_depth : 0xFFFFFFFFu;
         const lr_qbvh_info& I = sc->bvh->info;
         for (int k = 0; k < 3; ++k) {
             GA.lo[k] = I.scene_lo[k];
@@ -1636,7 +1643,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             } else if (int rc = lr::trace_closest_queue(sc->bvh, &rays, n, counts + cur, &hits, s)) {
                 return rc;
             }
-            if (guided) {
+            if (guided && gen < guide_depth) {
                 hipLaunchKernelGGL(k_guide_inputs, grd, blk, 0, s, S, q[cur], counts + cur, H, ps, GA, g_in);
                 if (int rc = lr_guide_infer(guide, g_in, n, counts + cur, g_mix, s)) return rc;
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), dim3((n + kAppendBlockWide - 1) / kAppendBlockWide),

@@ -275,25 +275,43 @@ def random_cube(n_tris: int, seed: int = 2, scale: float = 0.01, sun_scale: floa
     return cs, cam
 
 
-def batched_scenes(n_scenes: int = 64, seed: int = 11, min_tris: int = 10_000, max_tris: int = 2_000_000,
-                   sizes=(128, 256, 512, 1024), spps=(4, 16, 64), depth=(4, 16)):
-    """The batched uneven workload: `n_scenes` independent scenes with triangle
-    counts log-uniform in [min_tris, max_tris] (random-cube or terrain
-    generator, alternately), square images with sides drawn from `sizes`, spp
-    from `spps` and max depth uniform in `depth`.  Returns a list of dicts
-    (scene, camera, width, height, spp, max_depth, seed, n_tris)."""
+def batched_scene_specs(n_scenes: int = 64, seed: int = 11, min_tris: int = 10_000, max_tris: int = 2_000_000,
+                        sizes=(128, 256, 512, 1024), spps=(4, 16, 64), depth=(4, 16)):
+    """The draws of the batched uneven workload, without building a scene:
+    per scene (index, generator, triangle count asked for, image side, spp,
+    max depth, render seed).  Scene k uses the random-cube generator for even
+    k and the terrain generator for odd k, with generator seed seed * 1000 + k."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n_scenes):
         nt = int(round(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris)))))
-        if k % 2 == 0:
-            cs, cam = random_cube(nt, seed=seed * 1000 + k)
+        side = int(rng.choice(sizes))
+        out.append({"index": k, "generator": "random_cube" if k % 2 == 0 else "terrain", "tris_asked": nt,
+                    "gen_seed": seed * 1000 + k, "width": side, "height": side, "spp": int(rng.choice(spps)),
+                    "max_depth": int(rng.integers(depth[0], depth[1] + 1)), "seed": 100 + k})
+    return out
+
+
+def batched_scenes(n_scenes: int = 64, seed: int = 11, min_tris: int = 10_000, max_tris: int = 2_000_000,
+                   sizes=(128, 256, 512, 1024), spps=(4, 16, 64), depth=(4, 16), only=None):
+    """The batched uneven workload: `n_scenes` independent scenes with triangle
+    counts log-uniform in [min_tris, max_tris] (random-cube or terrain
+    generator, alternately), square images with sides drawn from `sizes`, spp
+    from `spps` and max depth uniform in `depth`.  Returns a list of dicts
+    (scene, camera, width, height, spp, max_depth, seed, n_tris, index);
+    `only`: the scene indices to build (the others are skipped; the draws of
+    every scene stay those of the full workload)."""
+    out = []
+    for sp in batched_scene_specs(n_scenes, seed, min_tris, max_tris, sizes, spps, depth):
+        k, nt = sp["index"], sp["tris_asked"]
+        if only is not None and k not in only:
+            continue
+        if sp["generator"] == "random_cube":
+            cs, cam = random_cube(nt, seed=sp["gen_seed"])
         else:
             n_side = max(2, int(round(np.sqrt(nt / 2.0))) + 1)
-            cs, cam = terrain(n_side, n_spheres=min(10000, nt // 200), seed=seed * 1000 + k)
-        side = int(rng.choice(sizes))
+            cs, cam = terrain(n_side, n_spheres=min(10000, nt // 200), seed=sp["gen_seed"])
         cam.aspectRatio = 1.0
-        out.append({"scene": cs, "camera": cam, "width": side, "height": side, "spp": int(rng.choice(spps)),
-                    "max_depth": int(rng.integers(depth[0], depth[1] + 1)), "seed": 100 + k,
-                    "n_tris": int(len(cs.tris))})
+        out.append({"scene": cs, "camera": cam, "width": sp["width"], "height": sp["height"], "spp": sp["spp"],
+                    "max_depth": sp["max_depth"], "seed": sp["seed"], "n_tris": int(len(cs.tris)), "index": k})
     return out

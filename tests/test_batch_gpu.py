@@ -71,3 +71,89 @@ def test_batch_films_equal_solo_renders(gpu_ctx, graph):
     batch.close()
     for gs in gss:
         gs.close()
+
+
+def _c4_subset():
+    """Scene indices of the 64-scene batched workload: its largest scene (by
+    triangles), the one with the most paths, and its two smallest."""
+    sp = W.batched_scene_specs()
+    by_tris = sorted(sp, key=lambda s: s["tris_asked"])
+    most_paths = max(sp, key=lambda s: s["width"] * s["height"] * s["spp"])
+    return [by_tris[-1]["index"], most_paths["index"], by_tris[0]["index"], by_tris[1]["index"]]
+
+
+def _c4_params(s, width=None, spp=None):
+    w = width or s["width"]
+    return E.render_params(w, w, spp or s["spp"], 1, s["max_depth"], 3, 0.5, (380.0, 780.0), s["seed"])
+
+
+def test_batched_workload_subset_graph_equals_solo(gpu_ctx):
+    """The batched uneven workload's largest, longest and two smallest scenes
+    at their full specifications through one graph-captured lr_render_batch:
+    every film bit-identical to the scene's own lr_render (the per-render
+    contract of RenderEngine::render, render_engine.cpp:170-247)."""
+    import torch
+
+    idx = _c4_subset()
+    print(f"[gpu] batched workload subset {idx}: building", flush=True)
+    specs = W.batched_scenes(only=set(idx))
+    gss = [E.GpuScene(gpu_ctx, s["scene"]) for s in specs]
+    batch = E.Batch(gpu_ctx, gss)
+    items, solo = [], []
+    for s, gs in zip(specs, gss):
+        prm = _c4_params(s)
+        fx, fw = _film(s["width"], s["height"])
+        items.append((E.camera_struct(s["camera"]), prm, fx, fw))
+        sx, sw = _film(s["width"], s["height"])
+        gs.render(E.camera_struct(s["camera"]), prm, sx, sw)
+        solo.append((sx, sw))
+    print(f"[gpu] batched subset: {[(s['index'], s['n_tris'], s['width'], s['spp']) for s in specs]}", flush=True)
+    for rep in range(2):  # the second call replays the updated graph
+        for _, _, fx, fw in items:
+            fx.zero_()
+            fw.zero_()
+        res = batch.render(items, graph=True)
+        torch.cuda.synchronize()
+        for s, (_, _, fx, fw), (sx, sw) in zip(specs, items, solo):
+            assert np.array_equal(fx.cpu().numpy(), sx.cpu().numpy()), f"scene {s['index']} xyz, call {rep}"
+            assert np.array_equal(fw.cpu().numpy(), sw.cpu().numpy()), f"scene {s['index']} weight, call {rep}"
+        assert res["paths"] == sum(s["width"] * s["height"] * s["spp"] for s in specs)
+    batch.close()
+    for gs in gss:
+        gs.close()
+
+
+def test_batched_workload_smallest_scenes_match_oracle(gpu_ctx):
+    """The two smallest scenes of the batched workload, at 48x48 and 8 spp,
+    rendered in one batch with the same seeds as the FP64 oracle: films agree
+    far below the Monte Carlo noise (only paths whose FP32 / FP64 decisions
+    differ diverge)."""
+    import torch
+
+    import oracle
+
+    idx = _c4_subset()[2:]
+    specs = W.batched_scenes(only=set(idx))
+    gss = [E.GpuScene(gpu_ctx, s["scene"]) for s in specs]
+    batch = E.Batch(gpu_ctx, gss)
+    items = []
+    for s in specs:
+        fx, fw = _film(48, 48)
+        items.append((E.camera_struct(s["camera"]), _c4_params(s, 48, 8), fx, fw))
+    batch.render(items, graph=True)
+    torch.cuda.synchronize()
+    for s, (_, _, fx, fw) in zip(specs, items):
+        ox, ow = oracle.render(s["scene"], s["camera"], 48, 48, 8, wavelengths=1, max_depth=s["max_depth"],
+                               rr_depth=3, seed=s["seed"])
+        gx, gw = fx.cpu().numpy().astype(np.float64), fw.cpu().numpy().astype(np.float64)
+        assert np.array_equal(gw, ow)
+        g, o = gx / gw, ox / ow
+        rel = abs(g[1].sum() - o[1].sum()) / o[1].sum()
+        diff = np.abs(g - o) / max(o[1].max(), 1e-12)
+        bad = float((diff.max(0) > 1e-4).mean())
+        print(f"[gpu] batched scene {s['index']} ({s['n_tris']} tris) vs oracle: total Y rel {rel:.2e}, "
+              f"pixels > 1e-4: {bad:.4f}", flush=True)
+        assert rel < 5e-3 and bad < 0.05
+    batch.close()
+    for gs in gss:
+        gs.close()
