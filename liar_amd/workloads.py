@@ -123,7 +123,7 @@ def terrain(n_side: int = 1581, n_spheres: int = 10000, seed: int = 7, tile: int
     """Heightfield terrain over [0,1]^2 (fBm, 8 octaves, gain 0.5) with
     `n_spheres` spheres of radius U[0.002, 0.02] resting on it.  Shaders are
     split 50/25/25 over Lambert / Cook-Torrance GGX gold (roughness
-    U[0.05, 0.5], 8 levels) / Dielectric 1.5, per 16x16-cell terrain tile and
+    U[0.05, 0.5] in 256 levels, 0.0018 apart) / Dielectric 1.5, per 16x16-cell terrain tile and
     per sphere.  Lights: a 0.5-degree sun disc (5800 K) and a constant sky.
     Returns (CompiledScene, camera) -- built directly in the engine's
     flattened form."""
@@ -135,14 +135,15 @@ def terrain(n_side: int = 1581, n_spheres: int = 10000, seed: int = 7, tile: int
     tris = np.empty((2 * (n - 1) ** 2, 3), np.uint32)
     N.check(N.lib().lr_gen_terrain(seed, n, 8, 0.5, 4.0, 0.1, N.ptr(verts), N.ptr(tris)), "lr_gen_terrain")
     rng = np.random.default_rng(seed)
-    # materials: 0 Lambert, 1..8 Cook-Torrance GGX gold (roughness levels), 9 Dielectric
+    # materials: 0 Lambert, 1..256 Cook-Torrance GGX gold (roughness levels), 257 Dielectric
     n_au, k_au = gold_nk()
     table = [spectra.Constant(0.45), n_au, k_au, spectra.Constant(1.5), spectra.Constant(1.0),
              spectra.BlackBody(5800.0, sun_scale), spectra.Constant(sky)]
     NONE = N.LR_NONE
     mats = [{"type": N.MAT_LAMBERT, "reflectance": 0, "transmittance": NONE, "eta": NONE, "kappa": NONE,
              "emission": NONE, "light": NONE, "roughness": 0.0}]
-    levels = np.linspace(0.05, 0.5, 8)
+    n_lv = 256
+    levels = 0.05 + 0.45 * (np.arange(n_lv) + 0.5) / n_lv  # midpoints of 256 equal bins of [0.05, 0.5]
     for r in levels:
         mats.append({"type": N.MAT_COOK_TORRANCE, "reflectance": 4, "transmittance": NONE, "eta": 1, "kappa": 2,
                      "emission": NONE, "light": NONE, "roughness": float(r)})
@@ -151,7 +152,7 @@ def terrain(n_side: int = 1581, n_spheres: int = 10000, seed: int = 7, tile: int
 
     def pick(k):
         u = rng.random(k)
-        m = np.where(u < 0.5, 0, np.where(u < 0.75, 1 + rng.integers(0, 8, k), 9))
+        m = np.where(u < 0.5, 0, np.where(u < 0.75, 1 + rng.integers(0, n_lv, k), n_lv + 1))
         return m.astype(np.uint32)
 
     cells = n - 1
