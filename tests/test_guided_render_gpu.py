@@ -83,11 +83,18 @@ def test_guided_cornell_unbiased(gpu_ctx, learned):
     bg, bp = np.stack([blocks(v) for v in yg]), np.stack([blocks(v) for v in yp])
     se = np.sqrt(bg.var(axis=0, ddof=1) / passes + bp.var(axis=0, ddof=1) / passes)
     z = np.abs(bg.mean(0) - bp.mean(0)) / np.maximum(se, 1e-12)
-    print(f"[gpu] guided cornell 1024 spp (learned selection {learned}): bias {bias:+.2e}, MRAE {mrae:.4f}, "
+    # the image means: standard errors from the per-pass spread; the 4096-spp
+    # reference's is taken as half the 1024-spp plain set's (4x the samples)
+    gm, pm = np.array([v.mean() for v in yg]), np.array([v.mean() for v in yp])
+    se_g, se_p = gm.std(ddof=1) / np.sqrt(passes), pm.std(ddof=1) / np.sqrt(passes)
+    z_ref = (y.mean() - ref.mean()) / np.hypot(se_g, se_p / 2)
+    z_plain = (gm.mean() - pm.mean()) / np.hypot(se_g, se_p)
+    print(f"[gpu] guided cornell 1024 spp (learned selection {learned}): bias {bias:+.2e} "
+          f"(z {z_ref:+.2f} vs reference, {z_plain:+.2f} vs plain), MRAE {mrae:.4f}, "
           f"block |z| vs plain: max {z.max():.2f} mean {z.mean():.2f}")
     # per-pixel MRAE is noise: plain 1024 spp scores ~0.04 against this reference; guided
     # sampling spends part of the samples on a network that is still learning
-    assert abs(bias) < 3e-3 and mrae < 0.07
+    assert abs(z_ref) < 3.5 and abs(z_plain) < 3.5 and abs(bias) < 1e-2 and mrae < 0.07
     assert z.max() < 5.0 and z.mean() < 1.3
     g.close()
     gs.close()
