@@ -112,7 +112,7 @@ def bench_render(args, la, oracle, torch, np):
         "camera_samples_per_s": round(paths / NL / secs, 1),
         "mrays_per_s": round(rays / secs / 1e6, 2),
         "rays_per_path": round(rays / paths, 3),
-        "trace_config": None if tuned is None else tuned["best"],
+        "trace_config": ctx.trace_config() if tuned is None else tuned["best"],
     }
     if not args.no_cpu:
         cw, ch = 64, 64
@@ -170,7 +170,7 @@ def bench_direct(args, la, oracle, torch, np):
         "paths_per_s": round(paths / secs, 1),
         "mrays_per_s": round(rays / secs / 1e6, 2),
         "rays_per_path": round(rays / paths, 3),
-        "trace_config": None if tuned is None else tuned["best"],
+        "trace_config": ctx.trace_config() if tuned is None else tuned["best"],
     }
     if not args.no_cpu:
         cw, ch = 128, 128
@@ -410,7 +410,7 @@ def bench_guided(args, la, torch, np):
                          "per-vertex strategy probabilities, at every bounce; online training in the first "
                          f"{GUIDED_TRAIN_PASSES} passes: 128 Adam steps (lr 1e-3, batch 4096) per 4M-path chunk on "
                          "the records of every bounce, weights clamped at 20x their mean",
-        "trace_config": None if tuned is None else tuned["best"],
+        "trace_config": ctx.trace_config() if tuned is None else tuned["best"],
     }
     gs.close()
     ctx.close()
@@ -457,7 +457,7 @@ def bench_terrain(args, la, torch, np):
         "samples_per_s": round(paths / secs, 1),
         "mrays_per_s": round(rays / secs / 1e6, 1),
         "rays_per_path": round(rays / paths, 3),
-        "trace_config": None if tuned is None else tuned["best"],
+        "trace_config": ctx.trace_config() if tuned is None else tuned["best"],
         "autotune_default_ms": None if tuned is None or tuned["default_ms"] is None else round(tuned["default_ms"], 3),
         "autotune_best_ms": None if tuned is None else round(tuned["best_ms"], 3),
     }
