@@ -52,6 +52,17 @@ def _traffic(kind: str):
         return None
 
 
+def _valu_issue():
+    """VALU issue figures of the closest-hit launch from the committed SQ
+    counter summary (busy share of the issue slots, active lanes)."""
+    f = ROOT / "profiles" / "traffic_random1M.json"
+    try:
+        v = json.loads(f.read_text())["valu_issue"]["closest"]
+        return {"valu_busy": v["valu_busy"], "lanes_active": v["lanes_active"]}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def pin_config(args, ctx) -> bool:
     """--trace-config: install the given traversal configuration in ctx and
     skip the autotuner (profiling runs time exactly one configuration)."""
@@ -825,6 +836,9 @@ def main() -> int:
             # give the lower device_frac
             "basis": "effective: canonical 32 + 16 + 128 N_node + 48 N_prim bytes per ray",
             "device_frac": round(N_RAYS * dev_b_ray / (kern_c * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            # what the SQ counters show bounds the launch: the VALU issue
+            # slots (busy share), with few lanes active per instruction
+            "issue": _valu_issue(),
         },
         "cpu_baseline": cpu,
         "detail": {
