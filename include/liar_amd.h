@@ -514,10 +514,10 @@ int lr_render(lr_scene* scene, const lr_camera* camera, const lr_render_params* 
  * own number of samples (lr_light.samples), each a light sample and a BSDF
  * sample combined by the power heuristic; otherwise secondary_light_samples
  * lights picked by power.  Rays that escape add nothing.  The random
- * numbers are fixed by the seed, but a path's unoccluded shadow-ray radiance
- * is added with float atomics in scheduling order, so films are reproducible
- * up to FP32 summation order (not bit for bit); the film is accumulated as by
- * lr_render. */
+ * numbers are fixed by the seed, and a path's radiance terms (from its ray
+ * tree's emission and unoccluded shadow rays) are summed in 64-bit fixed
+ * point (2^-32 per unit, up to 2^31), so films are bit-reproducible; the
+ * film is accumulated as by lr_render. */
 typedef struct lr_direct_params {
     uint32_t max_generation;           /* RayTracer::maxRayGeneration: rays of
                                           generation 0 (camera) .. this are cast

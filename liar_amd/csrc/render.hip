@@ -2115,9 +2115,34 @@ struct DLArgs {
     uint32_t qcap, scap;
     uint32_t* qover;  // branch entries that did not fit (the host grows the queues and repeats the chunk)
     uint32_t* sover;  // shadow rays that did not fit
+    unsigned long long* lq;  // per camera path: radiance in fixed point (kRadianceFx)
 };
 
 constexpr uint32_t kHQ = 0x80000000u;
+
+// A camera path's DirectLighting radiance arrives from many threads (its ray
+// tree's emission and shadow rays) in scheduling order.  It is summed in
+// 64-bit fixed point, 2^-32 per unit, with integer atomics, which are exact
+// and order-free, so films are bit-reproducible; a term is quantised to
+// 2.3e-10 absolute and a path's sum holds up to 2^31.  Terms are >= 0; a
+// non-finite or larger one saturates at 2^31.
+constexpr double kRadianceFx = 4294967296.0;
+__device__ inline unsigned long long radiance_fx(float c) {
+    const double v = (double)c * kRadianceFx;
+    return v >= 0.0 && v < 9.2e18 ? (unsigned long long)(v + 0.5) : 0x7FFFFFFFFFFFFFFFull;
+}
+
+// Chunk start: every camera path's fixed-point radiance at zero.
+__global__ void k_dl_zero(uint32_t n, unsigned long long* lq) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lq[i] = 0ull;
+}
+
+// Chunk end: the fixed-point sums into the path state the splat reads.
+__global__ void k_dl_fold(uint32_t n, const unsigned long long* lq, PathState ps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ps.L[i] = (float)((double)lq[i] * (1.0 / kRadianceFx));
+}
 
 // Power heuristic a^2 / (a^2 + b^2) for a > 0, written so that large pdfs
 // do not overflow in FP32 (ray_tracer.cpp:180-183).
@@ -2338,7 +2363,7 @@ __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t
             // (direct_lighting.cpp:248-255)
             if ((gen == 0 || m.light == LR_NONE) && m.emission != LR_NONE) {
                 const float Le = spectrum_eval(S, m.emission, H.lam, false);
-                if (Le != 0.0f) atomicAdd(&ps.L[pid], H.w * Le);
+                if (Le != 0.0f) atomicAdd(&A.lq[pid], radiance_fx(H.w * Le));
             }
             if (m.type != LR_MAT_NONE) {
                 surf = true;
@@ -2442,11 +2467,11 @@ __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t
 
 // Unoccluded shadow rays add their radiance to their camera path.
 __global__ void k_dl_resolve(const uint32_t* scount, uint32_t scap, ShadowQueue sq, const uint8_t* occluded,
-                             PathState ps, uint64_t* rays) {
+                             unsigned long long* lq, uint64_t* rays) {
     const uint32_t n = min(*scount, scap);
     if (blockIdx.x == 0 && threadIdx.x == 0) *rays += n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        if (!occluded || !occluded[i]) atomicAdd(&ps.L[sq.pid[i]], sq.c[i]);
+        if (!occluded || !occluded[i]) atomicAdd(&lq[sq.pid[i]], radiance_fx(sq.c[i]));
 }
 
 // Grows the DirectLighting workspace of a scene: two branch queues (11 words
@@ -2521,6 +2546,9 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
     uint64_t* rays_dev = reinterpret_cast<uint64_t*>(counts + 4);
     float* pw = sc->d_work;
     const PathState ps{pw, pw + cap, pw + 2 * (size_t)cap, pw + 3 * (size_t)cap};
+    // the camera paths' fixed-point radiance: words 4..5 of the path workspace
+    // (16-byte aligned), which the path tracer's queues use and this renderer does not
+    A.lq = reinterpret_cast<unsigned long long*>(pw + 4 * (size_t)cap);
     BatchArgs NB;
     std::memset(&NB, 0, sizeof(NB));
     const dim3 blk(256);
@@ -2566,6 +2594,7 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
                                dim3(kAppendBlock), 0, s, S, C, P, path0, n,
                                ps, cq, counts, NB);
             hipLaunchKernelGGL(k_dl_init, dim3((n + 255) / 256), blk, 0, s, counts, ps, q[0]);
+            hipLaunchKernelGGL(k_dl_zero, dim3((n + 255) / 256), blk, 0, s, n, A.lq);
             // no specular (or traced glossy) material: no hit continues, so
             // generations after the camera's are empty and not launched
             const uint32_t last_gen = (sc->has_specular || (A.glossy && sc->has_glossy)) ? A.max_gen : 0u;
@@ -2592,7 +2621,7 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
                     occ = sc->d_dlocc;
                 }
                 if (A.slots)
-                    hipLaunchKernelGGL(k_dl_resolve, dim3(grid_max), blk, 0, s, counts + 2, A.scap, sq, occ, ps,
+                    hipLaunchKernelGGL(k_dl_resolve, dim3(grid_max), blk, 0, s, counts + 2, A.scap, sq, occ, A.lq,
                                        rays_dev);
             }
             LR_HIP_CHECK(hipGetLastError());
@@ -2611,6 +2640,7 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
             if (qcap > (1ull << 30) || scap > (1ull << 30))
                 return lr::fail(LR_E_INVALID, "lr_render_direct: ray tree too large; use a smaller chunk_paths");
         }
+        hipLaunchKernelGGL(k_dl_fold, dim3((n + 255) / 256), blk, 0, s, n, A.lq, ps);
         hipLaunchKernelGGL(k_splat_tiles, dim3(splat_blocks(npix, P.spp * P.nl)), blk, 0, s, S, P, pix0, npix, ps,
                            film_xyz, film_w);
         LR_HIP_CHECK(hipGetLastError());
