@@ -296,7 +296,11 @@ int lr_autotune_trace(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n, lr_hits
  * in nanometres. */
 #define LR_SPEC_CONSTANT 0u  /* value                                         */
 #define LR_SPEC_SAMPLED 1u   /* piecewise linear through (w[i], v[i]), 0 outside
-                                (src/liar/kernel/spectral.cpp:185-210)         */
+                                (src/liar/kernel/spectral.cpp:185-210); as a
+                                reflectance divided by max(v) when above 1, or
+                                by `value` when value > 0 (a recovered XYZ
+                                spectrum's blended maximum,
+                                recovery_meng_simon.cpp:169-178)              */
 #define LR_SPEC_BINNED 2u    /* `count` equal bins over [lo, hi), constant per
                                 bin, 0 outside                                 */
 #define LR_SPEC_BLACKBODY 3u /* Planck at `temperature` K, peak = `scale`

@@ -13,9 +13,10 @@ from .engine import RenderEngine, compile_scene  # noqa: F401
 from .observer import Observer  # noqa: F401
 from .qbvh import Context, Qbvh, philox4x32, random_rays, random_triangles  # noqa: F401
 from .rgb_space import CIEXYZ, RgbSpace, sRGB  # noqa: F401
+from .spectra import Recovery, rgb  # noqa: F401
 
 # ``from liar_amd import *`` gives a script LiAR's top-level names
 __all__ = ["RenderEngine", "Observer", "cameras", "output", "samplers", "scenery", "shaders", "spectra", "textures",
-           "tracers", "sRGB", "CIEXYZ", "RgbSpace"]
+           "tracers", "sRGB", "CIEXYZ", "RgbSpace", "Recovery", "rgb"]
 
 __version__ = "0.1.0"

@@ -1045,7 +1045,8 @@ int build_spectra(const lr_spectrum* spectra, uint32_t n_spectra, std::vector<De
             d.offset = (uint32_t)table.size();
             table.insert(table.end(), s.w, s.w + s.count);
             table.insert(table.end(), s.v, s.v + s.count);
-            const float mx = *std::max_element(s.v, s.v + s.count);
+            // Reflectant divisor: the values' maximum, or the caller's (value > 0)
+            const float mx = s.value > 0.0f ? s.value : *std::max_element(s.v, s.v + s.count);
             d.refl_scale = mx > 1.0f ? 1.0f / mx : 1.0f;
             break;
         }

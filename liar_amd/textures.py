@@ -16,8 +16,9 @@ class Constant(Texture):
             self.spectrum = value
             self.value = None
         elif isinstance(value, (tuple, list)):
-            raise TypeError("RGB constants need spectral recovery, which is not implemented; "
-                            "pass a scalar or a liar_amd.spectra spectrum")
+            # Constant(const XYZ&) (constant.cpp:50-55): a tristimulus triple
+            self.spectrum = spectra.XYZ(value)
+            self.value = None
         else:
             self.value = float(value)
             self.spectrum = spectra.Constant(self.value)

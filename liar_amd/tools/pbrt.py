@@ -22,9 +22,8 @@ The subset of pbrt this engine can render is supported; the rest raises
 * AreaLightSource "diffuse" (L, nsamples) on a planar parallelogram (a
   two-triangle trianglemesh); LightSource "infinite" (L, nsamples, no map);
 * parameter types integer, float, point, vector, normal, bool, string, rgb /
-  color, blackbody, spectrum (inline pairs).  A grey rgb is a constant
-  spectrum; other colours become three bands (380-500, 500-580, 580-780 nm)
-  of the linear-sRGB values, an approximation of LiAR's RGB spectra.
+  color (``rgb`` of the linear-sRGB value, as LiAR's importer makes it), xyz,
+  blackbody, spectrum (inline pairs).
 
 pbrt's camera frame is left handed (right = up x forward); this engine's
 camera takes right = forward x sky, so the importer renders with sky = up and
@@ -41,6 +40,10 @@ from typing import Any
 import numpy as np
 
 from .. import cameras, engine, output, samplers, scenery, shaders, spectra, textures, tracers
+from ..rgb_space import sRGB
+
+# pbrt colours are linear sRGB values (LiAR's importer, pbrt.py:1395)
+_RGB_LINEAR = sRGB.linearSpace()
 
 __all__ = ["parse", "PbrtScene", "ParseError", "remap_roughness"]
 
@@ -141,14 +144,7 @@ def _spectrum(v):
     if isinstance(v, (int, float)):
         return float(v)
     r, g, b = (float(x) for x in v)
-    if r == g == b:
-        return r
-    return spectra.Binned(380.0, 780.0, _rgb_bands(r, g, b))
-
-
-def _rgb_bands(r, g, b):
-    # ten 40 nm bins over [380, 780): blue 380-500, green 500-580, red 580-780
-    return [b, b, b, g, g, r, r, r, r, r]
+    return spectra.rgb(r, g, b, _RGB_LINEAR)  # LiAR's importer: pbrt.py:1355-1362
 
 
 def _rotation(angle_deg, axis):
@@ -586,6 +582,10 @@ def _convert(kind, vals, path, line):
         if len(vals) != 3:
             raise ParseError(path, line, "an rgb value has three components")
         return tuple(float(x) for x in vals)
+    elif kind == "xyz":
+        if len(vals) != 3:
+            raise ParseError(path, line, "an xyz value has three components")
+        return spectra.XYZ(tuple(float(x) for x in vals))
     elif kind == "blackbody":
         t = float(vals[0])
         scale = float(vals[1]) if len(vals) > 1 else 1.0

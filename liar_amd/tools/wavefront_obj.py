@@ -14,9 +14,9 @@ the n cyclic triples.  When the normal's major component is negative LiAR
 clips the reversed polygon and applies the indices found there to the face in
 its original order; this module does the same, so the triangles match.
 
-Materials: this build has no RGB-to-spectrum recovery, so a library's ``Kd``
-becomes a Lambert of the colour's luminance (Rec. 709 weights); ``Ks``, ``Ns``
-and texture maps are ignored.  ``mtllib`` paths are tried as given, then next
+Materials: a library's ``Kd`` becomes a Lambert of ``rgb(Kd)`` (the colour
+in the default sRGB space, recovered as a spectrum, as LiAR's loader does,
+wavefront_obj.py:100-110); ``Ks``, ``Ns`` and texture maps are ignored.  ``mtllib`` paths are tried as given, then next
 to the OBJ file.  Negative ``vt`` / ``vn`` indices count back from the uv /
 normal lists (LiAR counts ``vt`` back from the normals and ``vn`` from the uvs).
 """
@@ -25,16 +25,11 @@ from __future__ import annotations
 import math
 import os
 
-from .. import scenery, shaders, textures
-
-
-def _luminance(rgb):
-    r, g, b = rgb
-    return 0.2126 * r + 0.7152 * g + 0.0722 * b
+from .. import scenery, shaders, spectra, textures
 
 
 def _default_material(kd=(1.0, 1.0, 1.0)):
-    return shaders.Lambert(textures.Constant(_luminance(kd)))
+    return shaders.Lambert(textures.Constant(spectra.rgb(kd)))
 
 
 # -- polygon triangulation -------------------------------------------------------

@@ -298,6 +298,13 @@ def _spec_struct(sp, keep):
         v = np.ascontiguousarray(sp.values, np.float32)
         keep += [w, v]
         return _Spec(1, len(v), w.ctypes.data, v.ctypes.data, 0, 0, 0, 0, 0)
+    if kind == "XYZ":
+        # the recovered table, with the recovery's Reflectant divisor
+        rec = sp.recovery_table()
+        w = np.ascontiguousarray(rec[0], np.float32)
+        v = np.ascontiguousarray(rec[1], np.float32)
+        keep += [w, v]
+        return _Spec(1, len(v), w.ctypes.data, v.ctypes.data, rec[2], 0, 0, 0, 0)
     if kind == "Binned":
         v = np.ascontiguousarray(sp.values, np.float32)
         keep.append(v)

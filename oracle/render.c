@@ -694,6 +694,7 @@ void or_spectrum_eval(const or_spectrum* sp, const double* w, uint32_t n, int re
     double mx = -INFINITY, scale = 1;
     if (sp->kind == 1 || sp->kind == 2)
         for (uint32_t k = 0; k < sp->count; ++k) mx = fmax(mx, sp->v[k]);
+    if (sp->kind == 1 && sp->value > 0) mx = sp->value;
     if (mx > 1) scale = 1 / mx;
     S.spec_reflscale = &scale;
     for (uint32_t i = 0; i < n; ++i) out[i] = spec_eval(&S, 0, w[i], reflectant);
@@ -1137,6 +1138,7 @@ static int render_impl(const float* verts, const uint32_t* tris, uint32_t nt, co
         double mx = -INFINITY;
         if (spectra[i].kind == 1 || spectra[i].kind == 2)
             for (uint32_t k = 0; k < spectra[i].count; ++k) mx = fmax(mx, spectra[i].v[k]);
+        if (spectra[i].kind == 1 && spectra[i].value > 0) mx = spectra[i].value;  /* caller's divisor */
         S.spec_reflscale[i] = mx > 1 ? 1 / mx : 1;
     }
     /* scene bounding sphere radius (half the AABB diagonal) */

@@ -173,9 +173,13 @@ def test_obj_reference_example_groups():
     assert sorted(mats) == ["", "black", "blue", "green", "red", "white", "yellow"]
     for g in groups:
         assert g.vertices().shape == (4, 3) and g.triangles().shape == (2, 3)
-    # materials follow usemtl: black, white, red, yellow, green, blue (Kd luminance)
-    lum = [g.shader.diffuse.spectrum.value for g in groups]
-    np.testing.assert_allclose(lum, [0.0, 1.0, 0.2126, 0.9278, 0.7152, 0.0722])
+    # materials follow usemtl: black, white, red, yellow, green, blue, each
+    # Lambert(Constant(rgb(Kd))) in sRGB: the XYZ of the Kd colour
+    from liar_amd import sRGB
+
+    kds = [(0, 0, 0), (1, 1, 1), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1)]
+    for g, kd in zip(groups, kds):
+        np.testing.assert_allclose(g.shader.diffuse.spectrum.value, sRGB.toXYZ(kd), atol=1e-12)
     # every group is a square of area 4 (side 2)
     for g in groups:
         v = g.vertices().astype(float)

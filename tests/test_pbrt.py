@@ -140,7 +140,9 @@ def test_area_light_and_infinite_light():
         WorldEnd""")
     objs = s.engine.scene.children
     sky = [o for o in objs if isinstance(o, scenery.LightSky)][0]
-    assert sky.radiance.scalar() == pytest.approx(0.5) and sky.numberOfEmissionSamples == 4
+    # "rgb L" [0.5 0.5 0.5]: rgb() of the linear-sRGB grey, the XYZ of E white at Y = 0.5
+    np.testing.assert_allclose(sky.radiance.spectrum.value, (0.5, 0.5, 0.5), atol=1e-7)
+    assert sky.numberOfEmissionSamples == 4
     placed = [o for o in objs if isinstance(o, scenery.Transformation)][0]
     light = placed.child
     assert isinstance(light, scenery.LightArea) and light.numberOfEmissionSamples == 9
