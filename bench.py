@@ -739,6 +739,20 @@ def main() -> int:
     mrays = N_RAYS * args.steps / wall_c / 1e6
     mrays_any = N_RAYS * args.steps / wall_a / 1e6
 
+    # this box's measured HBM bandwidth: a 1 GiB device-to-device copy (read +
+    # write), beside the 8 TB/s spec the roofline uses
+    cp_a = torch.zeros(1 << 28, dtype=torch.float32, device=dev)
+    cp_b = torch.empty_like(cp_a)
+    cp_b.copy_(cp_a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        cp_b.copy_(cp_a)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * cp_a.numel() * 4 * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del cp_a, cp_b
+
     # oracle counts + parity on a sample of the same rays
     nodes, prims = bvh.export()
     S = min(args.count_sample, N_RAYS)
@@ -759,8 +773,12 @@ def main() -> int:
     mism_any = int((occ[:S].cpu().numpy() != ref_any[0]).sum())
     # numerically tied hits (two or more primitives at the hit distance): the
     # GPU must break them as the oracle does, by the smaller primitive id
-    n_tied = int(tied.sum())
-    mism_tied = int((g_prim[tied != 0] != ref[0][tied != 0]).sum())
+    is_tie = (tied & 1) != 0
+    n_tied = int(is_tie.sum())
+    mism_tied = int((g_prim[is_tie] != ref[0][is_tie]).sum())
+    # rays whose answer LiAR's interval quirk decided (the mesh's nearest
+    # triangle within near (1 + 1e-6): reported as no hit)
+    n_quirk = int(((tied & 2) != 0).sum())
     nodes_per_ray = ref[4] / S
     prims_per_ray = ref[5] / S
     b_ray = 32 + 16 + NODE_BYTES * nodes_per_ray + 48 * prims_per_ray
@@ -771,6 +789,9 @@ def main() -> int:
     dev_b_ray_any = 32 + 1 + DEVICE_NODE_BYTES * cnt_any[1] / S + 48 * cnt_any[2] / S
     achieved = N_RAYS * b_ray / (kern_c * 1e-3) / 1e9
     achieved_any = N_RAYS * b_ray_any / (kern_a * 1e-3) / 1e9
+    # compulsory traffic: every ray and hit once plus the whole scene once
+    # (the device's 64 B nodes and 48 B primitive records)
+    compulsory = N_RAYS * (32 + 16) + info["n_nodes"] * DEVICE_NODE_BYTES + len(prims) * 48
 
     cpu = None
     if not args.no_cpu:
@@ -864,6 +885,11 @@ def main() -> int:
             "tied_hits_closest": n_tied,
             "tied_hit_fraction_closest": round(n_tied / S, 8),
             "tied_hit_mismatches_closest": mism_tied,
+            "interval_quirk_rays_closest": n_quirk,
+            "measured_copy_gbs": round(copy_gbs, 1),
+            "device_frac_of_measured_copy": round(N_RAYS * dev_b_ray / (kern_c * 1e-3) / 1e9 / copy_gbs, 4),
+            "compulsory_bytes_closest": int(compulsory),
+            "compulsory_ms_closest": round(compulsory / (HBM_PEAK_GBS * 1e9) * 1e3, 4),
             "hit_rate_closest": round(float((ref[0] != oracle.MISS).mean()), 4),
             "qbvh_build_s": round(info["build_seconds"], 3),
             "autotune_closest": tuned_c,

@@ -169,7 +169,9 @@ def qbvh_closest(nodes, prims, rays, threads: int | None = None):
 def qbvh_closest_ties(nodes, prims, rays, threads: int | None = None):
     """qbvh_closest plus a per-ray flag: 1 where the hit distance is reached
     by two or more primitives (a numerical tie, broken by the smaller id).
-    Returns (prim, t, u, v, tied uint8, nodes_visited, prims_tested)."""
+    Returns (prim, t, u, v, flags uint8, nodes_visited, prims_tested); flags
+    bit 0: the hit distance is shared by two or more primitives (a tie), bit
+    1: LiAR's interval quirk rejected the mesh's nearest triangle."""
     L = lib()
     if not getattr(L, "_ties_sig", False):
         P, U32 = C.c_void_p, C.c_uint32

@@ -378,7 +378,10 @@ static int qbvh_closest_one(const or_node* nodes, const or_prim* prims, uint32_t
     } else {
         *prim = OR_MISS; *th = INFINITY; *uh = 0.0f; *vh = 0.0f;
     }
-    if (tied) *tied = (uint8_t)tie;
+    /* bit 0: a tie; bit 1: LiAR's interval quirk decided the mesh's answer
+     * (its nearest triangle beyond near lay within near (1 + eps), so the
+     * mesh reports no hit, triangle_mesh.cpp:301-302) */
+    if (tied) *tied = (uint8_t)(tie | (tri_id != OR_MISS && doomed ? 2 : 0));
     return 0;
 }
 
@@ -680,8 +683,9 @@ int or_qbvh_closest(const void* nodes, uint32_t n_nodes, const void* prims,
     return run_threads(j, n, nthreads, nodes_visited, prims_tested);
 }
 
-/* or_qbvh_closest plus tied[i] = 1 where ray i's hit distance is shared by
- * two or more primitives. */
+/* or_qbvh_closest plus tied[i]: bit 0 where ray i's hit distance is shared
+ * by two or more primitives, bit 1 where the interval quirk rejected the
+ * mesh's nearest triangle. */
 int or_qbvh_closest_ties(const void* nodes, uint32_t n_nodes, const void* prims,
                          const float* const rays[8], uint32_t n, uint32_t* prim, float* t, float* u,
                          float* v, uint8_t* tied, int nthreads, uint64_t* nodes_visited, uint64_t* prims_tested) {

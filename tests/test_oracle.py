@@ -152,6 +152,23 @@ def test_interval_quirk_nearest_triangle_out_of_range_means_mesh_miss():
         assert q[0][0] == ref[0][0] and q[1][0] == ref[1][0]
 
 
+def test_quirk_and_tie_flags_are_counted():
+    """The oracle's per-ray flags (bench: tied_hits / interval_quirk_rays):
+    bit 1 on the constructed quirk case, bit 0 on duplicated geometry, none
+    on an ordinary hit."""
+    zA = np.float32(1.0) + np.float32(2.5e-7)
+    verts, tris = _stack(_quad_at(zA), _quad_at(2.0))
+    nodes, prims = la.Qbvh(None, verts, tris).export()
+    r = np.concatenate([rays_of([0.3, 0.2, 0.0], [0, 0, 1], tnear=1.0), rays_of([0.3, 0.2, 0.0], [0, 0, 1])], 1)
+    p, t, _, _, flags, _, _ = oracle.qbvh_closest_ties(nodes, prims, r)
+    assert p[0] == MISS and flags[0] == 2
+    assert p[1] != MISS and flags[1] == 0
+    verts, tris = _stack(_quad_at(1.0), _quad_at(1.0))
+    nodes, prims = la.Qbvh(None, verts, tris).export()
+    p, t, _, _, flags, _, _ = oracle.qbvh_closest_ties(nodes, prims, rays_of([0.7, 0.2, 0], [0, 0, 1]))
+    assert t[0] == 1.0 and flags[0] == 1
+
+
 def test_far_limit_is_open_and_shrunk():
     verts, tris = _quad_at(2.0)
     assert oracle.brute_closest(verts, tris, None, rays_of([0.3, 0.2, 0], [0, 0, 1], tfar=2.0))[0][0] == MISS
