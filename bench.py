@@ -318,9 +318,10 @@ def bench_guided(args, la, torch, np):
     closed box room lit by the sun and sky only through a 2 % ceiling slit;
     camera on the indirectly lit half; 1920x1080, depth 16.  Plain and guided
     renders accumulate 4-spp passes against a plain 4096-spp reference.  The
-    guided clock includes its online training (the first 2 passes).  Reports
-    the wall time at which the guided render first reaches the plain
-    tracer's 64-spp MSE, by two measures: the MSE over all pixels, and over
+    guided clock includes its online training (the first 2 passes).  Each
+    method runs 3 times (render and guide seeds) and its MSE curve is the
+    per-pass mean.  Reports the wall time at which the guided render first
+    reaches the plain tracer's 64-spp MSE, by two measures: the MSE over all pixels, and over
     all but the 0.1 % of pixels with the largest error (the sun caustics of
     the glass and gold spheres make single pixels -- of the renders and of
     the reference -- carry nearly all of the plain MSE).  The same figures
@@ -357,7 +358,7 @@ def bench_guided(args, la, torch, np):
         e = ((img - ref) ** 2).flatten()
         return float(e.mean()), float(torch.sort(e)[0][:keep].mean())
 
-    def run(guide, ml, ref, passes=32):
+    def run(guide, ml, ref, passes=32, seed0=100):
         fx, fw = film()
         # warm-up pass (allocation, code load) outside the clock
         if guide is None:
@@ -369,10 +370,10 @@ def bench_guided(args, la, torch, np):
         for p in range(passes):
             t0 = time.perf_counter()
             if guide is not None:
-                E.guided_render(gs, guide, camera, params(4, 100 + p, ml), fx, fw, train=p < GUIDED_TRAIN_PASSES,
+                E.guided_render(gs, guide, camera, params(4, seed0 + p, ml), fx, fw, train=p < GUIDED_TRAIN_PASSES,
                                 **GUIDED_CFG)
             else:
-                gs.render(camera, params(4, 100 + p, ml), fx, fw)
+                gs.render(camera, params(4, seed0 + p, ml), fx, fw)
             torch.cuda.synchronize()
             elapsed += time.perf_counter() - t0
             curve.append((elapsed, 4 * (p + 1)) + errors(fx[1] / fw, ref))
@@ -389,14 +390,27 @@ def bench_guided(args, la, torch, np):
         t_ref = time.perf_counter() - t0
         ref = (fx / fw)[1].clone()
         del fx, fw
-        log("[bench] guided: plain curve")
-        plain = run(None, ml, ref)
-        log("[bench] guided: guided curve")
-        g = Guide(ctx, seed=1, learning_rate=1e-3)
-        guided = run(g, ml, ref)
-        g.close()
-        out = {"reference": f"plain {ref_spp} spp ({t_ref:.1f} s)",
-               "plain_64spp_seconds": round(plain[15][0], 4), "guided_64spp_seconds": round(guided[15][0], 4)}
+        # each method run `reps` times (render seeds, guide seeds): one guide's
+        # training is one draw of a noisy process, and a single caustic
+        # outlier moves one run's MSE; the curves are averaged per pass
+        reps = max(1, args.guided_repeats)
+        plains, guideds = [], []
+        for r in range(reps):
+            log(f"[bench] guided: plain curve {r + 1}/{reps}")
+            plains.append(run(None, ml, ref, seed0=100 + 1000 * r))
+            log(f"[bench] guided: guided curve {r + 1}/{reps}")
+            g = Guide(ctx, seed=1 + r, learning_rate=1e-3)
+            guideds.append(run(g, ml, ref, seed0=100 + 1000 * r))
+            g.close()
+
+        def mean_curve(cs):
+            return [tuple(float(np.mean([c[p][k] for c in cs])) for k in range(4)) for p in range(len(cs[0]))]
+
+        plain, guided = mean_curve(plains), mean_curve(guideds)
+        out = {"reference": f"plain {ref_spp} spp ({t_ref:.1f} s)", "runs_per_method": reps,
+               "plain_64spp_seconds": round(plain[15][0], 4), "guided_64spp_seconds": round(guided[15][0], 4),
+               "equal_spp_mse_trimmed_ratio_per_run": [round(a[15][3] / b[15][3], 3)
+                                                       for a, b in zip(plains, guideds)]}
         for k, name in ((2, "mse"), (3, "mse_trimmed")):
             target = plain[15][k]
             hit = next((c[0] for c in guided if c[k] <= target), None)
@@ -636,6 +650,8 @@ def main() -> int:
                     help="run only a short slit-room plain + guided sequence for profilers")
     ap.add_argument("--guided-width", type=int, default=1920)
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
+    ap.add_argument("--guided-repeats", type=int, default=3,
+                    help="plain and guided curves per method (averaged)")
     ap.add_argument("--render-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (PMC byte-counter calibration)")

@@ -535,45 +535,48 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         cb /= rest;
                         cl = 0.0f;
                     }
+                    // the strategy picks the direction; the BSDF value and the
+                    // three densities of that direction are then evaluated
+                    // once, in code every lane shares (the same BSDF and
+                    // light-pdf evaluation whichever strategy sampled it)
                     V3 dw = v3(0, 0, 1);
                     float f = 0.0f, pb = 0.0f, pl = 0.0f, pg = 0.0f, pc = 0.0f;
+                    int chosen = -1;
+                    uint32_t lk = 0xFFFFFFFFu;  // the sampled light (light strategy)
+                    float pdf_lk = 0.0f;        // its density of the sampled direction
                     if (u_strat < cb) {
-                        ok = bsdf_sample(B, wi, u1, u2, u_comp, wo, f, pb);
-                        if (ok) {
+                        float fs, ps;
+                        if (bsdf_sample(B, wi, u1, u2, u_comp, wo, fs, ps)) {
                             dw = wo.x * tt + wo.y * bb + wo.z * nf;
-                            for (uint32_t k = 0; k < S.n_lights; ++k)
-                                pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], S.light_tris, x, dw);
-                            pg = vmf_mix_pdf(mx, dw);
-                            pc = cb * pb;
+                            chosen = 0;
                         }
                     } else if (u_strat < cb + cl) {
                         if (S.n_lights > 0) {
-                            const uint32_t k = light_pick(S, u_light);
-                            const DevLight& Lk = S.lights[k];
-                            float pdf_l = 0.0f;
+                            lk = light_pick(S, u_light);
+                            const DevLight& Lk = S.lights[lk];
                             float dist_l;
-                            dw = light_sample(Lk, S.light_tris, x, ul1, ul2, pdf_l, dist_l);
-                            if (Lk.sel_pdf > 0.0f && pdf_l > 0.0f) {
-                                pl = Lk.sel_pdf * pdf_l;
-                                for (uint32_t j2 = 0; j2 < S.n_lights; ++j2)
-                                    if (j2 != k) pl += S.lights[j2].sel_pdf * light_pdf(S.lights[j2], S.light_tris, x, dw);
+                            dw = light_sample(Lk, S.light_tris, x, ul1, ul2, pdf_lk, dist_l);
+                            if (Lk.sel_pdf > 0.0f && pdf_lk > 0.0f) {
                                 wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
-                                ok = bsdf_evaluate(B, wi, wo, ngl, f, pb) && f != 0.0f;
-                                pg = vmf_mix_pdf(mx, dw);
-                                pc = cl * pl;
+                                chosen = 1;
                             }
                         }
                     } else {
                         const lr::u4 r2 = rng(P, gp, gen, 3u);
                         dw = vmf_mix_sample(mx, lr::u01(r2.x), lr::u01(r2.y), lr::u01(r2.z));
                         wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
-                        ok = bsdf_evaluate(B, wi, wo, ngl, f, pb) && f != 0.0f;
-                        if (ok) {
-                            for (uint32_t k = 0; k < S.n_lights; ++k)
-                                pl += S.lights[k].sel_pdf * light_pdf(S.lights[k], S.light_tris, x, dw);
-                            pg = vmf_mix_pdf(mx, dw);
-                            pc = cg * pg;
-                        }
+                        chosen = 2;
+                    }
+                    ok = chosen >= 0 && bsdf_evaluate(B, wi, wo, ngl, f, pb) && f != 0.0f;
+                    if (ok) {
+                        // the sampled light's own density (re-evaluating it can
+                        // miss a direction at the edge of a small source such as
+                        // the sun's cone), the others' evaluated
+                        for (uint32_t k = 0; k < S.n_lights; ++k)
+                            pl += S.lights[k].sel_pdf *
+                                  (k == lk ? pdf_lk : light_pdf(S.lights[k], S.light_tris, x, dw));
+                        pg = vmf_mix_pdf(mx, dw);
+                        pc = chosen == 0 ? cb * pb : (chosen == 1 ? cl * pl : cg * pg);
                     }
                     if (ok && pc > 0.0f) {
                         const float qb = cb * pb, ql = cl * pl, qg = cg * pg;
