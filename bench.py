@@ -446,7 +446,8 @@ def bench_terrain(args, la, torch, np):
     """The plain path tracer at scale: the terrain workload (1581x1581 fBm
     heightfield, 4,992,800 triangles + 10k spheres; Lambert / GGX gold /
     dielectric; 0.5-degree sun + sky) at 1920x1080, 64 spp, depth 16, one
-    wavelength per sample; traversal autotuned on a 480x270 4-spp render."""
+    wavelength per sample; traversal autotuned on a 480x270 4-spp render,
+    the queue tile (paths per chunk) on a 16-spp frame."""
     from liar_amd import engine as E
     from liar_amd import workloads as W
 
@@ -462,6 +463,13 @@ def bench_terrain(args, la, torch, np):
         log("[bench] terrain: autotuning traversal")
         tuned = tune_render(gs, camera, E.render_params(480, 270, 4, 1, 16, 16, 0.5, (380.0, 780.0), 5), repeats=2)
     Wd, Hd, SPP = 1920, 1080, 64
+    chunk, chunk_table = 0, None
+    if not args.no_tune:
+        from liar_amd.autotune import tune_chunk
+
+        log("[bench] terrain: autotuning the queue tile (paths per chunk)")
+        tc = tune_chunk(gs, camera, E.render_params(Wd, Hd, 16, 1, 16, 16, 0.5, (380.0, 780.0), 6))
+        chunk, chunk_table = tc["best"], [(c, round(ms, 3)) for c, ms in tc["table"]]
     fx = torch.zeros((3, Hd, Wd), dtype=torch.float32, device="cuda")
     fw = torch.zeros((Hd, Wd), dtype=torch.float32, device="cuda")
     log(f"[bench] terrain: {Wd}x{Hd} x {SPP} spp frames")
@@ -469,7 +477,7 @@ def bench_terrain(args, la, torch, np):
     for k in range(1 + args.terrain_frames):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st = gs.render(camera, E.render_params(Wd, Hd, SPP, 1, 16, 16, 0.5, (380.0, 780.0), 1 + k), fx, fw)
+        st = gs.render(camera, E.render_params(Wd, Hd, SPP, 1, 16, 16, 0.5, (380.0, 780.0), 1 + k, chunk), fx, fw)
         torch.cuda.synchronize()
         if k >= 1:
             stats.append((time.perf_counter() - t0, st))
@@ -485,6 +493,8 @@ def bench_terrain(args, la, torch, np):
         "trace_config": ctx.trace_config() if tuned is None else tuned["best"],
         "autotune_default_ms": None if tuned is None or tuned["default_ms"] is None else round(tuned["default_ms"], 3),
         "autotune_best_ms": None if tuned is None else round(tuned["best_ms"], 3),
+        "chunk_paths": chunk or (1 << 22),
+        "chunk_autotune_ms_16spp": chunk_table,
     }
     gs.close()
     ctx.close()

@@ -64,3 +64,34 @@ def tune_render(gs: "E.GpuScene", camera, params, repeats: int = 2, configs=None
     ctx.set_trace_config(**best)
     default_ms = next((ms for c, ms in table if c == start), None)
     return {"best": best, "best_ms": best_ms, "default_ms": default_ms, "table": table}
+
+
+CHUNKS = (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
+
+
+def tune_chunk(gs: "E.GpuScene", camera, params, chunks=CHUNKS, repeats: int = 1) -> dict:
+    """The render's queue tile: times lr_render(gs, camera, params) with each
+    ``chunk_paths`` in ``chunks`` (paths traced together per wavefront, 4M ..
+    64M by default; whole pixels per chunk) and returns {"best", "best_ms",
+    "table"} with ``params.chunk_paths`` left at the fastest.  A chunk pays
+    every bounce launch's tail once, so larger chunks amortise it; their
+    queues take 104 bytes of HBM per path."""
+    import torch
+
+    w, h = params.width, params.height
+    fx = torch.zeros((3, h, w), dtype=torch.float32, device="cuda")
+    fw = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    table = []
+    for c in chunks:
+        params.chunk_paths = int(c)
+        gs.render(camera, params, fx, fw)  # warm-up: workspace for this chunk
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(repeats):
+            gs.render(camera, params, fx, fw)
+        b.record()
+        b.synchronize()
+        table.append((int(c), a.elapsed_time(b) / repeats))
+    best, best_ms = min(table, key=lambda x: x[1])
+    params.chunk_paths = best
+    return {"best": best, "best_ms": best_ms, "table": table}

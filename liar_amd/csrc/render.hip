@@ -1531,8 +1531,11 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     }
     const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
     const uint64_t n_pix = (uint64_t)R.width * R.height;
-    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (1u << 22);
-    const uint64_t pix_per_chunk = std::max<uint64_t>(1, chunk_req / ppp);
+    // default queue tile: 32M paths (every bounce launch's tail is paid once
+    // per chunk; 3.3 GB of queues), 4M for guided renders, whose chunk is also
+    // the training cadence; never more than the render's own paths
+    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (guide ? (1u << 22) : (1u << 25));
+    const uint64_t pix_per_chunk = std::min<uint64_t>(std::max<uint64_t>(1, chunk_req / ppp), std::max<uint64_t>(1, n_pix));
     const uint64_t cap64 = pix_per_chunk * ppp;
     if (cap64 >= (1ull << 31)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");
     const uint32_t cap = (uint32_t)cap64;
@@ -2538,8 +2541,9 @@ int lr_render_direct(lr_scene* sc, const lr_camera* cam, const lr_render_params*
     const uint64_t slots_eff = std::max(A.slots, 1u);
     if (slots_eff * ppp > kQueueMax)
         return lr::fail(LR_E_INVALID, "lr_render_direct: light samples * spp * wavelengths exceed 2^30 rays per pixel");
-    const uint64_t pix_per_chunk =
-        std::min<uint64_t>(std::max<uint64_t>(1, chunk_req / ppp), kQueueMax / (slots_eff * ppp));
+    const uint64_t pix_per_chunk = std::min<uint64_t>(
+        std::min<uint64_t>(std::max<uint64_t>(1, chunk_req / ppp), kQueueMax / (slots_eff * ppp)),
+        std::max<uint64_t>(1, n_pix));  // never more than the render's own pixels
     const uint64_t cap64 = pix_per_chunk * ppp;
     if (cap64 >= (1ull << 26)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");
     const uint32_t cap = (uint32_t)cap64;
