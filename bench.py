@@ -520,11 +520,11 @@ def bench_batch(args, la, torch, np):
     t_build = time.perf_counter() - t0
     batch = E.Batch(ctx, gss)
 
-    def items():
+    def items(chunk=0):
         out = []
         for s in specs:
             prm = E.render_params(s["width"], s["height"], s["spp"], 1, s["max_depth"], 3, 0.5, (380.0, 780.0),
-                                  s["seed"])
+                                  s["seed"], chunk)
             out.append((E.camera_struct(s["camera"]), prm,
                         torch.zeros((3, s["height"], s["width"]), device="cuda"),
                         torch.zeros((s["height"], s["width"]), device="cuda")))
@@ -540,14 +540,19 @@ def bench_batch(args, la, torch, np):
     else:
         res = gss[big].qbvh.autotune(tune_rays, repeats=2)
         ctx.set_trace_config(**res["best"])
-    batch.render(items(), graph=True)  # warm-up: code load, workspace, graph instantiation
-    torch.cuda.synchronize()
-    it = items()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    r = batch.render(it, graph=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    # the queue tile (paths per chunk; chunks span scenes): the fastest batch
+    # call of 4M, 16M and 32M paths, each after a warm-up call (code load,
+    # workspace, graph instantiation)
+    chunk_table = []
+    for chunk in (1 << 22, 1 << 24, 1 << 25):
+        batch.render(items(chunk), graph=True)
+        it = items(chunk)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = batch.render(it, graph=True)
+        torch.cuda.synchronize()
+        chunk_table.append((chunk, time.perf_counter() - t0, r))
+    chunk, wall, r = min(chunk_table, key=lambda c: c[1])
     ms = np.array(r["scene_ms"])
     it = items()
     torch.cuda.synchronize()
@@ -571,6 +576,8 @@ def bench_batch(args, la, torch, np):
         "sequential_seconds": round(wall_seq, 4),
         "sequential_mrays_per_s": round(r["rays"] / wall_seq / 1e6, 1),
         "trace_config": res["best"],
+        "chunk_paths": chunk,
+        "chunk_seconds": [(c, round(w, 4)) for c, w, _ in chunk_table],
         "build_seconds": round(t_build, 2),
     }
     batch.close()

@@ -1895,11 +1895,13 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
     std::vector<uint32_t> order(n);
     for (uint32_t k = 0; k < n; ++k) order[k] = k;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return paths[a] < paths[c]; });
-    // chunks of whole pixels, up to 4M paths each (or the smallest nonzero
-    // chunk_paths of the items), spanning scenes
-    uint64_t cap = 1u << 22;
+    // chunks of whole pixels, spanning scenes: the smallest nonzero
+    // chunk_paths of the items, 4M paths when none sets one
+    uint64_t cap = 0;
     for (uint32_t k = 0; k < n; ++k)
-        if (items[k].params.chunk_paths) cap = std::min<uint64_t>(cap, items[k].params.chunk_paths);
+        if (items[k].params.chunk_paths)
+            cap = cap ? std::min<uint64_t>(cap, items[k].params.chunk_paths) : items[k].params.chunk_paths;
+    if (!cap) cap = 1u << 22;
     std::vector<Segment> segs;
     std::vector<ChunkPlan> chunks;
     std::vector<uint32_t> last_chunk(n, 0);
