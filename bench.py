@@ -32,6 +32,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s closest-hit at 1 GPU; time-to-equal-MSE guided vs plain GPU tracer"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense
+# guide training, per row: forward, input gradients of layers 1..5, weight gradients
+TRAIN_FLOPS_PER_ROW = 2 * (5 * 64 * 64 + 48 * 64) + 2 * (4 * 64 * 64 + 48 * 64) + 2 * (5 * 64 * 64 + 48 * 64)
 NODE_BYTES = 128  # canonical: one 4-wide FP32 SoA node
 DEVICE_NODE_BYTES = 64  # lr_node_q8, what the GPU fetches
 N_TRIS = 1_000_000
@@ -442,6 +445,48 @@ def bench_guided(args, la, torch, np):
     return out
 
 
+def bench_train(args, la, torch, np):
+    """Guide training steps (k_train + k_adam) at the guided benchmark's batch
+    of 4096 rows and at larger batches; MFMA rate by algorithmic flops."""
+    from liar_amd.guide import Guide
+
+    ctx = la.Context(0)
+    g = Guide(ctx, seed=1, learning_rate=1e-3)
+    out = {"workload": "lr_guide_train_step on random records (8 inputs with light / BSDF pdfs, unit "
+                       "directions, weights U[0.2, 1.2))",
+           "flops_per_row": TRAIN_FLOPS_PER_ROW,
+           "flops_basis": "forward 2(5*64*64 + 48*64) + input gradients 2(4*64*64 + 48*64) + weight gradients "
+                          "2(5*64*64 + 48*64)",
+           "launches_per_step": 2, "steps": []}
+    for n in (4096, 32768, 262144):
+        rng = np.random.default_rng(n)
+        x = np.zeros((n, 8), np.float32)
+        x[:, :8] = rng.random((n, 8))
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        X = torch.from_numpy(x).cuda()
+        D = torch.from_numpy(d.astype(np.float32)).cuda()
+        W = torch.from_numpy((rng.random(n) + 0.2).astype(np.float32)).cuda()
+        for _ in range(5):
+            g.train_step(X, D, W)
+        steps = 200 if n <= 32768 else 50
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            g.train_step(X, D, W)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / steps
+        tf = n * TRAIN_FLOPS_PER_ROW / (us * 1e-6) / 1e12
+        out["steps"].append({"rows": n, "us_per_step": round(us, 2), "tflops": round(tf, 1),
+                             "frac_bf16_peak": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)})
+        log(f"[train] {n} rows: {us:.1f} us/step, {tf:.1f} TFLOP/s")
+    g.close()
+    ctx.close()
+    return out
+
+
 def bench_terrain(args, la, torch, np):
     """The plain path tracer at scale: the terrain workload (1581x1581 fBm
     heightfield, 4,992,800 triangles + 10k spheres; Lambert / GGX gold /
@@ -662,6 +707,7 @@ def main() -> int:
     ap.add_argument("--no-direct", action="store_true", help="skip the DirectLighting leg")
     ap.add_argument("--no-film", action="store_true", help="skip the film output (tone mapping) leg")
     ap.add_argument("--no-instanced", action="store_true", help="skip the instancing leg")
+    ap.add_argument("--no-train", action="store_true", help="skip the guide-training leg")
     ap.add_argument("--terrain-frames", type=int, default=2)
     ap.add_argument("--profile-workload", action="store_true",
                     help="run only a short slit-room plain + guided sequence for profilers")
@@ -676,7 +722,7 @@ def main() -> int:
                     help="block,lds_stack,blocks_per_cu,refill_min,schedule: use this traversal "
                          "configuration everywhere instead of autotuning (for profiler runs)")
     ap.add_argument("--only", default=None,
-                    choices=["render", "direct", "film", "instanced", "terrain", "guided", "batch"],
+                    choices=["render", "direct", "film", "instanced", "terrain", "guided", "batch", "train"],
                     help="run only this leg (no headline) and print its JSON object")
     args = ap.parse_args()
     if args.trace_config:
@@ -704,7 +750,8 @@ def main() -> int:
                 "instanced": lambda: bench_instanced(args, la, torch, np),
                 "terrain": lambda: bench_terrain(args, la, torch, np),
                 "guided": lambda: bench_guided(args, la, torch, np),
-                "batch": lambda: bench_batch(args, la, torch, np)}
+                "batch": lambda: bench_batch(args, la, torch, np),
+                "train": lambda: bench_train(args, la, torch, np)}
         print(json.dumps({args.only: legs[args.only]()}), flush=True)
         return 0
     dev = torch.device("cuda:0")
@@ -854,6 +901,7 @@ def main() -> int:
     terrain = None if args.no_terrain else bench_terrain(args, la, torch, np)
     guided = None if args.no_guided else bench_guided(args, la, torch, np)
     batch = None if args.no_batch else bench_batch(args, la, torch, np)
+    train = None if args.no_train else bench_train(args, la, torch, np)
 
     out = {
         "metric": METRIC,
@@ -935,6 +983,7 @@ def main() -> int:
         "terrain": terrain,
         "guided": guided,
         "batch": batch,
+        "train": train,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -1,31 +1,28 @@
 // guide.hip -- path-guiding network on gfx950 MFMA: a 6 -> 64 (frequency
-// encoding) -> 5 x 64 (ReLU) -> 40 MLP whose 40 outputs parameterise an
-// 8-lobe von Mises-Fisher mixture over incident directions.  BF16 weights and
-// activations, FP32 accumulation (v_mfma_f32_16x16x16_bf16), FP32 master
-// weights with Adam.
+// encoding) -> 5 x 64 (ReLU) -> 48 MLP whose outputs parameterise an 8-lobe
+// von Mises-Fisher mixture over incident directions and the strategy
+// selection of one-sample MIS.  BF16 weights and activations, FP32
+// accumulation (v_mfma_f32_16x16x32_bf16), FP32 master weights with Adam.
 //
 // Kernels
-//   k_infer       fully fused forward on v_mfma_f32_16x16x32_bf16 in the
-//                 transposed form Z^T = W^T-tiles x X^T: the accumulator of
-//                 one layer (lane = row, 4 consecutive features) is, after
-//                 bias + ReLU + bf16 packing, exactly the B operand of the
-//                 next, so activations never leave registers; weights sit in
-//                 LDS in per-lane fragment order; each lane ends holding two
-//                 whole lobes and the softmax needs two cross-lane exchanges
-//   k_train_fb    fused forward + loss + backward data path, register
-//                 resident like k_infer (backward on transposed fragments);
-//                 writes activations A_l and gradients dZ_l feature-major
-//                 (bf16) for the weight-gradient GEMMs
-//   k_wgrad       dW_l = dZ_l^T A_{l-1} on MFMA, split over row chunks
-//                 (partials), plus bias gradients
-//   k_adam        fixed-order reduction of the partials + Adam + bf16 copy
+//   k_infer       fully fused forward in the transposed form Z^T = W^T-tiles
+//                 x X^T: the accumulator of one layer (lane = row, 4
+//                 consecutive features) is, after bias + ReLU + bf16 packing,
+//                 exactly the B operand of the next, so activations never
+//                 leave registers; weights sit in LDS in per-lane fragment
+//                 order; each lane ends holding two whole lobes and the
+//                 softmax needs two cross-lane exchanges
+//   k_train       one Adam step's rows (given, or gathered from the tracer's
+//                 records): forward + loss + backward register resident like
+//                 k_infer, and the weight gradients dW_l = dZ_l^T X_l on MFMA
+//                 from a per-layer LDS stage, accumulated per workgroup
+//   k_adam        fixed-order reduction of the workgroup partials, Adam, bf16
+//                 copies
 //
-// MFMA 16x16x16 bf16 lane layout (wave64, lane = i + 16 j): A operand lane
-// holds A[i][4j..4j+3], B operand lane holds B[4j..4j+3][i], accumulator lane
-// holds D[4j..4j+3][i].  16x16x32 is the same with 8 K values per lane
-// (A[i][8j..8j+7], B[8j..8j+7][i]) and the same accumulator layout; the
-// contraction is order-free, so A and B may place any K permutation in those
-// slots as long as both use the same one.
+// MFMA 16x16x32 bf16 lane layout (wave64, lane = i + 16 j): A operand lane
+// holds A[i][8j..8j+7], B operand lane holds B[8j..8j+7][i], accumulator lane
+// holds D[4j..4j+3][i].  The contraction is order-free, so A and B may place
+// any K permutation in those slots as long as both use the same one.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -38,7 +35,6 @@
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -120,7 +116,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // Lobe concentration from its raw output s: kappa = 0.1 + e^min(s, 7.6)
 // (0.1 .. ~2000).  The log scale lets a lobe sharpen to the ~0.01 sr targets
@@ -146,24 +141,29 @@ __device__ __forceinline__ float encode(const float* v, int f) {
     return f == kIn + kIn * 2 * kFreq ? 1.0f : 0.0f;
 }
 
-__device__ __forceinline__ bf16x4 ld4(const uint16_t* p) {
-    const uint2 u = *reinterpret_cast<const uint2*>(p);  // 8-byte aligned
-    bf16x4 r;
-    r[0] = (short)(u.x & 0xffffu);
-    r[1] = (short)(u.x >> 16);
-    r[2] = (short)(u.y & 0xffffu);
-    r[3] = (short)(u.y >> 16);
-    return r;
-}
-
 // log vMF density at cosine c = mu . w:  log(k / (2 pi (1 - e^{-2k}))) + k (c - 1)
 __device__ inline float log_vmf(float kappa, float c) {
     return logf(kappa) - logf(6.283185307179586f * (-expm1f(-2.0f * kappa))) + kappa * (c - 1.0f);
 }
 
-constexpr int kTileRows = 16;
-constexpr int kWaves = 4;
-constexpr int kBlock = 64 * kWaves;
+// Copies N16 16-byte words from global memory to LDS, each thread issuing
+// all of its loads before its first store (one memory round trip per block
+// instead of one per word).
+template <int BLOCK, int N16>
+__device__ __forceinline__ void copy_to_lds(uint4* __restrict__ dst, const uint4* __restrict__ src) {
+    constexpr int kPer = (N16 + BLOCK - 1) / BLOCK;
+    uint4 r[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int e = (int)threadIdx.x + k * BLOCK;
+        if (e < N16) r[k] = src[e];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int e = (int)threadIdx.x + k * BLOCK;
+        if (e < N16) dst[e] = r[k];
+    }
+}
 
 // ---------------------------------------------------------------------------
 // inference
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restric
     float* s_b = reinterpret_cast<float*>(smem + sizeof(uint4) * kFrags * 64);
     if (n_dev) n = *n_dev;
     if (blockIdx.x * (uint32_t)(kInferWaves * kInferRows) >= n) return;  // no tile for this block
-    for (int e = threadIdx.x; e < kFrags * 64; e += kInferBlock) s_f[e] = reinterpret_cast<const uint4*>(frag)[e];
+    copy_to_lds<kInferBlock, kFrags * 64>(s_f, reinterpret_cast<const uint4*>(frag));
     for (int e = threadIdx.x; e < kBiasElems; e += kInferBlock) {
         const int p = e - kHidden * kW;
         s_b[e] = p < 0 ? bias[e] : bias[kHidden * kW + out_logical(p)];
@@ -320,23 +320,63 @@ __global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restric
 }
 
 // ---------------------------------------------------------------------------
-// training: fused forward + loss + backward data path
+// training: one fused launch per Adam step (rows -> forward -> loss ->
+// backward -> weight gradients), then k_adam
 // ---------------------------------------------------------------------------
-// Global scratch (feature-major, bf16): A[l][64][n_pad] for l = 0..5 (0 is
-// the encoding, 1..5 hidden outputs) and G[l][nout_l][n_pad] for l = 0..5
-// (dZ of layer l, the output layer's rows in logical order).  n_pad is a
-// multiple of 16.
+// A workgroup of WAVES waves takes kGroupRows = 16 WAVES rows per iteration, 16
+// per wave, in the register-resident form of k_infer: the forward keeps each
+// layer's packed activations x[l] (the next layer's B operand); each lane
+// computes the loss gradient of its two lobes (lane 0 of a row also that of
+// the selection logits); the backward runs dH^T = W^T-tiles x dZ^T on the
+// transposed fragments, the ReLU mask from the activation registers.
 //
-// Same register-resident form as k_infer, one 16-row tile per wave
-// iteration: the forward keeps every layer's packed activations in
-// registers; each lane computes the loss gradient of its two lobes (lane 0
-// of a row also that of the selection logits); the backward pass runs
-// dH^T = W^T-tiles x dZ^T on the transposed fragments, and the ReLU mask
-// comes from the activation registers.  A and dZ go to global memory for the
-// weight-gradient GEMMs.
-constexpr int kTrainWaves = 4;
-constexpr int kTrainBlock = 64 * kTrainWaves;
-constexpr size_t kTrainSmem = sizeof(uint4) * (kFrags + kFragsT) * 64 + sizeof(float) * kBiasElems;
+// Weight gradients dWt_l = dZ_l^T X_l contract over rows, which the
+// registers hold one per lane; so per layer every wave stages its x[l] and
+// dZ_l feature-major in LDS (a 16 x 16 transpose per fragment), and after one
+// barrier wave w forms output tile w of dWt_l for all of the group's rows
+// with v_mfma_f32_16x16x32_bf16 (A = dZ^T rows of LDS, B = X rows of LDS).
+// The stage is double buffered, so one barrier per layer separates a
+// buffer's reads from its next writes.  Each wave keeps its tile's
+// accumulators for all six layers across iterations; the workgroup writes
+// one partial (weight and bias gradients, sum of weights, sum of losses) at
+// the end.  Nothing per row goes to global memory.
+//
+// The gradient is that of the weighted loss sum_r w_r l_r; k_adam divides it
+// by sum_r w_r (Adam's step is nearly invariant to that scale, the reported
+// gradient and loss are those of the normalised weights).
+//
+// Two shapes: 4 waves (64 rows per iteration, 1 wave per SIMD, no spills in
+// 385 registers) finish a small step sooner; 8 waves (128 rows, 2 waves per
+// SIMD, each wave half the dWt tiles) run large steps faster.
+constexpr size_t kTrainFragBytes = sizeof(uint4) * (kFrags + kFragsT) * 64 + sizeof(float) * kBiasElems;
+template <int WAVES>
+struct TrainShape {
+    static constexpr int kBlock = 64 * WAVES;
+    static constexpr int kGroupRows = 16 * WAVES;          // rows per workgroup iteration
+    static constexpr int kKBlocks = kGroupRows / 32;       // 32-row MFMA K blocks per stage
+    static constexpr int kWTiles = 16 / WAVES;             // dWt output tiles (of 16 per layer) per wave
+    static constexpr int kStageStride = kGroupRows + 8;    // bf16 per feature row (+16 B spreads LDS banks)
+    static constexpr int kStageOperand = kW * kStageStride;
+    static constexpr size_t kSmem = kTrainFragBytes + 2 * 2 * sizeof(uint16_t) * kStageOperand;
+    static_assert(kTrainFragBytes % 16 == 0 && (kStageStride * 2) % 16 == 0, "16-byte LDS reads");
+    static_assert(kSmem <= 160 * 1024, "LDS");
+};
+constexpr uint32_t kTrainWideRows = 16384;  // steps above this many rows use 8 waves
+constexpr int kPartial = kParams + 16;  // per workgroup: gradients, then sum w, sum w * loss
+constexpr int kMaxChunks = 1024;         // partial slots (workgroups of one step)
+
+// Rows of one step: the rows themselves, or (GATHER) rows drawn uniformly
+// with replacement from min(*count, cap) records by Philox keyed by the
+// seed, counter (row, step, chunk), so that every bounce's records are
+// represented whatever order they were appended in.
+struct TrainRows {
+    const float* in;    // 8 floats per row / record
+    const float* dirs;  // 3
+    const float* w;     // 1
+    uint32_t n;         // rows in the step
+    const uint32_t* count;
+    uint32_t cap, seed_lo, seed_hi, chunk, step;
+};
 
 // Bits of element e of a bf16 vector (by a whole-vector bit cast: a bit cast
 // of one element of the __bf16 vector stored the wrong elements).
@@ -350,36 +390,67 @@ __device__ __forceinline__ bf16x8 pack8(const float* v) {
     return r;
 }
 
-__global__ __launch_bounds__(kTrainBlock) void k_train_fb(const uint16_t* __restrict__ frag,
-                                                          const uint16_t* __restrict__ fragT,
-                                                          const float* __restrict__ bias, const float* __restrict__ in,
-                                                          const float* __restrict__ dirs,
-                                                          const float* __restrict__ weight, uint32_t n, uint32_t n_pad,
-                                                          const float* __restrict__ inv_wsum, uint16_t* __restrict__ A,
-                                                          uint16_t* __restrict__ G, float* __restrict__ row_loss) {
+template <bool GATHER, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_train(const uint16_t* __restrict__ frag,
+                                                       const uint16_t* __restrict__ fragT,
+                                                       const float* __restrict__ bias, TrainRows R,
+                                                       float* __restrict__ partial) {
+    using T = TrainShape<WAVES>;
+    constexpr int kTrainBlock = T::kBlock, kGroupRows = T::kGroupRows, kKBlocks = T::kKBlocks,
+                  kWTiles = T::kWTiles, kStageStride = T::kStageStride, kStageOperand = T::kStageOperand;
     extern __shared__ uint8_t smem[];
+    __shared__ float s_red[2 * WAVES];
     uint4* s_f = reinterpret_cast<uint4*>(smem);
     uint4* s_t = s_f + kFrags * 64;
     float* s_b = reinterpret_cast<float*>(s_t + kFragsT * 64);
-    for (int e = threadIdx.x; e < kFrags * 64; e += kTrainBlock) s_f[e] = reinterpret_cast<const uint4*>(frag)[e];
-    for (int e = threadIdx.x; e < kFragsT * 64; e += kTrainBlock) s_t[e] = reinterpret_cast<const uint4*>(fragT)[e];
+    uint16_t* s_stage = reinterpret_cast<uint16_t*>(smem + kTrainFragBytes);
+    copy_to_lds<kTrainBlock, kFrags * 64>(s_f, reinterpret_cast<const uint4*>(frag));
+    copy_to_lds<kTrainBlock, kFragsT * 64>(s_t, reinterpret_cast<const uint4*>(fragT));
     for (int e = threadIdx.x; e < kBiasElems; e += kTrainBlock) {
         const int p = e - kHidden * kW;
         s_b[e] = p < 0 ? bias[e] : bias[kHidden * kW + out_logical(p)];
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
-    const float wnorm = *inv_wsum;
-    const uint32_t tiles = n_pad / kTileRows;
-    for (uint32_t tile = blockIdx.x * kTrainWaves + wave; tile < tiles; tile += gridDim.x * kTrainWaves) {
+    const uint32_t nrec = GATHER ? min(*R.count, R.cap) : 0u;
+    const uint32_t groups = (R.n + kGroupRows - 1) / kGroupRows;
+    // wave w forms output tiles (n tile w % 4, k tiles kWTiles (w / 4) + u) of
+    // every layer's dWt; the waves of n tile w % 4 with w < 4 also its bias
+    const int wn = wave & 3, wk = kWTiles * (wave >> 2);
+    // per-lane stage bases: st_w writes row 16 wave + i of feature 4 j (+ a
+    // constant); st_a reads the A operand (feature 16 wn + i, rows 8 j ..),
+    // st_b the B operand (feature 16 wk + i, rows 8 j ..)
+    uint16_t* const st_w = s_stage + 4 * j * kStageStride + 16 * wave + i;
+    const uint16_t* const st_a = s_stage + (16 * wn + i) * kStageStride + 8 * j;
+    const uint16_t* const st_b = s_stage + (16 * wk + i) * kStageStride + 8 * j;
+    f32x4 accW[kLayers][kWTiles];
+    float accB[kLayers];
+#pragma unroll
+    for (int l = 0; l < kLayers; ++l) {
+        accB[l] = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kWTiles; ++u) accW[l][u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    float sum_w = 0.0f, sum_loss = 0.0f;
+    for (uint32_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform over the workgroup
         asm volatile("" ::: "memory");  // fragment reads stay inside the loop (as in k_infer)
-        const uint32_t row = tile * kTileRows + i;
-        const bool valid = row < n;
+        const uint32_t row = grp * kGroupRows + 16 * wave + i;
+        const bool valid = row < R.n && (!GATHER || nrec > 0u);
+        uint32_t src = row;
+        if (GATHER && valid) {
+            const lr::u4 h = lr::philox4x32(R.seed_lo ^ 0x5bd1e995u, R.seed_hi, lr::u4{row, R.step, R.chunk, 0x6e1du});
+            src = (uint32_t)(((uint64_t)h.x * nrec) >> 32);
+        }
         float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float wx = 0.0f, wy = 0.0f, wz = 1.0f, w = 0.0f;
         if (valid) {
-            const float4 a = reinterpret_cast<const float4*>(in)[2 * (size_t)row];
-            const float4 b = reinterpret_cast<const float4*>(in)[2 * (size_t)row + 1];
+            const float4 a = reinterpret_cast<const float4*>(R.in)[2 * (size_t)src];
+            const float4 b = reinterpret_cast<const float4*>(R.in)[2 * (size_t)src + 1];
             v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            wx = R.dirs[3 * (size_t)src];
+            wy = R.dirs[3 * (size_t)src + 1];
+            wz = R.dirs[3 * (size_t)src + 2];
+            w = R.w[src];
         }
         // forward: x[0] the encoding, x[l] the output of hidden layer l - 1
         bf16x8 x[kLayers][2];
@@ -390,6 +461,9 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_fb(const uint16_t* __rest
         f32x4 acc[4];
 #pragma unroll
         for (int l = 0; l < kHidden; ++l) {
+            // each layer reads its own fragments: scheduled early, all 86 would
+            // need 344 registers
+            asm volatile("" ::: "memory");
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 acc[t] = bias4(s_b, l * kW + 16 * t + 4 * j);
@@ -401,6 +475,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_fb(const uint16_t* __rest
             x[l + 1][0] = pack_relu(acc[0], acc[1]);
             x[l + 1][1] = pack_relu(acc[2], acc[3]);
         }
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             acc[t] = bias4(s_b, kHidden * kW + 16 * t + 4 * j);
@@ -415,9 +490,6 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_fb(const uint16_t* __rest
         for (int sl = 0; sl < 12; ++sl) o[sl] = acc[sl >> 2][sl & 3];
 
         // loss and its gradient: this lane's lobes 2j, 2j + 1 (slots 0..4, 5..9)
-        const float wx = valid ? dirs[3 * (size_t)row] : 0.0f;
-        const float wy = valid ? dirs[3 * (size_t)row + 1] : 0.0f;
-        const float wz = valid ? dirs[3 * (size_t)row + 2] : 1.0f;
         float mx = fmaxf(o[0], o[5]);
         mx = fmaxf(mx, __shfl_xor(mx, 16));
         mx = fmaxf(mx, __shfl_xor(mx, 32));
@@ -444,12 +516,11 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_fb(const uint16_t* __rest
         ls += __shfl_xor(ls, 16);
         ls += __shfl_xor(ls, 32);
         const float logq = lmax + logf(ls);
-        const float w = valid ? weight[row] * wnorm : 0.0f;
         float g[12];
 #pragma unroll
         for (int sl = 0; sl < 12; ++sl) g[sl] = 0.0f;
-        float loss = -w * logq;
         if (valid) {
+            float loss = -w * logq;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const float resp = expf(lp[h] - logq);  // responsibility
@@ -477,108 +548,169 @@ __global__ __launch_bounds__(kTrainBlock) void k_train_fb(const uint16_t* __rest
                 g[10] = f * rl * (1.0f - rl) * (pl - rg * pg - (1.0f - rg) * pb);
                 g[11] = f * (1.0f - rl) * rg * (1.0f - rg) * (pg - pb);
             }
-            if (j == 0) row_loss[row] = loss;
+            if (j == 0) {
+                sum_w += w;
+                sum_loss += loss;
+            }
         }
-        // activations -> A[l] (feature-major)
-#pragma unroll
-        for (int l = 0; l < kLayers; ++l) {
-            uint16_t* Al = A + (size_t)l * kW * n_pad + row;
-#pragma unroll
-            for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-                for (int e = 0; e < 8; ++e)
-                    Al[(size_t)kslot_feature(kb, j, e) * n_pad] = bf_bits(x[l][kb], e);
-        }
-        // output-layer dZ (bf16) -> G[5] in logical row order, and as the B operand
+        // output-layer dZ in bf16: dz[kb][e] is physical output kslot_feature(kb, j, e)
         float gb[16];
 #pragma unroll
-        for (int sl = 0; sl < 12; ++sl) gb[sl] = (float)(__bf16)g[sl];
+        for (int sl = 0; sl < 12; ++sl) gb[sl] = g[sl];
 #pragma unroll
         for (int sl = 12; sl < 16; ++sl) gb[sl] = 0.0f;
-        {
-            uint16_t* G5 = G + (size_t)kHidden * kW * n_pad + row;
-#pragma unroll
-            for (int sl = 0; sl < 12; ++sl)
-                G5[(size_t)out_logical(16 * (sl >> 2) + 4 * j + (sl & 3)) * n_pad] =
-                    __builtin_bit_cast(uint16_t, (__bf16)gb[sl]);
-        }
         bf16x8 dz[2] = {pack8(gb), pack8(gb + 8)};
-        // backward: dH_l = W_l^T dZ_l, dZ_{l-1} = dH_l * (x[l] > 0)
+        // per layer, from the output layer down: stage X_l and dZ_l, form the
+        // weight gradients, then dZ_{l-1} = (W_l^T dZ_l) * (x[l] > 0)
 #pragma unroll
-        for (int l = kLayers - 1; l >= 1; --l) {
+        for (int l = kLayers - 1; l >= 0; --l) {
+            // stage buffer l & 1: X_l, then dZ_l; every offset below past the
+            // per-lane bases is a compile-time constant
+            const int buf = (l & 1) * 2 * kStageOperand;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int kb = 0; kb < 2; ++kb) {
+                // element pair (2h, 2h + 1) of a packed vector is one dword:
+                // low half to feature kslot_feature(kb, j, 2h), high half to the next
+                const uint4 ux = __builtin_bit_cast(uint4, x[l][kb]), ud = __builtin_bit_cast(uint4, dz[kb]);
+                const uint32_t wx4[4] = {ux.x, ux.y, ux.z, ux.w}, wd4[4] = {ud.x, ud.y, ud.z, ud.w};
 #pragma unroll
-                for (int kb = 0; kb < 2; ++kb)
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf16x8, s_t[fragT_id(l, t, kb) * 64 + lane]), dz[kb], acc[t], 0, 0, 0);
-            }
-            float d[16];
-            uint16_t* Gl = G + (size_t)(l - 1) * kW * n_pad + row;
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const float hv = (float)x[l][t >> 1][(t & 1) * 4 + r];
-                    const __bf16 b = (__bf16)(hv > 0.0f ? acc[t][r] : 0.0f);
-                    d[4 * t + r] = (float)b;
-                    Gl[(size_t)(16 * t + 4 * j + r) * n_pad] = __builtin_bit_cast(uint16_t, b);
+                for (int h = 0; h < 4; ++h) {
+                    const int f0 = kslot_feature(kb, 0, 2 * h);  // feature less its 4 j
+                    uint16_t* px = st_w + buf + f0 * kStageStride;
+                    px[0] = (uint16_t)wx4[h];
+                    px[kStageStride] = (uint16_t)(wx4[h] >> 16);
+                    px[kStageOperand] = (uint16_t)wd4[h];
+                    px[kStageOperand + kStageStride] = (uint16_t)(wd4[h] >> 16);
                 }
-            dz[0] = pack8(d);
-            dz[1] = pack8(d + 8);
+            }
+            __syncthreads();
+            if (16 * wn < layer_nout(l)) {
+#pragma unroll
+                for (int rb = 0; rb < kKBlocks; ++rb) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(st_a + buf + kStageOperand + 32 * rb);
+#pragma unroll
+                    for (int u = 0; u < kWTiles; ++u) {
+                        const bf16x8 b =
+                            *reinterpret_cast<const bf16x8*>(st_b + buf + 16 * u * kStageStride + 32 * rb);
+                        accW[l][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, accW[l][u], 0, 0, 0);
+                    }
+                    if (wk == 0) {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) accB[l] += (float)a[e];
+                    }
+                }
+            }
+            if (l >= 1) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int kb = 0; kb < 2; ++kb)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, s_t[fragT_id(l, t, kb) * 64 + lane]), dz[kb], acc[t], 0, 0, 0);
+                }
+                float d[16];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float hv = (float)x[l][t >> 1][(t & 1) * 4 + q];
+                        d[4 * t + q] = hv > 0.0f ? acc[t][q] : 0.0f;
+                    }
+                dz[0] = pack8(d);
+                dz[1] = pack8(d + 8);
+            }
         }
     }
-}
-
-// dWt_l[n][k] = sum_rows G_l[n][row] * A_l[k][row] over this block's row chunk;
-// bias gradient db_l[n] = sum_rows G_l[n][row].  One block per (layer, chunk);
-// wave w computes output rows n in [16 w, 16 w + 16) for all 4 k tiles.  The
-// bias sums come from the G values each lane already holds as its MFMA A
-// operand, reduced over the 4 lanes of a row at the end.
-__global__ __launch_bounds__(kBlock) void k_wgrad(const uint16_t* __restrict__ A, const uint16_t* __restrict__ G,
-                                                  uint32_t n_pad, uint32_t chunk_rows, float* __restrict__ partial) {
-    const int l = blockIdx.y;
-    const uint32_t chunk = blockIdx.x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
-    const int nout = layer_nout(l);
-    const uint16_t* Gl = G + (size_t)l * kW * n_pad;
-    const uint16_t* Al = A + (size_t)l * kW * n_pad;
-    float* out = partial + (size_t)chunk * kParams;
-    const uint32_t r0 = chunk * chunk_rows;
-    const uint32_t r1 = min(n_pad, r0 + chunk_rows);
-    if (16 * wave < nout) {
-        f32x4 acc[4];
-        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        float bsum = 0.0f;
-        for (uint32_t row = r0; row < r1; row += 16) {
-            const bf16x4 a = ld4(Gl + (size_t)(16 * wave + i) * n_pad + row + 4 * j);  // A_mfma[n][row]
+    // partial of this workgroup: accW[l][u][q] = dWt_l[physical 16 wn + 4 j + q][16 (wk + u) + i]
+    float* out = partial + (size_t)blockIdx.x * kPartial;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const bf16x4 b = ld4(Al + (size_t)(16 * t + i) * n_pad + row + 4 * j);  // B_mfma[row][k]
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) bsum += bf2f((uint16_t)a[e]);
-        }
+    for (int l = 0; l < kLayers; ++l) {
+        float bsum = accB[l];
         bsum += __shfl_xor(bsum, 16);
         bsum += __shfl_xor(bsum, 32);
-        // acc[t][r] = dWt[16 wave + 4 j + r][16 t + i]
-        for (int t = 0; t < 4; ++t)
-            for (int r = 0; r < 4; ++r)
-                out[wt_offset(l) + (16 * wave + 4 * j + r) * kW + 16 * t + i] = acc[t][r];
-        if (j == 0) out[bias_offset(l) + 16 * wave + i] = bsum;
+        if (16 * wn < layer_nout(l)) {
+#pragma unroll
+            for (int u = 0; u < kWTiles; ++u)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int p = 16 * wn + 4 * j + q;
+                    const int n = l < kHidden ? p : out_logical(p);
+                    out[wt_offset(l) + n * kW + 16 * (wk + u) + i] = accW[l][u][q];
+                }
+            if (j == 0 && wk == 0) {
+                const int p = 16 * wn + i;
+                out[bias_offset(l) + (l < kHidden ? p : out_logical(p))] = bsum;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        sum_w += __shfl_xor(sum_w, m);
+        sum_loss += __shfl_xor(sum_loss, m);
+    }
+    if (lane == 0) {
+        s_red[wave] = sum_w;
+        s_red[WAVES + wave] = sum_loss;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a = 0.0f, b = 0.0f;
+        for (int k = 0; k < WAVES; ++k) {
+            a += s_red[k];
+            b += s_red[WAVES + k];
+        }
+        out[kParams] = a;
+        out[kParams + 1] = b;
     }
 }
 
-// Fixed-order sum of the chunk partials, Adam, bf16 copy.
-__global__ void k_adam(const float* __restrict__ partial, uint32_t chunks, float* __restrict__ w,
-                       float* __restrict__ m, float* __restrict__ v, uint16_t* __restrict__ frag, uint16_t* __restrict__ fragT, float lr, float b1, float b2,
-                       float eps, float c1, float c2, float* __restrict__ grad_out) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (uint32_t)kParams) return;
-    float g = 0.0f;
-    for (uint32_t c = 0; c < chunks; ++c) g += partial[(size_t)c * kParams + p];
+// Fixed-order sum of the workgroup partials, normalised by the batch's
+// weight sum; Adam; both bf16 copies.  A block takes kAdamParams parameters;
+// its 4 x 64 threads each sum every 4th partial (4 loads in flight), then
+// the four sums are added in a fixed order.  loss_out (block 0): the
+// normalised loss.
+constexpr int kAdamParams = 64;
+__global__ __launch_bounds__(256) void k_adam(const float* __restrict__ partial, uint32_t chunks,
+                                              float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                                              uint16_t* __restrict__ frag, uint16_t* __restrict__ fragT, float lr,
+                                              float b1, float b2, float eps, float c1, float c2,
+                                              float* __restrict__ grad_out, float* __restrict__ loss_out) {
+    __shared__ float s_ws[256], s_ls[256], s_g[256];
+    float a = 0.0f, b = 0.0f;
+    for (uint32_t c = threadIdx.x; c < chunks; c += 256) {
+        a += partial[(size_t)c * kPartial + kParams];
+        b += partial[(size_t)c * kPartial + kParams + 1];
+    }
+    s_ws[threadIdx.x] = a;
+    s_ls[threadIdx.x] = b;
+    const uint32_t pl = threadIdx.x & (kAdamParams - 1), cl = threadIdx.x / kAdamParams;
+    const uint32_t p = blockIdx.x * kAdamParams + pl;
+    float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+    if (p < (uint32_t)kParams) {
+        const float* q = partial + p;
+        uint32_t c = cl;
+        for (; c + 12 < chunks; c += 16) {
+            g0 += q[(size_t)c * kPartial];
+            g1 += q[(size_t)(c + 4) * kPartial];
+            g2 += q[(size_t)(c + 8) * kPartial];
+            g3 += q[(size_t)(c + 12) * kPartial];
+        }
+        for (; c < chunks; c += 4) g0 += q[(size_t)c * kPartial];
+    }
+    s_g[threadIdx.x] = (g0 + g1) + (g2 + g3);
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            s_ws[threadIdx.x] += s_ws[threadIdx.x + k];
+            s_ls[threadIdx.x] += s_ls[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    const float inv = s_ws[0] > 0.0f ? 1.0f / s_ws[0] : 0.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) *loss_out = s_ls[0] * inv;
+    if (cl != 0 || p >= (uint32_t)kParams) return;
+    const float g = ((s_g[pl] + s_g[pl + 64]) + (s_g[pl + 128] + s_g[pl + 192])) * inv;
     if (grad_out) grad_out[p] = g;
     if (lr == 0.0f) return;  // gradient only
     const float mm = b1 * m[p] + (1.0f - b1) * g;
@@ -595,53 +727,6 @@ __global__ void k_adam(const float* __restrict__ partial, uint32_t chunks, float
     }
 }
 
-// Fixed-order sum of x[0..n) by one block of 1024 threads: each thread sums
-// a strided set of 4-wide groups (4 loads in flight), then a tree in LDS.
-// Elements before the first 16-byte boundary are summed on their own.
-__device__ float block_sum(const float* __restrict__ x, uint32_t n) {
-    __shared__ float s[1024];
-    const uint32_t t = threadIdx.x;
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-    const uint32_t head = min(n, (uint32_t)((4u - ((uint32_t)(reinterpret_cast<uintptr_t>(x) >> 2) & 3u)) & 3u));
-    if (t < head) a2 = x[t];
-    x += head;
-    n -= head;
-    const uint32_t n4 = n >> 2;
-    const float4* x4 = reinterpret_cast<const float4*>(x);
-    uint32_t g = t;
-    for (; g + 3 * 1024 < n4; g += 4 * 1024) {
-        const float4 p = x4[g], q = x4[g + 1024], r = x4[g + 2048], u = x4[g + 3072];
-        a0 += (p.x + p.y) + (p.z + p.w);
-        a1 += (q.x + q.y) + (q.z + q.w);
-        a2 += (r.x + r.y) + (r.z + r.w);
-        a3 += (u.x + u.y) + (u.z + u.w);
-    }
-    for (; g < n4; g += 1024) {
-        const float4 p = x4[g];
-        a0 += (p.x + p.y) + (p.z + p.w);
-    }
-    if (t < (n & 3u)) a1 += x[4 * n4 + t];
-    s[t] = (a0 + a1) + (a2 + a3);
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)t < w) s[t] += s[t + w];
-        __syncthreads();
-    }
-    return s[0];
-}
-
-// 1 / sum(weights)
-__global__ __launch_bounds__(1024) void k_wsum(const float* __restrict__ weight, uint32_t n, float* __restrict__ inv_wsum) {
-    const float total = block_sum(weight, n);
-    if (threadIdx.x == 0) *inv_wsum = total > 0.0f ? 1.0f / total : 0.0f;
-}
-
-__global__ __launch_bounds__(1024) void k_sum(const float* __restrict__ x, uint32_t n, float* __restrict__ out) {
-    const float total = block_sum(x, n);
-    if (threadIdx.x == 0) *out = total;
-}
-
-
 }  // namespace
 
 struct lr_guide {
@@ -653,14 +738,8 @@ struct lr_guide {
     uint16_t* fragT = nullptr; // bf16 copy of layers 1..5 in transposed fragment order
     uint32_t step = 0;
     float lr = 1e-3f;
-    // training scratch (grow-only)
-    size_t cap_rows = 0;
-    uint16_t* A = nullptr;
-    uint16_t* G = nullptr;
-    float* partial = nullptr;
-    uint32_t cap_chunks = 0;
-    float* row_loss = nullptr;
-    float* scalars = nullptr;  // [0] 1/sum w, [1] loss sum
+    float* partial = nullptr;  // training: one kPartial slot per workgroup (max_chunks)
+    uint32_t max_chunks = 0;
 };
 
 namespace {
@@ -668,7 +747,7 @@ namespace {
 void free_guide(lr_guide* g) {
     if (!g) return;
     (void)hipSetDevice(g->ctx->device);
-    void* ptrs[] = {g->w, g->m, g->v, g->frag, g->fragT, g->A, g->G, g->partial, g->row_loss, g->scalars};
+    void* ptrs[] = {g->w, g->m, g->v, g->frag, g->fragT, g->partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete g;
@@ -710,18 +789,27 @@ int lr_guide_create(lr_context* ctx, uint64_t seed, float lr, lr_guide** out) {
     if (int rc = lr::bind(ctx)) return rc;
     LR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_infer),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInferSmem));
-    LR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_train_fb),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTrainSmem));
+    {
+        const void* fns[] = {reinterpret_cast<const void*>(k_train<false, 4>),
+                             reinterpret_cast<const void*>(k_train<true, 4>),
+                             reinterpret_cast<const void*>(k_train<false, 8>),
+                             reinterpret_cast<const void*>(k_train<true, 8>)};
+        const size_t smem[] = {TrainShape<4>::kSmem, TrainShape<4>::kSmem, TrainShape<8>::kSmem, TrainShape<8>::kSmem};
+        for (int k = 0; k < 4; ++k)
+            LR_HIP_CHECK(hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem[k]));
+    }
     lr_guide* g = new (std::nothrow) lr_guide();
     if (!g) return lr::fail(LR_E_NOMEM, "out of host memory");
     g->ctx = ctx;
     g->lr = lr;
+    // a training step runs at most one workgroup per CU: one partial each
+    g->max_chunks = (uint32_t)std::max(1, std::min(ctx->n_cu, kMaxChunks));
     hipError_t e = hipMalloc(&g->w, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMalloc(&g->m, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMalloc(&g->v, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMalloc(&g->frag, sizeof(uint16_t) * kWtElems);
     if (e == hipSuccess) e = hipMalloc(&g->fragT, sizeof(uint16_t) * kFragsT * 512);
-    if (e == hipSuccess) e = hipMalloc(&g->scalars, sizeof(float) * 16);
+    if (e == hipSuccess) e = hipMalloc(&g->partial, sizeof(float) * kPartial * (size_t)g->max_chunks);
     if (e == hipSuccess) e = hipMemset(g->m, 0, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMemset(g->v, 0, sizeof(float) * kParams);
     if (e != hipSuccess) {
@@ -789,50 +877,41 @@ int lr_guide_infer(lr_guide* g, const float* inputs, uint32_t n, const uint32_t*
 
 namespace {
 
-// Row chunking of the weight-gradient pass: 16 .. 64 chunks of >= 256 rows,
-// enough blocks to spread a small batch over the chip, few partials for the
-// Adam reduction.
-void wgrad_chunks(uint32_t n_pad, uint32_t& chunk_rows, uint32_t& chunks) {
-    const uint32_t want = std::min<uint32_t>(64, std::max<uint32_t>(16, n_pad / 1024));
-    chunk_rows = std::max<uint32_t>(256, ((n_pad + want - 1) / want + 15) & ~15u);
-    chunks = (n_pad + chunk_rows - 1) / chunk_rows;
-}
-
-int grow_training(lr_guide* g, uint32_t n_pad, uint32_t chunks, hipStream_t s) {
-    if (g->A && g->G && g->partial && g->row_loss && g->cap_rows >= n_pad && g->cap_chunks >= chunks)
-        return LR_OK;
-    n_pad = std::max<uint32_t>(n_pad, (uint32_t)g->cap_rows);
-    chunks = std::max(chunks, g->cap_chunks);
-    {
-        LR_HIP_CHECK(hipStreamSynchronize(s));
-        if (g->A) LR_HIP_CHECK(hipFree(g->A));
-        if (g->G) LR_HIP_CHECK(hipFree(g->G));
-        if (g->partial) LR_HIP_CHECK(hipFree(g->partial));
-        if (g->row_loss) LR_HIP_CHECK(hipFree(g->row_loss));
-        g->A = g->G = nullptr;
-        g->partial = g->row_loss = nullptr;
-        // capacities drop with the buffers: a failed allocation below leaves
-        // a state that the next call grows again
-        g->cap_rows = 0;
-        g->cap_chunks = 0;
-        LR_HIP_CHECK(hipMalloc(&g->A, sizeof(uint16_t) * kLayers * kW * (size_t)n_pad));
-        LR_HIP_CHECK(hipMalloc(&g->G, sizeof(uint16_t) * kLayers * kW * (size_t)n_pad));
-        LR_HIP_CHECK(hipMalloc(&g->partial, sizeof(float) * kParams * (size_t)chunks));
-        LR_HIP_CHECK(hipMalloc(&g->row_loss, sizeof(float) * (size_t)n_pad));
-        g->cap_rows = n_pad;
-        g->cap_chunks = chunks;
-    }
+// One Adam step on the rows R: k_train (one workgroup per row group, at most
+// one per CU) and k_adam.  No allocation: the partials were allocated
+// with the guide.
+int train_rows(lr_guide* g, const TrainRows& R, bool gather, float* loss_out, float* grad_out, hipStream_t s) {
+    const bool wide = R.n > kTrainWideRows;
+    const uint32_t rows = wide ? TrainShape<8>::kGroupRows : TrainShape<4>::kGroupRows;
+    const uint32_t groups = (R.n + rows - 1) / rows;
+    const uint32_t grid = std::max(1u, std::min(groups, g->max_chunks));
+    auto launch = [&](auto fn, int waves, size_t smem) {
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * waves), smem, s, g->frag, g->fragT, g->w + kWtElems, R, g->partial);
+    };
+    if (wide)
+        gather ? launch(k_train<true, 8>, 8, TrainShape<8>::kSmem) : launch(k_train<false, 8>, 8, TrainShape<8>::kSmem);
+    else
+        gather ? launch(k_train<true, 4>, 4, TrainShape<4>::kSmem) : launch(k_train<false, 4>, 4, TrainShape<4>::kSmem);
+    g->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)g->step), c2 = 1.0f - std::pow(b2, (float)g->step);
+    hipLaunchKernelGGL(k_adam, dim3((kParams + kAdamParams - 1) / kAdamParams), dim3(256), 0, s, g->partial, grid,
+                       g->w, g->m, g->v,
+                       g->frag, g->fragT, g->lr, b1, b2, 1e-8f, c1, c2, grad_out, loss_out);
+    LR_HIP_CHECK(hipGetLastError());
     return LR_OK;
 }
 
 }  // namespace
 
-int lr_guide_reserve(lr_guide* g, uint32_t n, hipStream_t s) {
-    if (!g || n == 0) return LR_OK;
-    const uint32_t n_pad = (n + 15) & ~15u;
-    uint32_t chunk_rows, chunks;
-    wgrad_chunks(n_pad, chunk_rows, chunks);
-    return grow_training(g, n_pad, chunks, s);
+int lr_guide_train_records(lr_guide* g, const float* r_in, const float* r_dir, const float* r_w,
+                           const uint32_t* count, uint32_t cap, uint64_t seed, uint32_t chunk, uint32_t step,
+                           uint32_t batch, hipStream_t s) {
+    if (!g || !r_in || !r_dir || !r_w || !count) return lr::fail(LR_E_INVALID, "NULL argument");
+    if (batch == 0) return LR_OK;
+    if (int rc = lr::bind(g->ctx)) return rc;
+    const TrainRows R{r_in, r_dir, r_w, batch, count, cap, (uint32_t)seed, (uint32_t)(seed >> 32), chunk, step};
+    return train_rows(g, R, true, nullptr, nullptr, s);
 }
 
 extern "C" {
@@ -842,28 +921,8 @@ int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, con
     if (!g || (n && (!inputs || !dirs || !weights))) return lr::fail(LR_E_INVALID, "NULL argument");
     if (n == 0) return LR_OK;
     if (int rc = lr::bind(g->ctx)) return rc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint32_t n_pad = (n + 15) & ~15u;
-    uint32_t chunk_rows, chunks;
-    wgrad_chunks(n_pad, chunk_rows, chunks);
-    if (int rc = grow_training(g, n_pad, chunks, s)) return rc;
-    hipLaunchKernelGGL(k_wsum, dim3(1), dim3(1024), 0, s, weights, n, g->scalars);
-    const uint32_t tiles = n_pad / kTileRows;
-    // one workgroup per CU fits the 88 KB of fragments in LDS
-    const uint32_t grid = std::min<uint32_t>((tiles + kTrainWaves - 1) / kTrainWaves, (uint32_t)g->ctx->n_cu);
-    hipLaunchKernelGGL(k_train_fb, dim3(std::max(1u, grid)), dim3(kTrainBlock), kTrainSmem, s, g->frag, g->fragT,
-                       g->w + kWtElems, inputs, dirs, weights, n, n_pad, g->scalars, g->A, g->G, g->row_loss);
-    // rows n .. n_pad-1 are written by k_train_fb with zero gradients, so
-    // they add nothing to the weight gradients
-    hipLaunchKernelGGL(k_wgrad, dim3(chunks, kLayers), dim3(kBlock), 0, s, g->A, g->G, n_pad, chunk_rows, g->partial);
-    g->step += 1;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.0f - std::pow(b1, (float)g->step), c2 = 1.0f - std::pow(b2, (float)g->step);
-    hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, g->partial, chunks, g->w, g->m, g->v,
-                       g->frag, g->fragT, g->lr, b1, b2, 1e-8f, c1, c2, grad_out);
-    if (loss_out) hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, g->row_loss, n, loss_out);
-    LR_HIP_CHECK(hipGetLastError());
-    return LR_OK;
+    const TrainRows R{inputs, dirs, weights, n, nullptr, 0u, 0u, 0u, 0u, 0u};
+    return train_rows(g, R, false, loss_out, grad_out, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -67,6 +67,10 @@ int trace_any_queue(lr_qbvh* bvh, const lr_rays_soa* rays, uint32_t n_max, const
 int trace_reserve(lr_qbvh* bvh, uint32_t n_max, hipStream_t stream);
 }
 
-// Grows a guide's training scratch for batches of up to n rows (no
-// allocation in later lr_guide_train_step calls of at most n rows).
-int lr_guide_reserve(lr_guide* g, uint32_t n, hipStream_t stream);
+// One Adam step on `batch` rows drawn uniformly with replacement from the
+// min(*count, cap) training records (r_in 8, r_dir 3, r_w 1 floats each;
+// Philox keyed by seed, counter (row, step, chunk)).  The record count stays
+// on the device and nothing is allocated, so the step can be captured.
+int lr_guide_train_records(lr_guide* g, const float* r_in, const float* r_dir, const float* r_w,
+                           const uint32_t* count, uint32_t cap, uint64_t seed, uint32_t chunk, uint32_t step,
+                           uint32_t batch, hipStream_t stream);

@@ -391,38 +391,6 @@ __global__ void k_clamp_weights(uint32_t cap, const uint32_t* __restrict__ count
         w[r] = fminf(w[r], lim);
 }
 
-// One training batch of `batch` rows drawn uniformly with replacement from
-// the min(*count, cap) records (Philox keyed by seed, counter (chunk, step,
-// row)), so that every bounce's vertices are represented whatever the order
-// the records were appended in.  No records: zero weights.
-__global__ void k_gather_batch(uint32_t cap, const uint32_t* __restrict__ count, const float* __restrict__ r_in,
-                               const float* __restrict__ r_dir, const float* __restrict__ r_w, uint64_t seed,
-                               uint32_t chunk, uint32_t step, uint32_t batch, float* __restrict__ b_in,
-                               float* __restrict__ b_dir, float* __restrict__ b_w) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= batch) return;
-    const uint32_t n = min(*count, cap);
-    float4* o4 = reinterpret_cast<float4*>(b_in + 8 * (size_t)row);
-    if (n == 0) {
-        o4[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        o4[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        b_dir[3 * (size_t)row] = 0.0f;
-        b_dir[3 * (size_t)row + 1] = 0.0f;
-        b_dir[3 * (size_t)row + 2] = 1.0f;
-        b_w[row] = 0.0f;
-        return;
-    }
-    const lr::u4 h = lr::philox4x32((uint32_t)seed ^ 0x5bd1e995u, (uint32_t)(seed >> 32), lr::u4{row, step, chunk, 0x6e1du});
-    const uint32_t r = (uint32_t)(((uint64_t)h.x * n) >> 32);
-    const float4* i4 = reinterpret_cast<const float4*>(r_in + 8 * (size_t)r);
-    o4[0] = i4[0];
-    o4[1] = i4[1];
-    b_dir[3 * (size_t)row] = r_dir[3 * (size_t)r];
-    b_dir[3 * (size_t)row + 1] = r_dir[3 * (size_t)r + 1];
-    b_dir[3 * (size_t)row + 2] = r_dir[3 * (size_t)r + 2];
-    b_w[row] = r_w[r];
-}
-
 // -- one generation: emission + scatter (adjoint_photon_tracer.cpp:140-292) --
 template <bool GUIDED, bool BATCH>
 __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Queue q, const uint32_t* count,
@@ -877,7 +845,7 @@ struct lr_scene {
     uint32_t* d_counts = nullptr;  // [0..1] queue lengths, [4..5] u64 ray counter, [8] records
     // guided mode: inputs [cap][8], mixture [cap][40], records
     float* d_gwork = nullptr;
-    size_t gcap = 0, grec = 0, gbatch = 0;
+    size_t gcap = 0, grec = 0;
     // graph mode: capture stream, ordering events, instantiated graph
     hipStream_t cstream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -985,24 +953,21 @@ int build_observer(lr_scene* sc, float lo, float hi) {
 }
 
 // Guided-mode work memory: per path inputs [8] + mixture [48]; per record
-// inputs [8], direction [3], aux [4], weight [1]; per batch row inputs [8],
-// direction [3], weight [1].
-int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, size_t batch, hipStream_t s) {
-    if (sc->d_gwork && sc->gcap >= paths && sc->grec >= records && sc->gbatch >= batch) return LR_OK;
+// inputs [8], direction [3], aux [4], weight [1].
+int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, hipStream_t s) {
+    if (sc->d_gwork && sc->gcap >= paths && sc->grec >= records) return LR_OK;
     LR_HIP_CHECK(hipStreamSynchronize(s));
     paths = std::max(paths, sc->gcap);
     records = std::max(records, sc->grec);
-    batch = std::max(batch, sc->gbatch);
     // capacities drop to zero with the buffer, so a failed allocation below
     // leaves a state that the next call grows again instead of trusting
     if (sc->d_gwork) LR_HIP_CHECK(hipFree(sc->d_gwork));
     sc->d_gwork = nullptr;
-    sc->gcap = sc->grec = sc->gbatch = 0;
-    const size_t words = 56 * paths + 16 * records + 12 * batch;
+    sc->gcap = sc->grec = 0;
+    const size_t words = 56 * paths + 16 * records;
     LR_HIP_CHECK(hipMalloc(&sc->d_gwork, sizeof(float) * words));
     sc->gcap = paths;
     sc->grec = records;
-    sc->gbatch = batch;
     return LR_OK;
 }
 
@@ -1546,7 +1511,6 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     float* g_in = nullptr;
     float* g_mix = nullptr;
     float* r_w = nullptr;
-    float *b_in = nullptr, *b_dir = nullptr, *b_w = nullptr;
     uint32_t rec_cap = 0, batch = 0, guide_depth = 0;
     if (guided) {
         if (!gparams) return lr::fail(LR_E_INVALID, "guided render needs lr_guide_params");
@@ -1554,7 +1518,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             return lr::fail(LR_E_INVALID, "guide selection must be in [0, 1)");
         rec_cap = gparams->max_records ? gparams->max_records : (1u << 22);
         batch = gparams->train_batch ? gparams->train_batch : 65536u;
-        if (int rc = ensure_guide_work(sc, cap, rec_cap, batch, s)) return rc;
+        if (int rc = ensure_guide_work(sc, cap, rec_cap, s)) return rc;
         g_in = sc->d_gwork;
         g_mix = g_in + 8 * (size_t)cap;
         GA.in = g_in;
@@ -1583,9 +1547,6 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         GA.r_dir = GA.r_in + 8 * (size_t)rec_cap;
         GA.r_aux = GA.r_dir + 3 * (size_t)rec_cap;
         r_w = GA.r_aux + 4 * (size_t)rec_cap;
-        b_in = r_w + rec_cap;
-        b_dir = b_in + 8 * (size_t)batch;
-        b_w = b_dir + 3 * (size_t)batch;
     }
 
     const DevScene S = dev_scene(sc);
@@ -1620,8 +1581,6 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         // everything the captured launches use is allocated and queried first
         if (sc->bvh->info.n_nodes)
             if (int rc = lr::trace_reserve(sc->bvh, cap, s)) return rc;
-        if (training)
-            if (int rc = lr_guide_reserve(guide, batch, s)) return rc;
         LR_HIP_CHECK(hipEventRecord(sc->ev_in, caller));
         LR_HIP_CHECK(hipStreamWaitEvent(s, sc->ev_in, 0));
         LR_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1672,11 +1631,10 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
                 hipLaunchKernelGGL(k_clamp_weights, dim3(1024), blk, 0, s, rec_cap, nrec, wsum, gparams->weight_clamp,
                                    r_w);
             }
-            for (uint32_t st = 0; st < gparams->train_steps; ++st) {
-                hipLaunchKernelGGL(k_gather_batch, dim3((batch + 255) / 256), blk, 0, s, rec_cap, nrec, GA.r_in, GA.r_dir,
-                                   r_w, R.seed, chunks, st, batch, b_in, b_dir, b_w);
-                if (int rc = lr_guide_train_step(guide, b_in, b_dir, b_w, batch, nullptr, nullptr, s)) return rc;
-            }
+            for (uint32_t st = 0; st < gparams->train_steps; ++st)
+                if (int rc = lr_guide_train_records(guide, GA.r_in, GA.r_dir, r_w, nrec, rec_cap, R.seed, chunks, st,
+                                                    batch, s))
+                    return rc;
         }
         if (P.halton)
             hipLaunchKernelGGL(k_splat_scatter, grd, blk, 0, s, S, P, path0, n, ps, film_xyz, film_w);

@@ -52,7 +52,10 @@ def test_forward_matches_oracle(gpu_ctx):
     g.close()
 
 
-def test_gradient_matches_oracle(gpu_ctx):
+@pytest.mark.parametrize("n", [3000, 40000])
+def test_gradient_matches_oracle(gpu_ctx, n):
+    """n = 40000 spans several row groups per workgroup (the weight gradients
+    accumulate across them) and ends in a partial group."""
     import torch
 
     g = Guide(gpu_ctx, seed=5, learning_rate=0.0)
@@ -62,7 +65,6 @@ def test_gradient_matches_oracle(gpu_ctx):
     p[G.HIDDEN * G.W * G.W:G.WT] *= 6
     g.set_params(p.astype(np.float32))
     p = g.params().astype(np.float64)
-    n = 3000
     x = _inputs(n, 7)
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
@@ -80,8 +82,17 @@ def test_gradient_matches_oracle(gpu_ctx):
     go = G.grad(p, x.astype(np.float64), d, w.astype(np.float64), bf=True)
     rel = np.linalg.norm(gg - go) / np.linalg.norm(go)
     lo = G.loss(p, x.astype(np.float64), d, w.astype(np.float64), bf=True)
-    print(f"[gpu] guide gradient rel err {rel:.2e}, loss gpu {loss.item():.6f} oracle {lo:.6f}")
+    print(f"[gpu] guide gradient n={n} rel err {rel:.2e}, loss gpu {loss.item():.6f} oracle {lo:.6f}")
     assert rel < 1e-2
+    # every layer's weight and bias block on its own (a wrong tile or row
+    # mapping in one small block would hide in the global norm)
+    for layer in range(G.HIDDEN + 1):
+        nout = G.W if layer < G.HIDDEN else 48
+        blocks = [slice(layer * G.W * G.W, layer * G.W * G.W + nout * G.W),
+                  slice(G.WT + layer * G.W, G.WT + layer * G.W + nout)]
+        for sl in blocks:
+            e = np.linalg.norm(gg[sl] - go[sl]) / max(np.linalg.norm(go[sl]), 1e-30)
+            assert e < 3e-2, (layer, sl, e)
     assert abs(loss.item() - lo) < 1e-3 * max(1, abs(lo))
     assert np.array_equal(g.params(), p.astype(np.float32))  # learning rate 0: no update
     g.close()
