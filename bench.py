@@ -312,6 +312,11 @@ def bench_instanced(args, la, torch, np):
 
 GUIDED_CFG = dict(selection=0.5, light_share=0.5, learned_selection=True, train_steps=128, train_batch=4096,
                   weight_clamp=20.0, record_depth=0)
+# radiance-driven Russian roulette on the network's reflected-radiance
+# estimate (trained with the lobes, weight 1): from generation 3 on, a path's
+# roulette weight at most 5
+GUIDED_RR = dict(learned_rr=True, rr_scale=1.0, rr_min_survival=0.2, rr_start=3)
+GUIDED_RADIANCE_WEIGHT = 1.0
 GUIDED_TRAIN_PASSES = 2
 
 
@@ -361,28 +366,30 @@ def bench_guided(args, la, torch, np):
         e = ((img - ref) ** 2).flatten()
         return float(e.mean()), float(torch.sort(e)[0][:keep].mean())
 
-    def run(guide, ml, ref, passes=32, seed0=100):
+    def run(guide, ml, ref, passes=32, seed0=100, rr=True):
+        cfg = dict(GUIDED_CFG, **GUIDED_RR) if rr else GUIDED_CFG
         fx, fw = film()
         # warm-up pass (allocation, code load) outside the clock
         if guide is None:
             gs.render(camera, params(1, 1, ml), *film())
         else:
-            E.guided_render(gs, guide, camera, params(1, 1, ml), *film(), **GUIDED_CFG)
+            E.guided_render(gs, guide, camera, params(1, 1, ml), *film(), **cfg)
         torch.cuda.synchronize()
-        elapsed, curve = 0.0, []
+        elapsed, curve, rays = 0.0, [], 0
         for p in range(passes):
             t0 = time.perf_counter()
             if guide is not None:
-                E.guided_render(gs, guide, camera, params(4, seed0 + p, ml), fx, fw, train=p < GUIDED_TRAIN_PASSES,
-                                **GUIDED_CFG)
+                st = E.guided_render(gs, guide, camera, params(4, seed0 + p, ml), fx, fw,
+                                     train=p < GUIDED_TRAIN_PASSES, **cfg)
             else:
-                gs.render(camera, params(4, seed0 + p, ml), fx, fw)
+                st = gs.render(camera, params(4, seed0 + p, ml), fx, fw)
             torch.cuda.synchronize()
             elapsed += time.perf_counter() - t0
+            rays += st["rays"]
             curve.append((elapsed, 4 * (p + 1)) + errors(fx[1] / fw, ref))
-        return curve
+        return curve, rays / (passes * 4 * Wd * Hd)
 
-    def leg(ml):
+    def leg(ml, no_rr=False):
         fx, fw = film()
         ref_spp = args.guided_ref_spp
         log(f"[bench] guided: {ref_spp}-spp plain reference (clamp {ml or 'none'})")
@@ -397,36 +404,58 @@ def bench_guided(args, la, torch, np):
         # training is one draw of a noisy process, and a single caustic
         # outlier moves one run's MSE; the curves are averaged per pass
         reps = max(1, args.guided_repeats)
-        plains, guideds = [], []
+        plains, guideds, norrs = [], [], []
+        rpp = {"plain": [], "guided": [], "guided_no_roulette": []}
         for r in range(reps):
             log(f"[bench] guided: plain curve {r + 1}/{reps}")
-            plains.append(run(None, ml, ref, seed0=100 + 1000 * r))
+            c, k = run(None, ml, ref, seed0=100 + 1000 * r)
+            plains.append(c)
+            rpp["plain"].append(k)
             log(f"[bench] guided: guided curve {r + 1}/{reps}")
-            g = Guide(ctx, seed=1 + r, learning_rate=1e-3)
-            guideds.append(run(g, ml, ref, seed0=100 + 1000 * r))
+            g = Guide(ctx, seed=1 + r, learning_rate=1e-3, radiance_weight=GUIDED_RADIANCE_WEIGHT)
+            c, k = run(g, ml, ref, seed0=100 + 1000 * r)
+            guideds.append(c)
+            rpp["guided"].append(k)
             g.close()
+            if no_rr:
+                log(f"[bench] guided: guided curve without roulette {r + 1}/{reps}")
+                g = Guide(ctx, seed=1 + r, learning_rate=1e-3)
+                c, k = run(g, ml, ref, seed0=100 + 1000 * r, rr=False)
+                norrs.append(c)
+                rpp["guided_no_roulette"].append(k)
+                g.close()
 
         def mean_curve(cs):
             return [tuple(float(np.mean([c[p][k] for c in cs])) for k in range(4)) for p in range(len(cs[0]))]
 
+        def compare(plain, guided, out, suffix=""):
+            for k, name in ((2, "mse"), (3, "mse_trimmed")):
+                target = plain[15][k]
+                hit = next((c[0] for c in guided if c[k] <= target), None)
+                if not suffix:
+                    out[f"plain_64spp_{name}"] = target
+                out[f"guided{suffix}_64spp_{name}"] = guided[15][k]
+                out[f"equal_spp_{name}_ratio_plain_over_guided{suffix}"] = round(target / guided[15][k], 3)
+                out[f"guided{suffix}_seconds_to_plain_64spp_{name}"] = None if hit is None else round(hit, 4)
+                out[f"time_ratio_guided{suffix}_over_plain_{name}"] = None if hit is None else round(hit / plain[15][0], 3)
+
         plain, guided = mean_curve(plains), mean_curve(guideds)
         out = {"reference": f"plain {ref_spp} spp ({t_ref:.1f} s)", "runs_per_method": reps,
                "plain_64spp_seconds": round(plain[15][0], 4), "guided_64spp_seconds": round(guided[15][0], 4),
+               "rays_per_path": {k: round(float(np.mean(v)), 3) for k, v in rpp.items() if v},
                "equal_spp_mse_trimmed_ratio_per_run": [round(a[15][3] / b[15][3], 3)
                                                        for a, b in zip(plains, guideds)]}
-        for k, name in ((2, "mse"), (3, "mse_trimmed")):
-            target = plain[15][k]
-            hit = next((c[0] for c in guided if c[k] <= target), None)
-            out[f"plain_64spp_{name}"] = target
-            out[f"guided_64spp_{name}"] = guided[15][k]
-            out[f"equal_spp_{name}_ratio_plain_over_guided"] = round(target / guided[15][k], 3)
-            out[f"guided_seconds_to_plain_64spp_{name}"] = None if hit is None else round(hit, 4)
-            out[f"time_ratio_guided_over_plain_{name}"] = None if hit is None else round(hit / plain[15][0], 3)
-        out["curves"] = {"plain": [(round(c[0], 4), c[1], c[2], c[3]) for c in plain[::4]],
-                         "guided": [(round(c[0], 4), c[1], c[2], c[3]) for c in guided[::4]]}
+        compare(plain, guided, out)
+        curves = {"plain": plain, "guided": guided}
+        if norrs:
+            norr = mean_curve(norrs)
+            out["guided_no_roulette_64spp_seconds"] = round(norr[15][0], 4)
+            compare(plain, norr, out, "_no_roulette")
+            curves["guided_no_roulette"] = norr
+        out["curves"] = {k: [(round(c[0], 4), c[1], c[2], c[3]) for c in v[::4]] for k, v in curves.items()}
         return out
 
-    unclamped = leg(0.0)
+    unclamped = leg(0.0, no_rr=True)
     clamped = leg(0.5)
     out = {
         "workload": f"slit room (terrain 4,992,800 tris + 10k spheres in a closed box, 2% ceiling slit, sun + sky), "
@@ -437,7 +466,10 @@ def bench_guided(args, la, torch, np):
         "guided_config": "one-sample MIS of BSDF, light and guide sampling (power heuristic) with the network's "
                          "per-vertex strategy probabilities, at every bounce; online training in the first "
                          f"{GUIDED_TRAIN_PASSES} passes: 128 Adam steps (lr 1e-3, batch 4096) per 4M-path chunk on "
-                         "the records of every bounce, weights clamped at 20x their mean",
+                         "the records of every bounce, weights clamped at 20x their mean; radiance-driven Russian "
+                         "roulette on the network's reflected-radiance estimate (radiance loss weight 1) from "
+                         "generation 3 on, a path's roulette weight at most 5; 'no_roulette' figures: the same "
+                         "without the roulette or the radiance loss",
         "trace_config": ctx.trace_config() if tuned is None else tuned["best"],
     }
     gs.close()
@@ -672,14 +704,14 @@ def profile_workload(args, la, torch):
     camera = E.camera_struct(cam)
     fx = torch.zeros((3, Hd, Wd), device="cuda")
     fw = torch.zeros((Hd, Wd), device="cuda")
-    g = Guide(ctx, seed=1, learning_rate=1e-3)
+    g = Guide(ctx, seed=1, learning_rate=1e-3, radiance_weight=GUIDED_RADIANCE_WEIGHT)
     for p in range(6):
         prm = E.render_params(Wd, Hd, 4, 1, 16, 16, 0.5, (380.0, 780.0), 100 + p)
         t0 = time.perf_counter()
         if p < 2:
             st = gs.render(camera, prm, fx, fw)
         else:
-            st = E.guided_render(gs, g, camera, prm, fx, fw, train=p < 4, **GUIDED_CFG)
+            st = E.guided_render(gs, g, camera, prm, fx, fw, train=p < 4, **GUIDED_CFG, **GUIDED_RR)
         torch.cuda.synchronize()
         kind = "plain" if p < 2 else ("guided+train" if p < 4 else "guided")
         log(f"[profile] pass {p} {kind}: {1e3 * (time.perf_counter() - t0):.2f} ms, {st['rays']} rays")

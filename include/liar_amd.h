@@ -716,7 +716,34 @@ typedef struct lr_guide_params {
                            guide_depth bounces only; later ones sample as the
                            plain tracer (LiAR's 0.5 BSDF / 0.5 light) and run
                            no inference (0: every bounce) */
+    uint32_t learned_rr;   /* nonzero: radiance-driven Russian roulette at the
+                           guided diffuse / glossy vertices after a path's
+                           first one (adjoint-driven RR, Vorba & Krivanek
+                           2016): with L the network's estimate of the
+                           radiance a vertex reflects towards the previous one
+                           (mixture float 43) and I = throughput x L at the
+                           path's first diffuse / glossy vertex, the path
+                           survives with q = clamp(throughput x L / (rr_scale
+                           x I), B x rr_min_survival, 1) and its throughput is
+                           divided by q, B being the product of the path's
+                           earlier 1 / q (so that it stays <= 1 /
+                           rr_min_survival).
+                           Unbiased for any network; LiAR's roulette
+                           (rr_depth) keeps precedence */
+    float rr_scale;        /* see learned_rr (0: 1) */
+    float rr_min_survival; /* learned_rr: a path's roulette weight (the
+                              product of its 1 / q) is at most the inverse
+                              of this (0: 0.05) */
+    uint32_t rr_start;     /* learned_rr only at vertices of generation >=
+                              rr_start (0: from the path's second diffuse /
+                              glossy vertex on) */
 } lr_guide_params;
+/* Weight of the radiance loss in the guide's training (default 0: the
+ * radiance output, mixture float 43, is not trained).  Each training row adds
+ * weight x mean(w) x (1 - min(w / L, 50))^2, w the row's weight (a one-sample
+ * estimate of the radiance its vertex reflects) and L the output: the
+ * gradient pulls L to the mean of w.  No LiAR counterpart. */
+int lr_guide_set_radiance_weight(lr_guide* guide, float weight);
 int lr_render_guided(lr_scene* scene, const lr_camera* camera, const lr_render_params* params,
                      lr_guide* guide, const lr_guide_params* guide_params, float* film_xyz,
                      float* film_weight, lr_render_stats* stats, void* stream);

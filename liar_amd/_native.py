@@ -159,7 +159,9 @@ class GuideParams(C.Structure):
     _fields_ = [("selection", C.c_float), ("train", C.c_uint32), ("train_steps", C.c_uint32),
                 ("train_batch", C.c_uint32), ("max_records", C.c_uint32), ("weight_clamp", C.c_float),
                 ("light_share", C.c_float), ("learned_selection", C.c_uint32), ("balance_heuristic", C.c_uint32),
-                ("image_weighting", C.c_uint32), ("record_depth", C.c_uint32), ("guide_depth", C.c_uint32)]
+                ("image_weighting", C.c_uint32), ("record_depth", C.c_uint32), ("guide_depth", C.c_uint32),
+                ("learned_rr", C.c_uint32), ("rr_scale", C.c_float), ("rr_min_survival", C.c_float),
+                ("rr_start", C.c_uint32)]
 
 
 LR_NONE = 0xFFFFFFFF
@@ -210,6 +212,7 @@ SIGNATURES = {
     "lr_guide_set_params": (_I, [_P, _P]),
     "lr_guide_infer": (_I, [_P, _P, _U32, _P, _P, _P]),
     "lr_guide_train_step": (_I, [_P, _P, _P, _P, _U32, _P, _P, _P]),
+    "lr_guide_set_radiance_weight": (_I, [_P, _F]),
     "lr_set_standard_observer": (_I, [_P, _P, _U32]),
     "lr_scene_create": (_I, [_P, _P, _P, _P, _U32, _P, _U32, _P, _U32, C.POINTER(_P)]),
     "lr_scene_destroy": (_I, [_P]),

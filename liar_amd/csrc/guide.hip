@@ -1,7 +1,8 @@
 // guide.hip -- path-guiding network on gfx950 MFMA: a 6 -> 64 (frequency
 // encoding) -> 5 x 64 (ReLU) -> 48 MLP whose outputs parameterise an 8-lobe
-// von Mises-Fisher mixture over incident directions and the strategy
-// selection of one-sample MIS.  BF16 weights and activations, FP32
+// von Mises-Fisher mixture over incident directions, the strategy
+// selection of one-sample MIS and an estimate of the radiance a vertex
+// reflects towards the previous one (for Russian roulette).  BF16 weights and activations, FP32
 // accumulation (v_mfma_f32_16x16x32_bf16), FP32 master weights with Adam.
 //
 // Kernels
@@ -43,8 +44,9 @@ constexpr int kW = 64;        // width (encoding and hidden)
 constexpr int kHidden = 5;    // hidden layers
 constexpr int kOut = 40;      // 8 lobes x (logit, mean xyz, concentration)
 constexpr int kSel = 2;       // strategy-selection logits (outputs 40, 41)
+constexpr int kRad = 42;      // log of the reflected-radiance estimate
 constexpr int kOutPad = 48;   // padded to whole 16-column tiles
-constexpr int kMix = 48;      // floats per inferred row: 40 lobe values, c_light, c_guide, c_bsdf, 0 x 5
+constexpr int kMix = 48;      // floats per inferred row: 40 lobe values, c_light, c_guide, c_bsdf, radiance, 0 x 4
 constexpr float kSelFloor = 0.025f;  // least probability of each strategy
 constexpr int kLobes = 8;
 constexpr int kLayers = kHidden + 1;  // weight matrices
@@ -125,6 +127,15 @@ constexpr float kKappaLogMax = 7.6f;
 __device__ __forceinline__ float kappa_of(float s) { return 0.1f + expf(fminf(s, kKappaLogMax)); }
 __device__ __forceinline__ float dkappa_ds(float s) { return s < kKappaLogMax ? expf(s) : 0.0f; }
 
+// Reflected-radiance estimate from its raw output r: e^clamp(r, -20, 20).
+// Trained on the records' one-sample estimates y = f |cos| L_i / p_mix with
+// the relative loss (e^r - y')^2 / e^{2r}, the denominator held fixed and
+// y' = min(y, 50 e^r): the gradient 2 (1 - min(y e^-r, 50)) is scale free and
+// its zero in expectation is e^r = E[y].
+constexpr float kRadLogMax = 20.0f;
+constexpr float kRadRatioMax = 50.0f;
+__device__ __forceinline__ float rad_of(float r) { return expf(fminf(fmaxf(r, -kRadLogMax), kRadLogMax)); }
+
 // Frequency encoding of input row v[6] (each in [0, 1]) into 64 features:
 // f < 6 raw (2v - 1); 6 <= f < 54 sin / cos(2^k pi v_d), k < 4; f = 54 is 1;
 // the rest 0.
@@ -201,9 +212,11 @@ __host__ __device__ inline void selection_probs(float a0, float a1, float* c) {
 }
 
 // Per lane: the two lobes (logit, mean xyz, raw concentration) of one row in
-// o[0..9] (lane 0: selection logits in o[10..11]); the softmax over the row's
-// 8 logits spans lanes i, i+16, i+32, i+48.
+// o[0..9] (lane 0: selection logits in o[10..11], lane 16: the radiance
+// output in o[10]); the softmax over the row's 8 logits spans lanes i, i+16,
+// i+32, i+48.
 __device__ __forceinline__ void write_lobes(const float* o, bool valid, bool sel_lane, float* __restrict__ m) {
+    const float rad = __shfl(o[10], (int)(threadIdx.x & 15u) + 16);
     float mx = fmaxf(o[0], o[5]);
     mx = fmaxf(mx, __shfl_xor(mx, 16));
     mx = fmaxf(mx, __shfl_xor(mx, 32));
@@ -230,13 +243,13 @@ __device__ __forceinline__ void write_lobes(const float* o, bool valid, bool sel
         float c[3];
         selection_probs(o[10], o[11], c);
         float4* s4 = reinterpret_cast<float4*>(m + kOut);
-        s4[0] = make_float4(c[0], c[1], c[2], 0.0f);
+        s4[0] = make_float4(c[0], c[1], c[2], rad_of(rad));
         s4[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
 // mix[row][48] = lobes (pi, mu xyz, kappa) x 8, then (c_light, c_guide,
-// c_bsdf, 0 x 5), for rows < n (n from n_dev when given).  frag: the fragment-order bf16 weights; bias: FP32 biases in the
+// c_bsdf, reflected-radiance estimate, 0 x 4), for rows < n (n from n_dev when given).  frag: the fragment-order bf16 weights; bias: FP32 biases in the
 // master layout (the output layer's are permuted here).
 __global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restrict__ frag, const float* __restrict__ bias,
                                                        const float* __restrict__ in, uint32_t n,
@@ -376,6 +389,9 @@ struct TrainRows {
     uint32_t n;         // rows in the step
     const uint32_t* count;
     uint32_t cap, seed_lo, seed_hi, chunk, step;
+    // weight of each row's radiance loss (device scalar: the guide's radiance
+    // weight times the rows' or records' mean weight); null: no radiance loss
+    const float* rad_scale;
 };
 
 // Bits of element e of a bf16 vector (by a whole-vector bit cast: a bit cast
@@ -432,6 +448,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_train(const uint16_t* __restrict
         for (int u = 0; u < kWTiles; ++u) accW[l][u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     }
     float sum_w = 0.0f, sum_loss = 0.0f;
+    const float rad_s = R.rad_scale ? *R.rad_scale : 0.0f;
     for (uint32_t grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform over the workgroup
         asm volatile("" ::: "memory");  // fragment reads stay inside the loop (as in k_infer)
         const uint32_t row = grp * kGroupRows + 16 * wave + i;
@@ -552,6 +569,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_train(const uint16_t* __restrict
                 sum_w += w;
                 sum_loss += loss;
             }
+            // reflected radiance (lane 16 of the row, output 42): rad_s (1 -
+            // min(w / e^r, 50))^2, the record weight w being the estimate y;
+            // trains output row 42 only (see dz_shared below)
+            if (j == 1 && rad_s > 0.0f) {
+                const float r = o[10];
+                const float ratio = fminf(w / rad_of(r), kRadRatioMax);
+                const float d = 1.0f - ratio;
+                g[10] = (r > -kRadLogMax && r < kRadLogMax) ? rad_s * 2.0f * d : 0.0f;
+                sum_loss += rad_s * d * d;
+            }
         }
         // output-layer dZ in bf16: dz[kb][e] is physical output kslot_feature(kb, j, e)
         float gb[16];
@@ -560,6 +587,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_train(const uint16_t* __restrict
 #pragma unroll
         for (int sl = 12; sl < 16; ++sl) gb[sl] = 0.0f;
         bf16x8 dz[2] = {pack8(gb), pack8(gb + 8)};
+        // the radiance estimate is a linear readout of the last hidden layer:
+        // its loss trains output row 42 alone, not the layers the mixture and
+        // the selection share (the output layer's backward drops it)
+        if (j == 1) gb[10] = 0.0f;
+        const bf16x8 dz_shared = pack8(gb + 8);
         // per layer, from the output layer down: stage X_l and dZ_l, form the
         // weight gradients, then dZ_{l-1} = (W_l^T dZ_l) * (x[l] > 0)
 #pragma unroll
@@ -601,6 +633,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_train(const uint16_t* __restrict
                 }
             }
             if (l >= 1) {
+                if (l == kLayers - 1) dz[1] = dz_shared;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -727,6 +760,59 @@ __global__ __launch_bounds__(256) void k_adam(const float* __restrict__ partial,
     }
 }
 
+// Fixed-order sum of the first n floats of x (n = min(*count, cap) when
+// count is given): block b of kSumBlocks adds its contiguous slice, strided
+// over its threads and then as a tree in LDS; k_sum_finish adds the block
+// sums in order and writes mul * sum (mul * sum / n with `mean`, 0 when n = 0).
+constexpr int kSumBlocks = 256;
+__global__ __launch_bounds__(256) void k_sum_slices(const float* __restrict__ x, uint32_t n_host,
+                                                     const uint32_t* __restrict__ count, uint32_t cap,
+                                                     float* __restrict__ part) {
+    __shared__ float sh[256];
+    const uint32_t n = count ? min(*count, cap) : n_host;
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = blockIdx.x * per, hi = min(n, lo + per);
+    float a0 = 0.0f, a1 = 0.0f;
+    uint32_t r = lo + threadIdx.x;
+    for (; r + 256 < hi; r += 512) {
+        a0 += x[r];
+        a1 += x[r + 256];
+    }
+    if (r < hi) a0 += x[r];
+    sh[threadIdx.x] = a0 + a1;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_finish(const float* __restrict__ part, uint32_t n_host,
+                                                     const uint32_t* __restrict__ count, uint32_t cap, float mul,
+                                                     int mean, float* __restrict__ out) {
+    __shared__ float sh[256];
+    sh[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t n = count ? min(*count, cap) : n_host;
+        *out = mean ? (n ? mul * sh[0] / (float)n : 0.0f) : mul * sh[0];
+    }
+}
+
+// w = min(w, *lim) over the first min(*count, cap) records.
+__global__ void k_clamp_weights(uint32_t cap, const uint32_t* __restrict__ count, const float* __restrict__ lim,
+                                float* __restrict__ w) {
+    const uint32_t n = min(*count, cap);
+    const float l = *lim;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+        w[r] = fminf(w[r], l);
+}
+
 }  // namespace
 
 struct lr_guide {
@@ -740,6 +826,10 @@ struct lr_guide {
     float lr = 1e-3f;
     float* partial = nullptr;  // training: one kPartial slot per workgroup (max_chunks)
     uint32_t max_chunks = 0;
+    float rad_weight = 0.0f;   // weight of the radiance loss (0: the radiance output is not trained)
+    // device scalars: [0] radiance-loss scale of the step, [1] weight-clamp
+    // limit, then kSumBlocks partial sums
+    float* scal = nullptr;
 };
 
 namespace {
@@ -747,7 +837,7 @@ namespace {
 void free_guide(lr_guide* g) {
     if (!g) return;
     (void)hipSetDevice(g->ctx->device);
-    void* ptrs[] = {g->w, g->m, g->v, g->frag, g->fragT, g->partial};
+    void* ptrs[] = {g->w, g->m, g->v, g->frag, g->fragT, g->partial, g->scal};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete g;
@@ -776,11 +866,27 @@ int upload_params(lr_guide* g, const float* host) {
     return LR_OK;
 }
 
+// out = mul * (sum or mean) of the first n floats of x (n = min(*count, cap)
+// when count is given), in a fixed order; no allocation.
+void sum_floats(lr_guide* g, const float* x, uint32_t n, const uint32_t* count, uint32_t cap, float mul, bool mean,
+                float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_slices, dim3(kSumBlocks), dim3(256), 0, s, x, n, count, cap, g->scal + 2);
+    hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(kSumBlocks), 0, s, g->scal + 2, n, count, cap, mul, mean ? 1 : 0,
+                       out);
+}
+
 }  // namespace
 
 extern "C" {
 
 int lr_guide_param_count(void) { return kParams; }
+
+int lr_guide_set_radiance_weight(lr_guide* g, float weight) {
+    if (!g) return lr::fail(LR_E_INVALID, "NULL guide");
+    if (!(weight >= 0.0f && weight < INFINITY)) return lr::fail(LR_E_INVALID, "radiance weight must be >= 0");
+    g->rad_weight = weight;
+    return LR_OK;
+}
 
 int lr_guide_create(lr_context* ctx, uint64_t seed, float lr, lr_guide** out) {
     if (!ctx || !out) return lr::fail(LR_E_INVALID, "NULL context or out");
@@ -810,6 +916,8 @@ int lr_guide_create(lr_context* ctx, uint64_t seed, float lr, lr_guide** out) {
     if (e == hipSuccess) e = hipMalloc(&g->frag, sizeof(uint16_t) * kWtElems);
     if (e == hipSuccess) e = hipMalloc(&g->fragT, sizeof(uint16_t) * kFragsT * 512);
     if (e == hipSuccess) e = hipMalloc(&g->partial, sizeof(float) * kPartial * (size_t)g->max_chunks);
+    if (e == hipSuccess) e = hipMalloc(&g->scal, sizeof(float) * (2 + kSumBlocks));
+    if (e == hipSuccess) e = hipMemset(g->scal, 0, sizeof(float) * (2 + kSumBlocks));
     if (e == hipSuccess) e = hipMemset(g->m, 0, sizeof(float) * kParams);
     if (e == hipSuccess) e = hipMemset(g->v, 0, sizeof(float) * kParams);
     if (e != hipSuccess) {
@@ -910,8 +1018,22 @@ int lr_guide_train_records(lr_guide* g, const float* r_in, const float* r_dir, c
     if (!g || !r_in || !r_dir || !r_w || !count) return lr::fail(LR_E_INVALID, "NULL argument");
     if (batch == 0) return LR_OK;
     if (int rc = lr::bind(g->ctx)) return rc;
-    const TrainRows R{r_in, r_dir, r_w, batch, count, cap, (uint32_t)seed, (uint32_t)(seed >> 32), chunk, step};
+    const TrainRows R{r_in, r_dir, r_w, batch, count, cap, (uint32_t)seed, (uint32_t)(seed >> 32), chunk, step,
+                      g->rad_weight > 0.0f ? g->scal : nullptr};
     return train_rows(g, R, true, nullptr, nullptr, s);
+}
+
+int lr_guide_prepare_records(lr_guide* g, float* r_w, const uint32_t* count, uint32_t cap, float weight_clamp,
+                             hipStream_t s) {
+    if (!g || !r_w || !count) return lr::fail(LR_E_INVALID, "NULL argument");
+    if (int rc = lr::bind(g->ctx)) return rc;
+    if (weight_clamp > 0.0f) {
+        sum_floats(g, r_w, 0u, count, cap, weight_clamp, true, g->scal + 1, s);
+        hipLaunchKernelGGL(k_clamp_weights, dim3(1024), dim3(256), 0, s, cap, count, g->scal + 1, r_w);
+    }
+    if (g->rad_weight > 0.0f) sum_floats(g, r_w, 0u, count, cap, g->rad_weight, true, g->scal, s);
+    LR_HIP_CHECK(hipGetLastError());
+    return LR_OK;
 }
 
 extern "C" {
@@ -921,8 +1043,10 @@ int lr_guide_train_step(lr_guide* g, const float* inputs, const float* dirs, con
     if (!g || (n && (!inputs || !dirs || !weights))) return lr::fail(LR_E_INVALID, "NULL argument");
     if (n == 0) return LR_OK;
     if (int rc = lr::bind(g->ctx)) return rc;
-    const TrainRows R{inputs, dirs, weights, n, nullptr, 0u, 0u, 0u, 0u, 0u};
-    return train_rows(g, R, false, loss_out, grad_out, static_cast<hipStream_t>(stream));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (g->rad_weight > 0.0f) sum_floats(g, weights, n, nullptr, 0u, g->rad_weight, true, g->scal, s);
+    const TrainRows R{inputs, dirs, weights, n, nullptr, 0u, 0u, 0u, 0u, 0u, g->rad_weight > 0.0f ? g->scal : nullptr};
+    return train_rows(g, R, false, loss_out, grad_out, s);
 }
 
 }  // extern "C"

@@ -71,6 +71,14 @@ int trace_reserve(lr_qbvh* bvh, uint32_t n_max, hipStream_t stream);
 // min(*count, cap) training records (r_in 8, r_dir 3, r_w 1 floats each;
 // Philox keyed by seed, counter (row, step, chunk)).  The record count stays
 // on the device and nothing is allocated, so the step can be captured.
+// The radiance loss weighs each row by the guide's radiance weight times the
+// records' mean weight, as lr_guide_prepare_records last computed it.
 int lr_guide_train_records(lr_guide* g, const float* r_in, const float* r_dir, const float* r_w,
                            const uint32_t* count, uint32_t cap, uint64_t seed, uint32_t chunk, uint32_t step,
                            uint32_t batch, hipStream_t stream);
+// Before a chunk's training steps, over the min(*count, cap) record weights:
+// clamps each at weight_clamp times their mean (when weight_clamp > 0), then
+// stores the radiance-loss scale (radiance weight times the clamped mean
+// weight).  Fixed-order sums; nothing allocated, no host synchronisation.
+int lr_guide_prepare_records(lr_guide* g, float* r_w, const uint32_t* count, uint32_t cap, float weight_clamp,
+                             hipStream_t stream);

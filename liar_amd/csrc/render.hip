@@ -288,6 +288,14 @@ struct GuideArgs {
     float* r_in;
     float* r_dir;
     float* r_aux;
+    // radiance-driven Russian roulette: est[path] = throughput x the network's
+    // reflected-radiance estimate at the path's first diffuse / glossy vertex
+    // (0 before it); boost[path] = the product of the path's 1 / q so far,
+    // kept <= 1 / rr_min
+    uint32_t rr, rr_start;
+    float rr_scale, rr_min;
+    float* est;
+    float* boost;
 };
 
 __device__ inline float vmf_mix_pdf(const float* m, V3 w) {
@@ -357,40 +365,6 @@ __global__ void k_guide_targets(uint32_t cap, const uint32_t* __restrict__ count
     }
 }
 
-// Sum of the first min(*count, cap) floats of x by one block of 1024
-// threads (fixed order): strided partial sums, then a tree in LDS.
-__global__ __launch_bounds__(1024) void k_sum_f32(const float* __restrict__ x, uint32_t cap,
-                                                  const uint32_t* __restrict__ count, float* __restrict__ out) {
-    __shared__ float sh[1024];
-    const uint32_t n = min(*count, cap);
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-    uint32_t r = threadIdx.x;
-    for (; r + 3 * 1024 < n; r += 4 * 1024) {
-        a0 += x[r];
-        a1 += x[r + 1024];
-        a2 += x[r + 2048];
-        a3 += x[r + 3072];
-    }
-    for (; r < n; r += 1024) a0 += x[r];
-    sh[threadIdx.x] = (a0 + a1) + (a2 + a3);
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
-}
-
-// w = min(w, clamp * mean(w)) over the recorded vertices.
-__global__ void k_clamp_weights(uint32_t cap, const uint32_t* __restrict__ count, const float* __restrict__ sum,
-                                float clamp, float* __restrict__ w) {
-    const uint32_t n = min(*count, cap);
-    if (n == 0) return;
-    const float lim = clamp * (*sum) / (float)n;
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
-        w[r] = fminf(w[r], lim);
-}
-
 // -- one generation: emission + scatter (adjoint_photon_tracer.cpp:140-292) --
 template <bool GUIDED, bool BATCH>
 __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Queue q, const uint32_t* count,
@@ -421,6 +395,10 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
         const float tnear = q.r[6][i];
         float thr = ps.thr[pid];
         const float thr_in = thr;
+        if (GUIDED && G.rr && gen == 0) {  // no roulette reference or weight yet
+            G.est[pid] = 0.0f;
+            G.boost[pid] = 1.0f;
+        }
         float L = ps.L[pid];
         const float lam = ps.lam[pid];
         const uint32_t prim = h.prim[i];
@@ -619,6 +597,29 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                     else if (next >= P.rr_depth) {
                         if (u_rr < P.rr_p) thr /= P.rr_p;
                         else thr = 0.0f;
+                    } else if (GUIDED && G.rr && strategy < 1.0f) {
+                        // radiance-driven roulette: the expected contribution
+                        // of the rest of the path, thr_in x L_out, against the
+                        // one expected at its first diffuse / glossy vertex;
+                        // the path's total roulette weight stays <= 1 / rr_min
+                        // (an estimate that drops along a path would otherwise
+                        // compound the weights of successive survivals)
+                        const float lo = thr_in * G.mix[(size_t)i * 48 + 43];
+                        const float ref = gen == 0 ? 0.0f : G.est[pid];
+                        if (ref > 0.0f) {
+                            if (gen >= G.rr_start) {
+                                const float b = G.boost[pid];
+                                const float q = fminf(1.0f, fmaxf(b * G.rr_min, lo / (G.rr_scale * ref)));
+                                if (u_rr < q) {
+                                    thr /= q;
+                                    if (q < 1.0f) G.boost[pid] = b / q;
+                                } else {
+                                    thr = 0.0f;
+                                }
+                            }
+                        } else {
+                            G.est[pid] = lo;
+                        }
                     }
                     alive = thr > 0.0f && isfinite(thr);
                 }
@@ -953,7 +954,8 @@ int build_observer(lr_scene* sc, float lo, float hi) {
 }
 
 // Guided-mode work memory: per path inputs [8] + mixture [48]; per record
-// inputs [8], direction [3], aux [4], weight [1].
+// inputs [8], direction [3], aux [4], weight [1]; per path the roulette
+// reference and weight [2].
 int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, hipStream_t s) {
     if (sc->d_gwork && sc->gcap >= paths && sc->grec >= records) return LR_OK;
     LR_HIP_CHECK(hipStreamSynchronize(s));
@@ -964,7 +966,7 @@ int ensure_guide_work(lr_scene* sc, size_t paths, size_t records, hipStream_t s)
     if (sc->d_gwork) LR_HIP_CHECK(hipFree(sc->d_gwork));
     sc->d_gwork = nullptr;
     sc->gcap = sc->grec = 0;
-    const size_t words = 56 * paths + 16 * records;
+    const size_t words = 58 * paths + 16 * records;
     LR_HIP_CHECK(hipMalloc(&sc->d_gwork, sizeof(float) * words));
     sc->gcap = paths;
     sc->grec = records;
@@ -1547,6 +1549,15 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         GA.r_dir = GA.r_in + 8 * (size_t)rec_cap;
         GA.r_aux = GA.r_dir + 3 * (size_t)rec_cap;
         r_w = GA.r_aux + 4 * (size_t)rec_cap;
+        GA.est = r_w + rec_cap;
+        GA.boost = GA.est + cap;
+        GA.rr = gparams->learned_rr ? 1u : 0u;
+        GA.rr_scale = gparams->rr_scale > 0.0f ? gparams->rr_scale : 1.0f;
+        if (!(GA.rr_scale < INFINITY)) return lr::fail(LR_E_INVALID, "rr_scale must be finite");
+        // least survival probability: a survivor's weight is at most 1 / rr_min
+        GA.rr_min = gparams->rr_min_survival > 0.0f ? gparams->rr_min_survival : 0.05f;
+        GA.rr_start = gparams->rr_start;
+        if (!(GA.rr_min <= 1.0f)) return lr::fail(LR_E_INVALID, "rr_min_survival must be in (0, 1]");
     }
 
     const DevScene S = dev_scene(sc);
@@ -1625,12 +1636,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             // from all of them; the record count stays on the device
             const uint32_t* nrec = counts + 8;
             hipLaunchKernelGGL(k_guide_targets, dim3(1024), blk, 0, s, rec_cap, nrec, GA.r_aux, ps.L, r_w);
-            if (gparams->weight_clamp > 0.0f) {
-                float* wsum = reinterpret_cast<float*>(counts + 12);
-                hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(1024), 0, s, r_w, rec_cap, nrec, wsum);
-                hipLaunchKernelGGL(k_clamp_weights, dim3(1024), blk, 0, s, rec_cap, nrec, wsum, gparams->weight_clamp,
-                                   r_w);
-            }
+            if (int rc = lr_guide_prepare_records(guide, r_w, nrec, rec_cap, gparams->weight_clamp, s)) return rc;
             for (uint32_t st = 0; st < gparams->train_steps; ++st)
                 if (int rc = lr_guide_train_records(guide, GA.r_in, GA.r_dir, r_w, nrec, rec_cap, R.seed, chunks, st,
                                                     batch, s))

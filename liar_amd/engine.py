@@ -342,18 +342,22 @@ class Batch:
 def guided_render(gs: "GpuScene", guide, camera: N.Camera, params: N.RenderParams, film_xyz, film_w,
                   selection=0.5, train=False, train_steps=0, train_batch=65536, max_records=0, weight_clamp=0.0,
                   light_share=0.5, learned_selection=False, balance_heuristic=False, image_weighting=False,
-                  record_depth=0, guide_depth=0, stream=None) -> dict:
+                  record_depth=0, guide_depth=0, learned_rr=False, rr_scale=1.0, rr_min_survival=0.05,
+                  rr_start=0, stream=None) -> dict:
     """lr_render_guided on a GpuScene with a liar_amd.guide.Guide.  With
     ``learned_selection`` the network's per-vertex strategy probabilities
     replace ``selection`` / ``light_share``; ``guide_depth`` > 0 guides the
-    first guide_depth bounces only."""
+    first guide_depth bounces only; ``learned_rr`` plays radiance-driven
+    Russian roulette on the network's reflected-radiance estimate (train it
+    with ``Guide.set_radiance_weight``)."""
     import torch
 
     s = torch.cuda.current_stream(film_xyz.device).cuda_stream if stream is None else stream
     gp = N.GuideParams(float(selection), int(bool(train)), int(train_steps), int(train_batch), int(max_records),
                        float(weight_clamp), float(light_share), int(bool(learned_selection)),
                        int(bool(balance_heuristic)), int(bool(image_weighting)), int(record_depth),
-                       int(guide_depth))
+                       int(guide_depth), int(bool(learned_rr)), float(rr_scale),
+                       float(rr_min_survival), int(rr_start))
     st = N.RenderStats()
     N.check(N.lib().lr_render_guided(gs._h, C.byref(camera), C.byref(params), guide.handle, C.byref(gp),
                                      C.c_void_p(film_xyz.data_ptr()), C.c_void_p(film_w.data_ptr()), C.byref(st),

@@ -10,12 +10,20 @@ from . import _native as N
 
 
 class Guide:
-    def __init__(self, ctx, seed: int = 1, learning_rate: float = 1e-3):
+    def __init__(self, ctx, seed: int = 1, learning_rate: float = 1e-3, radiance_weight: float = 0.0):
         h = C.c_void_p()
         N.check(N.lib().lr_guide_create(ctx.handle, int(seed), float(learning_rate), C.byref(h)), "lr_guide_create")
         self._h = h
         self.ctx = ctx
         self.n_params = N.lib().lr_guide_param_count()
+        if radiance_weight:
+            self.set_radiance_weight(radiance_weight)
+
+    def set_radiance_weight(self, weight: float) -> None:
+        """Weight of the reflected-radiance loss in training (0: output 42,
+        the radiance estimate that radiance-driven roulette reads, is not
+        trained)."""
+        N.check(N.lib().lr_guide_set_radiance_weight(self._h, float(weight)), "lr_guide_set_radiance_weight")
 
     @property
     def handle(self):
@@ -50,7 +58,8 @@ class Guide:
 
     def infer(self, inputs, mixture=None, stream=None):
         """inputs: (n, 8) float32 CUDA -> (n, 48): 8 lobes x (weight, mean
-        xyz, kappa), then the strategy probabilities (light, guide, BSDF)."""
+        xyz, kappa), then the strategy probabilities (light, guide, BSDF)
+        and the reflected-radiance estimate."""
         import torch
 
         n = inputs.shape[0]

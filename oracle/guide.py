@@ -3,7 +3,8 @@
 Same parameter layout, encoding and mixture parameterisation as the GPU
 network (include/liar_amd.h, lr_guide_*).  Outputs 0..39 are the 8 vMF lobes,
 40 and 41 the strategy-selection logits (light / guide / BSDF probabilities
-of guided one-sample MIS), 42..47 padding.  With ``bf16=True`` the forward and
+of guided one-sample MIS), 42 the log of the reflected-radiance estimate
+(radiance-driven Russian roulette), 43..47 padding.  With ``bf16=True`` the forward and
 backward passes round to bfloat16 at the points where the GPU stores bf16
 (weights, encoding, hidden activations, output and hidden gradients), so GPU
 and oracle differ only by FP32-vs-FP64 accumulation.  The analytic gradient
@@ -15,6 +16,7 @@ import numpy as np
 
 W, HIDDEN, OUT, OUT_PAD, LOBES, FREQ, IN = 64, 5, 40, 48, 8, 4, 6
 SEL, SEL_FLOOR = 2, 0.025
+RAD, RAD_LOG_MAX, RAD_RATIO_MAX = 42, 20.0, 50.0
 WT = HIDDEN * W * W + OUT_PAD * W
 NPARAMS = WT + HIDDEN * W + OUT_PAD
 
@@ -114,11 +116,18 @@ def selection(out):
     return SEL_FLOOR + (1 - 3 * SEL_FLOOR) * ct
 
 
-def loss(p, inp, dirs, w, bf=False, pg=None):
+def radiance(out):
+    """Reflected-radiance estimate e^clamp(r, -20, 20) of output 42."""
+    return np.exp(np.clip(out[:, RAD], -RAD_LOG_MAX, RAD_LOG_MAX))
+
+
+def loss(p, inp, dirs, w, bf=False, pg=None, rad=0.0, rad_den=None):
     """sum_r w_r / sum(w) * (-log q(d_r) - [p_l + p_b > 0] log(c_l p_l + c_g
-    p_g + c_b p_b)), with p_l, p_b = inp[:, 6], inp[:, 7].  The guide pdf p_g
-    in the selection term is a constant of the optimisation (pass the value
-    at the point of differentiation as ``pg``; default: the current q)."""
+    p_g + c_b p_b)), with p_l, p_b = inp[:, 6], inp[:, 7], plus rad / n *
+    sum_r (L_r - min(w_r, 50 D_r))^2 / D_r^2 with L the radiance estimate and
+    D its value at the point of differentiation.  The guide pdf p_g in the
+    selection term and D are constants of the optimisation (pass their values
+    at that point as ``pg`` and ``rad_den``; default: the current ones)."""
     _, out = forward(p, inp, bf)
     wn = w / w.sum()
     lq = log_pdf(out, dirs)
@@ -129,11 +138,19 @@ def loss(p, inp, dirs, w, bf=False, pg=None):
     pgv = np.exp(lq) if pg is None else pg
     pmix = c[:, 0] * pl + c[:, 1] * pgv + c[:, 2] * pb
     sel = np.where(on, np.log(np.where(on, pmix, 1.0)), 0.0)
-    return float(-(wn * lq).sum() - (wn * sel).sum())
+    total = -(wn * lq).sum() - (wn * sel).sum()
+    if rad:
+        L = radiance(out)
+        D = L if rad_den is None else rad_den
+        y = np.minimum(np.asarray(w, np.float64), RAD_RATIO_MAX * D)
+        total += rad / len(L) * (((L - y) / D) ** 2).sum()
+    return float(total)
 
 
-def grad(p, inp, dirs, w, bf=False):
-    """Analytic gradient of loss() w.r.t. the flat parameter vector."""
+def grad(p, inp, dirs, w, bf=False, rad=0.0):
+    """Analytic gradient of loss() w.r.t. the flat parameter vector, except
+    that the radiance term's gradient stops at output row 42 (the hidden
+    layers get the gradient of loss(rad=0))."""
     Wt, b = unpack(p)
     Wtb = [bf16(x) for x in Wt] if bf else Wt
     acts, out = forward(p, inp, bf)
@@ -166,6 +183,11 @@ def grad(p, inp, dirs, w, bf=False):
     f = np.where(on, -wn * (1 - 3 * SEL_FLOOR) / pmix, 0.0)
     g[:, OUT] = f * rl * (1 - rl) * (pl - rg * pg - (1 - rg) * pb)
     g[:, OUT + 1] = f * (1 - rl) * rg * (1 - rg) * (pg - pb)
+    if rad:
+        r = out[:, RAD]
+        ratio = np.minimum(np.asarray(w, np.float64) / radiance(out), RAD_RATIO_MAX)
+        inside = (r > -RAD_LOG_MAX) & (r < RAD_LOG_MAX)
+        g[:, RAD] = np.where(inside, rad / n * 2 * (1 - ratio), 0.0)
     if bf:
         g = bf16(g)
     dW = [None] * (HIDDEN + 1)
@@ -176,6 +198,11 @@ def grad(p, inp, dirs, w, bf=False):
         db[l] = dz.sum(0)
         if l == 0:
             break
+        if l == HIDDEN:
+            # output 42 is a readout of the last hidden layer: its loss does
+            # not reach the layers the mixture and the selection share
+            dz = dz.copy()
+            dz[:, RAD] = 0.0
         da = dz @ Wtb[l]
         dz = np.where(acts[l] > 0, da, 0.0)
         if bf:

@@ -6,7 +6,8 @@
 - gradient: lr_guide_train_step's gradient (learning rate 0) equals the
   oracle's analytic gradient, itself pinned by finite differences;
 - training: Adam on samples of a known position-dependent vMF target lowers
-  the loss from ~log(4 pi) to near the target's entropy.
+  the loss from ~log(4 pi) to near the target's entropy; the reflected-
+  radiance output learns the conditional mean of noisy targets.
 """
 import numpy as np
 import pytest
@@ -41,7 +42,8 @@ def test_forward_matches_oracle(gpu_ctx):
         pi, mu, kappa = G.mixture(out)
         got = mix[:, :40].reshape(n, 8, 5)
         e_c = np.abs(mix[:, 40:43] - G.selection(out)).max()
-        assert e_c < 5e-3 and (mix[:, 43:] == 0).all()
+        e_r = np.abs(mix[:, 43] / G.radiance(out) - 1).max()
+        assert e_c < 5e-3 and e_r < 1e-2 and (mix[:, 44:] == 0).all()
         # single-ulp bf16 rounding flips (FP32 vs FP64 accumulation) move
         # values by ~1e-3; a wrong lane layout or wiring moves them by O(0.1)
         e_pi, e_mu = np.abs(got[..., 0] - pi).max(), np.abs(got[..., 1:4] - mu).max()
@@ -52,13 +54,14 @@ def test_forward_matches_oracle(gpu_ctx):
     g.close()
 
 
-@pytest.mark.parametrize("n", [3000, 40000])
-def test_gradient_matches_oracle(gpu_ctx, n):
+@pytest.mark.parametrize("n,rad", [(3000, 0.0), (40000, 0.0), (3000, 0.7), (40000, 0.7)])
+def test_gradient_matches_oracle(gpu_ctx, n, rad):
     """n = 40000 spans several row groups per workgroup (the weight gradients
-    accumulate across them) and ends in a partial group."""
+    accumulate across them) and ends in a partial group; rad > 0 adds the
+    reflected-radiance loss."""
     import torch
 
-    g = Guide(gpu_ctx, seed=5, learning_rate=0.0)
+    g = Guide(gpu_ctx, seed=5, learning_rate=0.0, radiance_weight=rad)
     p = g.params().astype(np.float64)
     rng = np.random.default_rng(2)
     p[G.WT:] = rng.normal(0, 0.05, p.size - G.WT)
@@ -79,10 +82,10 @@ def test_gradient_matches_oracle(gpu_ctx, n):
                  torch.from_numpy(w).cuda(), loss=loss, grad=grad)
     torch.cuda.synchronize()
     gg = grad.cpu().numpy().astype(np.float64)
-    go = G.grad(p, x.astype(np.float64), d, w.astype(np.float64), bf=True)
+    go = G.grad(p, x.astype(np.float64), d, w.astype(np.float64), bf=True, rad=rad)
     rel = np.linalg.norm(gg - go) / np.linalg.norm(go)
-    lo = G.loss(p, x.astype(np.float64), d, w.astype(np.float64), bf=True)
-    print(f"[gpu] guide gradient n={n} rel err {rel:.2e}, loss gpu {loss.item():.6f} oracle {lo:.6f}")
+    lo = G.loss(p, x.astype(np.float64), d, w.astype(np.float64), bf=True, rad=rad)
+    print(f"[gpu] guide gradient n={n} rad={rad} rel err {rel:.2e}, loss gpu {loss.item():.6f} oracle {lo:.6f}")
     assert rel < 1e-2
     # every layer's weight and bias block on its own (a wrong tile or row
     # mapping in one small block would hide in the global norm)
@@ -93,6 +96,13 @@ def test_gradient_matches_oracle(gpu_ctx, n):
         for sl in blocks:
             e = np.linalg.norm(gg[sl] - go[sl]) / max(np.linalg.norm(go[sl]), 1e-30)
             assert e < 3e-2, (layer, sl, e)
+    out0 = G.HIDDEN * G.W * G.W
+    row42 = slice(out0 + G.RAD * G.W, out0 + (G.RAD + 1) * G.W)
+    if rad:  # the radiance row on its own
+        assert np.linalg.norm(go[row42]) > 0
+        assert np.linalg.norm(gg[row42] - go[row42]) / np.linalg.norm(go[row42]) < 3e-2
+    else:
+        assert (gg[row42] == 0).all()
     assert abs(loss.item() - lo) < 1e-3 * max(1, abs(lo))
     assert np.array_equal(g.params(), p.astype(np.float32))  # learning rate 0: no update
     g.close()
@@ -108,6 +118,37 @@ def _sample_vmf(mu, kappa, rng):
     b = np.cross(mu, t)
     s = np.sqrt(np.maximum(0, 1 - wv ** 2))
     return wv[:, None] * mu + s[:, None] * (np.cos(phi)[:, None] * t + np.sin(phi)[:, None] * b)
+
+
+def test_training_learns_radiance_mean(gpu_ctx):
+    """The radiance output is a linear readout of the last hidden layer (its
+    loss does not train the shared layers), so it learns what the features the
+    lobes need can express: with the vMF target of the test below (its mean
+    direction a function of the position) and row weights y = m(x) * E, E ~
+    Exp(1) on 70% of the rows and 0 on the rest, m(x) = 0.2 + 3 x0^2, it
+    converges to the conditional mean 0.7 m(x), not to a log-space or median
+    fit (those sit at 0.48 m(x) and lower)."""
+    import torch
+
+    g = Guide(gpu_ctx, seed=12, learning_rate=3e-3, radiance_weight=1.0)
+    rng = np.random.default_rng(8)
+    for step in range(600):
+        n = 8192
+        x = _inputs(n, 5000 + step)
+        mu = x[:, :3] - 0.5 + np.array([0, 0, 0.3])
+        mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+        d = _sample_vmf(mu.astype(np.float64), 20.0, rng).astype(np.float32)
+        m = 0.2 + 3.0 * x[:, 0] ** 2
+        y = (m * rng.exponential(1.0, n) * (rng.random(n) < 0.7)).astype(np.float32)
+        g.train_step(torch.from_numpy(x).cuda(), torch.from_numpy(d).cuda(), torch.from_numpy(y).cuda())
+    x = _inputs(4096, 99)
+    mix = g.infer(torch.from_numpy(x).cuda()).cpu().numpy()
+    want = 0.7 * (0.2 + 3.0 * x[:, 0] ** 2)
+    rel = np.abs(mix[:, 43] / want - 1)
+    print(f"[gpu] radiance readout: mean rel err {rel.mean():.3f}, 95th pct {np.quantile(rel, 0.95):.3f}, "
+          f"mean ratio {np.mean(mix[:, 43] / want):.3f}")
+    assert rel.mean() < 0.15 and abs(np.mean(mix[:, 43] / want) - 1) < 0.1
+    g.close()
 
 
 def test_training_learns_a_vmf_target(gpu_ctx):

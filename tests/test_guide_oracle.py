@@ -49,6 +49,63 @@ def test_gradient_matches_finite_differences():
     assert (g0[out0 + G.OUT * G.W:out0 + (G.OUT + 2) * G.W] == 0).all()
 
 
+def test_radiance_gradient_matches_finite_differences():
+    """The reflected-radiance term (output 42, relative loss with its
+    denominator held fixed, targets clipped at 50x the estimate) against
+    central differences; one row's target is past the clip.  The term trains
+    the readout row only."""
+    p, inp, d, w = _case(seed=4, n=7)
+    w[2] = 400.0  # w / L > 50: the clipped branch
+    rad = 0.7
+    g = G.grad(p, inp, d, w, rad=rad)
+    out0 = G.HIDDEN * G.W * G.W
+    rng = np.random.default_rng(5)
+    idx = np.concatenate([rng.choice(out0, 24, replace=False),
+                          out0 + G.RAD * G.W + rng.choice(G.W, 12, replace=False),
+                          [G.WT + G.HIDDEN * G.W + G.RAD], G.WT + rng.choice(G.HIDDEN * G.W, 8, replace=False)])
+    _, out = G.forward(p, inp)
+    pg = np.exp(G.log_pdf(out, d))
+    D = G.radiance(out)
+    assert (np.minimum(w, 50 * D) < w).any()
+    # output row 42 and its bias: the whole loss; the hidden layers: the loss
+    # without the radiance term (its gradient stops at the readout row)
+    row42 = set(range(out0 + G.RAD * G.W, out0 + (G.RAD + 1) * G.W)) | {G.WT + G.HIDDEN * G.W + G.RAD}
+    fd = []
+    for k in idx:
+        e = np.zeros(p.size)
+        e[k] = 1e-6
+        r = rad if k in row42 else 0.0
+        fd.append((G.loss(p + e, inp, d, w, pg=pg, rad=r, rad_den=D)
+                   - G.loss(p - e, inp, d, w, pg=pg, rad=r, rad_den=D)) / 2e-6)
+    fd = np.array(fd)
+    assert np.max(np.abs(fd - g[idx])) < 1e-6 * max(1.0, np.abs(g[idx]).max())
+    assert np.abs(g[out0 + G.RAD * G.W:out0 + (G.RAD + 1) * G.W]).max() > 1e-3
+    # without the term output 42 gets no gradient, and every other parameter
+    # the same gradient as with it
+    g0 = G.grad(p, inp, d, w)
+    assert (g0[out0 + G.RAD * G.W:out0 + (G.RAD + 1) * G.W] == 0).all()
+    other = np.ones(p.size, bool)
+    other[sorted(row42)] = False
+    assert np.array_equal(g0[other], g[other])
+
+
+def test_radiance_stationary_point_is_the_mean():
+    """With output 42 a bias alone, the radiance gradient vanishes exactly
+    where e^b is the mean target and pushes b towards it from either side
+    (a log-space or unweighted-ratio loss would settle elsewhere)."""
+    p, inp, d, _ = _case(seed=6, n=50)
+    rng = np.random.default_rng(7)
+    w = rng.exponential(2.5, 50) * (rng.random(50) < 0.6)  # many zero targets
+    out0 = G.HIDDEN * G.W * G.W
+    p[out0 + G.RAD * G.W:out0 + (G.RAD + 1) * G.W] = 0.0
+    bi = G.WT + G.HIDDEN * G.W + G.RAD
+    grads = []
+    for off in (0.0, 0.2, -0.2):
+        p[bi] = np.log(w.mean()) + off
+        grads.append(G.grad(p, inp, d, w, rad=1.0)[bi])
+    assert abs(grads[0]) < 1e-12 and grads[1] > 1e-3 and grads[2] < -1e-3
+
+
 def test_selection_optimum_matches_closed_form():
     # With the target concentrated where only one strategy has density, the
     # selection loss drives that strategy's probability to its maximum

@@ -2,6 +2,10 @@
 
 - Unbiased for any network: an untrained (random) guide renders the white
   furnace to 1 and the Cornell box to the 4096-spp reference within noise.
+- Radiance-driven Russian roulette (learned_rr) is unbiased with an untrained
+  network and with a trained one inside an emitting, reflecting sphere whose
+  radiance has a closed form, and the trained radiance output reaches that
+  sphere's reflected radiance.
 - Useful when trained: with a hidden emitter that light sampling cannot see
   (an Unshaded quad, not a light), a guide trained for a few passes lowers the
   equal-spp MSE against a plain high-spp reference.
@@ -49,6 +53,69 @@ def test_guided_furnace_unbiased_with_untrained_guide(gpu_ctx):
     print(f"[gpu] guided furnace: mean {Y.mean():.5f}, 3 sigma {three_sigma:.1e}", flush=True)
     assert three_sigma < 5e-4
     assert abs(Y.mean() - 1.0) < 1e-3
+    g.close()
+    gs.close()
+
+
+def _cavity():
+    """Inside a unit sphere of Lambert 0.5 that also emits 1 (Unshaded):
+    the radiance is Le / (1 - rho) = 2 everywhere, of which rho * 2 = 1 is
+    reflected."""
+    sph = scenery.Sphere((0.0, 0.0, 0.0), 1.0)
+    sph.shader = shaders.Sum([shaders.Lambert(textures.Constant(0.5)), shaders.Unshaded(textures.Constant(1.0))])
+    cam = cameras.PerspectiveCamera()
+    cam.position = (0.0, 0.0, 0.0)
+    cam.lookAt((1.0, 0.2, 0.1))
+    cam.sky = (0.0, 0.0, 1.0)
+    cam.fovAngle = math.radians(60.0)
+    cam.aspectRatio = 1.0
+    return scenery.List([sph]), cam
+
+
+def test_learned_roulette_unbiased_in_emitting_cavity(gpu_ctx):
+    import torch
+
+    sc, cam = _cavity()
+    gs = E.GpuScene(gpu_ctx, la.compile_scene(sc))
+    camera = E.camera_struct(cam)
+    # no LiAR roulette before the 64-bounce cap (truncation 0.5^64)
+    P = lambda spp, seed: E.render_params(64, 64, spp, 1, 64, 64, 0.5, (380, 780), seed)
+
+    def mean_y(g, spp, seed, **kw):
+        fx, fw = _film(64, 64)
+        st = E.guided_render(gs, g, camera, P(spp, seed), fx, fw, selection=0.5, **kw)
+        y = _y(fx, fw)
+        return y.mean(), 3 * y.std() / math.sqrt(y.size), st["rays"] / st["paths"]
+
+    g = Guide(gpu_ctx, seed=9, learning_rate=3e-3, radiance_weight=1.0)
+    m0, s0, r0 = mean_y(g, 16384, 1)
+    m1, s1, r1 = mean_y(g, 16384, 2, learned_rr=True)
+    print(f"[gpu] cavity, untrained guide: no roulette {m0:.5f} +- {s0:.1e} ({r0:.2f} rays/path), "
+          f"learned roulette {m1:.5f} +- {s1:.1e} ({r1:.2f} rays/path)", flush=True)
+    for p in range(4):  # train the lobes and the radiance output
+        mean_y(g, 256, 10 + p, train=True, train_steps=64, train_batch=4096, learned_rr=True)
+    # the radiance a surface point reflects: 1 in every direction
+    rng = np.random.default_rng(3)
+    d = rng.normal(size=(4096, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = np.zeros((4096, 8), np.float32)
+    info = gs.qbvh.info()
+    lo, hi = np.array(info["scene_lo"]), np.array(info["scene_hi"])
+    x[:, :3] = (d - lo) / (hi - lo)
+    x[:, 3] = rng.random(4096)
+    x[:, 4] = rng.random(4096)
+    x[:, 5] = rng.random(4096)
+    L = g.infer(torch.from_numpy(x).cuda()).cpu().numpy()[:, 43]
+    m2, s2, r2 = mean_y(g, 16384, 3, learned_rr=True)
+    m3, s3, r3 = mean_y(g, 16384, 4)
+    print(f"[gpu] cavity, trained: reflected-radiance estimate mean {L.mean():.3f} (5-95% {np.quantile(L, 0.05):.3f}"
+          f"..{np.quantile(L, 0.95):.3f}, exact 1); learned roulette {m2:.5f} +- {s2:.1e} ({r2:.2f} rays/path), "
+          f"none {m3:.5f} +- {s3:.1e} ({r3:.2f} rays/path)")
+    for m, sg in ((m0, s0), (m1, s1), (m2, s2), (m3, s3)):
+        assert sg < 1e-3 and abs(m - 2.0) < 2e-3
+    assert r1 < r0 and r2 < r3
+    # a linear readout of the lobes' features: within 20% of the exact value
+    assert abs(L.mean() - 1.0) < 0.2 and np.quantile(np.abs(L - 1.0), 0.9) < 0.3
     g.close()
     gs.close()
 

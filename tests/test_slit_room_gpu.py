@@ -6,7 +6,7 @@ spheres, 256 x 144 pixels, depth 16.
    paths (adjoint_photon_tracer.cpp:140-196 at every vertex).
 2. Unbiasedness of the guided estimator (one-sample MIS of BSDF, light and
    the learned proposal, with the network's per-vertex strategy
-   probabilities): with a trained guide, 16 x 16-pixel block means agree with
+   probabilities; also with radiance-driven roulette): with a trained guide, 16 x 16-pixel block means agree with
    an independent plain render in units of their standard error.  On the
    room with every surface Lambert, so that no caustic outliers make the
    error estimate meaningless; the image mean is also checked on the room's
@@ -100,21 +100,29 @@ def test_slit_room_plain_same_seed_matches_oracle(gpu_ctx, rooms):
     assert rel < 5e-3 and bad < 0.05
 
 
-def _guided_and_plain(ctx, cs, cam, passes=32, spp=16, train_passes=4):
+# radiance-driven Russian roulette on the network's radiance estimate: the
+# benchmark's setting (from generation 3 on, a path's roulette weight <= 5)
+# and an aggressive one (from the second diffuse vertex on, weight <= 10)
+RR = {"bench": dict(learned_rr=True, rr_scale=1.0, rr_min_survival=0.2, rr_start=3),
+      "aggressive": dict(learned_rr=True, rr_scale=1.0, rr_min_survival=0.1, rr_start=0)}
+
+
+def _guided_and_plain(ctx, cs, cam, passes=32, spp=16, train_passes=4, rr=None):
+    cfg = dict(GCFG, **RR[rr]) if rr else GCFG
     gs = E.GpuScene(ctx, cs)
     camera = E.camera_struct(cam)
     fx, fw = _film()
     for k in range(8):
         gs.render(camera, _params(512, 10000 + k), fx, fw)
     ref = _y(fx, fw)
-    g = Guide(ctx, seed=1, learning_rate=1e-3)
+    g = Guide(ctx, seed=1, learning_rate=1e-3, radiance_weight=1.0 if rr else 0.0)
     for p in range(train_passes):
         fx, fw = _film()
-        E.guided_render(gs, g, camera, _params(4, 100 + p), fx, fw, train=True, **GCFG)
+        E.guided_render(gs, g, camera, _params(4, 100 + p), fx, fw, train=True, **cfg)
     yg, yp = [], []
     for p in range(passes):
         fx, fw = _film()
-        E.guided_render(gs, g, camera, _params(spp, 200 + p), fx, fw, train=False, **GCFG)
+        E.guided_render(gs, g, camera, _params(spp, 200 + p), fx, fw, train=False, **cfg)
         yg.append(_y(fx, fw))
         fx, fw = _film()
         gs.render(camera, _params(spp, 300 + p), fx, fw)
@@ -124,23 +132,27 @@ def _guided_and_plain(ctx, cs, cam, passes=32, spp=16, train_passes=4):
     return ref, yg, yp
 
 
-def test_slit_room_guided_unbiased(gpu_ctx, rooms):
+@pytest.mark.parametrize("rr", [None, "bench", "aggressive"])
+def test_slit_room_guided_unbiased(gpu_ctx, rooms, rr):
     """Every pass is an unbiased estimate (the guide is fixed after training),
     so the spread of the 32 per-pass block means gives the standard error of
     their average.  Lambert room: block |z| below 4.5 over the 144 blocks and
     their mean z within 0.5 of 0; both rooms: the image means agree within 3
-    standard errors."""
+    standard errors.  With ``rr`` the guided renders also play radiance-driven
+    roulette on the trained radiance estimate (its weight cap bounds the
+    tails: uncapped, compounded survivals of up to 20x each made the block
+    means skew low by ~4 standard errors)."""
     scenes, cam = rooms
     blocks = lambda a: a.reshape(HD // 16, 16, WD // 16, 16).mean(axis=(1, 3))
     for name in ("lambert", "materials"):
-        _, yg, yp = _guided_and_plain(gpu_ctx, scenes[name], cam)
+        _, yg, yp = _guided_and_plain(gpu_ctx, scenes[name], cam, rr=rr)
         n = len(yg)
         gm, pm = np.array([v.mean() for v in yg]), np.array([v.mean() for v in yp])
         z_all = (gm.mean() - pm.mean()) / np.sqrt(gm.var(ddof=1) / n + pm.var(ddof=1) / n)
         bg, bp = np.stack([blocks(v) for v in yg]), np.stack([blocks(v) for v in yp])
         se = np.sqrt(bg.var(axis=0, ddof=1) / n + bp.var(axis=0, ddof=1) / n)
         z = (bg.mean(0) - bp.mean(0)) / np.maximum(se, 1e-12)
-        say(f"[gpu] slit room ({name}) guided vs plain, {n} x 16 spp: image mean z {z_all:+.2f}; "
+        say(f"[gpu] slit room ({name}, roulette {rr}) guided vs plain, {n} x 16 spp: image mean z {z_all:+.2f}; "
             f"block z mean {z.mean():+.2f}, |z| max {np.abs(z).max():.2f}")
         assert abs(z_all) < 3.0
         if name == "lambert":
