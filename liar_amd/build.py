@@ -32,6 +32,9 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in VGPRs: the guide kernels read every accumulator with
+    # VALU code, and AGPR results cost one v_accvgpr_read each
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}",
     f"-I{CSRC}",

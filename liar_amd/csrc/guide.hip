@@ -136,20 +136,25 @@ constexpr float kRadLogMax = 20.0f;
 constexpr float kRadRatioMax = 50.0f;
 __device__ __forceinline__ float rad_of(float r) { return expf(fminf(fmaxf(r, -kRadLogMax), kRadLogMax)); }
 
-// Frequency encoding of input row v[6] (each in [0, 1]) into 64 features:
-// f < 6 raw (2v - 1); 6 <= f < 54 sin / cos(2^k pi v_d), k < 4; f = 54 is 1;
-// the rest 0.
-__device__ __forceinline__ float encode(const float* v, int f) {
-    if (f < kIn) return 2.0f * v[f] - 1.0f;
-    if (f < kIn + kIn * 2 * kFreq) {
-        const int idx = f - kIn;
-        const int d = idx / (2 * kFreq);
-        const int k = (idx % (2 * kFreq)) >> 1;
-        // sin / cos(2^k pi v) = v_sin / v_cos of 2^(k-1) v revolutions (|arg| <= 4)
-        const float rev = 0.5f * (float)(1 << k) * v[d];
-        return (idx & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+// Frequency encoding of input row v[6] (each in [0, 1]) into 64 features,
+// laid out so that every lane computes its 16 K slots with no indexing by
+// lane: slot (kb, j, e) of the first layer's B operand (logical feature
+// kslot_feature(kb, j, e)), s = 8 kb + e:
+//   s < 12        sin (s even) / cos (s odd) of 2^j pi v_{s/2}
+//   s >= 12, j=0  2 v_{s-12} - 1  (inputs 0..3)
+//   s >= 12, j=1  2 v_4 - 1, 2 v_5 - 1, 1, 0
+//   s >= 12, j>1  0
+// (6 raw inputs, 48 sines / cosines, a constant, 9 zeros).  rev_scale =
+// 2^(j-1): v_sin / v_cos take revolutions, |argument| <= 4.
+__device__ __forceinline__ float encode_slot(const float* v, float rev_scale, int j, int kb, int e) {
+    const int s = 8 * kb + e;
+    if (s < 12) {
+        const float rev = rev_scale * v[s >> 1];
+        return (s & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
     }
-    return f == kIn + kIn * 2 * kFreq ? 1.0f : 0.0f;
+    const float first = 2.0f * v[s - 12] - 1.0f;
+    const float second = s < 14 ? 2.0f * v[s - 8] - 1.0f : (s == 14 ? 1.0f : 0.0f);
+    return j == 0 ? first : (j == 1 ? second : 0.0f);
 }
 
 // log vMF density at cosine c = mu . w:  log(k / (2 pi (1 - e^{-2k}))) + k (c - 1)
@@ -185,12 +190,16 @@ constexpr int kInferRT = 2;  // 16-row tiles per wave iteration (share each weig
 constexpr int kInferRows = 16 * kInferRT;
 constexpr size_t kInferSmem = sizeof(uint4) * kFrags * 64 + sizeof(float) * kBiasElems;
 
+// max(x, 0) as an integer max on the bits (a negative float or -0 is a
+// negative int32): one v_max_i32 per activation, where fmaxf also quiets
+// signalling NaNs first (the same value for every non-NaN x).
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 __device__ __forceinline__ bf16x8 pack_relu(const f32x4& lo, const f32x4& hi) {
     bf16x8 r;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        r[e] = (__bf16)fmaxf(lo[e], 0.0f);
-        r[e + 4] = (__bf16)fmaxf(hi[e], 0.0f);
+        r[e] = (__bf16)relu(lo[e]);
+        r[e + 4] = (__bf16)relu(hi[e]);
     }
     return r;
 }
@@ -215,7 +224,17 @@ __host__ __device__ inline void selection_probs(float a0, float a1, float* c) {
 // o[0..9] (lane 0: selection logits in o[10..11], lane 16: the radiance
 // output in o[10]); the softmax over the row's 8 logits spans lanes i, i+16,
 // i+32, i+48.
-__device__ __forceinline__ void write_lobes(const float* o, bool valid, bool sel_lane, float* __restrict__ m) {
+//
+// The mixture goes to row-major rows of 48 floats (SOA false: float c of row r
+// at mix[48 r + c]) or to 48 columns of ld floats (SOA true: at mix[c ld + r],
+// so that a shading kernel's lanes read one value of consecutive rows
+// together).  The head's reciprocals and square root are the hardware's
+// approximate ones (1 ulp): sampling and evaluation read the same stored
+// values, so they stay consistent.
+template <bool SOA>
+__device__ __forceinline__ void write_lobes(const float* o, bool valid, bool sel_lane, float* __restrict__ mix,
+                                            uint32_t row, size_t ld) {
+    const int j = (int)((threadIdx.x & 63u) >> 4);
     const float rad = __shfl(o[10], (int)(threadIdx.x & 15u) + 16);
     float mx = fmaxf(o[0], o[5]);
     mx = fmaxf(mx, __shfl_xor(mx, 16));
@@ -225,35 +244,52 @@ __device__ __forceinline__ void write_lobes(const float* o, bool valid, bool sel
     sum += __shfl_xor(sum, 16);
     sum += __shfl_xor(sum, 32);
     if (!valid) return;
-    const float inv = 1.0f / sum;
+    const float inv = __builtin_amdgcn_rcpf(sum);
     float r[10];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const float* q = o + 5 * h;
-        const float nrm = 1.0f / (sqrtf(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + 1e-12f);
+        const float nrm = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + 1e-12f);
         r[5 * h + 0] = (h ? eb : ea) * inv;
         r[5 * h + 1] = q[1] * nrm;
         r[5 * h + 2] = q[2] * nrm;
         r[5 * h + 3] = q[3] * nrm;
         r[5 * h + 4] = kappa_of(q[4]);
     }
-#pragma unroll
-    for (int c = 0; c < 5; ++c) reinterpret_cast<float2*>(m)[c] = make_float2(r[2 * c], r[2 * c + 1]);
-    if (sel_lane) {  // lane 0 of the row: floats 40..47 of the row
-        float c[3];
+    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (sel_lane) {  // lane 0 of the row: floats 40..43 of the row (44..47 are 0)
         selection_probs(o[10], o[11], c);
-        float4* s4 = reinterpret_cast<float4*>(m + kOut);
-        s4[0] = make_float4(c[0], c[1], c[2], rad_of(rad));
-        s4[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        c[3] = rad_of(rad);
+    }
+    if (SOA) {
+        float* m = mix + row;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) m[(size_t)(10 * j + k) * ld] = r[k];
+        if (sel_lane)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) m[(size_t)(kOut + k) * ld] = k < 4 ? c[k] : 0.0f;
+    } else {
+        float* m = mix + (size_t)row * kMix;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) reinterpret_cast<float2*>(m + 10 * j)[k] = make_float2(r[2 * k], r[2 * k + 1]);
+        if (sel_lane) {
+            float4* s4 = reinterpret_cast<float4*>(m + kOut);
+            s4[0] = make_float4(c[0], c[1], c[2], c[3]);
+            s4[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
     }
 }
 
-// mix[row][48] = lobes (pi, mu xyz, kappa) x 8, then (c_light, c_guide,
-// c_bsdf, reflected-radiance estimate, 0 x 4), for rows < n (n from n_dev when given).  frag: the fragment-order bf16 weights; bias: FP32 biases in the
-// master layout (the output layer's are permuted here).
+// Per row: lobes (pi, mu xyz, kappa) x 8, then (c_light, c_guide, c_bsdf,
+// reflected-radiance estimate, 0 x 4), for rows < n (n from n_dev when
+// given), in the layout of write_lobes<SOA> (ld: the column length).  frag:
+// the fragment-order bf16 weights; bias: FP32 biases in the master layout
+// (the output layer's are permuted here).
+template <bool SOA>
 __global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restrict__ frag, const float* __restrict__ bias,
                                                        const float* __restrict__ in, uint32_t n,
-                                                       const uint32_t* __restrict__ n_dev, float* __restrict__ mix) {
+                                                       const uint32_t* __restrict__ n_dev, float* __restrict__ mix,
+                                                       size_t ld) {
     extern __shared__ uint8_t smem[];
     uint4* s_f = reinterpret_cast<uint4*>(smem);
     float* s_b = reinterpret_cast<float*>(smem + sizeof(uint4) * kFrags * 64);
@@ -266,6 +302,7 @@ __global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restric
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
+    const float rev_scale = 0.5f * (float)(1 << j);
     const uint32_t tiles = (n + kInferRows - 1) / kInferRows;
     for (uint32_t tile = blockIdx.x * kInferWaves + wave; tile < tiles; tile += gridDim.x * kInferWaves) {
         // keeps the fragment reads inside the loop: hoisted, the 46 fragments
@@ -284,7 +321,7 @@ __global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restric
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-                for (int e = 0; e < 8; ++e) x[rt][kb][e] = (__bf16)encode(v, kslot_feature(kb, j, e));
+                for (int e = 0; e < 8; ++e) x[rt][kb][e] = (__bf16)encode_slot(v, rev_scale, j, kb, e);
         }
         f32x4 acc[kInferRT][4];
 #pragma unroll
@@ -327,7 +364,7 @@ __global__ __launch_bounds__(kInferBlock) void k_infer(const uint16_t* __restric
             float o[12];
 #pragma unroll
             for (int sl = 0; sl < 12; ++sl) o[sl] = acc[rt][sl >> 2][sl & 3];
-            write_lobes(o, row < n, j == 0, mix + (size_t)row * kMix + 10 * j);
+            write_lobes<SOA>(o, row < n, j == 0, mix, row, ld);
         }
     }
 }
@@ -428,6 +465,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_train(const uint16_t* __restrict
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
+    const float rev_scale = 0.5f * (float)(1 << j);
     const uint32_t nrec = GATHER ? min(*R.count, R.cap) : 0u;
     const uint32_t groups = (R.n + kGroupRows - 1) / kGroupRows;
     // wave w forms output tiles (n tile w % 4, k tiles kWTiles (w / 4) + u) of
@@ -474,7 +512,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_train(const uint16_t* __restrict
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) x[0][kb][e] = (__bf16)encode(v, kslot_feature(kb, j, e));
+            for (int e = 0; e < 8; ++e) x[0][kb][e] = (__bf16)encode_slot(v, rev_scale, j, kb, e);
         f32x4 acc[4];
 #pragma unroll
         for (int l = 0; l < kHidden; ++l) {
@@ -875,7 +913,32 @@ void sum_floats(lr_guide* g, const float* x, uint32_t n, const uint32_t* count, 
                        out);
 }
 
+// k_infer over rows < n (or *n_dev): row-major mixture rows when ld is 0,
+// else columns of ld floats.
+int infer_rows(lr_guide* g, const float* inputs, uint32_t n, const uint32_t* n_dev, float* mixture, size_t ld,
+               hipStream_t s) {
+    const uint32_t tiles = (n + kInferRows - 1) / kInferRows;
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves,
+                                                          (uint32_t)g->ctx->n_cu * 3));
+    if (ld)
+        hipLaunchKernelGGL(k_infer<true>, dim3(grid), dim3(kInferBlock), kInferSmem, s, g->frag, g->w + kWtElems,
+                           inputs, n, n_dev, mixture, ld);
+    else
+        hipLaunchKernelGGL(k_infer<false>, dim3(grid), dim3(kInferBlock), kInferSmem, s, g->frag, g->w + kWtElems,
+                           inputs, n, n_dev, mixture, (size_t)0);
+    LR_HIP_CHECK(hipGetLastError());
+    return LR_OK;
+}
+
 }  // namespace
+
+int lr_guide_infer_columns(lr_guide* g, const float* inputs, uint32_t n, const uint32_t* n_dev, float* mixture,
+                           size_t ld, hipStream_t s) {
+    if (!g || (n && (!inputs || !mixture)) || ld < n) return lr::fail(LR_E_INVALID, "bad lr_guide_infer_columns arguments");
+    if (n == 0) return LR_OK;
+    if (int rc = lr::bind(g->ctx)) return rc;
+    return infer_rows(g, inputs, n, n_dev, mixture, ld, s);
+}
 
 extern "C" {
 
@@ -893,7 +956,9 @@ int lr_guide_create(lr_context* ctx, uint64_t seed, float lr, lr_guide** out) {
     *out = nullptr;
     if (!(lr >= 0.0f)) return lr::fail(LR_E_INVALID, "learning rate must be >= 0");
     if (int rc = lr::bind(ctx)) return rc;
-    LR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_infer),
+    LR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_infer<false>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInferSmem));
+    LR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_infer<true>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInferSmem));
     {
         const void* fns[] = {reinterpret_cast<const void*>(k_train<false, 4>),
@@ -972,13 +1037,7 @@ int lr_guide_infer(lr_guide* g, const float* inputs, uint32_t n, const uint32_t*
     if (!g || (n && (!inputs || !mixture))) return lr::fail(LR_E_INVALID, "NULL argument");
     if (n == 0) return LR_OK;
     if (int rc = lr::bind(g->ctx)) return rc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint32_t tiles = (n + kInferRows - 1) / kInferRows;
-    const uint32_t grid = std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, (uint32_t)g->ctx->n_cu * 3);
-    hipLaunchKernelGGL(k_infer, dim3(std::max(1u, grid)), dim3(kInferBlock), kInferSmem, s, g->frag,
-                       g->w + kWtElems, inputs, n, n_dev, mixture);
-    LR_HIP_CHECK(hipGetLastError());
-    return LR_OK;
+    return infer_rows(g, inputs, n, n_dev, mixture, 0, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

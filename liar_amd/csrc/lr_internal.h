@@ -71,6 +71,10 @@ int trace_reserve(lr_qbvh* bvh, uint32_t n_max, hipStream_t stream);
 // min(*count, cap) training records (r_in 8, r_dir 3, r_w 1 floats each;
 // Philox keyed by seed, counter (row, step, chunk)).  The record count stays
 // on the device and nothing is allocated, so the step can be captured.
+// lr_guide_infer with the mixture in 48 columns of ld (>= n) floats: value c
+// of row r at mixture[c * ld + r].
+int lr_guide_infer_columns(lr_guide* g, const float* inputs, uint32_t n, const uint32_t* n_dev, float* mixture,
+                           size_t ld, hipStream_t stream);
 // The radiance loss weighs each row by the guide's radiance weight times the
 // records' mean weight, as lr_guide_prepare_records last computed it.
 int lr_guide_train_records(lr_guide* g, const float* r_in, const float* r_dir, const float* r_w,

@@ -270,7 +270,8 @@ __global__ void k_fill_miss_queue(const uint32_t* count, Hits h) {
 // queue entry before shading.
 struct GuideArgs {
     const float* in;   // [queue][8] network inputs (written by k_guide_inputs)
-    const float* mix;  // [queue][48] mixture + strategy probabilities (lr_guide_infer)
+    const float* mix;  // [48][mix_ld] mixture, strategy probabilities, radiance (lr_guide_infer_columns)
+    uint32_t mix_ld;
     float sel;         // guide share of the non-light strategies
     float light_share; // probability of light sampling
     uint32_t learned;  // strategy probabilities from the network (mix[40..42]) instead of sel / light_share
@@ -298,27 +299,30 @@ struct GuideArgs {
     float* boost;
 };
 
-__device__ inline float vmf_mix_pdf(const float* m, V3 w) {
+// The mixture of one queue entry: value c at m[c * ld] (columns of the
+// inferred mixture).
+__device__ inline float vmf_mix_pdf(const float* m, uint32_t ld, V3 w) {
     float p = 0.0f;
     for (int k = 0; k < 8; ++k) {
-        const float kap = m[5 * k + 4];
-        const float c = m[5 * k + 1] * w.x + m[5 * k + 2] * w.y + m[5 * k + 3] * w.z;
-        p += m[5 * k] * kap / (2.0f * kPi * (-expm1f(-2.0f * kap))) * expf(kap * (c - 1.0f));
+        const float kap = m[(5 * k + 4) * (size_t)ld];
+        const float c = m[(5 * k + 1) * (size_t)ld] * w.x + m[(5 * k + 2) * (size_t)ld] * w.y +
+                        m[(5 * k + 3) * (size_t)ld] * w.z;
+        p += m[5 * k * (size_t)ld] * kap / (2.0f * kPi * (-expm1f(-2.0f * kap))) * expf(kap * (c - 1.0f));
     }
     return p;
 }
 
 // Lobe chosen by weight, then Jakob's stable vMF sampling ("Numerically
 // stable sampling of the von Mises-Fisher distribution on S^2", 2012).
-__device__ inline V3 vmf_mix_sample(const float* m, float u0, float u1, float u2) {
+__device__ inline V3 vmf_mix_sample(const float* m, uint32_t ld, float u0, float u1, float u2) {
     int k = 0;
     float acc = m[0];
     while (k < 7 && u0 >= acc) {
         ++k;
-        acc += m[5 * k];
+        acc += m[5 * k * (size_t)ld];
     }
-    const V3 mu = v3(m[5 * k + 1], m[5 * k + 2], m[5 * k + 3]);
-    const float kap = m[5 * k + 4];
+    const V3 mu = v3(m[(5 * k + 1) * (size_t)ld], m[(5 * k + 2) * (size_t)ld], m[(5 * k + 3) * (size_t)ld]);
+    const float kap = m[(5 * k + 4) * (size_t)ld];
     const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap;
     const float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     const float phi = 2.0f * kPi * u2;
@@ -464,12 +468,12 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                     // (power or balance heuristic over c_i p_i): the strategy
                     // probabilities are the network's (learned) or light
                     // sampling takes `light_share` and the guide `sel` of the rest
-                    const float* mx = G.mix + (size_t)i * 48;
+                    const float* mx = G.mix + i;  // column layout: value c at mx[c * G.mix_ld]
                     float cl, cg, cb;
                     if (G.learned) {
-                        cl = mx[40];
-                        cg = mx[41];
-                        cb = mx[42];
+                        cl = mx[40 * (size_t)G.mix_ld];
+                        cg = mx[41 * (size_t)G.mix_ld];
+                        cb = mx[42 * (size_t)G.mix_ld];
                     } else {
                         cl = G.light_share;
                         cg = (1.0f - cl) * G.sel;
@@ -509,7 +513,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         }
                     } else {
                         const lr::u4 r2 = rng(P, gp, gen, 3u);
-                        dw = vmf_mix_sample(mx, lr::u01(r2.x), lr::u01(r2.y), lr::u01(r2.z));
+                        dw = vmf_mix_sample(mx, G.mix_ld, lr::u01(r2.x), lr::u01(r2.y), lr::u01(r2.z));
                         wo = v3(dot(dw, tt), dot(dw, bb), dot(dw, nf));
                         chosen = 2;
                     }
@@ -521,7 +525,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         for (uint32_t k = 0; k < S.n_lights; ++k)
                             pl += S.lights[k].sel_pdf *
                                   (k == lk ? pdf_lk : light_pdf(S.lights[k], S.light_tris, x, dw));
-                        pg = vmf_mix_pdf(mx, dw);
+                        pg = vmf_mix_pdf(mx, G.mix_ld, dw);
                         pc = chosen == 0 ? cb * pb : (chosen == 1 ? cl * pl : cg * pg);
                     }
                     if (ok && pc > 0.0f) {
@@ -604,7 +608,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         // the path's total roulette weight stays <= 1 / rr_min
                         // (an estimate that drops along a path would otherwise
                         // compound the weights of successive survivals)
-                        const float lo = thr_in * G.mix[(size_t)i * 48 + 43];
+                        const float lo = thr_in * G.mix[43 * (size_t)G.mix_ld + i];
                         const float ref = gen == 0 ? 0.0f : G.est[pid];
                         if (ref > 0.0f) {
                             if (gen >= G.rr_start) {
@@ -1525,6 +1529,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
         g_mix = g_in + 8 * (size_t)cap;
         GA.in = g_in;
         GA.mix = g_mix;
+        GA.mix_ld = cap;
         GA.sel = gparams->selection;
         GA.light_share = gparams->light_share;
         if (!(GA.light_share >= 0.0f && GA.light_share < 1.0f))
@@ -1622,7 +1627,7 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             }
             if (guided && gen < guide_depth) {
                 hipLaunchKernelGGL(k_guide_inputs, grd, blk, 0, s, S, q[cur], counts + cur, H, ps, GA, g_in);
-                if (int rc = lr_guide_infer(guide, g_in, n, counts + cur, g_mix, s)) return rc;
+                if (int rc = lr_guide_infer_columns(guide, g_in, n, counts + cur, g_mix, cap, s)) return rc;
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), dim3((n + kAppendBlockWide - 1) / kAppendBlockWide),
                                    dim3(kAppendBlockWide), 0, s, S, P, gen, path0, q[cur], counts + cur, H,
                                    geom_args(sc), ps, q[nxt], counts + nxt, GA, NB);

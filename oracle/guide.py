@@ -44,19 +44,32 @@ def unpack(p):
     return Wt, b
 
 
+def kslot_of(f):
+    """K slot (kb, j, e) of logical feature f of a layer's input (the GPU's
+    MFMA operand layout): kb = f / 32, j = (f % 16) / 4, e = f % 4 (+4 past
+    16 within the 32-block)."""
+    kb, r = f >> 5, f & 31
+    return kb, (r & 15) >> 2, (r & 3) + (4 if r >= 16 else 0)
+
+
 def encode(inp):
+    """64 features of the inputs' first 6 values, feature f at K slot (kb, j,
+    e) = kslot_of(f), s = 8 kb + e: s < 12 sin (s even) / cos (s odd) of
+    2^j pi v_{s/2}; s >= 12: j = 0 2 v_{s-12} - 1, j = 1 (2 v_4 - 1, 2 v_5 - 1,
+    1, 0), j > 1 zero."""
     v = np.asarray(inp, np.float64)[:, :IN]
     n = v.shape[0]
     f = np.zeros((n, W))
-    f[:, :IN] = 2 * v - 1
-    c = IN
-    for d in range(IN):
-        for k in range(FREQ):
-            a = np.float32(np.pi) * (2 ** k) * v[:, d]
-            f[:, c] = np.sin(a)
-            f[:, c + 1] = np.cos(a)
-            c += 2
-    f[:, c] = 1.0
+    for c in range(W):
+        kb, j, e = kslot_of(c)
+        s = 8 * kb + e
+        if s < 12:
+            a = np.float32(np.pi) * (2 ** j) * v[:, s >> 1]
+            f[:, c] = np.cos(a) if s & 1 else np.sin(a)
+        elif j == 0:
+            f[:, c] = 2 * v[:, s - 12] - 1
+        elif j == 1:
+            f[:, c] = 2 * v[:, s - 8] - 1 if s < 14 else (1.0 if s == 14 else 0.0)
     return f
 
 

@@ -106,6 +106,28 @@ def test_radiance_stationary_point_is_the_mean():
     assert abs(grads[0]) < 1e-12 and grads[1] > 1e-3 and grads[2] < -1e-3
 
 
+def test_encoding_features():
+    """Every row's 64 features are, in some order, the 6 inputs as 2v - 1,
+    sin and cos of 2^k pi v_d for k < 4 and every input d, one constant 1
+    and 9 zeros; feature f's position is the GPU's K slot of f (the order the
+    kernels compute them in), checked on three hand-placed slots."""
+    rng = np.random.default_rng(11)
+    v = rng.random((5, 8))
+    f = G.encode(v)
+    for r in range(5):
+        want = [2 * v[r, d] - 1 for d in range(6)] + [1.0] + [0.0] * 9
+        for d in range(6):
+            for k in range(4):
+                a = np.float32(np.pi) * 2 ** k * v[r, d]
+                want += [np.sin(a), np.cos(a)]
+        assert np.allclose(np.sort(f[r]), np.sort(want), atol=1e-15)
+    # f = 11 is slot (kb 0, j 2, e 3): s = 3, cos(4 pi v_1); f = 54 is (1, 1, 6):
+    # s = 14 with j = 1, the constant; f = 1 is (0, 0, 1): s = 1, cos(pi v_0)
+    assert np.allclose(f[:, 4 * 2 + 3], np.cos(np.float32(np.pi) * 4 * v[:, 1]))
+    assert (f[:, 32 + 16 + 4 * 1 + 2] == 1.0).all()
+    assert np.allclose(f[:, 1], np.cos(np.float32(np.pi) * v[:, 0]))
+
+
 def test_selection_optimum_matches_closed_form():
     # With the target concentrated where only one strategy has density, the
     # selection loss drives that strategy's probability to its maximum
