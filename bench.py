@@ -310,7 +310,7 @@ def bench_instanced(args, la, torch, np):
             "build_s": round(build_s, 3)}
 
 
-GUIDED_CFG = dict(selection=0.5, light_share=0.5, learned_selection=True, train_steps=128, train_batch=4096,
+GUIDED_CFG = dict(selection=0.5, light_share=0.5, learned_selection=True, train_steps=512, train_batch=8192,
                   weight_clamp=20.0, record_depth=0)
 # radiance-driven Russian roulette on the network's reflected-radiance
 # estimate (trained with the lobes, weight 1): from generation 3 on, a path's
@@ -327,7 +327,7 @@ def bench_guided(args, la, torch, np):
     camera on the indirectly lit half; 1920x1080, depth 16.  Plain and guided
     renders accumulate 4-spp passes against a plain 4096-spp reference.  The
     guided clock includes its online training (the first 2 passes).  Each
-    method runs 3 times (render and guide seeds) and its MSE curve is the
+    method runs 5 times (render and guide seeds) and its MSE curve is the
     per-pass mean.  Reports the wall time at which the guided render first
     reaches the plain tracer's 64-spp MSE, by two measures: the MSE over all pixels, and over
     all but the 0.1 % of pixels with the largest error (the sun caustics of
@@ -465,7 +465,7 @@ def bench_guided(args, la, torch, np):
         "clamped_0.5": clamped,
         "guided_config": "one-sample MIS of BSDF, light and guide sampling (power heuristic) with the network's "
                          "per-vertex strategy probabilities, at every bounce; online training in the first "
-                         f"{GUIDED_TRAIN_PASSES} passes: 128 Adam steps (lr 1e-3, batch 4096) per 4M-path chunk on "
+                         f"{GUIDED_TRAIN_PASSES} passes: 512 Adam steps (lr 1e-3, batch 8192) per 4M-path chunk on "
                          "the records of every bounce, weights clamped at 20x their mean; radiance-driven Russian "
                          "roulette on the network's reflected-radiance estimate (radiance loss weight 1) from "
                          "generation 3 on, a path's roulette weight at most 5; 'no_roulette' figures: the same "
@@ -745,7 +745,7 @@ def main() -> int:
                     help="run only a short slit-room plain + guided sequence for profilers")
     ap.add_argument("--guided-width", type=int, default=1920)
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
-    ap.add_argument("--guided-repeats", type=int, default=3,
+    ap.add_argument("--guided-repeats", type=int, default=5,
                     help="plain and guided curves per method (averaged)")
     ap.add_argument("--render-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
