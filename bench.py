@@ -570,7 +570,7 @@ def bench_terrain(args, la, torch, np):
         "trace_config": ctx.trace_config() if tuned is None else tuned["best"],
         "autotune_default_ms": None if tuned is None or tuned["default_ms"] is None else round(tuned["default_ms"], 3),
         "autotune_best_ms": None if tuned is None else round(tuned["best_ms"], 3),
-        "chunk_paths": chunk or (1 << 22),
+        "chunk_paths": chunk or (1 << 25),
         "chunk_autotune_ms_16spp": chunk_table,
     }
     gs.close()
