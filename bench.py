@@ -310,8 +310,8 @@ def bench_instanced(args, la, torch, np):
             "build_s": round(build_s, 3)}
 
 
-GUIDED_CFG = dict(selection=0.5, light_share=0.5, learned_selection=True, train_steps=512, train_batch=8192,
-                  weight_clamp=20.0, record_depth=0)
+GUIDED_CFG = dict(selection=0.5, light_share=0.5, learned_selection=True, balance_heuristic=True, train_steps=512,
+                  train_batch=8192, weight_clamp=20.0, record_depth=0)
 # radiance-driven Russian roulette on the network's reflected-radiance
 # estimate (trained with the lobes, weight 1): from generation 3 on, a path's
 # roulette weight at most 5
@@ -463,7 +463,7 @@ def bench_guided(args, la, torch, np):
         "mse_trimmed": "MSE over all pixels but the 0.1% with the largest squared error",
         **{k: v for k, v in unclamped.items()},
         "clamped_0.5": clamped,
-        "guided_config": "one-sample MIS of BSDF, light and guide sampling (power heuristic) with the network's "
+        "guided_config": "one-sample MIS of BSDF, light and guide sampling (balance heuristic) with the network's "
                          "per-vertex strategy probabilities, at every bounce; online training in the first "
                          f"{GUIDED_TRAIN_PASSES} passes: 512 Adam steps (lr 1e-3, batch 8192) per 4M-path chunk on "
                          "the records of every bounce, weights clamped at 20x their mean; radiance-driven Russian "

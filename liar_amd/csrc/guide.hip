@@ -738,10 +738,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_train(const uint16_t* __restrict
 
 // Fixed-order sum of the workgroup partials, normalised by the batch's
 // weight sum; Adam; both bf16 copies.  A block takes kAdamParams parameters;
-// its 4 x 64 threads each sum every 4th partial (4 loads in flight), then
-// the four sums are added in a fixed order.  loss_out (block 0): the
+// each parameter's kAdamLanes threads sum every kAdamLanes-th partial (four
+// loads in flight, so a step's 128 partials take two rounds of loads), then
+// the lane sums are added in a fixed tree.  loss_out (block 0): the
 // normalised loss.
-constexpr int kAdamParams = 64;
+constexpr int kAdamParams = 16;
+constexpr int kAdamLanes = 256 / kAdamParams;
 __global__ __launch_bounds__(256) void k_adam(const float* __restrict__ partial, uint32_t chunks,
                                               float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
                                               uint16_t* __restrict__ frag, uint16_t* __restrict__ fragT, float lr,
@@ -761,13 +763,13 @@ __global__ __launch_bounds__(256) void k_adam(const float* __restrict__ partial,
     if (p < (uint32_t)kParams) {
         const float* q = partial + p;
         uint32_t c = cl;
-        for (; c + 12 < chunks; c += 16) {
+        for (; c + 3 * kAdamLanes < chunks; c += 4 * kAdamLanes) {
             g0 += q[(size_t)c * kPartial];
-            g1 += q[(size_t)(c + 4) * kPartial];
-            g2 += q[(size_t)(c + 8) * kPartial];
-            g3 += q[(size_t)(c + 12) * kPartial];
+            g1 += q[(size_t)(c + kAdamLanes) * kPartial];
+            g2 += q[(size_t)(c + 2 * kAdamLanes) * kPartial];
+            g3 += q[(size_t)(c + 3 * kAdamLanes) * kPartial];
         }
-        for (; c < chunks; c += 4) g0 += q[(size_t)c * kPartial];
+        for (; c < chunks; c += kAdamLanes) g0 += q[(size_t)c * kPartial];
     }
     s_g[threadIdx.x] = (g0 + g1) + (g2 + g3);
     __syncthreads();
@@ -776,12 +778,14 @@ __global__ __launch_bounds__(256) void k_adam(const float* __restrict__ partial,
             s_ws[threadIdx.x] += s_ws[threadIdx.x + k];
             s_ls[threadIdx.x] += s_ls[threadIdx.x + k];
         }
+        // the lane sums of each parameter: s_g[pl + kAdamParams * lane], a tree over lanes
+        if ((int)threadIdx.x < k && k >= kAdamParams) s_g[threadIdx.x] += s_g[threadIdx.x + k];
         __syncthreads();
     }
     const float inv = s_ws[0] > 0.0f ? 1.0f / s_ws[0] : 0.0f;
     if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) *loss_out = s_ls[0] * inv;
     if (cl != 0 || p >= (uint32_t)kParams) return;
-    const float g = ((s_g[pl] + s_g[pl + 64]) + (s_g[pl + 128] + s_g[pl + 192])) * inv;
+    const float g = s_g[pl] * inv;
     if (grad_out) grad_out[p] = g;
     if (lr == 0.0f) return;  // gradient only
     const float mm = b1 * m[p] + (1.0f - b1) * g;

@@ -32,8 +32,8 @@ pytestmark = pytest.mark.gpu
 
 WD, HD = 256, 144
 # the guided configuration of the slit-room benchmark
-GCFG = dict(selection=0.5, light_share=0.5, learned_selection=True, train_steps=128, train_batch=4096,
-            weight_clamp=20.0, record_depth=0)
+GCFG = dict(selection=0.5, light_share=0.5, learned_selection=True, balance_heuristic=True, train_steps=128,
+            train_batch=4096, weight_clamp=20.0, record_depth=0)
 
 
 def say(*a):
