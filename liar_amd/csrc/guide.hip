@@ -190,18 +190,20 @@ constexpr int kInferRT = 2;  // 16-row tiles per wave iteration (share each weig
 constexpr int kInferRows = 16 * kInferRT;
 constexpr size_t kInferSmem = sizeof(uint4) * kFrags * 64 + sizeof(float) * kBiasElems;
 
-// max(x, 0) as an integer max on the bits (a negative float or -0 is a
-// negative int32): one v_max_i32 per activation, where fmaxf also quiets
-// signalling NaNs first (the same value for every non-NaN x).
-__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+// bf16(max(x, 0)) as max(bf16(x), 0) on the bits: a negative bf16 or -0 is
+// a negative int16, so one packed integer max clamps a converted pair (the
+// same bits for every non-NaN x; fmaxf would also quieten signalling NaNs,
+// two instructions per value).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_pair(float a, float b) {
+    const bf16x2 r = __builtin_convertvector((f32x2){a, b}, bf16x2);  // one v_cvt_pk_bf16_f32
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, r), (s16x2)0));
+}
 __device__ __forceinline__ bf16x8 pack_relu(const f32x4& lo, const f32x4& hi) {
-    bf16x8 r;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        r[e] = (__bf16)relu(lo[e]);
-        r[e + 4] = (__bf16)relu(hi[e]);
-    }
-    return r;
+    return __builtin_bit_cast(bf16x8, make_uint4(relu_pair(lo[0], lo[1]), relu_pair(lo[2], lo[3]),
+                                                 relu_pair(hi[0], hi[1]), relu_pair(hi[2], hi[3])));
 }
 
 __device__ __forceinline__ f32x4 bias4(const float* s_b, int idx) {
