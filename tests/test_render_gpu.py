@@ -181,6 +181,27 @@ def test_terrain_small_same_seed_matches_oracle(gpu_ctx):
     assert frac_bad < 0.05
 
 
+def test_terrain_full_size_same_seed_matches_oracle(gpu_ctx):
+    """The terrain workload at full size (4,992,800 triangles, 10,000
+    spheres, all three shaders, sun + sky) through its own camera at 32x18
+    pixels x 2 spp, depth 16, against the oracle's FP64 brute-force render
+    of the same paths (same seed).  The GPU traverses its FP32 QBVH, the
+    oracle tests every primitive in FP64, so a path whose nearest hit ties
+    to FP32 precision may continue differently: the bar is the share of
+    pixels that differ (and the image total), not bit equality."""
+    import time
+
+    say("[gpu] full terrain 32x18 x 2 spp vs oracle (the oracle takes ~12 s on 16 threads)")
+    cs, cam = W.terrain()
+    gx, gw, st = gpu_render(gpu_ctx, cs, cam, 32, 18, 2, nl=1, max_depth=16, rr_depth=16, seed=11)
+    t0 = time.time()
+    ox, ow = oracle.render(cs, cam, 32, 18, 2, wavelengths=1, max_depth=16, rr_depth=16, seed=11)
+    say(f"[gpu] full terrain: oracle took {time.time() - t0:.1f} s; {st['rays']} GPU rays for {st['paths']} paths")
+    rel_total, frac_bad, _ = compare(gx, gw, ox, ow, "terrain-full")
+    assert frac_bad < 0.02
+    assert rel_total < 1e-2
+
+
 def test_terrain_full_size_renders(gpu_ctx):
     """Full terrain workload (4,992,800 triangles + 10,000 spheres) at reduced
     resolution: finite, non-negative, deterministic image."""
