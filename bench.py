@@ -88,6 +88,17 @@ def host_info() -> dict:
     return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
 
 
+def _film_error(fx, fw, ox, ow, np) -> dict:
+    """FP32 GPU film against the FP64 oracle's of the same paths: relative
+    error of the image total (Y) and the largest per-pixel difference
+    relative to the brightest pixel."""
+    g = fx.cpu().numpy().astype(np.float64) / fw.cpu().numpy().astype(np.float64)
+    o = ox / ow
+    return {"pixels": int(o.shape[1] * o.shape[2]),
+            "rel_total_y": float(f"{abs(g[1].sum() - o[1].sum()) / o[1].sum():.3e}"),
+            "max_pixel_diff_rel_max": float(f"{np.abs(g - o).max() / max(o[1].max(), 1e-30):.3e}")}
+
+
 def bench_render(args, la, oracle, torch, np):
     """Path-tracer leg: the Cornell-box workload (256x256, 16 spp, 4 stratified
     wavelengths, depth <= 8, Russian roulette from generation 3) rendered
@@ -132,12 +143,18 @@ def bench_render(args, la, oracle, torch, np):
         cw, ch = 64, 64
         threads = min(oracle.default_threads(), 16)
         t0 = time.perf_counter()
-        oracle.render(cs, cam, cw, ch, SPP, wavelengths=NL, max_depth=8, rr_depth=3, seed=1, threads=threads)
+        ox, ow = oracle.render(cs, cam, cw, ch, SPP, wavelengths=NL, max_depth=8, rr_depth=3, seed=1,
+                               threads=threads)
         t_cpu = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(cw * ch * SPP * NL / t_cpu, 1), "unit": "paths/s",
                                "cores": threads, "kind": "port", "host": host_info(),
                                "sample": f"{cw}x{ch} pixels of the same frame geometry (FP64 oracle), "
                                          f"{t_cpu:.2f} s"}
+        # the FP32 GPU render of the same frame and seed against the FP64 oracle
+        fx = torch.zeros((3, ch, cw), dtype=torch.float32, device="cuda")
+        fw = torch.zeros((ch, cw), dtype=torch.float32, device="cuda")
+        gs.render(camera, E.render_params(cw, ch, SPP, NL, 8, 3, 0.5, (380.0, 780.0), 1), fx, fw)
+        out["fp32_vs_fp64"] = _film_error(fx, fw, ox, ow, np)
     gs.close()
     ctx.close()
     return out
@@ -190,12 +207,16 @@ def bench_direct(args, la, oracle, torch, np):
         cw, ch = 128, 128
         threads = min(oracle.default_threads(), 16)
         t0 = time.perf_counter()
-        oracle.render_direct(cs, cam, cw, ch, SPP, wavelengths=NL, seed=1, threads=threads)
+        ox, ow = oracle.render_direct(cs, cam, cw, ch, SPP, wavelengths=NL, seed=1, threads=threads)
         t_cpu = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(cw * ch * SPP * NL / t_cpu, 1), "unit": "paths/s",
                                "cores": threads, "kind": "port", "host": host_info(),
                                "sample": f"{cw}x{ch} pixels of the same frame geometry (FP64 oracle, brute-force "
                                          f"shadow rays), {t_cpu:.2f} s"}
+        fx = torch.zeros((3, ch, cw), dtype=torch.float32, device="cuda")
+        fw = torch.zeros((ch, cw), dtype=torch.float32, device="cuda")
+        gs.render_direct(camera, E.render_params(cw, ch, SPP, NL, 0, 8, 0.5, (380.0, 780.0), 1), fx, fw)
+        out["fp32_vs_fp64"] = _film_error(fx, fw, ox, ow, np)
     gs.close()
     ctx.close()
     return out
@@ -729,6 +750,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--fp64-sample", type=int, default=2048,
+                    help="rays whose FP32 hits are compared with an FP64 brute force over the scene")
     ap.add_argument("--count-sample", type=int, default=1 << 18,
                     help="rays used for the oracle's nodes/prims-per-ray counts and parity")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracer leg")
@@ -901,6 +924,17 @@ def main() -> int:
     dev_b_ray_any = 32 + 1 + DEVICE_NODE_BYTES * cnt_any[1] / S + 48 * cnt_any[2] / S
     achieved = N_RAYS * b_ray / (kern_c * 1e-3) / 1e9
     achieved_any = N_RAYS * b_ray_any / (kern_a * 1e-3) / 1e9
+    # the FP32 results' error against an FP64 restatement of the same
+    # candidates (brute force over every primitive, all arithmetic in FP64)
+    F = min(S, args.fp64_sample)
+    log(f"[bench] FP32 vs FP64: brute force over {N_TRIS} triangles for {F} rays")
+    fp, ft, fu, fv = oracle.brute_closest_f64(verts, tris, np.zeros((0, 4), np.float32), rays_h[:, :F])
+    both = (g_prim[:F] == fp) & (fp != oracle.MISS)
+    rel_t = np.abs(g_tuv[0][:F][both].astype(np.float64) - ft[both]) / np.abs(ft[both])
+    err_uv = np.maximum(np.abs(g_tuv[1][:F][both] - fu[both]), np.abs(g_tuv[2][:F][both] - fv[both]))
+    fp64 = {"sample": F, "id_mismatches": int((g_prim[:F] != fp).sum()), "hits": int(both.sum()),
+            "max_rel_t": float(f"{rel_t.max():.3e}"), "median_rel_t": float(f"{np.median(rel_t):.3e}"),
+            "max_abs_uv": float(f"{err_uv.max():.3e}")}
     # compulsory traffic: every ray and hit once plus the whole scene once
     # (the device's 64 B nodes and 48 B primitive records)
     compulsory = N_RAYS * (32 + 16) + info["n_nodes"] * DEVICE_NODE_BYTES + len(prims) * 48
@@ -999,6 +1033,7 @@ def main() -> int:
             "tied_hit_fraction_closest": round(n_tied / S, 8),
             "tied_hit_mismatches_closest": mism_tied,
             "interval_quirk_rays_closest": n_quirk,
+            "fp32_vs_fp64_closest": fp64,
             "measured_copy_gbs": round(copy_gbs, 1),
             "device_frac_of_measured_copy": round(N_RAYS * dev_b_ray / (kern_c * 1e-3) / 1e9 / copy_gbs, 4),
             "compulsory_bytes_closest": int(compulsory),
