@@ -162,22 +162,21 @@ __device__ inline float log_vmf(float kappa, float c) {
     return logf(kappa) - logf(6.283185307179586f * (-expm1f(-2.0f * kappa))) + kappa * (c - 1.0f);
 }
 
-// Copies N16 16-byte words from global memory to LDS, each thread issuing
-// all of its loads before its first store (one memory round trip per block
-// instead of one per word).
+// Copies N16 16-byte words from global memory to LDS with the direct
+// global-to-LDS load (global_load_lds_dwordx4: lane l of a wave writes LDS
+// word base + l, no registers), every load of the block in flight at once;
+// the caller's __syncthreads() waits for them (s_waitcnt vmcnt(0) precedes
+// its barrier).  N16 is a multiple of 64, so every wave's words are whole.
+// (A copy through registers held 22 words per thread for the training
+// kernel, which the compiler kept in scratch.)
 template <int BLOCK, int N16>
 __device__ __forceinline__ void copy_to_lds(uint4* __restrict__ dst, const uint4* __restrict__ src) {
-    constexpr int kPer = (N16 + BLOCK - 1) / BLOCK;
-    uint4 r[kPer];
+    static_assert(N16 % 64 == 0 && BLOCK % 64 == 0, "whole waves");
+    const int wbase = (int)(threadIdx.x & ~63u), lane = (int)(threadIdx.x & 63u);
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int e = (int)threadIdx.x + k * BLOCK;
-        if (e < N16) r[k] = src[e];
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int e = (int)threadIdx.x + k * BLOCK;
-        if (e < N16) dst[e] = r[k];
+    for (int k = 0; k < (N16 + BLOCK - 1) / BLOCK; ++k) {
+        const int e0 = wbase + k * BLOCK;  // wave-uniform
+        if (e0 < N16) __builtin_amdgcn_global_load_lds(src + e0 + lane, dst + e0, 16, 0, 0);
     }
 }
 
