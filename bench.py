@@ -511,7 +511,7 @@ def bench_train(args, la, torch, np):
            "flops_basis": "forward 2(5*64*64 + 48*64) + input gradients 2(4*64*64 + 48*64) + weight gradients "
                           "2(5*64*64 + 48*64)",
            "launches_per_step": 2, "steps": []}
-    for n in (4096, 32768, 262144):
+    for n in (4096, 8192, 32768, 262144):
         rng = np.random.default_rng(n)
         x = np.zeros((n, 8), np.float32)
         x[:, :8] = rng.random((n, 8))
