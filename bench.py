@@ -338,7 +338,7 @@ GUIDED_CFG = dict(selection=0.5, light_share=0.5, learned_selection=True, balanc
 # roulette weight at most 5
 GUIDED_RR = dict(learned_rr=True, rr_scale=1.0, rr_min_survival=0.2, rr_start=3)
 GUIDED_RADIANCE_WEIGHT = 1.0
-GUIDED_TRAIN_PASSES = 2
+GUIDED_TRAIN_PASSES = 1
 
 
 def bench_guided(args, la, torch, np):
@@ -347,7 +347,7 @@ def bench_guided(args, la, torch, np):
     closed box room lit by the sun and sky only through a 2 % ceiling slit;
     camera on the indirectly lit half; 1920x1080, depth 16.  Plain and guided
     renders accumulate 4-spp passes against a plain 4096-spp reference.  The
-    guided clock includes its online training (the first 2 passes).  Each
+    guided clock includes its online training (the first pass).  Each
     method runs 5 times (render and guide seeds) and its MSE curve is the
     per-pass mean.  Reports the wall time at which the guided render first
     reaches the plain tracer's 64-spp MSE, by two measures: the MSE over all pixels, and over
@@ -485,8 +485,8 @@ def bench_guided(args, la, torch, np):
         **{k: v for k, v in unclamped.items()},
         "clamped_0.5": clamped,
         "guided_config": "one-sample MIS of BSDF, light and guide sampling (balance heuristic) with the network's "
-                         "per-vertex strategy probabilities, at every bounce; online training in the first "
-                         f"{GUIDED_TRAIN_PASSES} passes: 512 Adam steps (lr 1e-3, batch 8192) per 4M-path chunk on "
+                         "per-vertex strategy probabilities, at every bounce; online training during the first "
+                         f"{GUIDED_TRAIN_PASSES} pass(es): 512 Adam steps (lr 1e-3, batch 8192) per 4M-path chunk on "
                          "the records of every bounce, weights clamped at 20x their mean; radiance-driven Russian "
                          "roulette on the network's reflected-radiance estimate (radiance loss weight 1) from "
                          "generation 3 on, a path's roulette weight at most 5; 'no_roulette' figures: the same "
