@@ -644,16 +644,19 @@ int lr_render_batch(lr_batch* batch, const lr_batch_item* items, uint32_t flags,
  * 48 outputs) in BF16 on MFMA with FP32 accumulation.  Outputs 0..39 are an
  * 8-lobe von Mises-Fisher mixture over incident directions, 40 and 41 the
  * logits of the per-vertex strategy probabilities (light, guide, BSDF) =
- * 0.025 + 0.925 (r_l, (1 - r_l) r_g, (1 - r_l)(1 - r_g)), r = sigmoid.
+ * 0.025 + 0.925 (r_l, (1 - r_l) r_g, (1 - r_l)(1 - r_g)), r = sigmoid, and
+ * 42 the log of an estimate of the radiance the vertex reflects towards the
+ * previous one (a linear readout of the last hidden layer, trained when the
+ * radiance weight is > 0: lr_guide_set_radiance_weight).
  * Trained online with Adam on radiance samples the tracer produces.
  * Inputs: device float [n][8], per row (px, py, pz, theta_o/pi, phi_o/(2 pi),
  * (lambda - lambda_min)/(lambda_max - lambda_min), p_light, p_bsdf), the
  * first six in [0, 1]; the last two are read by training only (below).
  * Mixture: device float [n][48], per lobe (weight = softmax of the logits,
  * unit mean, kappa = 0.1 + e^min(s, 7.6)), then
- * (c_light, c_guide, c_bsdf, 0, 0, 0, 0, 0).
+ * (c_light, c_guide, c_bsdf, radiance e^clamp(out42, -20, 20), 0, 0, 0, 0).
  * Parameters (lr_guide_param_count() floats): per layer l (0..5) Wt_l
- * [n_out][64] row-major (n_out = 64, output layer 48 with rows 42..47 unused),
+ * [n_out][64] row-major (n_out = 64, output layer 48 with rows 43..47 unused),
  * then the biases (5 x 64 + 48). */
 typedef struct lr_guide lr_guide;
 int lr_guide_param_count(void);
@@ -667,7 +670,10 @@ int lr_guide_infer(lr_guide* guide, const float* inputs, uint32_t n, const uint3
 /* One Adam step on sum_i w_i (-log q(dir_i | input_i) - [p_l + p_b > 0]
  * log(c_l p_l + c_g q + c_b p_b)) / sum_i w_i, where p_l, p_b are inputs 6
  * and 7 of row i (the light and BSDF sampling pdfs of dir_i) and q is held
- * fixed inside the second term, which trains the strategy probabilities.
+ * fixed inside the second term, which trains the strategy probabilities;
+ * with a radiance weight lambda > 0, plus lambda / n sum_i (1 - min(w_i / L_i,
+ * 50))^2 with L_i the radiance output held fixed in the denominator, whose
+ * gradient trains output row 42 only.
  * dirs: device float [n][3] unit vectors; weights: device float [n] >= 0.
  * loss_out (device float, optional): the summed normalised loss.
  * grad_out (device float [param_count], optional): the gradient.  A guide
