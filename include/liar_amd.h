@@ -657,7 +657,10 @@ int lr_render_batch(lr_batch* batch, const lr_batch_item* items, uint32_t flags,
  * (c_light, c_guide, c_bsdf, radiance e^clamp(out42, -20, 20), 0, 0, 0, 0).
  * Parameters (lr_guide_param_count() floats): per layer l (0..5) Wt_l
  * [n_out][64] row-major (n_out = 64, output layer 48 with rows 43..47 unused),
- * then the biases (5 x 64 + 48). */
+ * then the biases (5 x 64 + 48).
+ * A guide's training scratch (workgroup partials, weight sums) is its own and
+ * shared by its calls: calls that train one guide on different streams at the
+ * same time race; inference alone may overlap. */
 typedef struct lr_guide lr_guide;
 int lr_guide_param_count(void);
 int lr_guide_create(lr_context* ctx, uint64_t seed, float learning_rate, lr_guide** out);
