@@ -500,7 +500,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                             dw = wo.x * tt + wo.y * bb + wo.z * nf;
                             chosen = 0;
                         }
-                    } else if (u_strat < cb + cl) {
+                    } else if (u_strat < cb + cl || !(cg > 0.0f)) {  // (cb + cl may round below 1 when cg = 0)
                         if (S.n_lights > 0) {
                             lk = light_pick(S, u_light);
                             const DevLight& Lk = S.lights[lk];
@@ -525,7 +525,7 @@ __global__ void k_shade(DevScene S0, Params P0, uint32_t gen, uint64_t path0, Qu
                         for (uint32_t k = 0; k < S.n_lights; ++k)
                             pl += S.lights[k].sel_pdf *
                                   (k == lk ? pdf_lk : light_pdf(S.lights[k], S.light_tris, x, dw));
-                        pg = vmf_mix_pdf(mx, G.mix_ld, dw);
+                        pg = cg > 0.0f ? vmf_mix_pdf(mx, G.mix_ld, dw) : 0.0f;  // no guide share: no mixture
                         pc = chosen == 0 ? cb * pb : (chosen == 1 ? cl * pl : cg * pg);
                     }
                     if (ok && pc > 0.0f) {
@@ -1583,6 +1583,10 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     BatchArgs NB;
     std::memset(&NB, 0, sizeof(NB));
     const bool training = guided && gparams->train && gparams->train_steps > 0;
+    // the mixture is read only by the guide strategy, learned selection and
+    // the roulette: without them (a call that samples with the plain
+    // strategies and only records, say) the inference is skipped
+    const bool infer = guided && (GA.learned || GA.sel > 0.0f || GA.rr);
 
     // graph mode: every launch below is captured on the scene's own stream
     // (the caller's may be the legacy null stream, which cannot capture)
@@ -1627,7 +1631,8 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
             }
             if (guided && gen < guide_depth) {
                 hipLaunchKernelGGL(k_guide_inputs, grd, blk, 0, s, S, q[cur], counts + cur, H, ps, GA, g_in);
-                if (int rc = lr_guide_infer_columns(guide, g_in, n, counts + cur, g_mix, cap, s)) return rc;
+                if (infer)
+                    if (int rc = lr_guide_infer_columns(guide, g_in, n, counts + cur, g_mix, cap, s)) return rc;
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true, false>), dim3((n + kAppendBlockWide - 1) / kAppendBlockWide),
                                    dim3(kAppendBlockWide), 0, s, S, P, gen, path0, q[cur], counts + cur, H,
                                    geom_args(sc), ps, q[nxt], counts + nxt, GA, NB);

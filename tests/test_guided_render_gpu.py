@@ -39,18 +39,22 @@ def _y(fx, fw):
     return (fx / fw)[1].cpu().numpy().astype(np.float64)
 
 
-def test_guided_furnace_unbiased_with_untrained_guide(gpu_ctx):
+@pytest.mark.parametrize("selection", [0.5, 0.0])
+def test_guided_furnace_unbiased_with_untrained_guide(gpu_ctx, selection):
+    """selection 0 (no guide share, no learned selection or roulette) runs no
+    inference: the shading must then never read the mixture."""
     sc, cam = W.furnace_sphere(shaders.Lambert(textures.Constant(1.0)))
     gs = E.GpuScene(gpu_ctx, la.compile_scene(sc))
     g = Guide(gpu_ctx, seed=7)
     fx, fw = _film(128, 128)
     E.guided_render(gs, g, E.camera_struct(cam), E.render_params(128, 128, 32768, 1, 0, 8, 0.5, (380, 780), 3),
-                    fx, fw, selection=0.5)
+                    fx, fw, selection=selection)
     y = _y(fx, fw)
     yy, xx = np.mgrid[0:128, 0:128]
     Y = y[(xx - 63.5) ** 2 + (yy - 63.5) ** 2 < 36 ** 2]
     three_sigma = 3 * Y.std() / math.sqrt(Y.size)
-    print(f"[gpu] guided furnace: mean {Y.mean():.5f}, 3 sigma {three_sigma:.1e}", flush=True)
+    print(f"[gpu] guided furnace (selection {selection}): mean {Y.mean():.5f}, 3 sigma {three_sigma:.1e}",
+          flush=True)
     assert three_sigma < 5e-4
     assert abs(Y.mean() - 1.0) < 1e-3
     g.close()
