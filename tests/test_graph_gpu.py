@@ -67,25 +67,28 @@ def test_graph_guided_inference_matches_eager_and_training_runs(gpu_ctx):
     sc, cam = W.cornell_box()
     gs = E.GpuScene(gpu_ctx, la.compile_scene(sc))
     cs = E.camera_struct(cam)
-    g = Guide(gpu_ctx, seed=4, learning_rate=2e-3)
-    # train a little (eager), then compare inference-only renders
+    g = Guide(gpu_ctx, seed=4, learning_rate=2e-3, radiance_weight=1.0)
+    # train a little (eager), then compare inference-only renders (with the
+    # radiance-driven roulette, whose per-path state is reset by the kernels)
     E.guided_render(gs, g, cs, E.render_params(64, 64, 16, 1, 8, 3, 0.5, (380, 780), 1), *_film(64, 64),
                     train=True, train_steps=8, train_batch=8192)
     for call in range(3):
         prm = E.render_params(64, 64, 16, 2, 8, 3, 0.5, (380, 780), 20 + call, 1 << 15)
         fe = _film(64, 64)
-        E.guided_render(gs, g, cs, prm, *fe, learned_selection=True, balance_heuristic=True)
+        E.guided_render(gs, g, cs, prm, *fe, learned_selection=True, balance_heuristic=True, learned_rr=True)
         eager = _np(*fe)
         prm.graph = 1
         fg = _film(64, 64)
-        E.guided_render(gs, g, cs, prm, *fg, learned_selection=True, balance_heuristic=True)
+        E.guided_render(gs, g, cs, prm, *fg, learned_selection=True, balance_heuristic=True, learned_rr=True)
         got = _np(*fg)
         assert np.array_equal(eager[0], got[0]) and np.array_equal(eager[1], got[1]), f"call {call}"
-    # training inside the graph: the guide's parameters move and stay finite
+    # training inside the graph (records, the clamp and radiance sums, Adam):
+    # the guide's parameters move and stay finite
     p0 = g.params()
     for call in range(3):
         prm = E.render_params(64, 64, 16, 1, 8, 3, 0.5, (380, 780), 40 + call, 1 << 15, graph=True)
-        E.guided_render(gs, g, cs, prm, *_film(64, 64), train=True, train_steps=4, train_batch=4096)
+        E.guided_render(gs, g, cs, prm, *_film(64, 64), train=True, train_steps=4, train_batch=4096,
+                        weight_clamp=20.0, learned_rr=True)
     torch.cuda.synchronize()
     p1 = g.params()
     assert np.isfinite(p1).all() and not np.array_equal(p0, p1)
