@@ -1,6 +1,10 @@
 """Pins the FP64 guiding-network oracle (oracle/guide.py): analytic gradient
-vs central finite differences, mixture pdf normalised on the sphere, bf16
-rounding matches the IEEE round-to-nearest-even definition."""
+vs central finite differences (lobes, selection and the reflected-radiance
+term, whose gradient stops at its readout row), the radiance term's
+stationary point at the mean target, the encoding's feature set and slot
+order, the selection optimum against a brute-force grid, mixture pdf
+normalised on the sphere, bf16 rounding matches the IEEE
+round-to-nearest-even definition."""
 import numpy as np
 
 from oracle import guide as G
