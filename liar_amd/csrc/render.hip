@@ -2144,6 +2144,12 @@ __device__ inline lr::u4 dl_rng(const Params& P, uint64_t gp, uint32_t gen, uint
     return rng(P, gp, gen | (br << 8), purpose);
 }
 
+// Fifth uniform of a 128-bit block whose four words also give four: the low
+// bytes of x, y, z as a 24-bit fraction (u01 reads the top 24 bits).
+__device__ inline uint32_t dl_pair_fifth(const lr::u4& r) {
+    return ((r.x & 0xffu) | ((r.y & 0xffu) << 8) | ((r.z & 0xffu) << 16)) << 8;
+}
+
 struct DLHit {
     BsdfParams B;
     V3 wi, tt, bb, nf, ngl, start;
@@ -2163,8 +2169,14 @@ struct DLHit {
 // le_k / le_v cache the radiance of the last light a candidate of this hit
 // used (candidates of one light are consecutive slots): a sampled spectrum
 // costs a binary search through global memory.
+// The two candidates of a pair (2s, 2s+1: same light, same pick) share their
+// random numbers: the even one draws the pick (purpose 10 + s, secondary
+// samples only) and one 128-bit block (purpose 100 + s) into rs / rp, the
+// odd one reads them.  Whatever ends the even candidate before a draw ends
+// the odd one too, so the odd one never reads a stale block.
 __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DLArgs& A, const DLHit& H, uint32_t j,
-                                    float& c, V3& dir, float& far, uint32_t& le_k, float& le_v) {
+                                    float& c, V3& dir, float& far, uint32_t& le_k, float& le_v, lr::u4& rs,
+                                    lr::u4& rp) {
     uint32_t k = 0, local = 0;
     float nl = 1.0f, scale = 1.0f;
     if (H.hq) {
@@ -2183,7 +2195,7 @@ __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DL
         nl = (float)S.lights[k].samples;
     } else {
         if (j >= 2u * A.secondary || S.n_lights == 0) return false;
-        const lr::u4 rs = dl_rng(P, H.gp, H.gen, H.br, 10u + (j >> 1));
+        if ((j & 1u) == 0u) rs = dl_rng(P, H.gp, H.gen, H.br, 10u + (j >> 1));
         k = light_pick(S, lr::u01(rs.x));
         const float psel = S.lights[k].sel_pdf;
         if (!(psel > 0.0f)) return false;
@@ -2197,12 +2209,12 @@ __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DL
     }
     const float Le = le_v;
     if (Le == 0.0f) return false;
-    const lr::u4 r = dl_rng(P, H.gp, H.gen, H.br, 100u + j);
+    if ((j & 1u) == 0u) rp = dl_rng(P, H.gp, H.gen, H.br, 100u + (j >> 1));
     float f = 0.0f, pb = 0.0f, pl = 0.0f;
     float dist = INFINITY;  // to the mesh light's surface along dir
     if ((local & 1u) == 0u) {
         // light sample, weighted against BSDF sampling (ray_tracer.cpp:211-231)
-        dir = light_sample(Lk, S.light_tris, H.start, lr::u01(r.x), lr::u01(r.y), pl, dist);
+        dir = light_sample(Lk, S.light_tris, H.start, lr::u01(rp.x), lr::u01(rp.y), pl, dist);
         if (!(pl > 0.0f)) return false;
         // sampleEmission with the target's normal rejects lights behind the
         // shading normal (light_area.cpp:242-252)
@@ -2213,8 +2225,8 @@ __device__ inline bool dl_candidate(const DevScene& S, const Params& P, const DL
     } else {
         // BSDF sample that reaches the light (ray_tracer.cpp:234-264)
         V3 wo;
-        if (!bsdf_sample(H.B, H.wi, lr::u01(r.x), lr::u01(r.y), lr::u01(r.z), wo, f, pb) || !(pb > 0.0f) ||
-            f == 0.0f)
+        if (!bsdf_sample(H.B, H.wi, lr::u01(rp.z), lr::u01(rp.w), lr::u01(dl_pair_fifth(rp)), wo, f, pb) ||
+            !(pb > 0.0f) || f == 0.0f)
             return false;
         dir = normalize(wo.x * H.tt + wo.y * H.bb + wo.z * H.nf);
         pl = light_pdf(Lk, S.light_tris, H.start, dir);
@@ -2367,10 +2379,11 @@ __global__ __launch_bounds__(256) void k_dl_shade(DevScene S, Params P, uint32_t
         }
         uint32_t le_k = ~0u;
         float le_v = 0.0f;
+        lr::u4 rs{0u, 0u, 0u, 0u}, rp{0u, 0u, 0u, 0u};
         for (uint32_t j = 0; j < A.slots; ++j) {
             float c = 0.0f, far = 0.0f;
             V3 dir = v3(0, 0, 1);
-            const bool alive = surf && dl_candidate(S, P, A, H, j, c, dir, far, le_k, le_v);
+            const bool alive = surf && dl_candidate(S, P, A, H, j, c, dir, far, le_k, le_v, rs, rp);
             // block-local compaction into the stage: one barrier per slot
             const uint64_t mask = __ballot(alive);
             if (lane == 0) s_wcnt[parity][wave] = (uint32_t)__popcll(mask);

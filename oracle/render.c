@@ -867,7 +867,16 @@ typedef struct {
  * power and whose estimate is divided by (samples x pick pdf)
  * (direct_lighting.cpp:259-310).  Shadow rays to a parallelogram end 1e-4
  * (relative) short of its plane. */
-static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
+static uint32_t dl_pair_fifth(const uint32_t* r) {
+    return ((r[0] & 0xffu) | ((r[1] & 0xffu) << 8) | ((r[2] & 0xffu) << 16)) << 8;
+}
+
+/* The two candidates of a pair (2s, 2s+1) share the light pick (purpose
+ * 10 + s) and one 128-bit block (purpose 100 + s): the light sample takes
+ * words 0, 1, the BSDF sample words 2, 3 and the low bytes of words 0-2 as a
+ * fifth 24-bit uniform.  The even candidate draws into rs / rp, the odd one
+ * reads them. */
+static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j, uint32_t* rs, uint32_t* rp) {
     const Scene* S = c->S;
     uint32_t k = 0, local = 0;
     double nl = 1, scale = 1;
@@ -883,8 +892,7 @@ static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
         nl = S->light_ns[k];
     } else {
         if (j >= 2 * c->dl->secondary || S->n_lights == 0) return 0;
-        uint32_t rs[4];
-        rng(c, H->p, H->key, 10u + (j >> 1), rs);
+        if ((j & 1u) == 0) rng(c, H->p, H->key, 10u + (j >> 1), rs);
         k = light_pick(S, u01(rs[0]));
         double psel = S->light_sel[k];
         if (!(psel > 0)) return 0;
@@ -894,12 +902,11 @@ static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
     const or_light* L = &S->lights[k];
     double Le = spec_eval(S, L->radiance, H->lam, 0);
     if (Le == 0) return 0;
-    uint32_t r[4];
-    rng(c, H->p, H->key, 100u + j, r);
+    if ((j & 1u) == 0) rng(c, H->p, H->key, 100u + (j >> 1), rp);
     double f = 0, pb = 0, pl = 0, contrib, dist = INFINITY;
     V dir;
     if ((local & 1u) == 0) {
-        dir = light_sample(S, k, H->start, u01(r[0]), u01(r[1]), &pl, &dist);
+        dir = light_sample(S, k, H->start, u01(rp[0]), u01(rp[1]), &pl, &dist);
         if (!(pl > 0)) return 0;
         /* sampleEmission with the target normal (light_area.cpp:242-252) */
         if (!(dot(dir, H->nf) > 0)) return 0;
@@ -909,7 +916,8 @@ static double dl_candidate(const Ctx* c, const DLHit* H, uint32_t j) {
         contrib = f * Le * wt * fabs(wo.z) / (nl * pl * scale);
     } else {
         V wo;
-        if (!bsdf_sample(&H->B, H->wi, u01(r[0]), u01(r[1]), u01(r[2]), &wo, &f, &pb, 3u) || !(pb > 0) || f == 0)
+        if (!bsdf_sample(&H->B, H->wi, u01(rp[2]), u01(rp[3]), u01(dl_pair_fifth(rp)), &wo, &f, &pb, 3u) ||
+            !(pb > 0) || f == 0)
             return 0;
         dir = nrm(add(add(mul(wo.x, H->tt), mul(wo.y, H->bb)), mul(wo.z, H->nf)));
         pl = light_pdf(S, k, H->start, dir);
@@ -979,7 +987,8 @@ static double dl_cast(const Ctx* c, uint64_t p, V o, V d, double tn, double lam,
         for (uint32_t k = 0; k < S->n_lights; ++k) slots += 2 * S->light_ns[k];
         if (2 * A->secondary > slots) slots = 2 * A->secondary;
     }
-    for (uint32_t j = 0; j < slots; ++j) L += dl_candidate(c, &H, j);
+    uint32_t rs[4] = {0, 0, 0, 0}, rp[4] = {0, 0, 0, 0};
+    for (uint32_t j = 0; j < slots; ++j) L += dl_candidate(c, &H, j, rs, rp);
     /* specular / glossy continuations (direct_lighting.cpp:314-377) */
     int glossy = m->type == 5 || m->type == 6;
     if (gen < A->max_gen && (is_specular(m->type) || (glossy && A->glossy))) {
