@@ -427,8 +427,9 @@ typedef struct lr_render_params {
     float lambda_min, lambda_max; /* wavelength sampling domain [nm]       */
     uint64_t seed;
     uint32_t chunk_paths;  /* paths per wavefront chunk, whole pixels (0: 32M
-                              for lr_render, 4M for lr_render_guided and
-                              lr_render_direct)                              */
+                              for lr_render and lr_render_guided without
+                              training, 4M for lr_render_guided with
+                              training and for lr_render_direct)             */
     float max_sample_luminance; /* Raster::maxSampleLuminance: a sample whose
                                    Y exceeds it is scaled down to it
                                    (raster.cpp:567-570); 0 = no clamp       */

@@ -1503,9 +1503,12 @@ static int render_impl(lr_scene* sc, const lr_camera* cam, const lr_render_param
     const uint64_t ppp = (uint64_t)R.spp * R.wavelengths;
     const uint64_t n_pix = (uint64_t)R.width * R.height;
     // default queue tile: 32M paths (every bounce launch's tail is paid once
-    // per chunk; 3.3 GB of queues), 4M for guided renders, whose chunk is also
-    // the training cadence; never more than the render's own paths
-    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (guide ? (1u << 22) : (1u << 25));
+    // per chunk; 3.3 GB of queues), 4M for guided renders that train, whose
+    // chunk is also the training cadence; never more than the render's own
+    // paths.  Without training the tile changes no result (whole pixels per
+    // chunk, random numbers keyed by the global path id).
+    const bool trains = guide && gparams && gparams->train;
+    const uint64_t chunk_req = R.chunk_paths ? R.chunk_paths : (trains ? (1u << 22) : (1u << 25));
     const uint64_t pix_per_chunk = std::min<uint64_t>(std::max<uint64_t>(1, chunk_req / ppp), std::max<uint64_t>(1, n_pix));
     const uint64_t cap64 = pix_per_chunk * ppp;
     if (cap64 >= (1ull << 31)) return lr::fail(LR_E_INVALID, "spp * wavelengths too large for one chunk");

@@ -206,3 +206,29 @@ def test_trained_guide_lowers_mse_on_hidden_emitter(gpu_ctx):
     assert abs(y.mean() - ref.mean()) / ref.mean() < 0.03
     g.close()
     gs.close()
+
+
+def test_inference_render_independent_of_chunking(gpu_ctx):
+    """Without training the queue tile changes no result: a guided render in
+    one chunk (the default tile) and in many small chunks gives bit-identical
+    films, with learned selection, the balance heuristic and the roulette."""
+    import torch
+
+    print("[gpu] guided inference render: one chunk vs many", flush=True)
+    sc, cam = W.cornell_box()
+    gs = E.GpuScene(gpu_ctx, la.compile_scene(sc))
+    cs = E.camera_struct(cam)
+    g = Guide(gpu_ctx, seed=6, learning_rate=2e-3, radiance_weight=1.0)
+    E.guided_render(gs, g, cs, E.render_params(64, 48, 16, 1, 8, 3, 0.5, (380, 780), 1), *_film(64, 48),
+                    train=True, train_steps=8, train_batch=4096)
+    films = []
+    for chunk in (0, 1 << 12):
+        fx, fw = _film(64, 48)
+        st = E.guided_render(gs, g, cs, E.render_params(64, 48, 8, 2, 8, 3, 0.5, (380, 780), 9, chunk), fx, fw,
+                             learned_selection=True, balance_heuristic=True, learned_rr=True)
+        torch.cuda.synchronize()
+        films.append((fx.cpu().numpy(), fw.cpu().numpy(), st["chunks"]))
+    assert films[0][2] == 1 and films[1][2] > 1
+    assert np.array_equal(films[0][0], films[1][0]) and np.array_equal(films[0][1], films[1][1])
+    g.close()
+    gs.close()
