@@ -641,10 +641,10 @@ def bench_batch(args, la, torch, np):
         res = gss[big].qbvh.autotune(tune_rays, repeats=2)
         ctx.set_trace_config(**res["best"])
     # the queue tile (paths per chunk; chunks span scenes): the fastest batch
-    # call of 4M, 16M and 32M paths, each after a warm-up call (code load,
-    # workspace, graph instantiation)
+    # call of 4M, 16M, 32M and 64M paths, each after a warm-up call (code
+    # load, workspace, graph instantiation)
     chunk_table = []
-    for chunk in (1 << 22, 1 << 24, 1 << 25):
+    for chunk in (1 << 22, 1 << 24, 1 << 25, 1 << 26):
         batch.render(items(chunk), graph=True)
         it = items(chunk)
         torch.cuda.synchronize()

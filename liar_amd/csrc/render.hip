@@ -1938,6 +1938,7 @@ int lr_render_batch(lr_batch* b, const lr_batch_item* items, uint32_t flags, flo
     if (b->cap < cap) {
         if (b->d_work) LR_HIP_CHECK(hipFree(b->d_work));
         b->d_work = nullptr;
+        b->cap = 0;  // a failed allocation below leaves no stale capacity behind
         LR_HIP_CHECK(hipMalloc(&b->d_work, sizeof(float) * 28 * cap));
         b->cap = cap;
     }
